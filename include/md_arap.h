@@ -1,0 +1,175 @@
+/*
+ * md_arap.h -- C ABI of libmdarap.so, the MI355X (gfx950) As-Rigid-As-Possible
+ * deformation library.
+ *
+ * The ABI is the drop-in boundary for the deformation path of cheind/mesh-deform:
+ * include/deform/arap.h (the C++ class AsRigidAsPossibleDeformation) forwards every call
+ * to it, and the Python tests and bench drive it through ctypes. Signatures use plain C
+ * types only: pointers to host memory, sizes and opaque handles. Every function returns an
+ * md_status (0 = success); md_last_error() gives a message for the calling thread.
+ *
+ * Threading: a context and everything created from it belong to one host thread at a time
+ * (the reference class is single-threaded too). A server (md_server_*) owns an internal
+ * worker thread; its submit/poll/wait calls may come from any thread.
+ * Ownership: the caller owns every host buffer it passes; the library owns device memory.
+ */
+#ifndef MD_ARAP_H
+#define MD_ARAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MD_OK = 0,
+    MD_ERR_INVALID = 1,   /* bad argument: null pointer, size, out-of-range vertex index */
+    MD_ERR_NO_DEVICE = 2, /* no usable GPU, or the device index is out of range */
+    MD_ERR_HIP = 3,       /* a HIP runtime call failed */
+    MD_ERR_FACTOR = 4,    /* the system is not positive definite (deform() returns false) */
+    MD_ERR_NOMEM = 5,     /* host or device allocation failed */
+    MD_ERR_BUDGET = 6,    /* the request does not fit the server's HBM budget */
+    MD_ERR_STATE = 7,     /* the call is not valid in the object's current state */
+    MD_ERR_TIMEOUT = 8    /* md_server_wait timed out */
+} md_status;
+
+typedef enum {
+    MD_PREC_FP64 = 0,        /* every step in FP64 (the reference's double PrecisionType) */
+    MD_PREC_FP32_REFINE = 1  /* FP32 local step and solve, FP64 factor and residual */
+} md_precision;
+
+typedef enum {
+    MD_SOLVER_LDLT = 0, /* supernodal block LDL^T, level-scheduled triangular sweeps */
+    MD_SOLVER_PCG = 1   /* Jacobi-preconditioned CG on the residual, warm-started */
+} md_solver;
+
+typedef enum {
+    MD_REST_REFERENCE = 0, /* each dirty deform() takes the current mesh as the rest pose
+                              (inc/deform/arap.h:102-120,162-168) */
+    MD_REST_FIXED = 1      /* the rest pose, weights and factor of the first deform() are
+                              kept; later calls only move targets and warm-start */
+} md_rest_mode;
+
+typedef struct {
+    int precision;     /* md_precision, default MD_PREC_FP64 */
+    int solver;        /* md_solver, default MD_SOLVER_LDLT */
+    int refine_steps;  /* residual/solve steps per ARAP iteration (>= 1), default 1 */
+    int rest_mode;     /* md_rest_mode, default MD_REST_REFERENCE */
+    int pcg_max_iters; /* MD_SOLVER_PCG: CG iterations per solve, default 200 */
+    double pcg_tol;    /* MD_SOLVER_PCG: relative residual target, default 1e-7 */
+    int use_graph;     /* replay iterations from a captured hipGraph, default 1 */
+} md_config;
+
+typedef struct {
+    int n, nf, F;            /* vertices, free vertices, faces */
+    int nslices;             /* 64-vertex one-ring slices */
+    int supernodes, levels;  /* supernodal tree size and height */
+    int max_sn_width, max_sn_rows;
+    int64_t slots;           /* one-ring slots incl. padding (64 * slice widths) */
+    int64_t cotan_nnz;       /* nonzeros of the cotan weight matrix (2E) */
+    int64_t panel_entries;   /* stored factor entries (Dinv + G blocks) */
+    int64_t factor_nnz;      /* strictly-lower entries of the equivalent LDL^T factor */
+    int64_t fwd_items, bwd_items;
+    int64_t device_bytes;    /* device memory held by the mesh */
+    double setup_ms_host;    /* last analysis + factorization time on the host */
+} md_mesh_stats;
+
+/* ---- library ------------------------------------------------------------------------ */
+void md_config_default(md_config *cfg);
+const char *md_last_error(void);
+const char *md_version(void);
+int md_device_count(int *count);
+
+/* ---- context: one GPU, one HIP stream ------------------------------------------------ */
+typedef struct md_ctx md_ctx;
+int md_ctx_create(int device, md_ctx **out);
+int md_ctx_destroy(md_ctx *ctx);
+int md_ctx_synchronize(md_ctx *ctx);
+
+/* ---- one mesh: the AsRigidAsPossibleDeformation object ------------------------------ */
+typedef struct md_mesh md_mesh;
+
+/* Replaces the constructor + initializeMeshTopology (inc/deform/arap.h:66-70,149-155):
+ * n vertices, F faces given as 3F vertex indices. Positions are read at deform time. */
+int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const md_config *cfg, md_mesh **out);
+int md_mesh_destroy(md_mesh *mesh);
+
+/* Replaces setConstraint (inc/deform/arap.h:81-85): insert or overwrite a target and mark
+ * the object dirty. Indices outside [0, n) are MD_ERR_INVALID (undefined in the reference). */
+int md_mesh_set_constraint(md_mesh *mesh, int vidx, const double *xyz3);
+int md_mesh_set_constraints(md_mesh *mesh, int k, const int *idx, const double *xyz);
+
+/* Replaces deform (inc/deform/arap.h:101-138). xyz is the adapter's vertex storage (3n
+ * doubles, original vertex order): read when the object is dirty, overwritten by the
+ * write-back. *ok receives the reference's bool: 1, or 0 when the factorization failed
+ * (then the status is MD_ERR_FACTOR and xyz is untouched). With no constraints it returns
+ * with *ok = 1 and leaves xyz untouched, like arap.h:113-114. */
+int md_mesh_deform(md_mesh *mesh, int iterations, double *xyz, int *ok);
+
+/* Device-resident variants for measurement and batching: prepare runs the dirty setup of
+ * deform() (positions from xyz) without iterating; iterate runs iterations on the device
+ * state without any host copy; download copies the current positions (original order). */
+int md_mesh_prepare(md_mesh *mesh, const double *xyz);
+int md_mesh_iterate(md_mesh *mesh, int iterations);
+int md_mesh_download(md_mesh *mesh, double *xyz);
+
+/* The reference's PrivateAccessor hook (tests/accessor.h:16-22): the cotan weight matrix
+ * of the current rest pose as CSR (original numbering, ascending columns, no diagonal). */
+int md_mesh_cotan_nnz(md_mesh *mesh, int64_t *nnz);
+int md_mesh_cotan_csr(md_mesh *mesh, int *rowptr, int *col, double *val);
+
+/* Rotations of the last local step, row-major 3x3 per vertex in original order. */
+int md_mesh_rotations(md_mesh *mesh, double *R);
+/* ARAP energy sum_i sum_j w_ij |(p'_i - p'_j) - R_i (p_i - p_j)|^2 of the current state. */
+int md_mesh_energy(md_mesh *mesh, double *energy);
+int md_mesh_get_stats(md_mesh *mesh, md_mesh_stats *out);
+
+/* Times `iterations` device iterations (after `warmup` untimed ones) with HIP events on the
+ * context's stream: total milliseconds, and per-phase milliseconds summed over the timed
+ * iterations: [0] local step, [1] residual, [2] forward sweeps, [3] backward sweeps. */
+int md_mesh_time_iterations(md_mesh *mesh, int warmup, int iterations, double *total_ms,
+                            double *phase_ms4);
+
+/* ---- batches: many prepared meshes iterated in the same launches -------------------- */
+typedef struct md_batch md_batch;
+int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, md_batch **out);
+int md_batch_iterate(md_batch *batch, int iterations);
+int md_batch_time_iterations(md_batch *batch, int warmup, int iterations, double *total_ms);
+int md_batch_destroy(md_batch *batch);
+
+/* ---- serving: resident meshes, continuous batching of drag requests ----------------- */
+typedef struct md_server md_server;
+typedef struct {
+    size_t hbm_budget_bytes; /* device bytes the resident meshes may hold (0 = unlimited) */
+    int max_active;          /* requests iterated together (0 = unlimited) */
+} md_server_config;
+
+int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_server **out);
+/* Registers a prepared mesh (md_mesh_prepare done, MD_REST_FIXED); fails with
+ * MD_ERR_BUDGET when it does not fit the budget. The server takes no ownership. */
+int md_server_add_mesh(md_server *srv, md_mesh *mesh, int *mesh_id);
+/* Queues a drag: move the k constrained vertices idx to xyz, then run `iterations`
+ * warm-started iterations. Indices must already be constrained on that mesh. */
+int md_server_submit(md_server *srv, int mesh_id, int k, const int *idx, const double *xyz,
+                     int iterations, uint64_t *ticket);
+/* done = 1 when finished; positions (3n, original order, may be NULL) and the latency in
+ * milliseconds from submit to completion are returned once done. */
+int md_server_poll(md_server *srv, uint64_t ticket, int *done, double *xyz, double *latency_ms);
+int md_server_wait(md_server *srv, uint64_t ticket, double timeout_ms, double *xyz, double *latency_ms);
+int md_server_destroy(md_server *srv);
+
+/* ---- host-only diagnostics (no GPU needed) ------------------------------------------ */
+/* Factors the SPD matrix given as full CSR (n rows, ascending columns), with coordinates
+ * xyz (3 per row) for the ordering, by the same host analysis and FP64 block-LDL^T the
+ * library uploads, then solves nrhs right-hand sides (b[r*n + i]) on the host. */
+int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
+                         const double *xyz, int nrhs, const double *b, double *x,
+                         int64_t *panel_entries, int *levels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MD_ARAP_H */

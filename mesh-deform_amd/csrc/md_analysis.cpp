@@ -1,0 +1,644 @@
+// Host-side analysis for the ARAP solve: see md_analysis.h.
+#include "md_analysis.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace md {
+
+namespace {
+
+// Free-free adjacency in "free rank" numbering (rank among free vertices, ascending id).
+void free_graph(int n, int F, const int* tri, const std::vector<char>& is_con,
+                std::vector<int>& rank_of, std::vector<int>& vert_of, std::vector<int>& ptr,
+                std::vector<int>& adj) {
+    rank_of.assign(n, -1);
+    vert_of.clear();
+    for (int v = 0; v < n; ++v)
+        if (!is_con[v]) { rank_of[v] = (int)vert_of.size(); vert_of.push_back(v); }
+    int m = (int)vert_of.size();
+    std::vector<std::pair<int, int>> e;
+    e.reserve((size_t)F * 6);
+    for (int f = 0; f < F; ++f)
+        for (int k = 0; k < 3; ++k) {
+            int a = rank_of[tri[3 * f + k]], b = rank_of[tri[3 * f + (k + 1) % 3]];
+            if (a >= 0 && b >= 0 && a != b) { e.emplace_back(a, b); e.emplace_back(b, a); }
+        }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    ptr.assign(m + 1, 0);
+    adj.resize(e.size());
+    for (size_t k = 0; k < e.size(); ++k) { ptr[e[k].first + 1]++; adj[k] = e[k].second; }
+    for (int i = 0; i < m; ++i) ptr[i + 1] += ptr[i];
+}
+
+// ---- geometric nested dissection ------------------------------------------------------
+
+struct ND {
+    const std::vector<int>& ptr;
+    const std::vector<int>& adj;
+    const std::vector<double>& xyz;  // 3 per free rank
+    std::vector<int> stamp, side;
+    int stamp_ctr = 0;
+    std::vector<int> out;
+    static constexpr int kLeaf = 8;
+
+    ND(const std::vector<int>& p, const std::vector<int>& a, const std::vector<double>& x)
+        : ptr(p), adj(a), xyz(x), stamp(p.size(), 0), side(p.size(), 0) {}
+
+    void run(std::vector<int>& v) {
+        int cnt = (int)v.size();
+        if (cnt <= kLeaf) {
+            out.insert(out.end(), v.begin(), v.end());
+            return;
+        }
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int x : v)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], xyz[3 * x + a]);
+                hi[a] = std::max(hi[a], xyz[3 * x + a]);
+            }
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+        int half = cnt / 2;
+        std::nth_element(v.begin(), v.begin() + half, v.end(), [&](int i, int j) {
+            double a = xyz[3 * i + axis], b = xyz[3 * j + axis];
+            return a < b || (a == b && i < j);
+        });
+        int st = ++stamp_ctr;
+        for (int k = 0; k < cnt; ++k) { stamp[v[k]] = st; side[v[k]] = k < half ? 0 : 1; }
+        // boundary of each side; the smaller one is the separator
+        std::vector<int> bnd[2], rest[2];
+        for (int k = 0; k < cnt; ++k) {
+            int x = v[k], sx = side[x];
+            bool b = false;
+            for (int e = ptr[x]; e < ptr[x + 1]; ++e) {
+                int y = adj[e];
+                if (stamp[y] == st && side[y] != sx) { b = true; break; }
+            }
+            (b ? bnd[sx] : rest[sx]).push_back(x);
+        }
+        int ss = bnd[0].size() <= bnd[1].size() ? 0 : 1;
+        std::vector<int> left, right, sep = bnd[ss];
+        left = rest[0];
+        right = rest[1];
+        if (ss == 0) right.insert(right.end(), bnd[1].begin(), bnd[1].end());
+        else left.insert(left.end(), bnd[0].begin(), bnd[0].end());
+        std::vector<int>().swap(v);
+        run(left);
+        run(right);
+        out.insert(out.end(), sep.begin(), sep.end());
+    }
+};
+
+// Elimination tree of the symmetric pattern (ptr/adj) under the order `perm` (new -> old).
+std::vector<int> etree(const std::vector<int>& ptr, const std::vector<int>& adj,
+                       const std::vector<int>& perm, const std::vector<int>& iperm) {
+    int m = (int)perm.size();
+    std::vector<int> parent(m, -1), anc(m, -1);
+    for (int k = 0; k < m; ++k) {
+        int old = perm[k];
+        for (int e = ptr[old]; e < ptr[old + 1]; ++e) {
+            int i = iperm[adj[e]];
+            while (i != -1 && i < k) {
+                int nxt = anc[i];
+                anc[i] = k;
+                if (nxt == -1) { parent[i] = k; break; }
+                i = nxt;
+            }
+        }
+    }
+    return parent;
+}
+
+std::vector<int> postorder(const std::vector<int>& parent) {
+    int m = (int)parent.size();
+    std::vector<int> head(m, -1), next(m, -1), post;
+    post.reserve(m);
+    for (int j = m - 1; j >= 0; --j)
+        if (parent[j] != -1) { next[j] = head[parent[j]]; head[parent[j]] = j; }
+    std::vector<int> stack;
+    for (int r = 0; r < m; ++r) {
+        if (parent[r] != -1) continue;
+        stack.push_back(r);
+        while (!stack.empty()) {
+            int p = stack.back();
+            int c = head[p];
+            if (c == -1) { stack.pop_back(); post.push_back(p); }
+            else { head[p] = next[c]; stack.push_back(c); }
+        }
+    }
+    return post;
+}
+
+}  // namespace
+
+namespace {
+
+// Nested dissection of the graph (ptr/adj, m vertices with coordinates fx) followed by an
+// elimination-tree postorder; returns new -> old.
+std::vector<int> nd_postorder(int m, const std::vector<int>& ptr, const std::vector<int>& adj,
+                              const std::vector<double>& fx) {
+    ND nd(ptr, adj, fx);
+    std::vector<int> all(m);
+    std::iota(all.begin(), all.end(), 0);
+    nd.out.reserve(m);
+    nd.run(all);
+    std::vector<int> perm = nd.out, iperm(m);
+    for (int k = 0; k < m; ++k) iperm[perm[k]] = k;
+    // postorder the elimination tree: same fill, contiguous subtrees
+    std::vector<int> par = etree(ptr, adj, perm, iperm);
+    std::vector<int> post = postorder(par);
+    std::vector<int> result(m);
+    for (int k = 0; k < m; ++k) result[k] = perm[post[k]];
+    return result;
+}
+
+}  // namespace
+
+std::vector<int> order_graph(int m, const int* rowptr, const int* col, const double* xyz) {
+    std::vector<int> ptr(m + 1, 0), adj;
+    for (int i = 0; i < m; ++i) {
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e)
+            if (col[e] != i) adj.push_back(col[e]);
+        ptr[i + 1] = (int)adj.size();
+    }
+    std::vector<double> fx(xyz, xyz + 3 * (size_t)m);
+    return nd_postorder(m, ptr, adj, fx);
+}
+
+std::vector<int> order_free_vertices(int n, int F, const int* tri, const double* xyz,
+                                     const std::vector<char>& is_con) {
+    std::vector<int> rank_of, vert_of, ptr, adj;
+    free_graph(n, F, tri, is_con, rank_of, vert_of, ptr, adj);
+    int m = (int)vert_of.size();
+    std::vector<double> fx(3 * (size_t)m);
+    for (int k = 0; k < m; ++k)
+        for (int a = 0; a < 3; ++a) fx[3 * k + a] = xyz[3 * (size_t)vert_of[k] + a];
+    std::vector<int> perm = nd_postorder(m, ptr, adj, fx);
+    for (int k = 0; k < m; ++k) perm[k] = vert_of[perm[k]];
+    return perm;
+}
+
+void build_topology(int n, int F, const int* tri, const std::vector<int>& perm,
+                    const std::vector<char>& is_con, Topology& t) {
+    t.n = n;
+    t.F = F;
+    t.nf = (int)perm.size();
+    t.internal_of.assign(n, -1);
+    t.original_of.assign(n, -1);
+    int next = 0;
+    for (int v : perm) { t.internal_of[v] = next; t.original_of[next] = v; ++next; }
+    for (int v = 0; v < n; ++v)
+        if (is_con[v]) { t.internal_of[v] = next; t.original_of[next] = v; ++next; }
+    // undirected edge list with the face edges that touch it (3f + k), in face order
+    std::vector<int64_t> key;
+    key.reserve((size_t)F * 3);
+    std::vector<int> fe_order;
+    for (int f = 0; f < F; ++f)
+        for (int k = 0; k < 3; ++k) {
+            int a = tri[3 * f + k], b = tri[3 * f + (k + 1) % 3];
+            if (a == b) { key.push_back(-1); continue; }
+            int lo = std::min(a, b), hi = std::max(a, b);
+            key.push_back((int64_t)lo * n + hi);
+        }
+    // CSR adjacency in original ids (ascending neighbour ids), with per-directed-edge face lists
+    std::vector<int> deg(n, 0);
+    std::vector<int64_t> ukeys;
+    ukeys.reserve(key.size());
+    for (int64_t k : key)
+        if (k >= 0) ukeys.push_back(k);
+    std::sort(ukeys.begin(), ukeys.end());
+    ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
+    for (int64_t k : ukeys) { deg[k / n]++; deg[k % n]++; }
+    std::vector<int> rp(n + 1, 0);
+    for (int v = 0; v < n; ++v) rp[v + 1] = rp[v] + deg[v];
+    std::vector<int> ci(rp[n]);
+    std::vector<int> fill(rp.begin(), rp.end() - 1);
+    for (int64_t k : ukeys) {
+        int a = (int)(k / n), b = (int)(k % n);
+        ci[fill[a]++] = b;
+        ci[fill[b]++] = a;
+    }
+    for (int v = 0; v < n; ++v) std::sort(ci.begin() + rp[v], ci.begin() + rp[v + 1]);
+    // slices over internal ids
+    t.nslices = (n + kSlice - 1) / kSlice;
+    t.slice_ptr.assign(t.nslices + 1, 0);
+    for (int s = 0; s < t.nslices; ++s) {
+        int wmax = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            int iv = s * kSlice + l;
+            if (iv >= n) break;
+            int v = t.original_of[iv];
+            wmax = std::max(wmax, rp[v + 1] - rp[v]);
+        }
+        t.slice_ptr[s + 1] = t.slice_ptr[s] + wmax;
+    }
+    int64_t slots = t.total_slots();
+    t.sell_col.assign(slots, 0);
+    // directed edge (original a -> original b) -> slot
+    std::vector<int64_t> slot_of_csr(rp[n]);
+    for (int s = 0; s < t.nslices; ++s) {
+        int wdt = t.slice_ptr[s + 1] - t.slice_ptr[s];
+        for (int l = 0; l < kSlice; ++l) {
+            int iv = s * kSlice + l;
+            for (int k = 0; k < wdt; ++k) {
+                int64_t slot = ((int64_t)t.slice_ptr[s] + k) * kSlice + l;
+                if (iv >= n) { t.sell_col[slot] = 0; continue; }
+                int v = t.original_of[iv];
+                int d = rp[v + 1] - rp[v];
+                if (k < d) {
+                    t.sell_col[slot] = t.internal_of[ci[rp[v] + k]];
+                    slot_of_csr[rp[v] + k] = slot;
+                } else {
+                    t.sell_col[slot] = iv;  // padding: self, zero weight
+                }
+            }
+        }
+    }
+    // face edges feeding each slot, in ascending face order (deterministic summation that
+    // matches setFromTriplets' insertion order, arap.h:224-238)
+    std::vector<int> cnt(slots + 1, 0);
+    auto csr_find = [&](int a, int b) {
+        auto it = std::lower_bound(ci.begin() + rp[a], ci.begin() + rp[a + 1], b);
+        return (int)(it - ci.begin());
+    };
+    std::vector<std::pair<int64_t, int>> pairs;
+    pairs.reserve((size_t)F * 6);
+    for (int f = 0; f < F; ++f)
+        for (int k = 0; k < 3; ++k) {
+            int a = tri[3 * f + k], b = tri[3 * f + (k + 1) % 3];
+            if (a == b) continue;
+            pairs.emplace_back(slot_of_csr[csr_find(a, b)], 3 * f + k);
+            pairs.emplace_back(slot_of_csr[csr_find(b, a)], 3 * f + k);
+        }
+    std::stable_sort(pairs.begin(), pairs.end(),
+                     [](const std::pair<int64_t, int>& x, const std::pair<int64_t, int>& y) {
+                         return x.first < y.first;
+                     });
+    t.slot_fe_ptr.assign(slots + 1, 0);
+    t.slot_fe.resize(pairs.size());
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        t.slot_fe_ptr[pairs[k].first + 1]++;
+        t.slot_fe[k] = pairs[k].second;
+    }
+    for (int64_t s = 0; s < slots; ++s) t.slot_fe_ptr[s + 1] += t.slot_fe_ptr[s];
+    t.tri_internal.resize(3 * (size_t)F);
+    for (size_t k = 0; k < 3 * (size_t)F; ++k) t.tri_internal[k] = t.internal_of[tri[k]];
+}
+
+// ---- symbolic ---------------------------------------------------------------------------
+
+int g_amalg_small = 16;
+double g_amalg_ratio = 0.3;
+
+void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, Factor& f) {
+    f = Factor();
+    f.n = n;
+    std::vector<int> ident(n);
+    std::iota(ident.begin(), ident.end(), 0);
+    std::vector<int> par = etree(ptr, adj, ident, ident);
+    // column counts (with diagonal) by row-subtree traversal
+    std::vector<int> cc(n, 1), mark(n, -1);
+    for (int i = 0; i < n; ++i) {
+        mark[i] = i;
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+            int k = adj[e];
+            if (k >= i) continue;
+            while (k != -1 && k < i && mark[k] != i) {
+                cc[k]++;
+                mark[k] = i;
+                k = par[k];
+            }
+        }
+    }
+    std::vector<int> nchild(n, 0);
+    for (int j = 0; j < n; ++j)
+        if (par[j] != -1) nchild[par[j]]++;
+    // fundamental supernodes
+    std::vector<int> first;
+    for (int j = 0; j < n; ++j) {
+        bool join = j > 0 && par[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1 &&
+                    (j - first.back()) < kMaxSnWidth;
+        if (!join) first.push_back(j);
+    }
+    int ns = (int)first.size();
+    first.push_back(n);
+    std::vector<int> sn_of(n);
+    for (int s = 0; s < ns; ++s)
+        for (int j = first[s]; j < first[s + 1]; ++j) sn_of[j] = s;
+    // supernodal structure (off-diagonal rows), children before parents
+    std::vector<std::vector<int>> rows(ns);
+    std::vector<int> spar(ns, -1);
+    std::vector<std::vector<int>> kids(ns);
+    std::vector<int> marker(n, -1);
+    for (int s = 0; s < ns; ++s) {
+        int lo = first[s], hi = first[s + 1];
+        std::vector<int>& r = rows[s];
+        for (int j = lo; j < hi; ++j)
+            for (int e = ptr[j]; e < ptr[j + 1]; ++e) {
+                int i = adj[e];
+                if (i >= hi && marker[i] != s) { marker[i] = s; r.push_back(i); }
+            }
+        for (int c : kids[s])
+            for (int i : rows[c])
+                if (i >= hi && marker[i] != s) { marker[i] = s; r.push_back(i); }
+        std::sort(r.begin(), r.end());
+        if (!r.empty()) {
+            spar[s] = sn_of[r[0]];
+            kids[spar[s]].push_back(s);
+        }
+    }
+    // relaxed amalgamation: merge the last child into its parent when the panel grows little
+    std::vector<int> lo(ns), w(ns);
+    for (int s = 0; s < ns; ++s) { lo[s] = first[s]; w[s] = first[s + 1] - first[s]; }
+    std::vector<char> alive(ns, 1);
+    // explicit zeros each supernode's panel carries after merges
+    std::vector<int64_t> zeros(ns, 0);
+    for (int p = 0; p < ns; ++p) {
+        for (;;) {
+            if (kids[p].empty()) break;
+            int c = kids[p].back();  // children are appended in increasing order
+            if (lo[c] + w[c] != lo[p]) break;
+            int64_t wc = w[c], wp = w[p], nc = (int64_t)rows[c].size(), np_ = (int64_t)rows[p].size();
+            int64_t before = wc * (wc + nc) + wp * (wp + np_);
+            int64_t after = (wc + wp) * (wc + wp + np_);
+            int64_t z = zeros[c] + zeros[p] + (after - before);
+            double frac = (double)z / (double)after;
+            int64_t wm = wc + wp;
+            if (wm > kMaxSnWidth) break;
+            // relaxed amalgamation: tiny supernodes always merge, wider ones only while the
+            // share of explicit zeros stays small
+            bool ok = wm <= g_amalg_small || (wm <= 16 && frac < 0.5) ||
+                      (wm <= 48 && frac < g_amalg_ratio) || frac < 0.5 * g_amalg_ratio;
+            if (!ok) break;
+            zeros[p] = z;
+            // merge c into p
+            kids[p].pop_back();
+            for (int g : kids[c]) { spar[g] = p; }
+            kids[p].insert(kids[p].end(), kids[c].begin(), kids[c].end());
+            std::sort(kids[p].begin(), kids[p].end());
+            lo[p] = lo[c];
+            w[p] += w[c];
+            alive[c] = 0;
+            std::vector<int>().swap(rows[c]);
+        }
+    }
+    // renumber surviving supernodes (column order = topological order)
+    std::vector<int> newid(ns, -1);
+    int nn = 0;
+    for (int s = 0; s < ns; ++s)
+        if (alive[s]) newid[s] = nn++;
+    f.ns = nn;
+    f.c0.resize(nn); f.w.resize(nn); f.noff.resize(nn); f.parent.assign(nn, -1);
+    f.roff.assign(nn + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        if (!alive[s]) continue;
+        int t = newid[s];
+        f.c0[t] = lo[s];
+        f.w[t] = w[s];
+        f.noff[t] = (int)rows[s].size();
+        f.parent[t] = spar[s] >= 0 ? newid[spar[s]] : -1;
+        f.roff[t + 1] = f.noff[t];
+    }
+    for (int t = 0; t < nn; ++t) f.roff[t + 1] += f.roff[t];
+    f.offrows.resize(f.roff[nn]);
+    for (int s = 0; s < ns; ++s)
+        if (alive[s]) std::copy(rows[s].begin(), rows[s].end(), f.offrows.begin() + f.roff[newid[s]]);
+    f.uoff = f.roff;
+    // children, heights
+    f.child_ptr.assign(nn + 1, 0);
+    for (int t = 0; t < nn; ++t)
+        if (f.parent[t] >= 0) f.child_ptr[f.parent[t] + 1]++;
+    for (int t = 0; t < nn; ++t) f.child_ptr[t + 1] += f.child_ptr[t];
+    f.child.resize(f.child_ptr[nn]);
+    {
+        std::vector<int> fill(f.child_ptr.begin(), f.child_ptr.end() - 1);
+        for (int t = 0; t < nn; ++t)
+            if (f.parent[t] >= 0) f.child[fill[f.parent[t]]++] = t;
+    }
+    f.height.assign(nn, 0);
+    f.nlevels = nn > 0 ? 1 : 0;
+    for (int t = 0; t < nn; ++t) {
+        for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k)
+            f.height[t] = std::max(f.height[t], f.height[f.child[k]] + 1);
+        f.nlevels = std::max(f.nlevels, f.height[t] + 1);
+    }
+    // rel: position of each off row of t in its parent's row list [0, m_p)
+    f.rel.assign(f.offrows.size(), -1);
+    for (int t = 0; t < nn; ++t) {
+        int p = f.parent[t];
+        if (p < 0) continue;
+        const int* prow = f.offrows.data() + f.roff[p];
+        int pn = f.noff[p];
+        for (int k = f.roff[t]; k < f.roff[t + 1]; ++k) {
+            int r = f.offrows[k];
+            if (r < f.c0[p] + f.w[p]) {
+                f.rel[k] = r - f.c0[p];
+            } else {
+                int pos = (int)(std::lower_bound(prow, prow + pn, r) - prow);
+                f.rel[k] = f.w[p] + pos;
+            }
+        }
+    }
+    f.poff.assign(nn + 1, 0);
+    for (int t = 0; t < nn; ++t) f.poff[t + 1] = f.poff[t] + (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
+    // work items by level
+    f.fwd_ptr.assign(f.nlevels + 1, 0);
+    f.bwd_ptr.assign(f.nlevels + 1, 0);
+    for (int t = 0; t < nn; ++t) {
+        f.fwd_ptr[f.height[t] + 1] += std::max(1, (f.noff[t] + kFwdRows - 1) / kFwdRows);
+        f.bwd_ptr[f.height[t] + 1] += (f.w[t] + kBwdCols - 1) / kBwdCols;
+    }
+    for (int l = 0; l < f.nlevels; ++l) {
+        f.fwd_ptr[l + 1] += f.fwd_ptr[l];
+        f.bwd_ptr[l + 1] += f.bwd_ptr[l];
+    }
+    f.fwd_s.resize(f.fwd_ptr[f.nlevels]); f.fwd_r0.resize(f.fwd_ptr[f.nlevels]);
+    f.bwd_s.resize(f.bwd_ptr[f.nlevels]); f.bwd_q0.resize(f.bwd_ptr[f.nlevels]);
+    std::vector<int> ff(f.fwd_ptr.begin(), f.fwd_ptr.end() - 1), bf(f.bwd_ptr.begin(), f.bwd_ptr.end() - 1);
+    for (int t = 0; t < nn; ++t) {
+        int l = f.height[t];
+        int nfw = std::max(1, (f.noff[t] + kFwdRows - 1) / kFwdRows);
+        for (int k = 0; k < nfw; ++k) { f.fwd_s[ff[l]] = t; f.fwd_r0[ff[l]] = k * kFwdRows; ff[l]++; }
+        for (int q = 0; q < f.w[t]; q += kBwdCols) { f.bwd_s[bf[l]] = t; f.bwd_q0[bf[l]] = q; bf[l]++; }
+    }
+}
+
+// ---- numeric ----------------------------------------------------------------------------
+
+bool numeric(const std::vector<int>& ptr, const std::vector<int>& adj, const std::vector<double>& val,
+             const std::vector<double>& diag, Factor& f, std::string& err) {
+    int n = f.n;
+    f.panel.assign(f.poff[f.ns], 0.0);
+    double amax = 0.0;
+    for (int i = 0; i < n; ++i) amax = std::max(amax, std::fabs(diag[i]));
+    std::vector<std::vector<double>> upd(f.ns);
+    std::vector<int> pos(n, -1);
+    std::vector<double> front;
+    for (int s = 0; s < f.ns; ++s) {
+        const int c0 = f.c0[s], w = f.w[s], no = f.noff[s], m = w + no;
+        const int* rows = f.offrows.data() + f.roff[s];
+        for (int k = 0; k < w; ++k) pos[c0 + k] = k;
+        for (int k = 0; k < no; ++k) pos[rows[k]] = w + k;
+        front.assign((size_t)m * m, 0.0);  // column-major, lower part used
+        auto F = [&](int r, int c) -> double& { return front[(size_t)c * m + r]; };
+        for (int k = 0; k < w; ++k) {
+            int j = c0 + k;
+            F(k, k) += diag[j];
+            for (int e = ptr[j]; e < ptr[j + 1]; ++e) {
+                int i = adj[e];
+                if (i > j) F(pos[i], k) += val[e];
+            }
+        }
+        for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
+            int c = f.child[q];
+            int nc = f.noff[c];
+            const int* rl = f.rel.data() + f.roff[c];
+            const std::vector<double>& U = upd[c];
+            for (int b = 0; b < nc; ++b)
+                for (int a = b; a < nc; ++a) {
+                    int ra = rl[a], rb = rl[b];
+                    // (ra >= rb since rel is increasing)
+                    F(ra, rb) += U[(size_t)b * nc + a];
+                }
+            std::vector<double>().swap(upd[c]);
+        }
+        // Cholesky of the pivot block (left-looking, lower)
+        std::vector<double> Ls((size_t)w * w, 0.0);
+        for (int c = 0; c < w; ++c)
+            for (int r = c; r < w; ++r) Ls[(size_t)c * w + r] = F(r, c);
+        for (int c = 0; c < w; ++c) {
+            for (int k = 0; k < c; ++k) {
+                double lck = Ls[(size_t)k * w + c];
+                if (lck == 0.0) continue;
+                for (int r = c; r < w; ++r) Ls[(size_t)c * w + r] -= Ls[(size_t)k * w + r] * lck;
+            }
+            double d = Ls[(size_t)c * w + c];
+            if (!(d > 1e-14 * amax)) {
+                err = "factorization failed: pivot block of supernode " + std::to_string(s) +
+                      " is not positive definite (a mesh component without constraints?)";
+                return false;
+            }
+            d = std::sqrt(d);
+            Ls[(size_t)c * w + c] = d;
+            for (int r = c + 1; r < w; ++r) Ls[(size_t)c * w + r] /= d;
+        }
+        // Linv = Ls^{-1} (lower), column by column: Ls * Linv[:,c] = e_c
+        std::vector<double> Li((size_t)w * w, 0.0);
+        for (int c = 0; c < w; ++c) {
+            double* x = &Li[(size_t)c * w];
+            x[c] = 1.0 / Ls[(size_t)c * w + c];
+            for (int k = c; k < w; ++k) {
+                double xk = (k == c) ? x[c] : x[k] / Ls[(size_t)k * w + k];
+                x[k] = xk;
+                for (int r = k + 1; r < w; ++r) x[r] -= Ls[(size_t)k * w + r] * xk;
+            }
+        }
+        // Dinv = Linv^T Linv
+        double* P = f.panel.data() + f.poff[s];
+        for (int c = 0; c < w; ++c)
+            for (int r = c; r < w; ++r) {
+                double acc = 0.0;
+                const double* lr = &Li[(size_t)r * w];
+                const double* lc = &Li[(size_t)c * w];
+                for (int k = r; k < w; ++k) acc += lr[k] * lc[k];
+                P[(size_t)c * m + r] = acc;
+                P[(size_t)r * m + c] = acc;
+            }
+        if (no > 0) {
+            // G = F_os * Dinv  (no x w), stored in panel rows [w, m)
+            for (int q = 0; q < w; ++q) {
+                double* g = P + (size_t)q * m + w;
+                for (int k = 0; k < w; ++k) {
+                    double dkq = P[(size_t)q * m + k];
+                    const double* fo = &front[(size_t)k * m + w];
+                    for (int r = 0; r < no; ++r) g[r] += fo[r] * dkq;
+                }
+            }
+            // U = F_oo - G F_os^T (lower part, column-major no x no)
+            std::vector<double>& U = upd[s];
+            U.assign((size_t)no * no, 0.0);
+            for (int b = 0; b < no; ++b)
+                for (int a = b; a < no; ++a) U[(size_t)b * no + a] = F(w + a, w + b);
+            for (int k = 0; k < w; ++k) {
+                const double* g = P + (size_t)k * m + w;
+                const double* fo = &front[(size_t)k * m + w];
+                for (int b = 0; b < no; ++b) {
+                    double fb = fo[b];
+                    if (fb == 0.0) continue;
+                    double* u = &U[(size_t)b * no];
+                    for (int a = b; a < no; ++a) u[a] -= g[a] * fb;
+                }
+            }
+        }
+        for (int k = 0; k < w; ++k) pos[c0 + k] = -1;
+        for (int k = 0; k < no; ++k) pos[rows[k]] = -1;
+    }
+    return true;
+}
+
+void solve_host(const Factor& f, const double* b, double* x, int nrhs) {
+    int n = f.n;
+    for (int r = 0; r < nrhs; ++r) {
+        const double* br = b + (size_t)r * n;
+        double* xr = x + (size_t)r * n;
+        std::vector<double> z(n), u(f.roff[f.ns]);
+        for (int s = 0; s < f.ns; ++s) {
+            int c0 = f.c0[s], w = f.w[s], no = f.noff[s], m = w + no;
+            std::vector<double> v(m, 0.0);
+            for (int k = 0; k < w; ++k) v[k] = br[c0 + k];
+            for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
+                int c = f.child[q];
+                for (int k = 0; k < f.noff[c]; ++k) v[f.rel[f.roff[c] + k]] += u[f.uoff[c] + k];
+            }
+            const double* P = f.panel.data() + f.poff[s];
+            for (int k = 0; k < w; ++k) z[c0 + k] = v[k];
+            for (int rr = 0; rr < no; ++rr) {
+                double acc = v[w + rr];
+                for (int q = 0; q < w; ++q) acc -= P[(size_t)q * m + w + rr] * v[q];
+                u[f.uoff[s] + rr] = acc;
+            }
+        }
+        for (int s = f.ns - 1; s >= 0; --s) {
+            int c0 = f.c0[s], w = f.w[s], no = f.noff[s], m = w + no;
+            const int* rows = f.offrows.data() + f.roff[s];
+            const double* P = f.panel.data() + f.poff[s];
+            for (int q = 0; q < w; ++q) {
+                double acc = 0.0;
+                for (int p = 0; p < w; ++p) acc += P[(size_t)q * m + p] * z[c0 + p];
+                for (int rr = 0; rr < no; ++rr) acc -= P[(size_t)q * m + w + rr] * xr[rows[rr]];
+                xr[c0 + q] = acc;
+            }
+        }
+    }
+}
+
+AnalysisStats stats(const Topology& t, const Factor& f) {
+    AnalysisStats s{};
+    s.n = t.n;
+    s.nf = t.nf;
+    s.nslices = t.nslices;
+    s.ns = f.ns;
+    s.nlevels = f.nlevels;
+    s.slots = t.slice_ptr.empty() ? 0 : t.total_slots();
+    s.panel_entries = f.factor_nnz();
+    int64_t strict = 0;
+    for (int k = 0; k < f.ns; ++k)
+        strict += (int64_t)f.w[k] * (f.w[k] - 1) / 2 + (int64_t)f.w[k] * f.noff[k];
+    s.strict_nnz_equiv = strict;
+    s.fwd_items = f.fwd_ptr.empty() ? 0 : f.fwd_ptr.back();
+    s.bwd_items = f.bwd_ptr.empty() ? 0 : f.bwd_ptr.back();
+    s.max_width = 0;
+    s.max_rows = 0;
+    for (int k = 0; k < f.ns; ++k) {
+        s.max_width = std::max(s.max_width, f.w[k]);
+        s.max_rows = std::max(s.max_rows, f.w[k] + f.noff[k]);
+    }
+    return s;
+}
+
+}  // namespace md

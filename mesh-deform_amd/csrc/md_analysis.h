@@ -1,0 +1,109 @@
+// Host-side analysis of one ARAP mesh: vertex renumbering, one-ring slices, fill-reducing
+// ordering and the supernodal block-LDL^T factor consumed by the HIP solve kernels.
+//
+// Replaces, for the path of inc/deform/arap.h (cheind/mesh-deform):
+//   initializeFreeVariableMapping (arap.h:261-272) -> free vertices first in solve order,
+//   setupLinearSystem (arap.h:292-340)             -> supernodal symbolic + FP64 numeric
+//                                                      factorization (Eigen SimplicialLDLT
+//                                                      with AMD ordering in the reference).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace md {
+
+// Maximum columns of one supernode (wider separators are split into chains).
+constexpr int kMaxSnWidth = 1024;
+// Rows of one forward work item and columns of one backward work item.
+constexpr int kFwdRows = 64;
+constexpr int kBwdCols = 16;
+// Vertices per one-ring slice (one wavefront).
+constexpr int kSlice = 64;
+
+// One-ring topology in the order the kernels use ("internal" vertex ids):
+//   internal ids [0, nf) are the free vertices in solve (elimination) order,
+//   internal ids [nf, n) are the constrained vertices in ascending original id.
+// Each slice of 64 internal vertices stores its neighbours column by column
+// (slot k of lane l at (slice_ptr[s] + k) * 64 + l), padded with the vertex itself.
+struct Topology {
+    int n = 0, F = 0, nf = 0;
+    std::vector<int> internal_of;   // original id -> internal id
+    std::vector<int> original_of;   // internal id -> original id
+    int nslices = 0;
+    std::vector<int> slice_ptr;     // nslices + 1, in 64-wide slot columns
+    std::vector<int> sell_col;      // slots * 64, internal neighbour id (self for padding)
+    std::vector<int> sell_orig_rank;// not stored on device: for the weight build
+    // cotan build: each face edge (3 per face) writes its half-cotangent to fe_val[3f+k];
+    // each slot sums the face edges listed in slot_fe[slot_fe_ptr[slot] ..] in face order.
+    std::vector<int> tri_internal;  // 3F internal ids
+    std::vector<int> slot_fe_ptr;   // total_slots + 1
+    std::vector<int> slot_fe;       // face-edge ids (3f + k)
+    int64_t total_slots() const { return (int64_t)slice_ptr[nslices] * kSlice; }
+};
+
+// Supernodal block LDL^T: A = Lb * D * Lb^T with Lb unit block-lower-triangular.
+// For supernode s with columns [c0, c0 + w) and off-diagonal rows offrows[roff .. roff+noff)
+// the panel (m = w + noff rows, w columns, column-major) holds
+//   rows [0, w):  Dinv_s = D_s^{-1} (dense symmetric),
+//   rows [w, m):  G_s = L_os L_ss^{-1} (the block of Lb below D_s).
+// Forward:  v = b[c0..] + children updates; z_s = v_d; u_s = v_o - G_s v_d.
+// Backward: x_s = Dinv_s z_s - G_s^T x[offrows].
+struct Factor {
+    int n = 0;                       // dimension (free vertices)
+    int ns = 0;                      // supernodes
+    std::vector<int> c0, w, noff;    // per supernode
+    std::vector<int> parent;         // -1 for roots
+    std::vector<int> height;         // 0 for leaves
+    int nlevels = 0;
+    std::vector<int> roff;           // ns + 1: offsets into offrows / rel
+    std::vector<int> offrows;        // ascending, solve numbering
+    std::vector<int> rel;            // per off row of s: position in parent's rows [0, m_p)
+    std::vector<int> child_ptr, child;
+    std::vector<int64_t> poff;       // ns + 1: panel offsets
+    std::vector<int> uoff;           // ns + 1: update-vector offsets (= roff)
+    std::vector<double> panel;       // FP64 panels
+    // work items, grouped by height level: fwd item = (s, first off row), bwd item =
+    // (s, first column); level l occupies [fwd_ptr[l], fwd_ptr[l+1]).
+    std::vector<int> fwd_ptr, fwd_s, fwd_r0;
+    std::vector<int> bwd_ptr, bwd_s, bwd_q0;
+    int64_t factor_nnz() const { return poff.empty() ? 0 : poff.back(); }
+};
+
+struct AnalysisStats {
+    int n, nf, nslices, ns, nlevels;
+    int64_t slots, panel_entries, strict_nnz_equiv, fwd_items, bwd_items;
+    int max_width, max_rows;
+};
+
+// Builds the topology for faces `tri` (3F original ids) of an n-vertex mesh with the
+// constrained flags `is_con` (n entries) and the solve order `perm` of the free vertices
+// (perm[k] = original id of the k-th free vertex in elimination order).
+void build_topology(int n, int F, const int* tri, const std::vector<int>& perm,
+                    const std::vector<char>& is_con, Topology& topo);
+
+// Fill-reducing order of the free vertices: geometric nested dissection on the free-free
+// graph, followed by an elimination-tree postorder. Returns original vertex ids.
+std::vector<int> order_free_vertices(int n, int F, const int* tri, const double* xyz,
+                                     const std::vector<char>& is_con);
+
+// The same ordering for a general symmetric pattern (full CSR, m rows) with coordinates.
+std::vector<int> order_graph(int m, const int* rowptr, const int* col, const double* xyz);
+
+// Symbolic supernodal analysis of the free-free graph in solve order.
+// adj_ptr/adj (CSR over solve indices, no diagonal needed) describe A's pattern.
+void symbolic(int n, const std::vector<int>& adj_ptr, const std::vector<int>& adj, Factor& f);
+
+// FP64 multifrontal numeric factorization. A is given by its diagonal (diag[i]) and the
+// strictly-off-diagonal entries adj/adj_val (CSR in solve order, symmetric pattern).
+// Returns false (and sets err) when a pivot block is not positive definite.
+bool numeric(const std::vector<int>& adj_ptr, const std::vector<int>& adj,
+             const std::vector<double>& adj_val, const std::vector<double>& diag, Factor& f,
+             std::string& err);
+
+// Host reference of the GPU solve on the same panels (used by host tests).
+void solve_host(const Factor& f, const double* b, double* x, int nrhs);
+
+AnalysisStats stats(const Topology& t, const Factor& f);
+
+}  // namespace md

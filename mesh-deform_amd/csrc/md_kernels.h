@@ -1,0 +1,68 @@
+// Device-side data layout and kernel launchers of the ARAP path (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace md {
+
+struct alignas(16) f4 { float x, y, z, w; };
+struct alignas(32) d4 { double x, y, z, w; };
+
+template <typename T> struct V4T;
+template <> struct V4T<float> { using type = f4; };
+template <> struct V4T<double> { using type = d4; };
+
+// Device view of one mesh. All pointers are device pointers into the mesh's arena.
+// Internal vertex ids: [0, nf) free vertices in solve order, [nf, n) constrained ones.
+struct MeshDev {
+    int n, nf, nslices, ns;
+    int nslices_free;           // slices covering internal ids [0, nf)
+    int F;
+    long long slots;            // 64 * slice_ptr[nslices]
+    const int *slice_ptr;       // nslices + 1 (64-wide slot columns)
+    const int *sell_col;        // slots: neighbour internal id (self = padding)
+    float *a32;                 // 3 planes of slots: w_ij (p_i - p_j), FP32
+    double *a64;                // 3 planes of slots, FP64
+    double *w64;                // slots: cotan weight w_ij
+    const int *tri;             // 3F internal ids
+    double *fe_val;             // 3F half-cotangents (face edge 3f+k)
+    const int *slot_fe_ptr;     // slots + 1
+    const int *slot_fe;         // face-edge ids feeding each slot, face order
+    d4 *p;                      // rest positions (internal order)
+    d4 *pp;                     // current positions p'
+    void *quat;                 // rotations as quaternions (x, y, z, w=scalar): f4 or d4
+    void *r, *z, *x, *u;        // solve vectors (V4 of the solve type): nf, nf, nf, noff total
+    const int *sn_c0, *sn_w, *sn_noff, *sn_roff;
+    const long long *sn_poff;
+    const int *child_ptr, *child;
+    const int *offrows, *rel;
+    const void *panel;          // float or double panels
+    const int *fwd_ptr, *fwd_s, *fwd_r0;
+    const int *bwd_ptr, *bwd_s, *bwd_q0;
+    double *energy_part;        // one partial sum per 256 vertices
+};
+
+// A set of meshes processed by one launch. Block b belongs to mesh k where
+// prefix[k] <= b < prefix[k+1]; `meshes` and `prefix` are device arrays.
+struct LaunchSet {
+    const MeshDev *meshes;
+    const int *prefix;
+    int nactive;
+};
+
+// Launchers (grid = total blocks of the launch, from the host-side prefix table).
+void launch_cotan_face(hipStream_t s, LaunchSet L, int blocks);
+void launch_cotan_slot(hipStream_t s, LaunchSet L, int blocks);
+void launch_local(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_residual(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
+void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
+void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+
+// Threads per block and work per block of each kernel (used for the prefix tables).
+constexpr int kBlock = 256;
+constexpr int kSlicesPerBlock = kBlock / 64;  // one 64-vertex slice per wavefront
+
+}  // namespace md

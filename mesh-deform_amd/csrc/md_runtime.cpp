@@ -1,0 +1,822 @@
+// C ABI of the ARAP library: context, mesh (the AsRigidAsPossibleDeformation object of
+// inc/deform/arap.h), launch plans and graph replay. Batches and the server are in
+// md_serve.cpp.
+#include "md_runtime.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+namespace md {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+#define MD_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            md::set_error(std::string(#call) + ": " + hipGetErrorString(e_));           \
+            return MD_ERR_HIP;                                                          \
+        }                                                                               \
+    } while (0)
+
+#define MD_HIP_E(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) return e_;                                                \
+    } while (0)
+
+Plan::~Plan() {
+    drop_graph();
+    if (d_meshes) hipFree(d_meshes);
+    if (d_prefix) hipFree(d_prefix);
+}
+
+void Plan::drop_graph() {
+    if (graph) hipGraphExecDestroy(graph);
+    graph = nullptr;
+}
+
+static int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
+                      const std::vector<const Factor *> &factors, bool fp64, int refine_steps,
+                      hipStream_t stream) {
+    plan.drop_graph();
+    plan.fp64 = fp64;
+    plan.refine_steps = std::max(1, refine_steps);
+    int na = (int)meshes.size();
+    plan.nactive = na;
+    plan.nlevels = 0;
+    for (const Factor *f : factors) plan.nlevels = std::max(plan.nlevels, f->nlevels);
+    int nl = plan.nlevels;
+    int ntables = 4 + 2 * nl;
+    plan.h_prefix.assign((size_t)ntables * (na + 1), 0);
+    plan.off_fwd.assign(nl, 0);
+    plan.off_bwd.assign(nl, 0);
+    plan.blk_fwd.assign(nl, 0);
+    plan.blk_bwd.assign(nl, 0);
+    auto table = [&](int t, int &off, int &total, auto count) {
+        off = t * (na + 1);
+        int *p = plan.h_prefix.data() + off;
+        p[0] = 0;
+        for (int k = 0; k < na; ++k) p[k + 1] = p[k] + count(k);
+        total = p[na];
+    };
+    table(0, plan.off_face, plan.blk_face, [&](int k) { return cdiv(meshes[k]->F, kBlock); });
+    table(1, plan.off_slice, plan.blk_slice,
+          [&](int k) { return cdiv(meshes[k]->nslices, kSlicesPerBlock); });
+    table(2, plan.off_resid, plan.blk_resid,
+          [&](int k) { return cdiv(meshes[k]->nslices_free, kSlicesPerBlock); });
+    table(3, plan.off_ident, plan.blk_ident, [&](int k) { return cdiv(meshes[k]->n, kBlock); });
+    for (int l = 0; l < nl; ++l) {
+        table(4 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
+            const Factor *f = factors[k];
+            return l < f->nlevels ? f->fwd_ptr[l + 1] - f->fwd_ptr[l] : 0;
+        });
+        table(5 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
+            const Factor *f = factors[k];
+            return l < f->nlevels ? f->bwd_ptr[l + 1] - f->bwd_ptr[l] : 0;
+        });
+    }
+    plan.h_meshes.resize(na);
+    for (int k = 0; k < na; ++k) plan.h_meshes[k] = *meshes[k];
+    if ((size_t)na > plan.cap_meshes) {
+        if (plan.d_meshes) hipFree(plan.d_meshes);
+        plan.d_meshes = nullptr;
+        MD_HIP_E(hipMalloc(&plan.d_meshes, sizeof(MeshDev) * (size_t)na));
+        plan.cap_meshes = na;
+    }
+    if (plan.h_prefix.size() > plan.cap_prefix) {
+        if (plan.d_prefix) hipFree(plan.d_prefix);
+        plan.d_prefix = nullptr;
+        MD_HIP_E(hipMalloc(&plan.d_prefix, sizeof(int) * plan.h_prefix.size()));
+        plan.cap_prefix = plan.h_prefix.size();
+    }
+    if (na > 0) MD_HIP_E(hipMemcpyAsync(plan.d_meshes, plan.h_meshes.data(), sizeof(MeshDev) * na,
+                                        hipMemcpyHostToDevice, stream));
+    if (!plan.h_prefix.empty())
+        MD_HIP_E(hipMemcpyAsync(plan.d_prefix, plan.h_prefix.data(), sizeof(int) * plan.h_prefix.size(),
+                                hipMemcpyHostToDevice, stream));
+    return hipStreamSynchronize(stream);
+}
+
+void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
+    if (ev) hipEventRecord(ev[0], stream);
+    launch_local(stream, plan.set(plan.off_slice), plan.blk_slice, plan.fp64);
+    if (ev) hipEventRecord(ev[1], stream);
+    for (int r = 0; r < plan.refine_steps; ++r) {
+        launch_residual(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
+        if (ev && r == 0) hipEventRecord(ev[2], stream);
+        for (int l = 0; l < plan.nlevels; ++l)
+            launch_fwd(stream, plan.set(plan.off_fwd[l]), plan.blk_fwd[l], l, plan.fp64);
+        if (ev && r == 0) hipEventRecord(ev[3], stream);
+        for (int l = plan.nlevels - 1; l >= 0; --l)
+            launch_bwd(stream, plan.set(plan.off_bwd[l]), plan.blk_bwd[l], l, plan.fp64);
+        if (ev && r == 0) hipEventRecord(ev[4], stream);
+    }
+}
+
+hipError_t run_iterations(Plan &plan, hipStream_t stream, int iters, bool use_graph) {
+    if (iters <= 0) return hipSuccess;
+    if (!use_graph) {
+        for (int i = 0; i < iters; ++i) enqueue_iteration(plan, stream, nullptr);
+        return hipGetLastError();
+    }
+    if (!plan.graph) {
+        hipGraph_t g = nullptr;
+        MD_HIP_E(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(plan, stream, nullptr);
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess) return e;
+        e = hipGraphInstantiate(&plan.graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) { plan.graph = nullptr; return e; }
+    }
+    for (int i = 0; i < iters; ++i) MD_HIP_E(hipGraphLaunch(plan.graph, stream));
+    return hipGetLastError();
+}
+
+}  // namespace md
+
+using namespace md;
+
+// ------------------------------------------------------------------------------------------
+// library
+// ------------------------------------------------------------------------------------------
+
+extern "C" void md_config_default(md_config *cfg) {
+    if (!cfg) return;
+    cfg->precision = MD_PREC_FP64;
+    cfg->solver = MD_SOLVER_LDLT;
+    cfg->refine_steps = 1;
+    cfg->rest_mode = MD_REST_REFERENCE;
+    cfg->pcg_max_iters = 200;
+    cfg->pcg_tol = 1e-7;
+    cfg->use_graph = 1;
+}
+
+extern "C" const char *md_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *md_version(void) { return "mdarap 0.1 (gfx950)"; }
+
+extern "C" int md_device_count(int *count) {
+    if (!count) { set_error("md_device_count: null pointer"); return MD_ERR_INVALID; }
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return MD_OK;
+}
+
+extern "C" int md_ctx_create(int device, md_ctx **out) {
+    if (!out) { set_error("md_ctx_create: null pointer"); return MD_ERR_INVALID; }
+    *out = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || device < 0 || device >= c) {
+        set_error("md_ctx_create: no GPU with index " + std::to_string(device));
+        return MD_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    MD_HIP(hipGetDeviceProperties(&prop, device));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0) {
+        set_error("md_ctx_create: device " + std::to_string(device) + " is " + arch +
+                  "; this library is built for gfx950 (MI355X) only");
+        return MD_ERR_NO_DEVICE;
+    }
+    MD_HIP(hipSetDevice(device));
+    std::unique_ptr<md_ctx> ctx(new md_ctx());
+    ctx->device = device;
+    ctx->arch = arch;
+    MD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+    return MD_OK;
+}
+
+extern "C" int md_ctx_destroy(md_ctx *ctx) {
+    if (!ctx) return MD_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MD_OK;
+}
+
+extern "C" int md_ctx_synchronize(md_ctx *ctx) {
+    if (!ctx) { set_error("md_ctx_synchronize: null context"); return MD_ERR_INVALID; }
+    MD_HIP(hipStreamSynchronize(ctx->stream));
+    return MD_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// mesh
+// ------------------------------------------------------------------------------------------
+
+extern "C" int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const md_config *cfg,
+                              md_mesh **out) {
+    if (!ctx || !out || n < 0 || F < 0 || (F > 0 && !tri)) {
+        set_error("md_mesh_create: invalid argument");
+        return MD_ERR_INVALID;
+    }
+    *out = nullptr;
+    for (int64_t k = 0; k < 3 * (int64_t)F; ++k)
+        if (tri[k] < 0 || tri[k] >= n) {
+            set_error("md_mesh_create: face " + std::to_string(k / 3) + " references vertex " +
+                      std::to_string(tri[k]) + " outside [0, " + std::to_string(n) + ")");
+            return MD_ERR_INVALID;
+        }
+    std::unique_ptr<md_mesh> m(new md_mesh());
+    m->ctx = ctx;
+    if (cfg) m->cfg = *cfg; else md_config_default(&m->cfg);
+    if (m->cfg.precision != MD_PREC_FP64 && m->cfg.precision != MD_PREC_FP32_REFINE) {
+        set_error("md_mesh_create: unknown precision");
+        return MD_ERR_INVALID;
+    }
+    if (m->cfg.solver != MD_SOLVER_LDLT) {
+        set_error("md_mesh_create: only the LDLT solver is available in this build");
+        return MD_ERR_INVALID;
+    }
+    m->cfg.refine_steps = std::max(1, m->cfg.refine_steps);
+    m->n = n;
+    m->F = F;
+    m->tri.assign(tri, tri + 3 * (size_t)F);
+    m->is_con.assign(n, 0);
+    m->con.assign(3 * (size_t)n, 0.0);
+    *out = m.release();
+    return MD_OK;
+}
+
+extern "C" int md_mesh_destroy(md_mesh *m) {
+    if (!m) return MD_OK;
+    hipSetDevice(m->ctx->device);
+    hipStreamSynchronize(m->ctx->stream);
+    delete m;
+    return MD_OK;
+}
+
+extern "C" int md_mesh_set_constraint(md_mesh *m, int vidx, const double *xyz) {
+    if (!m || !xyz) { set_error("md_mesh_set_constraint: null pointer"); return MD_ERR_INVALID; }
+    if (vidx < 0 || vidx >= m->n) {
+        set_error("md_mesh_set_constraint: vertex index " + std::to_string(vidx) +
+                  " outside [0, " + std::to_string(m->n) + ")");
+        return MD_ERR_INVALID;
+    }
+    if (!m->is_con[vidx]) { m->is_con[vidx] = 1; m->ncon++; m->con_set_changed = true; }
+    std::memcpy(&m->con[3 * (size_t)vidx], xyz, 3 * sizeof(double));
+    m->dirty = true;
+    return MD_OK;
+}
+
+extern "C" int md_mesh_set_constraints(md_mesh *m, int k, const int *idx, const double *xyz) {
+    if (!m || k < 0 || (k > 0 && (!idx || !xyz))) {
+        set_error("md_mesh_set_constraints: invalid argument");
+        return MD_ERR_INVALID;
+    }
+    for (int i = 0; i < k; ++i)
+        if (idx[i] < 0 || idx[i] >= m->n) {
+            set_error("md_mesh_set_constraints: vertex index " + std::to_string(idx[i]) +
+                      " outside [0, " + std::to_string(m->n) + ")");
+            return MD_ERR_INVALID;
+        }
+    for (int i = 0; i < k; ++i) md_mesh_set_constraint(m, idx[i], xyz + 3 * (size_t)i);
+    return MD_OK;
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Host analysis for the current constrained set: ordering, topology, symbolic factor and
+// the device arena with every static array uploaded. Positions come from m->rest.
+int analyze(md_mesh *m) {
+    auto t0 = Clock::now();
+    int n = m->n;
+    std::vector<int> perm = order_free_vertices(n, m->F, m->tri.data(), m->rest.data(), m->is_con);
+    m->topo = Topology();
+    build_topology(n, m->F, m->tri.data(), perm, m->is_con, m->topo);
+    const Topology &t = m->topo;
+    int nf = t.nf;
+    // free-free pattern in solve order, from the slots (ascending original neighbour id)
+    m->adj_ptr.assign(nf + 1, 0);
+    m->adj.clear();
+    m->adj_slot.clear();
+    for (int i = 0; i < nf; ++i) {
+        int s = i / kSlice, l = i % kSlice;
+        for (int q = t.slice_ptr[s]; q < t.slice_ptr[s + 1]; ++q) {
+            int64_t g = (int64_t)q * kSlice + l;
+            int j = t.sell_col[g];
+            if (j == i) continue;
+            if (j < nf) { m->adj.push_back(j); m->adj_slot.push_back(g); }
+        }
+        m->adj_ptr[i + 1] = (int)m->adj.size();
+    }
+    m->fac = Factor();
+    if (nf > 0 && nf < n) symbolic(nf, m->adj_ptr, m->adj, m->fac);
+    else m->fac.n = nf;
+    const Factor &f = m->fac;
+    // arena
+    bool fp64 = m->fp64();
+    size_t vsz = fp64 ? sizeof(d4) : sizeof(f4);
+    size_t psz = fp64 ? sizeof(double) : sizeof(float);
+    int64_t slots = t.total_slots();
+    Arena &A = m->arena;
+    A.reset();
+    size_t o_slice = A.reserve(sizeof(int) * (t.nslices + 1));
+    size_t o_col = A.reserve(sizeof(int) * slots);
+    size_t o_a32 = A.reserve(sizeof(float) * 3 * slots);
+    size_t o_a64 = A.reserve(sizeof(double) * 3 * slots);
+    size_t o_w64 = A.reserve(sizeof(double) * slots);
+    size_t o_tri = A.reserve(sizeof(int) * 3 * (size_t)m->F);
+    size_t o_fe = A.reserve(sizeof(double) * 3 * (size_t)m->F);
+    size_t o_sfp = A.reserve(sizeof(int) * (slots + 1));
+    size_t o_sf = A.reserve(sizeof(int) * std::max<size_t>(1, t.slot_fe.size()));
+    size_t o_p = A.reserve(sizeof(d4) * n);
+    size_t o_pp = A.reserve(sizeof(d4) * n);
+    size_t o_q = A.reserve(vsz * n);
+    size_t o_r = A.reserve(vsz * std::max(1, nf));
+    size_t o_z = A.reserve(vsz * std::max(1, nf));
+    size_t o_x = A.reserve(vsz * std::max(1, nf));
+    size_t nrows_off = f.roff.empty() ? 0 : (size_t)f.roff.back();
+    size_t o_u = A.reserve(vsz * std::max<size_t>(1, nrows_off));
+    int ns = f.ns;
+    size_t o_c0 = A.reserve(sizeof(int) * std::max(1, ns));
+    size_t o_w = A.reserve(sizeof(int) * std::max(1, ns));
+    size_t o_no = A.reserve(sizeof(int) * std::max(1, ns));
+    size_t o_ro = A.reserve(sizeof(int) * (ns + 1));
+    size_t o_po = A.reserve(sizeof(long long) * (ns + 1));
+    size_t o_cp = A.reserve(sizeof(int) * (ns + 1));
+    size_t o_ch = A.reserve(sizeof(int) * std::max<size_t>(1, f.child.size()));
+    size_t o_or = A.reserve(sizeof(int) * std::max<size_t>(1, f.offrows.size()));
+    size_t o_rel = A.reserve(sizeof(int) * std::max<size_t>(1, f.rel.size()));
+    size_t o_pan = A.reserve(psz * std::max<int64_t>(1, f.factor_nnz()));
+    int nl = f.nlevels;
+    size_t o_fp = A.reserve(sizeof(int) * (nl + 1));
+    size_t o_fs = A.reserve(sizeof(int) * std::max<size_t>(1, f.fwd_s.size()));
+    size_t o_fr = A.reserve(sizeof(int) * std::max<size_t>(1, f.fwd_r0.size()));
+    size_t o_bp = A.reserve(sizeof(int) * (nl + 1));
+    size_t o_bs = A.reserve(sizeof(int) * std::max<size_t>(1, f.bwd_s.size()));
+    size_t o_bq = A.reserve(sizeof(int) * std::max<size_t>(1, f.bwd_q0.size()));
+    int eblocks = std::max(1, cdiv(t.nslices, kSlicesPerBlock));
+    size_t o_en = A.reserve(sizeof(double) * eblocks);
+    if (A.allocate() != hipSuccess) {
+        set_error("device allocation of " + std::to_string(A.size()) + " bytes failed");
+        A.reset();
+        return MD_ERR_NOMEM;
+    }
+    MeshDev &d = m->dev;
+    d = MeshDev{};
+    d.n = n; d.nf = nf; d.nslices = t.nslices; d.ns = ns; d.F = m->F;
+    d.nslices_free = cdiv(nf, kSlice);
+    d.slots = slots;
+    d.slice_ptr = (const int *)A.at(o_slice);
+    d.sell_col = (const int *)A.at(o_col);
+    d.a32 = (float *)A.at(o_a32);
+    d.a64 = (double *)A.at(o_a64);
+    d.w64 = (double *)A.at(o_w64);
+    d.tri = (const int *)A.at(o_tri);
+    d.fe_val = (double *)A.at(o_fe);
+    d.slot_fe_ptr = (const int *)A.at(o_sfp);
+    d.slot_fe = (const int *)A.at(o_sf);
+    d.p = (d4 *)A.at(o_p);
+    d.pp = (d4 *)A.at(o_pp);
+    d.quat = A.at(o_q);
+    d.r = A.at(o_r); d.z = A.at(o_z); d.x = A.at(o_x); d.u = A.at(o_u);
+    d.sn_c0 = (const int *)A.at(o_c0); d.sn_w = (const int *)A.at(o_w);
+    d.sn_noff = (const int *)A.at(o_no); d.sn_roff = (const int *)A.at(o_ro);
+    d.sn_poff = (const long long *)A.at(o_po);
+    d.child_ptr = (const int *)A.at(o_cp); d.child = (const int *)A.at(o_ch);
+    d.offrows = (const int *)A.at(o_or); d.rel = (const int *)A.at(o_rel);
+    d.panel = A.at(o_pan);
+    d.fwd_ptr = (const int *)A.at(o_fp); d.fwd_s = (const int *)A.at(o_fs); d.fwd_r0 = (const int *)A.at(o_fr);
+    d.bwd_ptr = (const int *)A.at(o_bp); d.bwd_s = (const int *)A.at(o_bs); d.bwd_q0 = (const int *)A.at(o_bq);
+    d.energy_part = (double *)A.at(o_en);
+    hipStream_t st = m->ctx->stream;
+    auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
+        if (bytes == 0) return hipSuccess;
+        return hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, st);
+    };
+    std::vector<long long> poff(f.poff.begin(), f.poff.end());
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(up(d.slice_ptr, t.slice_ptr.data(), sizeof(int) * (t.nslices + 1)));
+    chk(up(d.sell_col, t.sell_col.data(), sizeof(int) * slots));
+    chk(up(d.tri, t.tri_internal.data(), sizeof(int) * 3 * (size_t)m->F));
+    chk(up(d.slot_fe_ptr, t.slot_fe_ptr.data(), sizeof(int) * (slots + 1)));
+    chk(up(d.slot_fe, t.slot_fe.data(), sizeof(int) * t.slot_fe.size()));
+    if (ns > 0) {
+        chk(up(d.sn_c0, f.c0.data(), sizeof(int) * ns));
+        chk(up(d.sn_w, f.w.data(), sizeof(int) * ns));
+        chk(up(d.sn_noff, f.noff.data(), sizeof(int) * ns));
+        chk(up(d.sn_roff, f.roff.data(), sizeof(int) * (ns + 1)));
+        chk(up(d.sn_poff, poff.data(), sizeof(long long) * (ns + 1)));
+        chk(up(d.child_ptr, f.child_ptr.data(), sizeof(int) * (ns + 1)));
+        chk(up(d.child, f.child.data(), sizeof(int) * f.child.size()));
+        chk(up(d.offrows, f.offrows.data(), sizeof(int) * f.offrows.size()));
+        chk(up(d.rel, f.rel.data(), sizeof(int) * f.rel.size()));
+        chk(up(d.fwd_ptr, f.fwd_ptr.data(), sizeof(int) * (nl + 1)));
+        chk(up(d.fwd_s, f.fwd_s.data(), sizeof(int) * f.fwd_s.size()));
+        chk(up(d.fwd_r0, f.fwd_r0.data(), sizeof(int) * f.fwd_r0.size()));
+        chk(up(d.bwd_ptr, f.bwd_ptr.data(), sizeof(int) * (nl + 1)));
+        chk(up(d.bwd_s, f.bwd_s.data(), sizeof(int) * f.bwd_s.size()));
+        chk(up(d.bwd_q0, f.bwd_q0.data(), sizeof(int) * f.bwd_q0.size()));
+    }
+    chk(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+        set_error(std::string("upload of the mesh analysis failed: ") + hipGetErrorString(e));
+        return MD_ERR_HIP;
+    }
+    std::vector<const MeshDev *> ms{&m->dev};
+    std::vector<const Factor *> fs{&m->fac};
+    e = build_plan(m->plan, ms, fs, fp64, m->cfg.refine_steps, st);
+    if (e != hipSuccess) {
+        set_error(std::string("launch plan upload failed: ") + hipGetErrorString(e));
+        return MD_ERR_HIP;
+    }
+    m->analyzed = true;
+    m->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    return MD_OK;
+}
+
+// Uploads positions (original order, 3 per vertex) as d4 in internal order.
+hipError_t upload_positions(md_mesh *m, const double *xyz, d4 *dst) {
+    std::vector<d4> h(m->n);
+    for (int iv = 0; iv < m->n; ++iv) {
+        int v = m->topo.original_of[iv];
+        h[iv] = d4{xyz[3 * (size_t)v], xyz[3 * (size_t)v + 1], xyz[3 * (size_t)v + 2], 0.0};
+    }
+    hipError_t e = hipMemcpyAsync(dst, h.data(), sizeof(d4) * m->n, hipMemcpyHostToDevice, m->ctx->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(m->ctx->stream);
+}
+
+hipError_t download_positions(md_mesh *m, double *xyz) {
+    std::vector<d4> h(m->n);
+    hipError_t e = hipMemcpyAsync(h.data(), m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, m->ctx->stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(m->ctx->stream);
+    if (e != hipSuccess) return e;
+    for (int iv = 0; iv < m->n; ++iv) {
+        int v = m->topo.original_of[iv];
+        xyz[3 * (size_t)v] = h[iv].x;
+        xyz[3 * (size_t)v + 1] = h[iv].y;
+        xyz[3 * (size_t)v + 2] = h[iv].z;
+    }
+    return hipSuccess;
+}
+
+// Cotan weights of the uploaded rest pose (per-face kernel + per-slot deterministic sum).
+hipError_t compute_cotan(md_mesh *m) {
+    const Plan &P = m->plan;
+    hipStream_t st = m->ctx->stream;
+    launch_cotan_face(st, P.set(P.off_face), P.blk_face);
+    launch_cotan_slot(st, P.set(P.off_slice), P.blk_slice);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
+// FP64 numeric factorization on the host from the device cotan weights; uploads panels.
+int factorize(md_mesh *m) {
+    auto t0 = Clock::now();
+    const Topology &t = m->topo;
+    int nf = t.nf;
+    int64_t slots = t.total_slots();
+    std::vector<double> w(slots);
+    if (hipMemcpy(w.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("download of cotan weights failed");
+        return MD_ERR_HIP;
+    }
+    std::vector<double> diag(nf, 0.0), val(m->adj.size());
+    for (int i = 0; i < nf; ++i) {
+        int s = i / kSlice, l = i % kSlice;
+        double acc = 0.0;  // L_ii sums every neighbour weight in ascending id (arap.h:332)
+        for (int q = t.slice_ptr[s]; q < t.slice_ptr[s + 1]; ++q) {
+            int64_t g = (int64_t)q * kSlice + l;
+            if (t.sell_col[g] != i) acc += w[g];
+        }
+        diag[i] = acc;
+    }
+    for (size_t k = 0; k < m->adj.size(); ++k) val[k] = -w[m->adj_slot[k]];
+    std::string err;
+    if (!numeric(m->adj_ptr, m->adj, val, diag, m->fac, err)) {
+        set_error(err);
+        m->factored = false;
+        return MD_ERR_FACTOR;
+    }
+    hipError_t e;
+    size_t ne = (size_t)m->fac.factor_nnz();
+    if (m->fp64()) {
+        e = hipMemcpy(const_cast<void *>(m->dev.panel), m->fac.panel.data(), sizeof(double) * ne,
+                      hipMemcpyHostToDevice);
+    } else {
+        std::vector<float> p32(ne);
+        for (size_t k = 0; k < ne; ++k) p32[k] = (float)m->fac.panel[k];
+        e = hipMemcpy(const_cast<void *>(m->dev.panel), p32.data(), sizeof(float) * ne, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        set_error("panel upload failed");
+        return MD_ERR_HIP;
+    }
+    m->factored = true;
+    m->setup_ms_host += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    return MD_OK;
+}
+
+// The dirty branch of deform (arap.h:102-120). Returns MD_OK with *early = true when no
+// vertex is constrained (arap.h:113-114).
+int setup(md_mesh *m, const double *xyz, bool *early) {
+    *early = false;
+    bool full = m->cfg.rest_mode == MD_REST_REFERENCE || !m->initialized;
+    if (full) m->rest.assign(xyz, xyz + 3 * (size_t)m->n);
+    bool need_analysis = !m->analyzed || m->con_set_changed;
+    if (need_analysis) {
+        m->factored = false;
+        int rc = analyze(m);
+        if (rc) return rc;
+    }
+    hipError_t e;
+    if (full || need_analysis) {
+        e = upload_positions(m, m->rest.data(), m->dev.p);
+        if (e == hipSuccess) e = compute_cotan(m);
+        if (e != hipSuccess) { set_error(std::string("cotan weights: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    }
+    // initial guess: the current mesh with the targets pasted (arap.h:162-168, 277-281)
+    std::vector<double> init(xyz, xyz + 3 * (size_t)m->n);
+    for (int v = 0; v < m->n; ++v)
+        if (m->is_con[v]) std::memcpy(&init[3 * (size_t)v], &m->con[3 * (size_t)v], 3 * sizeof(double));
+    e = upload_positions(m, init.data(), m->dev.pp);
+    if (e != hipSuccess) { set_error("position upload failed"); return MD_ERR_HIP; }
+    // identity rotations (arap.h:246-249)
+    launch_identity_quat(m->ctx->stream, m->plan.set(m->plan.off_ident), m->plan.blk_ident, m->fp64());
+    m->initialized = true;
+    m->iterated = false;
+    if (m->ncon == 0) { *early = true; return MD_OK; }
+    if (m->topo.nf > 0 && (full || need_analysis || !m->factored)) {
+        int rc = factorize(m);
+        if (rc) return rc;
+    }
+    m->dirty = false;
+    m->con_set_changed = false;
+    return MD_OK;
+}
+
+int iterate(md_mesh *m, int iterations) {
+    if (iterations <= 0) return MD_OK;
+    hipError_t e = run_iterations(m->plan, m->ctx->stream, iterations, m->cfg.use_graph != 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
+    if (e != hipSuccess) { set_error(std::string("iteration failed: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    m->iterated = true;
+    return MD_OK;
+}
+
+}  // namespace
+
+extern "C" int md_mesh_prepare(md_mesh *m, const double *xyz) {
+    if (!m || !xyz) { set_error("md_mesh_prepare: null pointer"); return MD_ERR_INVALID; }
+    hipSetDevice(m->ctx->device);
+    if (!m->dirty) return MD_OK;
+    bool early = false;
+    int rc = setup(m, xyz, &early);
+    if (rc) return rc;
+    if (early) { set_error("md_mesh_prepare: no constrained vertex"); return MD_ERR_STATE; }
+    return MD_OK;
+}
+
+extern "C" int md_mesh_deform(md_mesh *m, int iterations, double *xyz, int *ok) {
+    if (ok) *ok = 0;
+    if (!m || !xyz || iterations < 0) { set_error("md_mesh_deform: invalid argument"); return MD_ERR_INVALID; }
+    hipSetDevice(m->ctx->device);
+    if (m->dirty) {
+        bool early = false;
+        int rc = setup(m, xyz, &early);
+        if (rc == MD_ERR_FACTOR) return rc;   // deform() == false (arap.h:116-117)
+        if (rc) return rc;
+        if (early) { if (ok) *ok = 1; return MD_OK; }
+    }
+    int rc = iterate(m, iterations);  // arap.h:122-129
+    if (rc) return rc;
+    hipError_t e = download_positions(m, xyz);  // write-back, arap.h:133-135
+    if (e != hipSuccess) { set_error("position download failed"); return MD_ERR_HIP; }
+    if (ok) *ok = 1;
+    return MD_OK;
+}
+
+extern "C" int md_mesh_iterate(md_mesh *m, int iterations) {
+    if (!m || iterations < 0) { set_error("md_mesh_iterate: invalid argument"); return MD_ERR_INVALID; }
+    if (m->dirty) { set_error("md_mesh_iterate: mesh is not prepared"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    return iterate(m, iterations);
+}
+
+extern "C" int md_mesh_download(md_mesh *m, double *xyz) {
+    if (!m || !xyz) { set_error("md_mesh_download: null pointer"); return MD_ERR_INVALID; }
+    if (!m->analyzed) { set_error("md_mesh_download: no device state yet"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    if (download_positions(m, xyz) != hipSuccess) { set_error("position download failed"); return MD_ERR_HIP; }
+    return MD_OK;
+}
+
+extern "C" int md_mesh_cotan_nnz(md_mesh *m, int64_t *nnz) {
+    if (!m || !nnz) { set_error("md_mesh_cotan_nnz: null pointer"); return MD_ERR_INVALID; }
+    if (!m->analyzed) { set_error("md_mesh_cotan_nnz: call deform first"); return MD_ERR_STATE; }
+    int64_t c = 0;
+    const Topology &t = m->topo;
+    for (int s = 0; s < t.nslices; ++s)
+        for (int l = 0; l < kSlice; ++l) {
+            int iv = s * kSlice + l;
+            if (iv >= t.n) break;
+            for (int q = t.slice_ptr[s]; q < t.slice_ptr[s + 1]; ++q)
+                if (t.sell_col[(int64_t)q * kSlice + l] != iv) ++c;
+        }
+    *nnz = c;
+    return MD_OK;
+}
+
+extern "C" int md_mesh_cotan_csr(md_mesh *m, int *rowptr, int *col, double *val) {
+    if (!m || !rowptr || !col || !val) { set_error("md_mesh_cotan_csr: null pointer"); return MD_ERR_INVALID; }
+    if (!m->analyzed) { set_error("md_mesh_cotan_csr: call deform first"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    const Topology &t = m->topo;
+    int64_t slots = t.total_slots();
+    std::vector<double> w(slots);
+    if (hipMemcpy(w.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("download of cotan weights failed");
+        return MD_ERR_HIP;
+    }
+    rowptr[0] = 0;
+    int64_t nnz = 0;
+    for (int v = 0; v < m->n; ++v) {
+        int iv = t.internal_of[v];
+        int s = iv / kSlice, l = iv % kSlice;
+        for (int q = t.slice_ptr[s]; q < t.slice_ptr[s + 1]; ++q) {
+            int64_t g = (int64_t)q * kSlice + l;
+            int j = t.sell_col[g];
+            if (j == iv) continue;
+            col[nnz] = t.original_of[j];
+            val[nnz] = w[g];
+            ++nnz;
+        }
+        rowptr[v + 1] = (int)nnz;
+    }
+    return MD_OK;
+}
+
+extern "C" int md_mesh_rotations(md_mesh *m, double *R) {
+    if (!m || !R) { set_error("md_mesh_rotations: null pointer"); return MD_ERR_INVALID; }
+    if (!m->analyzed) { set_error("md_mesh_rotations: call deform first"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    int n = m->n;
+    std::vector<double> q(4 * (size_t)n);
+    if (m->fp64()) {
+        if (hipMemcpy(q.data(), m->dev.quat, sizeof(d4) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("rotation download failed"); return MD_ERR_HIP;
+        }
+    } else {
+        std::vector<float> qf(4 * (size_t)n);
+        if (hipMemcpy(qf.data(), m->dev.quat, sizeof(f4) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("rotation download failed"); return MD_ERR_HIP;
+        }
+        for (size_t k = 0; k < qf.size(); ++k) q[k] = qf[k];
+    }
+    for (int iv = 0; iv < n; ++iv) {
+        double x = q[4 * iv], y = q[4 * iv + 1], z = q[4 * iv + 2], w = q[4 * iv + 3];
+        double s = 2.0 / (x * x + y * y + z * z + w * w);
+        double *o = R + 9 * (size_t)m->topo.original_of[iv];
+        o[0] = 1 - s * (y * y + z * z); o[1] = s * (x * y - w * z); o[2] = s * (x * z + w * y);
+        o[3] = s * (x * y + w * z); o[4] = 1 - s * (x * x + z * z); o[5] = s * (y * z - w * x);
+        o[6] = s * (x * z - w * y); o[7] = s * (y * z + w * x); o[8] = 1 - s * (x * x + y * y);
+    }
+    return MD_OK;
+}
+
+extern "C" int md_mesh_energy(md_mesh *m, double *energy) {
+    if (!m || !energy) { set_error("md_mesh_energy: null pointer"); return MD_ERR_INVALID; }
+    if (!m->analyzed) { set_error("md_mesh_energy: call deform first"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    const Plan &P = m->plan;
+    launch_energy(m->ctx->stream, P.set(P.off_slice), P.blk_slice, m->fp64());
+    int nb = std::max(1, cdiv(m->topo.nslices, kSlicesPerBlock));
+    std::vector<double> part(nb);
+    MD_HIP(hipMemcpyAsync(part.data(), m->dev.energy_part, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                          m->ctx->stream));
+    MD_HIP(hipStreamSynchronize(m->ctx->stream));
+    double e = 0.0;
+    for (int b = 0; b < P.blk_slice; ++b) e += part[b];
+    *energy = e;
+    return MD_OK;
+}
+
+extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
+    if (!m || !out) { set_error("md_mesh_get_stats: null pointer"); return MD_ERR_INVALID; }
+    std::memset(out, 0, sizeof(*out));
+    out->n = m->n;
+    out->F = m->F;
+    if (!m->analyzed) return MD_OK;
+    AnalysisStats s = stats(m->topo, m->fac);
+    out->nf = s.nf;
+    out->nslices = s.nslices;
+    out->supernodes = s.ns;
+    out->levels = s.nlevels;
+    out->max_sn_width = s.max_width;
+    out->max_sn_rows = s.max_rows;
+    out->slots = s.slots;
+    int64_t nnz = 0;
+    md_mesh_cotan_nnz(m, &nnz);
+    out->cotan_nnz = nnz;
+    out->panel_entries = s.panel_entries;
+    out->factor_nnz = s.strict_nnz_equiv;
+    out->fwd_items = s.fwd_items;
+    out->bwd_items = s.bwd_items;
+    out->device_bytes = (int64_t)m->arena.size();
+    out->setup_ms_host = m->setup_ms_host;
+    return MD_OK;
+}
+
+extern "C" int md_mesh_time_iterations(md_mesh *m, int warmup, int iterations, double *total_ms,
+                                       double *phase_ms4) {
+    if (!m || iterations < 0 || warmup < 0) { set_error("md_mesh_time_iterations: invalid argument"); return MD_ERR_INVALID; }
+    if (m->dirty) { set_error("md_mesh_time_iterations: mesh is not prepared"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    hipStream_t st = m->ctx->stream;
+    bool g = m->cfg.use_graph != 0;
+    MD_HIP(run_iterations(m->plan, st, warmup, g));
+    MD_HIP(hipStreamSynchronize(st));
+    hipEvent_t a, b;
+    MD_HIP(hipEventCreate(&a));
+    MD_HIP(hipEventCreate(&b));
+    MD_HIP(hipEventRecord(a, st));
+    MD_HIP(run_iterations(m->plan, st, iterations, g));
+    MD_HIP(hipEventRecord(b, st));
+    MD_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    if (total_ms) *total_ms = ms;
+    if (phase_ms4) {
+        // phases from a separate pass with events between them (no graph)
+        hipEvent_t ev[5];
+        for (auto &x : ev) hipEventCreate(&x);
+        double acc[4] = {0, 0, 0, 0};
+        for (int i = 0; i < iterations; ++i) {
+            enqueue_iteration(m->plan, st, ev);
+            hipEventSynchronize(ev[4]);
+            for (int k = 0; k < 4; ++k) {
+                float t = 0.f;
+                hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+                acc[k] += t;
+            }
+        }
+        for (int k = 0; k < 4; ++k) phase_ms4[k] = acc[k];
+        for (auto &x : ev) hipEventDestroy(x);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    m->iterated = true;
+    return MD_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// host-only diagnostics
+// ------------------------------------------------------------------------------------------
+
+extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
+                                    const double *xyz, int nrhs, const double *b, double *x,
+                                    int64_t *panel_entries, int *levels) {
+    if (n <= 0 || !rowptr || !col || !val || !xyz || nrhs < 0 || (nrhs > 0 && (!b || !x))) {
+        set_error("md_host_factor_solve: invalid argument");
+        return MD_ERR_INVALID;
+    }
+    std::vector<int> perm = order_graph(n, rowptr, col, xyz);
+    std::vector<int> iperm(n);
+    for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
+    std::vector<int> ptr(n + 1, 0), adj;
+    std::vector<double> av, diag(n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        int i = perm[k];
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+            int j = col[e];
+            if (j == i) diag[k] += val[e];
+            else { adj.push_back(iperm[j]); av.push_back(val[e]); }
+        }
+        ptr[k + 1] = (int)adj.size();
+    }
+    Factor f;
+    symbolic(n, ptr, adj, f);
+    std::string err;
+    if (!numeric(ptr, adj, av, diag, f, err)) { set_error(err); return MD_ERR_FACTOR; }
+    std::vector<double> bp((size_t)n * nrhs), xp((size_t)n * nrhs);
+    for (int r = 0; r < nrhs; ++r)
+        for (int k = 0; k < n; ++k) bp[(size_t)r * n + k] = b[(size_t)r * n + perm[k]];
+    solve_host(f, bp.data(), xp.data(), nrhs);
+    for (int r = 0; r < nrhs; ++r)
+        for (int k = 0; k < n; ++k) x[(size_t)r * n + perm[k]] = xp[(size_t)r * n + k];
+    if (panel_entries) *panel_entries = f.factor_nnz();
+    if (levels) *levels = f.nlevels;
+    return MD_OK;
+}

@@ -1,0 +1,104 @@
+// Host runtime objects behind the C ABI (include/md_arap.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "md_analysis.h"
+#include "md_arap.h"
+#include "md_kernels.h"
+
+namespace md {
+
+void set_error(const std::string &msg);
+
+// One device allocation carved into 256-byte aligned buffers.
+class Arena {
+public:
+    ~Arena() { release(); }
+    // two passes: reserve sizes, then allocate once and hand out pointers
+    size_t reserve(size_t bytes) {
+        size_t off = (size_ + 255) & ~size_t(255);
+        size_ = off + bytes;
+        return off;
+    }
+    hipError_t allocate() {
+        release();
+        if (size_ == 0) return hipSuccess;
+        return hipMalloc(&base_, size_);
+    }
+    void *at(size_t off) const { return static_cast<char *>(base_) + off; }
+    size_t size() const { return size_; }
+    void release() {
+        if (base_) hipFree(base_);
+        base_ = nullptr;
+    }
+    void reset() { release(); size_ = 0; }
+
+private:
+    void *base_ = nullptr;
+    size_t size_ = 0;
+};
+
+// Prefix tables and device mesh table for one set of meshes launched together.
+struct Plan {
+    int nactive = 0;
+    int nlevels = 0;
+    int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0;
+    std::vector<int> blk_fwd, blk_bwd;
+    int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0;
+    std::vector<int> off_fwd, off_bwd;
+    MeshDev *d_meshes = nullptr;
+    int *d_prefix = nullptr;
+    size_t cap_meshes = 0, cap_prefix = 0;
+    std::vector<int> h_prefix;
+    std::vector<MeshDev> h_meshes;
+    bool fp64 = true;
+    int refine_steps = 1;
+    hipGraphExec_t graph = nullptr;
+    ~Plan();
+    void drop_graph();
+    LaunchSet set(int off) const { return LaunchSet{d_meshes, d_prefix + off, nactive}; }
+};
+
+// Builds (or rebuilds) a plan for the given meshes and uploads its tables on `stream`.
+hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
+                      const std::vector<const Factor *> &factors, bool fp64, int refine_steps,
+                      hipStream_t stream);
+// Enqueues one ARAP iteration (local step + refine_steps x (residual, sweeps)).
+void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *phase_events);
+// Runs `iters` iterations, from a captured graph when use_graph is set.
+hipError_t run_iterations(Plan &plan, hipStream_t stream, int iters, bool use_graph);
+
+}  // namespace md
+
+struct md_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string arch;
+};
+
+struct md_mesh {
+    md_ctx *ctx = nullptr;
+    md_config cfg{};
+    int n = 0, F = 0;
+    std::vector<int> tri;          // original faces
+    std::vector<char> is_con;      // constrained flags (the key set of the reference's map)
+    std::vector<double> con;       // targets, 3 per vertex
+    int ncon = 0;
+    bool dirty = true, con_set_changed = true, initialized = false;
+    bool analyzed = false, factored = false, iterated = false;
+    std::vector<double> rest;      // rest positions, original order
+    md::Topology topo;
+    md::Factor fac;
+    md::Arena arena;
+    md::MeshDev dev{};
+    md::Plan plan;                 // single-mesh launch plan
+    std::vector<int> adj_ptr, adj; // free-free pattern in solve order
+    std::vector<int64_t> adj_slot; // slot of each adj entry
+    std::vector<int64_t> diag_slot_ptr; // unused placeholder for symmetry with adj
+    double setup_ms_host = 0.0;
+    bool fp64() const { return cfg.precision == MD_PREC_FP64; }
+};

@@ -1,0 +1,230 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+Tolerances (position error = max |gpu - oracle| / max |oracle| over all coordinates):
+  * FP64 path: 1e-9 (positions), 1e-12 relative (cotan weights), 1e-9 (rotations, energy).
+  * FP32 local step + FP32 block LDL^T + FP64 residual refinement: 1e-5 (positions).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import meshgen
+import mdarap
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL_FP64 = 1e-9
+POS_TOL_FP32 = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if mdarap.device_count() == 0:
+        pytest.fail("no GPU visible: the HIP path cannot run")
+    c = mdarap.Context(0)
+    yield c
+    c.close()
+
+
+def gold(golden_dir, name):
+    return np.load(os.path.join(golden_dir, name + ".npz"))
+
+
+def run_gpu(ctx, xyz, tri, idx, tgt, iters, precision=mdarap.PREC_FP64, **kw):
+    m = mdarap.Mesh(ctx, xyz, tri, mdarap.config(precision=precision, **kw))
+    m.set_constraints(idx, tgt)
+    ok = m.deform(iters)
+    return m, ok
+
+
+# ---- cotan weights (arap.h:182-239) -------------------------------------------------------
+
+def test_cotan_known_answer(ctx, golden_dir):
+    """tests/test_cotan.cpp:20-54 through the PrivateAccessor-equivalent export."""
+    say("case: unit square cotan")
+    g = gold(golden_dir, "unit_square")
+    m = mdarap.Mesh(ctx, g["xyz"], g["tri"])
+    assert m.deform(0) is True  # no constraints: returns true (arap.h:113-114)
+    rp, ci, v = m.cotan()
+    import scipy.sparse as sp
+    W = sp.csr_matrix((v, ci, rp), shape=(4, 4)).toarray().astype(np.float32)
+    exp = g["expected"]
+    assert np.linalg.norm(W - exp) <= 1e-4 * min(np.linalg.norm(W), np.linalg.norm(exp))
+    np.testing.assert_array_equal(m.mesh, g["xyz"])  # mesh untouched
+
+
+@pytest.mark.parametrize("name", ["sphere_obj", "bar_obj", "plane_obj", "uv_sphere"])
+def test_cotan_matches_oracle(ctx, golden_dir, name):
+    say("case: cotan", name)
+    g = gold(golden_dir, name)
+    m, ok = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], 0)
+    assert ok
+    rp, ci, v = m.cotan()
+    assert np.array_equal(rp, g["rp"]) and np.array_equal(ci, g["ci"])
+    np.testing.assert_allclose(v, g["w"], rtol=1e-12, atol=0)
+
+
+# ---- local step (arap.h:354-384) ----------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["sphere_obj", "bar_obj", "plane_obj", "uv_sphere"])
+def test_first_rotations_match_oracle(ctx, golden_dir, name):
+    say("case: rotations", name)
+    g = gold(golden_dir, name)
+    m, ok = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], 1)
+    R = m.rotations()
+    np.testing.assert_allclose(R, g["R1"], atol=1e-9)
+    m32, _ = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], 1, precision=mdarap.PREC_FP32_REFINE)
+    np.testing.assert_allclose(m32.rotations(), g["R1"], atol=2e-4)
+
+
+# ---- whole deform (arap.h:101-138) --------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["sphere_obj", "bar_obj", "plane_obj", "uv_sphere"])
+def test_deform_fp64_matches_golden(ctx, golden_dir, name):
+    say("case: deform fp64", name)
+    g = gold(golden_dir, name)
+    m, ok = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], int(g["iters"]))
+    assert ok
+    err = meshgen.relative_error(m.mesh, g["pos"])
+    say("  rel err", err)
+    assert err < POS_TOL_FP64
+    # energy of the final state (rotations of the last local step)
+    assert m.energy() == pytest.approx(float(g["energy"][-1]), rel=1e-9)
+
+
+@pytest.mark.parametrize("name", ["sphere_obj", "bar_obj", "plane_obj", "uv_sphere"])
+def test_deform_fp32_refine_matches_golden(ctx, golden_dir, name):
+    say("case: deform fp32+refine", name)
+    g = gold(golden_dir, name)
+    m, ok = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], int(g["iters"]),
+                    precision=mdarap.PREC_FP32_REFINE)
+    assert ok
+    err = meshgen.relative_error(m.mesh, g["pos"])
+    say("  rel err", err)
+    assert err < POS_TOL_FP32
+
+
+def test_energy_trace_monotone(ctx):
+    say("case: energy trace")
+    pr = meshgen.sphere_problem()
+    idx, tgt = pr.constraints()
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri)
+    m.set_constraints(idx, tgt)
+    m.deform(0)
+    a = oracle.Arap(pr.xyz, pr.tri)
+    a.set_constraints(idx, tgt)
+    _, en = a.deform(8, energy=True)
+    got = []
+    for _ in range(8):
+        m.iterate(1)
+        got.append(m.energy())
+    np.testing.assert_allclose(got, en, rtol=1e-9)
+    assert np.all(np.diff(got) <= 1e-12 * got[0])
+
+
+# ---- deform() semantics --------------------------------------------------------------------
+
+def test_semantics(ctx):
+    say("case: semantics")
+    pr = meshgen.sphere_problem(8, 12)
+    idx, tgt = pr.constraints()
+    # deform(0) pastes targets into the rest pose (arap.h:122-135)
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri)
+    m.set_constraints(idx, tgt)
+    assert m.deform(0)
+    exp = pr.xyz.copy()
+    exp[idx] = tgt
+    np.testing.assert_array_equal(m.mesh, exp)
+    # out-of-range index is an error
+    with pytest.raises(mdarap.MDError) as e:
+        m.set_constraint(pr.n, [0, 0, 0])
+    assert e.value.code == mdarap.MD_ERR_INVALID
+    # all vertices constrained: targets are written back
+    m2 = mdarap.Mesh(ctx, pr.xyz, pr.tri)
+    m2.set_constraints(np.arange(pr.n), pr.xyz + 1.0)
+    assert m2.deform(3)
+    np.testing.assert_allclose(m2.mesh, pr.xyz + 1.0)
+
+
+def test_singular_returns_false(ctx):
+    say("case: singular")
+    x1, t1 = meshgen.uv_sphere(6, 8)
+    xyz = np.concatenate([x1, x1 + [5.0, 0, 0]])
+    tri = np.concatenate([t1, t1 + x1.shape[0]])
+    m = mdarap.Mesh(ctx, xyz, tri)
+    m.set_constraint(0, xyz[0] + [0, 0, 0.1])
+    m.set_constraint(x1.shape[0] - 1, xyz[x1.shape[0] - 1])
+    before = m.mesh.copy()
+    assert m.deform(2) is False
+    np.testing.assert_array_equal(m.mesh, before)
+
+
+@pytest.mark.parametrize("rest_mode", [mdarap.REST_REFERENCE, mdarap.REST_FIXED])
+def test_interactive_frames_match_oracle(ctx, golden_dir, rest_mode):
+    """Frame loop of examples/deform_sphere.cpp:71-91: setConstraint + deform(5) per frame."""
+    say("case: frames rest_mode", rest_mode)
+    g = gold(golden_dir, "sphere_obj")
+    xyz, tri = g["xyz"], g["tri"]
+    m = mdarap.Mesh(ctx, xyz, tri, mdarap.config(rest_mode=rest_mode))
+    a = oracle.Arap(xyz, tri)
+    m.set_constraint(37, xyz[37])
+    a.set_constraint(37, xyz[37])
+    for frame in range(6):
+        tgt = xyz[32] + [0, 0, np.sin(-np.pi / 2 + 0.3 * frame)]
+        m.set_constraint(32, tgt)
+        a.set_constraint(32, tgt)
+        assert m.deform(5)
+        assert a.deform(5, rest_mode=rest_mode)
+        err = meshgen.relative_error(m.mesh, a.mesh)
+        assert err < POS_TOL_FP64, (frame, err)
+
+
+# ---- full-size workloads ------------------------------------------------------------------
+
+@pytest.mark.parametrize("which,iters", [("bar", 5), ("ico", 5)])
+def test_large_workloads_fp64_and_fp32(ctx, which, iters):
+    pr = {"bar": meshgen.bar_problem, "ico": meshgen.noisy_icosphere_problem}[which]()
+    say("case: full size", pr.name, "n", pr.n)
+    idx, tgt = pr.constraints()
+    a = oracle.Arap(pr.xyz, pr.tri)
+    a.set_constraints(idx, tgt)
+    a.deform(iters)
+    for prec, tol in ((mdarap.PREC_FP64, POS_TOL_FP64), (mdarap.PREC_FP32_REFINE, POS_TOL_FP32)):
+        m, ok = run_gpu(ctx, pr.xyz, pr.tri, idx, tgt, iters, precision=prec)
+        assert ok
+        err = meshgen.relative_error(m.mesh, a.mesh)
+        say("  precision", prec, "rel err", err, "stats", m.stats())
+        assert err < tol
+
+
+def test_batch_equals_individual(ctx):
+    say("case: batch")
+    probs = meshgen.mixed_batch(6, 500, 6000, seed=7)
+    outs, meshes = [], []
+    for pr in probs:
+        idx, tgt = pr.constraints()
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+        m.set_constraints(idx, tgt)
+        m.prepare(pr.xyz)
+        m.iterate(4)
+        outs.append(m.download())
+        meshes.append(m)
+    bm = []
+    for pr in probs:
+        idx, tgt = pr.constraints()
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+        m.set_constraints(idx, tgt)
+        m.prepare(pr.xyz)
+        bm.append(m)
+    b = mdarap.Batch(ctx, bm)
+    b.iterate(4)
+    for m, ref in zip(bm, outs):
+        np.testing.assert_array_equal(m.download(), ref)
