@@ -408,7 +408,6 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     f.offrows.resize(f.roff[nn]);
     for (int s = 0; s < ns; ++s)
         if (alive[s]) std::copy(rows[s].begin(), rows[s].end(), f.offrows.begin() + f.roff[newid[s]]);
-    f.uoff = f.roff;
     // children, heights
     f.child_ptr.assign(nn + 1, 0);
     for (int t = 0; t < nn; ++t)
@@ -446,25 +445,72 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     }
     f.poff.assign(nn + 1, 0);
     for (int t = 0; t < nn; ++t) f.poff[t + 1] = f.poff[t] + (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
-    // work items by level
-    f.fwd_ptr.assign(f.nlevels + 1, 0);
-    f.bwd_ptr.assign(f.nlevels + 1, 0);
+    // device layout: tiles, partial vectors, bucket pointers, row locators
+    auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
+    f.kc.resize(nn); f.kr.resize(nn);
+    f.pc_off.assign(nn + 1, 0); f.pr_off.assign(nn + 1, 0);
+    f.uoff.assign(nn + 1, 0); f.xoff.assign(nn + 1, 0);
     for (int t = 0; t < nn; ++t) {
-        f.fwd_ptr[f.height[t] + 1] += std::max(1, (f.noff[t] + kFwdRows - 1) / kFwdRows);
-        f.bwd_ptr[f.height[t] + 1] += (f.w[t] + kBwdCols - 1) / kBwdCols;
+        int m = f.w[t] + f.noff[t];
+        f.kc[t] = cdiv(f.w[t], kTile);
+        f.kr[t] = cdiv(m, kTile);
+        f.pc_off[t + 1] = f.pc_off[t] + (int64_t)f.noff[t] * f.w[t];
+        f.pr_off[t + 1] = f.pr_off[t] + (int64_t)m * f.w[t];
+        f.uoff[t + 1] = f.uoff[t] + (int64_t)f.kc[t] * f.noff[t];
+        f.xoff[t + 1] = f.xoff[t] + (int64_t)f.kr[t] * f.w[t];
+    }
+    f.bp_off.assign(nn, 0);
+    f.bp.clear();
+    for (int t = 0; t < nn; ++t) {
+        int p = f.parent[t];
+        f.bp_off[t] = (int)f.bp.size();
+        if (p < 0) { f.bp.push_back(0); continue; }
+        int nb = cdiv(f.w[p] + f.noff[p], kTile);
+        const int* rl = f.rel.data() + f.roff[t];
+        int e = 0, cn = f.noff[t];
+        for (int b = 0; b <= nb; ++b) {
+            while (e < cn && rl[e] < b * kTile) ++e;
+            f.bp.push_back(e);
+        }
+    }
+    f.rowinfo.assign(4 * (size_t)n, 0);
+    for (int t = 0; t < nn; ++t)
+        for (int q = 0; q < f.w[t]; ++q) {
+            int row = f.c0[t] + q;
+            f.rowinfo[4 * (size_t)row] = (int)(f.xoff[t] + q);
+            f.rowinfo[4 * (size_t)row + 1] = f.w[t];
+            f.rowinfo[4 * (size_t)row + 2] = f.kr[t];
+        }
+    f.xloc.assign(4 * f.offrows.size(), 0);
+    for (size_t e = 0; e < f.offrows.size(); ++e)
+        for (int k = 0; k < 4; ++k) f.xloc[4 * e + k] = f.rowinfo[4 * (size_t)f.offrows[e] + k];
+    f.ft_ptr.assign(f.nlevels + 1, 0);
+    f.bt_ptr.assign(f.nlevels + 1, 0);
+    for (int t = 0; t < nn; ++t) {
+        int m = f.w[t] + f.noff[t];
+        f.ft_ptr[f.height[t] + 1] += std::max(1, cdiv(f.noff[t], kTile)) * f.kc[t];
+        f.bt_ptr[f.height[t] + 1] += f.kc[t] * cdiv(m, kTile);
     }
     for (int l = 0; l < f.nlevels; ++l) {
-        f.fwd_ptr[l + 1] += f.fwd_ptr[l];
-        f.bwd_ptr[l + 1] += f.bwd_ptr[l];
+        f.ft_ptr[l + 1] += f.ft_ptr[l];
+        f.bt_ptr[l + 1] += f.bt_ptr[l];
     }
-    f.fwd_s.resize(f.fwd_ptr[f.nlevels]); f.fwd_r0.resize(f.fwd_ptr[f.nlevels]);
-    f.bwd_s.resize(f.bwd_ptr[f.nlevels]); f.bwd_q0.resize(f.bwd_ptr[f.nlevels]);
-    std::vector<int> ff(f.fwd_ptr.begin(), f.fwd_ptr.end() - 1), bf(f.bwd_ptr.begin(), f.bwd_ptr.end() - 1);
+    f.ft.assign(4 * (size_t)f.ft_ptr[f.nlevels], 0);
+    f.bt.assign(4 * (size_t)f.bt_ptr[f.nlevels], 0);
+    std::vector<int> ff(f.ft_ptr.begin(), f.ft_ptr.end() - 1), bf(f.bt_ptr.begin(), f.bt_ptr.end() - 1);
     for (int t = 0; t < nn; ++t) {
-        int l = f.height[t];
-        int nfw = std::max(1, (f.noff[t] + kFwdRows - 1) / kFwdRows);
-        for (int k = 0; k < nfw; ++k) { f.fwd_s[ff[l]] = t; f.fwd_r0[ff[l]] = k * kFwdRows; ff[l]++; }
-        for (int q = 0; q < f.w[t]; q += kBwdCols) { f.bwd_s[bf[l]] = t; f.bwd_q0[bf[l]] = q; bf[l]++; }
+        int l = f.height[t], m = f.w[t] + f.noff[t];
+        int rbs = std::max(1, cdiv(f.noff[t], kTile));
+        for (int rb = 0; rb < rbs; ++rb)
+            for (int cb = 0; cb < f.kc[t]; ++cb) {
+                int* tt = &f.ft[4 * (size_t)ff[l]++];
+                tt[0] = t; tt[1] = rb * kTile; tt[2] = cb * kTile;
+            }
+        for (int cb = 0; cb < f.kc[t]; ++cb)
+            for (int pb = 0; pb < cdiv(m, kTile); ++pb) {
+                int* tt = &f.bt[4 * (size_t)bf[l]++];
+                tt[0] = t; tt[1] = cb * kTile; tt[2] = pb * kTile;
+            }
     }
 }
 
@@ -593,14 +639,14 @@ void solve_host(const Factor& f, const double* b, double* x, int nrhs) {
             for (int k = 0; k < w; ++k) v[k] = br[c0 + k];
             for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
                 int c = f.child[q];
-                for (int k = 0; k < f.noff[c]; ++k) v[f.rel[f.roff[c] + k]] += u[f.uoff[c] + k];
+                for (int k = 0; k < f.noff[c]; ++k) v[f.rel[f.roff[c] + k]] += u[f.roff[c] + k];
             }
             const double* P = f.panel.data() + f.poff[s];
             for (int k = 0; k < w; ++k) z[c0 + k] = v[k];
             for (int rr = 0; rr < no; ++rr) {
                 double acc = v[w + rr];
                 for (int q = 0; q < w; ++q) acc -= P[(size_t)q * m + w + rr] * v[q];
-                u[f.uoff[s] + rr] = acc;
+                u[f.roff[s] + rr] = acc;
             }
         }
         for (int s = f.ns - 1; s >= 0; --s) {
@@ -617,6 +663,141 @@ void solve_host(const Factor& f, const double* b, double* x, int nrhs) {
     }
 }
 
+// Emulates the device sweeps tile by tile (same tile lists, bucket pointers, partial
+// vectors and locators as k_fwd / k_bwd / k_apply), checking every index against the
+// buffer it reads or writes. Returns false on an out-of-range access.
+bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& err) {
+    const int n = f.n;
+    const int64_t nu = f.uoff.empty() ? 0 : f.uoff.back(), nx = f.xoff.empty() ? 0 : f.xoff.back();
+    std::vector<double> z(3 * (size_t)n, 0.0), up(3 * (size_t)nu, 0.0), xp(3 * (size_t)nx, 0.0);
+    std::vector<double> pc(f.pc_off.back()), pr(f.pr_off.back());
+    for (int s = 0; s < f.ns; ++s) {
+        int w = f.w[s], no = f.noff[s], mm = w + no;
+        const double* P = f.panel.data() + f.poff[s];
+        for (int q = 0; q < w; ++q) {
+            for (int r = 0; r < no; ++r) pc[f.pc_off[s] + (size_t)q * no + r] = P[(size_t)q * mm + w + r];
+            for (int p = 0; p < mm; ++p) pr[f.pr_off[s] + (size_t)p * w + q] = P[(size_t)q * mm + p];
+        }
+    }
+#define MD_CHECK(cond, what)                       \
+    if (!(cond)) {                                 \
+        err = std::string("tile emulation: ") + what; \
+        return false;                              \
+    }
+    auto sum_part = [&](const std::vector<double>& buf, int64_t base, int cnt, int stride, int64_t e, int c,
+                        double& out) -> bool {
+        out = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            int64_t idx = base + (int64_t)k * stride + e;
+            if (idx < 0 || 3 * idx + c >= (int64_t)buf.size()) return false;
+            out += buf[3 * idx + c];
+        }
+        return true;
+    };
+    for (int l = 0; l < f.nlevels; ++l) {
+        for (int t = f.ft_ptr[l]; t < f.ft_ptr[l + 1]; ++t) {
+            int s = f.ft[4 * (size_t)t], r0 = f.ft[4 * (size_t)t + 1], q0 = f.ft[4 * (size_t)t + 2];
+            MD_CHECK(s >= 0 && s < f.ns && f.height[s] == l, "forward tile supernode");
+            int w = f.w[s], no = f.noff[s], c0 = f.c0[s];
+            int qn = std::min(kTile, w - q0), rn = std::max(0, std::min(kTile, no - r0)), kcb = q0 / kTile;
+            MD_CHECK(qn > 0 && q0 % kTile == 0 && r0 % kTile == 0, "forward tile origin");
+            double vd[3][kTile] = {}, vo[3][kTile] = {};
+            for (int i = 0; i < qn; ++i)
+                for (int c = 0; c < 3; ++c) vd[c][i] = b[(size_t)c * n + c0 + q0 + i];
+            int lo = w + r0, hi = w + r0 + rn;
+            for (int ci = f.child_ptr[s]; ci < f.child_ptr[s + 1]; ++ci) {
+                int ch = f.child[ci];
+                const int* bp = f.bp.data() + f.bp_off[ch];
+                int nb = (f.w[s] + f.noff[s] + kTile - 1) / kTile;
+                MD_CHECK(kcb + 1 <= nb, "bucket index (columns)");
+                for (int e = bp[kcb]; e < bp[kcb + 1]; ++e) {
+                    MD_CHECK(e >= 0 && e < f.noff[ch], "child entry");
+                    int pos = f.rel[f.roff[ch] + e];
+                    MD_CHECK(pos >= q0 && pos < q0 + kTile, "child position outside column block");
+                    if (pos >= q0 + qn) continue;
+                    for (int c = 0; c < 3; ++c) {
+                        double v;
+                        MD_CHECK(sum_part(up, f.uoff[ch], f.kc[ch], f.noff[ch], e, c, v), "update partial");
+                        vd[c][pos - q0] += v;
+                    }
+                }
+                if (kcb == 0 && rn > 0) {
+                    MD_CHECK((hi - 1) / kTile + 1 <= nb, "bucket index (rows)");
+                    for (int e = bp[lo / kTile]; e < bp[(hi - 1) / kTile + 1]; ++e) {
+                        int pos = f.rel[f.roff[ch] + e];
+                        if (pos < lo || pos >= hi) continue;
+                        for (int c = 0; c < 3; ++c) {
+                            double v;
+                            MD_CHECK(sum_part(up, f.uoff[ch], f.kc[ch], f.noff[ch], e, c, v), "update partial");
+                            vo[c][pos - lo] += v;
+                        }
+                    }
+                }
+            }
+            if (r0 == 0)
+                for (int i = 0; i < qn; ++i)
+                    for (int c = 0; c < 3; ++c) z[3 * (size_t)(c0 + q0 + i) + c] = vd[c][i];
+            for (int lane = 0; lane < rn; ++lane) {
+                double acc[3] = {0, 0, 0};
+                for (int i = 0; i < qn; ++i) {
+                    int64_t idx = f.pc_off[s] + (int64_t)(q0 + i) * no + r0 + lane;
+                    MD_CHECK(idx < f.pc_off[s + 1], "forward panel");
+                    for (int c = 0; c < 3; ++c) acc[c] += pc[idx] * vd[c][i];
+                }
+                int64_t o = f.uoff[s] + (int64_t)kcb * no + r0 + lane;
+                MD_CHECK(o < f.uoff[s + 1], "update output");
+                for (int c = 0; c < 3; ++c) up[3 * o + c] = (kcb == 0 ? vo[c][lane] : 0.0) - acc[c];
+            }
+        }
+    }
+    for (int l = f.nlevels - 1; l >= 0; --l) {
+        for (int t = f.bt_ptr[l]; t < f.bt_ptr[l + 1]; ++t) {
+            int s = f.bt[4 * (size_t)t], q0 = f.bt[4 * (size_t)t + 1], p0 = f.bt[4 * (size_t)t + 2];
+            MD_CHECK(s >= 0 && s < f.ns && f.height[s] == l, "backward tile supernode");
+            int w = f.w[s], mm = w + f.noff[s];
+            int qn = std::min(kTile, w - q0), pn = std::min(kTile, mm - p0);
+            MD_CHECK(qn > 0 && pn > 0, "backward tile extent");
+            double tv[3][kTile] = {};
+            for (int lane = 0; lane < pn; ++lane) {
+                int p = p0 + lane;
+                for (int c = 0; c < 3; ++c) {
+                    if (p < w) {
+                        tv[c][lane] = z[3 * (size_t)(f.c0[s] + p) + c];
+                    } else {
+                        size_t e = (size_t)f.roff[s] + p - w;
+                        MD_CHECK(e < f.offrows.size(), "off row entry");
+                        const int* xl = f.xloc.data() + 4 * e;
+                        double v;
+                        MD_CHECK(sum_part(xp, xl[0], xl[2], xl[1], 0, c, v), "solution partial");
+                        tv[c][lane] = -v;
+                    }
+                }
+            }
+            for (int lane = 0; lane < qn; ++lane) {
+                double acc[3] = {0, 0, 0};
+                for (int i = 0; i < pn; ++i) {
+                    int64_t idx = f.pr_off[s] + (int64_t)(p0 + i) * w + q0 + lane;
+                    MD_CHECK(idx < f.pr_off[s + 1], "backward panel");
+                    for (int c = 0; c < 3; ++c) acc[c] += pr[idx] * tv[c][i];
+                }
+                int64_t o = f.xoff[s] + (int64_t)(p0 / kTile) * w + q0 + lane;
+                MD_CHECK(o < f.xoff[s + 1], "solution output");
+                for (int c = 0; c < 3; ++c) xp[3 * o + c] = acc[c];
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        const int* ri = f.rowinfo.data() + 4 * (size_t)i;
+        for (int c = 0; c < 3; ++c) {
+            double v;
+            MD_CHECK(sum_part(xp, ri[0], ri[2], ri[1], 0, c, v), "final partial");
+            x[(size_t)c * n + i] = v;
+        }
+    }
+#undef MD_CHECK
+    return true;
+}
+
 AnalysisStats stats(const Topology& t, const Factor& f) {
     AnalysisStats s{};
     s.n = t.n;
@@ -630,8 +811,8 @@ AnalysisStats stats(const Topology& t, const Factor& f) {
     for (int k = 0; k < f.ns; ++k)
         strict += (int64_t)f.w[k] * (f.w[k] - 1) / 2 + (int64_t)f.w[k] * f.noff[k];
     s.strict_nnz_equiv = strict;
-    s.fwd_items = f.fwd_ptr.empty() ? 0 : f.fwd_ptr.back();
-    s.bwd_items = f.bwd_ptr.empty() ? 0 : f.bwd_ptr.back();
+    s.fwd_items = f.ft_ptr.empty() ? 0 : f.ft_ptr.back();
+    s.bwd_items = f.bt_ptr.empty() ? 0 : f.bt_ptr.back();
     s.max_width = 0;
     s.max_rows = 0;
     for (int k = 0; k < f.ns; ++k) {

@@ -15,9 +15,8 @@ namespace md {
 
 // Maximum columns of one supernode (wider separators are split into chains).
 constexpr int kMaxSnWidth = 1024;
-// Rows of one forward work item and columns of one backward work item.
-constexpr int kFwdRows = 64;
-constexpr int kBwdCols = 16;
+// Edge of the square panel tile one wavefront streams in the triangular sweeps.
+constexpr int kTile = 64;
 // Vertices per one-ring slice (one wavefront).
 constexpr int kSlice = 64;
 
@@ -49,6 +48,14 @@ struct Topology {
 //   rows [w, m):  G_s = L_os L_ss^{-1} (the block of Lb below D_s).
 // Forward:  v = b[c0..] + children updates; z_s = v_d; u_s = v_o - G_s v_d.
 // Backward: x_s = Dinv_s z_s - G_s^T x[offrows].
+//
+// On the device both sweeps are cut into 64 x 64 tiles, one wavefront each:
+//   forward tile (s, r0, q0): rows [r0, r0+64) of G_s times columns [q0, q0+64) of v_d,
+//     read from a column-major copy of G_s; it writes partial kc = q0/64 of u_s, so u_s is
+//     stored as kc_s = ceil(w/64) partial vectors that the parent sums in order;
+//   backward tile (s, q0, p0): rows [p0, p0+64) of [Dinv_s; G_s]^T [z_s; -x_off], read
+//     from a row-major copy; it writes partial kb = p0/64 of x_s, so x_s is stored as
+//     kr_s = ceil(m/64) partials that every reader (descendants, the final update) sums.
 struct Factor {
     int n = 0;                       // dimension (free vertices)
     int ns = 0;                      // supernodes
@@ -60,13 +67,21 @@ struct Factor {
     std::vector<int> offrows;        // ascending, solve numbering
     std::vector<int> rel;            // per off row of s: position in parent's rows [0, m_p)
     std::vector<int> child_ptr, child;
-    std::vector<int64_t> poff;       // ns + 1: panel offsets
-    std::vector<int> uoff;           // ns + 1: update-vector offsets (= roff)
+    std::vector<int64_t> poff;       // ns + 1: FP64 panel offsets (m x w, column-major)
     std::vector<double> panel;       // FP64 panels
-    // work items, grouped by height level: fwd item = (s, first off row), bwd item =
-    // (s, first column); level l occupies [fwd_ptr[l], fwd_ptr[l+1]).
-    std::vector<int> fwd_ptr, fwd_s, fwd_r0;
-    std::vector<int> bwd_ptr, bwd_s, bwd_q0;
+    // device layout
+    std::vector<int> kc, kr;         // partial counts of u_s and x_s
+    std::vector<int64_t> pc_off;     // ns + 1: column-major G_s (noff x w)
+    std::vector<int64_t> pr_off;     // ns + 1: row-major [Dinv_s; G_s] (m x w)
+    std::vector<int64_t> uoff;       // ns + 1: kc * noff partial update entries
+    std::vector<int64_t> xoff;       // ns + 1: kr * w partial solution entries
+    std::vector<int> bp_off;         // per supernode: its bucket pointers (as a child)
+    std::vector<int> bp;             // bp[bp_off[c] + b] = first off row e of c with rel >= 64 b
+    std::vector<int> rowinfo;        // 4 per row: first partial, stride, count, 0
+    std::vector<int> xloc;           // 4 per off row entry: rowinfo of that row
+    // tiles grouped by height level: level l occupies [ft_ptr[l], ft_ptr[l+1]) (4 ints each)
+    std::vector<int> ft_ptr, ft;     // forward (s, r0, q0, 0)
+    std::vector<int> bt_ptr, bt;     // backward (s, q0, p0, 0)
     int64_t factor_nnz() const { return poff.empty() ? 0 : poff.back(); }
 };
 
@@ -103,6 +118,10 @@ bool numeric(const std::vector<int>& adj_ptr, const std::vector<int>& adj,
 
 // Host reference of the GPU solve on the same panels (used by host tests).
 void solve_host(const Factor& f, const double* b, double* x, int nrhs);
+
+// Tile-by-tile emulation of the device sweeps on 3 right-hand sides (b and x are 3 x n,
+// coordinate-major), with every index checked against its buffer.
+bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& err);
 
 AnalysisStats stats(const Topology& t, const Factor& f);
 

@@ -13,6 +13,14 @@ template <typename T> struct V4T;
 template <> struct V4T<float> { using type = f4; };
 template <> struct V4T<double> { using type = d4; };
 
+// Per-supernode record of the block LDL^T sweeps (see Factor in md_analysis.h).
+struct SnDev {
+    int c0, w, noff, roff;     // columns [c0, c0+w), off rows offrows[roff .. roff+noff)
+    int kc, kr, bp_off, pad;   // partial counts of u_s / x_s, bucket pointers as a child
+    long long pc_off, pr_off;  // column-major G_s, row-major [Dinv_s; G_s]
+    long long uoff, xoff;      // partial update / partial solution entries
+};
+
 // Device view of one mesh. All pointers are device pointers into the mesh's arena.
 // Internal vertex ids: [0, nf) free vertices in solve order, [nf, n) constrained ones.
 struct MeshDev {
@@ -32,14 +40,17 @@ struct MeshDev {
     d4 *p;                      // rest positions (internal order)
     d4 *pp;                     // current positions p'
     void *quat;                 // rotations as quaternions (x, y, z, w=scalar): f4 or d4
-    void *r, *z, *x, *u;        // solve vectors (V4 of the solve type): nf, nf, nf, noff total
-    const int *sn_c0, *sn_w, *sn_noff, *sn_roff;
-    const long long *sn_poff;
+    void *r, *z;                // residual and forward result (V4 of the solve type), nf each
+    void *upart, *xpart;        // partial update / solution vectors (V4)
+    const SnDev *sn;
     const int *child_ptr, *child;
-    const int *offrows, *rel;
-    const void *panel;          // float or double panels
-    const int *fwd_ptr, *fwd_s, *fwd_r0;
-    const int *bwd_ptr, *bwd_s, *bwd_q0;
+    const int *rel;             // per off row entry: position in the parent's rows
+    const int *bp;              // bucket pointers into rel
+    const int4 *rowinfo;        // per row: (first partial, stride, count, 0)
+    const int4 *xloc;           // per off row entry: rowinfo of that row
+    const void *pc, *pr;        // panels (float or double), forward and backward layouts
+    const int *ft_ptr, *bt_ptr; // tiles per level
+    const int4 *ft, *bt;        // (s, r0, q0, 0) / (s, q0, p0, 0)
     double *energy_part;        // one partial sum per 256 vertices
 };
 
@@ -58,6 +69,7 @@ void launch_local(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_residual(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
+void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 

@@ -265,153 +265,165 @@ __global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
 
 // ---- block LDL^T sweeps (arap.h:416-429) --------------------------------------------------
 
-template <typename T> __device__ __forceinline__ const T *panels(const MeshDev &M) {
-    return reinterpret_cast<const T *>(M.panel);
+template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
+    return reinterpret_cast<const T *>(p);
 }
 
-// Forward sweep, one work item = (supernode s, 64 off-diagonal rows from r0):
-//   v = r[c0 .. c0+w) + children's update vectors (extend-add through rel, child order),
-//   z_s = v_d (first item only),  u_s[r0 .. r0+64) = v_o - G_s v_d.
-// The 4 wavefronts split the w columns of G_s; partial sums meet in LDS.
+// Sum of the kc partial update vectors of a child at entry e (fixed order).
+template <typename V>
+__device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e) {
+    V a = U[e];
+    for (int k = 1; k < cnt; ++k) {
+        V b = U[(long long)k * stride + e];
+        a.x += b.x; a.y += b.y; a.z += b.z;
+    }
+    return a;
+}
+
+// Forward tile (s, r0, q0), one wavefront:
+//   v_d[q0 .. q0+64) = r[c0 + q] + children's updates landing on those columns,
+//   v_o[r0 .. r0+64) = children's updates landing on those off rows (first column tile),
+//   z_s[q0 ..] = v_d (first row tile),
+//   partial q0/64 of u_s on rows [r0, r0+64) = (q0 == 0 ? v_o : 0) - G_s[rows, cols] v_d.
+// Children's updates arrive as kc partial vectors; bucket pointers give the contiguous range
+// of each child's entries that lands in a 64-position block of this supernode's rows.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fwd(LaunchSet L, int level) {
+__global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     using V = typename V4T<T>::type;
-    __shared__ T vd[3][kMaxSnWidth];
-    __shared__ T vo[3][kFwdRows];
-    __shared__ T part[4][3][kFwdRows];
+    __shared__ T vd[3][kTile];
+    __shared__ T vo[3][kTile];
     int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    int item = M.fwd_ptr[level] + (blockIdx.x - L.prefix[k]);
-    int s = M.fwd_s[item], r0 = M.fwd_r0[item];
-    int c0 = M.sn_c0[s], w = M.sn_w[s], no = M.sn_noff[s];
-    int m = w + no;
-    int nrows = min(kFwdRows, no - r0);
+    const int4 tl = M.ft[M.ft_ptr[level] + (blockIdx.x - L.prefix[k])];
+    const SnDev S = M.sn[tl.x];
+    const int r0 = tl.y, q0 = tl.z, lane = threadIdx.x;
+    const int w = S.w, no = S.noff;
+    const int qn = min(kTile, w - q0), rn = max(0, min(kTile, no - r0));
+    const int kcb = q0 / kTile;
     const V *R = reinterpret_cast<const V *>(M.r);
-    const V *U = reinterpret_cast<const V *>(M.u);
-    int tid = threadIdx.x;
-    for (int p = tid; p < w; p += kBlock) {
-        V b = R[c0 + p];
-        vd[0][p] = b.x; vd[1][p] = b.y; vd[2][p] = b.z;
+    const V *U = reinterpret_cast<const V *>(M.upart);
+    {
+        V b;
+        if (lane < qn) b = R[S.c0 + q0 + lane]; else { b.x = 0; b.y = 0; b.z = 0; }
+        vd[0][lane] = b.x; vd[1][lane] = b.y; vd[2][lane] = b.z;
+        vo[0][lane] = 0; vo[1][lane] = 0; vo[2][lane] = 0;
     }
-    if (tid < kFwdRows) { vo[0][tid] = 0; vo[1][tid] = 0; vo[2][tid] = 0; }
     __syncthreads();
-    for (int ci = M.child_ptr[s]; ci < M.child_ptr[s + 1]; ++ci) {
-        int c = M.child[ci];
-        int cn = M.sn_noff[c], cr = M.sn_roff[c];
-        for (int e = tid; e < cn; e += kBlock) {
-            int pos = M.rel[cr + e];
-            V uv = U[cr + e];
-            if (pos < w) {
-                vd[0][pos] += uv.x; vd[1][pos] += uv.y; vd[2][pos] += uv.z;
-            } else {
-                int t = pos - w - r0;
-                if (t >= 0 && t < nrows) { vo[0][t] += uv.x; vo[1][t] += uv.y; vo[2][t] += uv.z; }
-            }
+    const int lo = w + r0, hi = w + r0 + rn;
+    for (int ci = M.child_ptr[tl.x]; ci < M.child_ptr[tl.x + 1]; ++ci) {
+        const SnDev C = M.sn[M.child[ci]];
+        const int *rel = M.rel + C.roff;
+        const int *bp = M.bp + C.bp_off;
+        const V *Uc = U + C.uoff;
+        int e0 = bp[kcb], e1 = bp[kcb + 1];
+        for (int e = e0 + lane; e < e1; e += kTile) {
+            int pos = rel[e];
+            if (pos >= q0 + qn) continue;  // the block's tail past column w holds off rows
+            V u = sum_partials<V>(Uc, C.kc, C.noff, e);
+            vd[0][pos - q0] += u.x; vd[1][pos - q0] += u.y; vd[2][pos - q0] += u.z;
         }
-        __syncthreads();
-    }
-    if (r0 == 0) {
-        V *Z = reinterpret_cast<V *>(M.z);
-        for (int p = tid; p < w; p += kBlock) {
-            V zz;
-            zz.x = vd[0][p]; zz.y = vd[1][p]; zz.z = vd[2][p]; zz.w = 0;
-            Z[c0 + p] = zz;
-        }
-    }
-    if (nrows <= 0) return;
-    int wave = tid >> 6, lane = tid & 63;
-    const T *P = panels<T>(M) + M.sn_poff[s] + w + r0;
-    int qb = (w * wave) / 4, qe = (w * (wave + 1)) / 4;
-    T ax = 0, ay = 0, az = 0;
-    if (lane < nrows) {
-        for (int q = qb; q < qe; ++q) {
-            T g = P[(long long)q * m + lane];
-            ax += g * vd[0][q]; ay += g * vd[1][q]; az += g * vd[2][q];
-        }
-    }
-    part[wave][0][lane] = ax; part[wave][1][lane] = ay; part[wave][2][lane] = az;
-    __syncthreads();
-    if (wave == 0 && lane < nrows) {
-        V out;
-        out.x = vo[0][lane] - (part[0][0][lane] + part[1][0][lane] + part[2][0][lane] + part[3][0][lane]);
-        out.y = vo[1][lane] - (part[0][1][lane] + part[1][1][lane] + part[2][1][lane] + part[3][1][lane]);
-        out.z = vo[2][lane] - (part[0][2][lane] + part[1][2][lane] + part[2][2][lane] + part[3][2][lane]);
-        out.w = 0;
-        reinterpret_cast<V *>(M.u)[M.sn_roff[s] + r0 + lane] = out;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Backward sweep, one work item = (supernode s, 16 columns from q0):
-//   x_s[q] = sum_p Dinv_s[p, q] z_s[p] - sum_r G_s[r, q] x[offrow_r]
-// (the panel's column q dotted with [z_s; -x_off]); the correction is added to p'.
-// Each wavefront owns 4 columns; lanes stride the rows; rows are staged through LDS.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bwd(LaunchSet L, int level) {
-    using V = typename V4T<T>::type;
-    constexpr int kStage = 1024;
-    __shared__ T tv[3][kStage];
-    int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
-    int item = M.bwd_ptr[level] + (blockIdx.x - L.prefix[k]);
-    int s = M.bwd_s[item], q0 = M.bwd_q0[item];
-    int c0 = M.sn_c0[s], w = M.sn_w[s], no = M.sn_noff[s], roff = M.sn_roff[s];
-    int m = w + no;
-    int ncols = min(kBwdCols, w - q0);
-    const V *Z = reinterpret_cast<const V *>(M.z);
-    V *X = reinterpret_cast<V *>(M.x);
-    const T *P = panels<T>(M) + M.sn_poff[s];
-    int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    T acc[4][3];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) { acc[i][0] = 0; acc[i][1] = 0; acc[i][2] = 0; }
-    for (int pb = 0; pb < m; pb += kStage) {
-        int cnt = min(kStage, m - pb);
-        for (int p = tid; p < cnt; p += kBlock) {
-            int gp = pb + p;
-            V t;
-            if (gp < w) {
-                t = Z[c0 + gp];
-            } else {
-                t = X[M.offrows[roff + gp - w]];
-                t.x = -t.x; t.y = -t.y; t.z = -t.z;
-            }
-            tv[0][p] = t.x; tv[1][p] = t.y; tv[2][p] = t.z;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            int qc = wave + 4 * i;
-            if (qc < ncols) {
-                const T *col = P + (long long)(q0 + qc) * m + pb;
-                for (int p = lane; p < cnt; p += 64) {
-                    T g = col[p];
-                    acc[i][0] += g * tv[0][p]; acc[i][1] += g * tv[1][p]; acc[i][2] += g * tv[2][p];
+        if (kcb == 0 && rn > 0) {
+            e0 = bp[lo / kTile];
+            e1 = bp[(hi - 1) / kTile + 1];
+            for (int e = e0 + lane; e < e1; e += kTile) {
+                int pos = rel[e];
+                if (pos >= lo && pos < hi) {
+                    V u = sum_partials<V>(Uc, C.kc, C.noff, e);
+                    vo[0][pos - lo] += u.x; vo[1][pos - lo] += u.y; vo[2][pos - lo] += u.z;
                 }
             }
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        int qc = wave + 4 * i;
-        T sx = wave_sum<T>(acc[i][0]), sy = wave_sum<T>(acc[i][1]), sz = wave_sum<T>(acc[i][2]);
-        if (qc < ncols && lane == 0) {
-            int col = c0 + q0 + qc;
-            V out;
-            out.x = sx; out.y = sy; out.z = sz; out.w = 0;
-            X[col] = out;
-            d4 pv = M.pp[col];
-            pv.x += (double)sx; pv.y += (double)sy; pv.z += (double)sz;
-            M.pp[col] = pv;
-        }
+    if (r0 == 0 && lane < qn) {
+        V zz;
+        zz.x = vd[0][lane]; zz.y = vd[1][lane]; zz.z = vd[2][lane]; zz.w = 0;
+        reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = zz;
     }
+    if (rn == 0) return;
+    // column-major G_s: entry (row r, column q) at q * no + r
+    const T *G = tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1);
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int i = 0; i < kTile; ++i) {
+        int ii = min(i, qn - 1);
+        T g = G[(long long)ii * no];
+        T m = i < qn ? T(1) : T(0);
+        g *= m;
+        ax += g * vd[0][ii]; ay += g * vd[1][ii]; az += g * vd[2][ii];
+    }
+    if (lane < rn) {
+        V out;
+        out.x = (kcb == 0 ? vo[0][lane] : T(0)) - ax;
+        out.y = (kcb == 0 ? vo[1][lane] : T(0)) - ay;
+        out.z = (kcb == 0 ? vo[2][lane] : T(0)) - az;
+        out.w = 0;
+        reinterpret_cast<V *>(M.upart)[S.uoff + (long long)kcb * no + r0 + lane] = out;
+    }
+}
+
+// Backward tile (s, q0, p0), one wavefront: partial p0/64 of x_s on columns [q0, q0+64):
+//   sum over rows p in [p0, p0+64) of [Dinv_s; G_s][p, q] * t[p],  t = [z_s; -x_off],
+// read from the row-major copy (lanes on consecutive columns). x at off rows is the sum of
+// the owner's partials (xloc gives where they are).
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
+    using V = typename V4T<T>::type;
+    __shared__ T tv[3][kTile];
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    const int4 tl = M.bt[M.bt_ptr[level] + (blockIdx.x - L.prefix[k])];
+    const SnDev S = M.sn[tl.x];
+    const int q0 = tl.y, p0 = tl.z, lane = threadIdx.x;
+    const int w = S.w, m = S.w + S.noff;
+    const int qn = min(kTile, w - q0), pn = min(kTile, m - p0);
+    const V *X = reinterpret_cast<const V *>(M.xpart);
+    {
+        V t;
+        t.x = 0; t.y = 0; t.z = 0;
+        int p = p0 + lane;
+        if (lane < pn) {
+            if (p < w) {
+                t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
+            } else {
+                int4 xl = M.xloc[S.roff + p - w];
+                V a = sum_partials<V>(X + xl.x, xl.z, xl.y, 0);
+                t.x = -a.x; t.y = -a.y; t.z = -a.z;
+            }
+        }
+        tv[0][lane] = t.x; tv[1][lane] = t.y; tv[2][lane] = t.z;
+    }
+    __syncthreads();
+    const T *P = tptr<T>(M.pr) + S.pr_off + (long long)p0 * w + q0 + min(lane, qn - 1);
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int i = 0; i < kTile; ++i) {
+        int ii = min(i, pn - 1);
+        T g = P[(long long)ii * w];
+        T m1 = i < pn ? T(1) : T(0);
+        g *= m1;
+        ax += g * tv[0][ii]; ay += g * tv[1][ii]; az += g * tv[2][ii];
+    }
+    if (lane < qn) {
+        V out;
+        out.x = ax; out.y = ay; out.z = az; out.w = 0;
+        reinterpret_cast<V *>(M.xpart)[S.xoff + (long long)(p0 / kTile) * w + q0 + lane] = out;
+    }
+}
+
+// p'_i += x_i for every free vertex, x_i the sum of its supernode's partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int i = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
+    if (i >= M.nf) return;
+    int4 ri = M.rowinfo[i];
+    V a = sum_partials<V>(reinterpret_cast<const V *>(M.xpart) + ri.x, ri.z, ri.y, 0);
+    d4 pv = M.pp[i];
+    pv.x += (double)a.x; pv.y += (double)a.y; pv.z += (double)a.z;
+    M.pp[i] = pv;
 }
 
 // ---- ARAP energy -------------------------------------------------------------------------
@@ -481,13 +493,18 @@ void launch_residual(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
 }
 void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
     if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_fwd<double>, dim3(blocks), dim3(kBlock), 0, s, L, level);
-    else hipLaunchKernelGGL(k_fwd<float>, dim3(blocks), dim3(kBlock), 0, s, L, level);
+    if (fp64) hipLaunchKernelGGL(k_fwd<double>, dim3(blocks), dim3(kTile), 0, s, L, level);
+    else hipLaunchKernelGGL(k_fwd<float>, dim3(blocks), dim3(kTile), 0, s, L, level);
 }
 void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
     if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_bwd<double>, dim3(blocks), dim3(kBlock), 0, s, L, level);
-    else hipLaunchKernelGGL(k_bwd<float>, dim3(blocks), dim3(kBlock), 0, s, L, level);
+    if (fp64) hipLaunchKernelGGL(k_bwd<double>, dim3(blocks), dim3(kTile), 0, s, L, level);
+    else hipLaunchKernelGGL(k_bwd<float>, dim3(blocks), dim3(kTile), 0, s, L, level);
+}
+void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
+    if (blocks <= 0) return;
+    if (fp64) hipLaunchKernelGGL(k_apply<double>, dim3(blocks), dim3(kBlock), 0, s, L);
+    else hipLaunchKernelGGL(k_apply<float>, dim3(blocks), dim3(kBlock), 0, s, L);
 }
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
     if (blocks <= 0) return;

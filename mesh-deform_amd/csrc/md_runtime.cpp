@@ -57,7 +57,7 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     plan.nlevels = 0;
     for (const Factor *f : factors) plan.nlevels = std::max(plan.nlevels, f->nlevels);
     int nl = plan.nlevels;
-    int ntables = 4 + 2 * nl;
+    int ntables = 5 + 2 * nl;
     plan.h_prefix.assign((size_t)ntables * (na + 1), 0);
     plan.off_fwd.assign(nl, 0);
     plan.off_bwd.assign(nl, 0);
@@ -76,14 +76,15 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     table(2, plan.off_resid, plan.blk_resid,
           [&](int k) { return cdiv(meshes[k]->nslices_free, kSlicesPerBlock); });
     table(3, plan.off_ident, plan.blk_ident, [&](int k) { return cdiv(meshes[k]->n, kBlock); });
+    table(4, plan.off_apply, plan.blk_apply, [&](int k) { return cdiv(meshes[k]->nf, kBlock); });
     for (int l = 0; l < nl; ++l) {
-        table(4 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
+        table(5 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
             const Factor *f = factors[k];
-            return l < f->nlevels ? f->fwd_ptr[l + 1] - f->fwd_ptr[l] : 0;
+            return l < f->nlevels ? f->ft_ptr[l + 1] - f->ft_ptr[l] : 0;
         });
-        table(5 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
+        table(6 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
             const Factor *f = factors[k];
-            return l < f->nlevels ? f->bwd_ptr[l + 1] - f->bwd_ptr[l] : 0;
+            return l < f->nlevels ? f->bt_ptr[l + 1] - f->bt_ptr[l] : 0;
         });
     }
     plan.h_meshes.resize(na);
@@ -120,6 +121,7 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
         if (ev && r == 0) hipEventRecord(ev[3], stream);
         for (int l = plan.nlevels - 1; l >= 0; --l)
             launch_bwd(stream, plan.set(plan.off_bwd[l]), plan.blk_bwd[l], l, plan.fp64);
+        launch_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[4], stream);
     }
 }
@@ -342,27 +344,24 @@ int analyze(md_mesh *m) {
     size_t o_q = A.reserve(vsz * n);
     size_t o_r = A.reserve(vsz * std::max(1, nf));
     size_t o_z = A.reserve(vsz * std::max(1, nf));
-    size_t o_x = A.reserve(vsz * std::max(1, nf));
-    size_t nrows_off = f.roff.empty() ? 0 : (size_t)f.roff.back();
-    size_t o_u = A.reserve(vsz * std::max<size_t>(1, nrows_off));
     int ns = f.ns;
-    size_t o_c0 = A.reserve(sizeof(int) * std::max(1, ns));
-    size_t o_w = A.reserve(sizeof(int) * std::max(1, ns));
-    size_t o_no = A.reserve(sizeof(int) * std::max(1, ns));
-    size_t o_ro = A.reserve(sizeof(int) * (ns + 1));
-    size_t o_po = A.reserve(sizeof(long long) * (ns + 1));
+    auto last = [](const std::vector<int64_t> &v) { return v.empty() ? (int64_t)0 : v.back(); };
+    size_t o_up = A.reserve(vsz * std::max<int64_t>(1, last(f.uoff)));
+    size_t o_xp = A.reserve(vsz * std::max<int64_t>(1, last(f.xoff)));
+    size_t o_sn = A.reserve(sizeof(SnDev) * std::max(1, ns));
     size_t o_cp = A.reserve(sizeof(int) * (ns + 1));
     size_t o_ch = A.reserve(sizeof(int) * std::max<size_t>(1, f.child.size()));
-    size_t o_or = A.reserve(sizeof(int) * std::max<size_t>(1, f.offrows.size()));
     size_t o_rel = A.reserve(sizeof(int) * std::max<size_t>(1, f.rel.size()));
-    size_t o_pan = A.reserve(psz * std::max<int64_t>(1, f.factor_nnz()));
+    size_t o_bp = A.reserve(sizeof(int) * std::max<size_t>(1, f.bp.size()));
+    size_t o_ri = A.reserve(sizeof(int) * std::max<size_t>(4, f.rowinfo.size()));
+    size_t o_xl = A.reserve(sizeof(int) * std::max<size_t>(4, f.xloc.size()));
+    size_t o_pc = A.reserve(psz * std::max<int64_t>(1, last(f.pc_off)));
+    size_t o_pr = A.reserve(psz * std::max<int64_t>(1, last(f.pr_off)));
     int nl = f.nlevels;
-    size_t o_fp = A.reserve(sizeof(int) * (nl + 1));
-    size_t o_fs = A.reserve(sizeof(int) * std::max<size_t>(1, f.fwd_s.size()));
-    size_t o_fr = A.reserve(sizeof(int) * std::max<size_t>(1, f.fwd_r0.size()));
-    size_t o_bp = A.reserve(sizeof(int) * (nl + 1));
-    size_t o_bs = A.reserve(sizeof(int) * std::max<size_t>(1, f.bwd_s.size()));
-    size_t o_bq = A.reserve(sizeof(int) * std::max<size_t>(1, f.bwd_q0.size()));
+    size_t o_ftp = A.reserve(sizeof(int) * (nl + 1));
+    size_t o_btp = A.reserve(sizeof(int) * (nl + 1));
+    size_t o_ft = A.reserve(sizeof(int) * std::max<size_t>(4, f.ft.size()));
+    size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
     int eblocks = std::max(1, cdiv(t.nslices, kSlicesPerBlock));
     size_t o_en = A.reserve(sizeof(double) * eblocks);
     if (A.allocate() != hipSuccess) {
@@ -387,22 +386,28 @@ int analyze(md_mesh *m) {
     d.p = (d4 *)A.at(o_p);
     d.pp = (d4 *)A.at(o_pp);
     d.quat = A.at(o_q);
-    d.r = A.at(o_r); d.z = A.at(o_z); d.x = A.at(o_x); d.u = A.at(o_u);
-    d.sn_c0 = (const int *)A.at(o_c0); d.sn_w = (const int *)A.at(o_w);
-    d.sn_noff = (const int *)A.at(o_no); d.sn_roff = (const int *)A.at(o_ro);
-    d.sn_poff = (const long long *)A.at(o_po);
+    d.r = A.at(o_r); d.z = A.at(o_z);
+    d.upart = A.at(o_up); d.xpart = A.at(o_xp);
+    d.sn = (const SnDev *)A.at(o_sn);
     d.child_ptr = (const int *)A.at(o_cp); d.child = (const int *)A.at(o_ch);
-    d.offrows = (const int *)A.at(o_or); d.rel = (const int *)A.at(o_rel);
-    d.panel = A.at(o_pan);
-    d.fwd_ptr = (const int *)A.at(o_fp); d.fwd_s = (const int *)A.at(o_fs); d.fwd_r0 = (const int *)A.at(o_fr);
-    d.bwd_ptr = (const int *)A.at(o_bp); d.bwd_s = (const int *)A.at(o_bs); d.bwd_q0 = (const int *)A.at(o_bq);
+    d.rel = (const int *)A.at(o_rel); d.bp = (const int *)A.at(o_bp);
+    d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
+    d.pc = A.at(o_pc); d.pr = A.at(o_pr);
+    d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
+    d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
     d.energy_part = (double *)A.at(o_en);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
         return hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, st);
     };
-    std::vector<long long> poff(f.poff.begin(), f.poff.end());
+    std::vector<SnDev> snh(ns);
+    for (int k = 0; k < ns; ++k) {
+        SnDev &x = snh[k];
+        x.c0 = f.c0[k]; x.w = f.w[k]; x.noff = f.noff[k]; x.roff = f.roff[k];
+        x.kc = f.kc[k]; x.kr = f.kr[k]; x.bp_off = f.bp_off[k]; x.pad = 0;
+        x.pc_off = f.pc_off[k]; x.pr_off = f.pr_off[k]; x.uoff = f.uoff[k]; x.xoff = f.xoff[k];
+    }
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
     chk(up(d.slice_ptr, t.slice_ptr.data(), sizeof(int) * (t.nslices + 1)));
@@ -411,21 +416,17 @@ int analyze(md_mesh *m) {
     chk(up(d.slot_fe_ptr, t.slot_fe_ptr.data(), sizeof(int) * (slots + 1)));
     chk(up(d.slot_fe, t.slot_fe.data(), sizeof(int) * t.slot_fe.size()));
     if (ns > 0) {
-        chk(up(d.sn_c0, f.c0.data(), sizeof(int) * ns));
-        chk(up(d.sn_w, f.w.data(), sizeof(int) * ns));
-        chk(up(d.sn_noff, f.noff.data(), sizeof(int) * ns));
-        chk(up(d.sn_roff, f.roff.data(), sizeof(int) * (ns + 1)));
-        chk(up(d.sn_poff, poff.data(), sizeof(long long) * (ns + 1)));
+        chk(up(d.sn, snh.data(), sizeof(SnDev) * ns));
         chk(up(d.child_ptr, f.child_ptr.data(), sizeof(int) * (ns + 1)));
         chk(up(d.child, f.child.data(), sizeof(int) * f.child.size()));
-        chk(up(d.offrows, f.offrows.data(), sizeof(int) * f.offrows.size()));
         chk(up(d.rel, f.rel.data(), sizeof(int) * f.rel.size()));
-        chk(up(d.fwd_ptr, f.fwd_ptr.data(), sizeof(int) * (nl + 1)));
-        chk(up(d.fwd_s, f.fwd_s.data(), sizeof(int) * f.fwd_s.size()));
-        chk(up(d.fwd_r0, f.fwd_r0.data(), sizeof(int) * f.fwd_r0.size()));
-        chk(up(d.bwd_ptr, f.bwd_ptr.data(), sizeof(int) * (nl + 1)));
-        chk(up(d.bwd_s, f.bwd_s.data(), sizeof(int) * f.bwd_s.size()));
-        chk(up(d.bwd_q0, f.bwd_q0.data(), sizeof(int) * f.bwd_q0.size()));
+        chk(up(d.bp, f.bp.data(), sizeof(int) * f.bp.size()));
+        chk(up(d.rowinfo, f.rowinfo.data(), sizeof(int) * f.rowinfo.size()));
+        chk(up(d.xloc, f.xloc.data(), sizeof(int) * f.xloc.size()));
+        chk(up(d.ft_ptr, f.ft_ptr.data(), sizeof(int) * (nl + 1)));
+        chk(up(d.bt_ptr, f.bt_ptr.data(), sizeof(int) * (nl + 1)));
+        chk(up(d.ft, f.ft.data(), sizeof(int) * f.ft.size()));
+        chk(up(d.bt, f.bt.data(), sizeof(int) * f.bt.size()));
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
@@ -510,15 +511,33 @@ int factorize(md_mesh *m) {
         m->factored = false;
         return MD_ERR_FACTOR;
     }
+    // device layouts: column-major G_s for the forward sweep, row-major [Dinv_s; G_s] for the
+    // backward sweep
+    const Factor &f = m->fac;
+    size_t npc = (size_t)f.pc_off[f.ns], npr = (size_t)f.pr_off[f.ns];
+    std::vector<double> pc(npc), pr(npr);
+    for (int s = 0; s < f.ns; ++s) {
+        int w = f.w[s], no = f.noff[s], mm = w + no;
+        const double *P = f.panel.data() + f.poff[s];
+        double *C = pc.data() + f.pc_off[s];
+        double *Rm = pr.data() + f.pr_off[s];
+        for (int q = 0; q < w; ++q) {
+            for (int r = 0; r < no; ++r) C[(size_t)q * no + r] = P[(size_t)q * mm + w + r];
+            for (int p = 0; p < mm; ++p) Rm[(size_t)p * w + q] = P[(size_t)q * mm + p];
+        }
+    }
     hipError_t e;
-    size_t ne = (size_t)m->fac.factor_nnz();
     if (m->fp64()) {
-        e = hipMemcpy(const_cast<void *>(m->dev.panel), m->fac.panel.data(), sizeof(double) * ne,
-                      hipMemcpyHostToDevice);
+        e = hipMemcpy(const_cast<void *>(m->dev.pc), pc.data(), sizeof(double) * npc, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(const_cast<void *>(m->dev.pr), pr.data(), sizeof(double) * npr, hipMemcpyHostToDevice);
     } else {
-        std::vector<float> p32(ne);
-        for (size_t k = 0; k < ne; ++k) p32[k] = (float)m->fac.panel[k];
-        e = hipMemcpy(const_cast<void *>(m->dev.panel), p32.data(), sizeof(float) * ne, hipMemcpyHostToDevice);
+        std::vector<float> c32(npc), r32(npr);
+        for (size_t k = 0; k < npc; ++k) c32[k] = (float)pc[k];
+        for (size_t k = 0; k < npr; ++k) r32[k] = (float)pr[k];
+        e = hipMemcpy(const_cast<void *>(m->dev.pc), c32.data(), sizeof(float) * npc, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(const_cast<void *>(m->dev.pr), r32.data(), sizeof(float) * npr, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         set_error("panel upload failed");
@@ -813,7 +832,14 @@ extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, co
     std::vector<double> bp((size_t)n * nrhs), xp((size_t)n * nrhs);
     for (int r = 0; r < nrhs; ++r)
         for (int k = 0; k < n; ++k) bp[(size_t)r * n + k] = b[(size_t)r * n + perm[k]];
-    solve_host(f, bp.data(), xp.data(), nrhs);
+    // three right-hand sides at a time through the tile emulation of the device sweeps
+    for (int r0 = 0; r0 < nrhs; r0 += 3) {
+        std::vector<double> b3(3 * (size_t)n, 0.0), x3(3 * (size_t)n, 0.0);
+        int cnt = std::min(3, nrhs - r0);
+        std::copy(bp.begin() + (size_t)r0 * n, bp.begin() + (size_t)(r0 + cnt) * n, b3.begin());
+        if (!solve_tiles_host(f, b3.data(), x3.data(), err)) { set_error(err); return MD_ERR_INVALID; }
+        std::copy(x3.begin(), x3.begin() + (size_t)cnt * n, xp.begin() + (size_t)r0 * n);
+    }
     for (int r = 0; r < nrhs; ++r)
         for (int k = 0; k < n; ++k) x[(size_t)r * n + perm[k]] = xp[(size_t)r * n + k];
     if (panel_entries) *panel_entries = f.factor_nnz();

@@ -46,9 +46,9 @@ private:
 struct Plan {
     int nactive = 0;
     int nlevels = 0;
-    int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0;
+    int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0, blk_apply = 0;
     std::vector<int> blk_fwd, blk_bwd;
-    int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0;
+    int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0, off_apply = 0;
     std::vector<int> off_fwd, off_bwd;
     MeshDev *d_meshes = nullptr;
     int *d_prefix = nullptr;
