@@ -74,6 +74,9 @@ typedef struct {
     int64_t fwd_items, bwd_items;
     int64_t device_bytes;    /* device memory held by the mesh */
     double setup_ms_host;    /* last analysis + factorization time on the host */
+    /* algorithmic bytes of one ARAP iteration with one refinement step, per phase: every
+     * array element the phase must read or write counted once */
+    int64_t bytes_local, bytes_residual, bytes_forward, bytes_backward;
 } md_mesh_stats;
 
 /* ---- library ------------------------------------------------------------------------ */

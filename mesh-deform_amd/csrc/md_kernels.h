@@ -72,6 +72,7 @@ void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k);
 
 // Threads per block and work per block of each kernel (used for the prefix tables).
 constexpr int kBlock = 256;

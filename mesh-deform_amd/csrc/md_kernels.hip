@@ -468,7 +468,20 @@ __global__ __launch_bounds__(kBlock) void k_energy(LaunchSet L) {
     if (threadIdx.x == 0) M.energy_part[lb] = red[0];
 }
 
+// p'[iv[k]] = xyz[k] for k targets (internal ids), used to move handles without a re-setup.
+__global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *iv, const double *xyz, int k) {
+    int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= k) return;
+    d4 v;
+    v.x = xyz[3 * t]; v.y = xyz[3 * t + 1]; v.z = xyz[3 * t + 2]; v.w = 0.0;
+    pp[iv[t]] = v;
+}
+
 }  // namespace
+
+void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {
+    if (k > 0) hipLaunchKernelGGL(k_scatter_targets, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pp, iv, xyz, k);
+}
 
 void launch_cotan_face(hipStream_t s, LaunchSet L, int blocks) {
     if (blocks > 0) hipLaunchKernelGGL(k_cotan_face, dim3(blocks), dim3(kBlock), 0, s, L);

@@ -364,6 +364,8 @@ int analyze(md_mesh *m) {
     size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
     int eblocks = std::max(1, cdiv(t.nslices, kSlicesPerBlock));
     size_t o_en = A.reserve(sizeof(double) * eblocks);
+    size_t o_ti = A.reserve(sizeof(int) * std::max(1, n - nf));
+    size_t o_tx = A.reserve(sizeof(double) * 3 * std::max(1, n - nf));
     if (A.allocate() != hipSuccess) {
         set_error("device allocation of " + std::to_string(A.size()) + " bytes failed");
         A.reset();
@@ -396,6 +398,8 @@ int analyze(md_mesh *m) {
     d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
     d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
     d.energy_part = (double *)A.at(o_en);
+    m->d_tgt_idx = (int *)A.at(o_ti);
+    m->d_tgt_xyz = (double *)A.at(o_tx);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -755,6 +759,33 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->bwd_items = s.bwd_items;
     out->device_bytes = (int64_t)m->arena.size();
     out->setup_ms_host = m->setup_ms_host;
+    // algorithmic bytes per iteration (see the byte model in the design notes)
+    const Topology &t = m->topo;
+    const Factor &f = m->fac;
+    const int64_t es = m->fp64() ? 8 : 4;      // bytes per panel / edge scalar
+    const int64_t vs = 4 * es;                 // bytes per solve vector entry (x, y, z, pad)
+    const int64_t ps = 32;                     // FP64 position (x, y, z, pad)
+    int64_t slots = t.total_slots();
+    int64_t free_slots = t.nslices > 0 ? (int64_t)t.slice_ptr[(t.nf + kSlice - 1) / kSlice] * kSlice : 0;
+    // local: neighbour ids + weighted rest edges per slot, p' and the rotation per vertex
+    out->bytes_local = slots * (4 + 3 * es) + (int64_t)t.n * (ps + vs);
+    // residual: ids, edges and FP64 weights per slot, p' and rotation per vertex, residual out
+    out->bytes_residual = free_slots * (4 + 3 * es + 8) + (int64_t)t.n * (ps + vs) + (int64_t)t.nf * vs;
+    int64_t gw = 0, dw = 0, upd = 0, xp = 0, offr = 0, xloc_reads = 0;
+    for (int k = 0; k < f.ns; ++k) {
+        gw += (int64_t)f.noff[k] * f.w[k];
+        dw += (int64_t)f.w[k] * f.w[k];
+        upd += (int64_t)f.kc[k] * f.noff[k];
+        xp += (int64_t)f.kr[k] * f.w[k];
+        offr += f.noff[k];
+    }
+    for (size_t e = 0; e < f.offrows.size(); ++e) xloc_reads += f.xloc[4 * e + 2];
+    // forward: G panel, r in, z out, update partials out and read back by the parent, rel
+    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + offr * 4;
+    // backward + update: Dinv and G panels, z in, x partials of ancestors gathered (with
+    // their locators), x partials out, final sum and the p' update
+    out->bytes_backward = (dw + gw) * es + (int64_t)t.nf * vs + xloc_reads * vs + offr * 16 + 2 * xp * vs +
+                          (int64_t)t.nf * (16 + 2 * ps);
     return MD_OK;
 }
 

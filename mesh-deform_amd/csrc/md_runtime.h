@@ -100,5 +100,7 @@ struct md_mesh {
     std::vector<int64_t> adj_slot; // slot of each adj entry
     std::vector<int64_t> diag_slot_ptr; // unused placeholder for symmetry with adj
     double setup_ms_host = 0.0;
+    int *d_tgt_idx = nullptr;      // staging for moved targets (ncon entries)
+    double *d_tgt_xyz = nullptr;
     bool fp64() const { return cfg.precision == MD_PREC_FP64; }
 };

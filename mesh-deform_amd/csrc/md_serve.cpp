@@ -1,7 +1,14 @@
-// Batches: many prepared meshes iterated by the same launches (one launch per kernel per
-// level covers every mesh of the batch), replayed from one captured hipGraph.
+// Batches (many prepared meshes iterated by the same launches, one launch per kernel per
+// level covering every mesh, replayed from one captured hipGraph) and the drag-request server.
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 
 #include "md_runtime.h"
 
@@ -83,20 +90,280 @@ extern "C" int md_batch_destroy(md_batch *b) {
 }
 
 // ---- server (continuous batching) ---------------------------------------------------------
+//
+// One worker thread owns the context's stream while the server runs. Requests queue in
+// submission order; at every iteration boundary the worker admits queued requests whose mesh
+// is idle (one request in flight per mesh, at most max_active in flight), rebuilds the launch
+// plan when the set changed, and runs one ARAP iteration for every admitted request in the
+// same launches. A request leaves the set when its iterations are done; its positions are
+// downloaded and its waiters woken. Meshes are resident from md_server_add_mesh on, within the
+// HBM budget.
+
+namespace {
+using SClock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket = 0;
+    int mesh_id = -1;
+    std::vector<int> idx;
+    std::vector<double> xyz;
+    int iters = 0, left = 0;
+    SClock::time_point t_submit, t_done;
+    bool admitted = false, done = false;
+    int status = MD_OK;
+    std::string error;
+    std::vector<double> result;
+};
+}  // namespace
 
 struct md_server {
     md_ctx *ctx = nullptr;
+    md_server_config cfg{};
+    std::vector<md_mesh *> meshes;
+    size_t resident = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::map<uint64_t, std::shared_ptr<Request>> tickets;
+    std::vector<std::shared_ptr<Request>> active;
+    std::vector<char> busy;  // per mesh
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+    Plan plan;
+    std::vector<int> plan_meshes;
+    std::string fatal;
 };
 
-extern "C" int md_server_create(md_ctx *ctx, const md_server_config *, md_server **out) {
-    (void)ctx; (void)out;
-    set_error("md_server_create: not available yet");
-    return MD_ERR_STATE;
+namespace {
+
+int apply_targets(md_mesh *m, const Request &r) {
+    // paste the new targets into p' (fixed rest pose: the previous result is the initial
+    // guess) with one upload and one scatter launch
+    int k = (int)r.idx.size();
+    if (k == 0) return MD_OK;
+    std::vector<int> iv(k);
+    for (int i = 0; i < k; ++i) {
+        iv[i] = m->topo.internal_of[r.idx[i]];
+        std::memcpy(&m->con[3 * (size_t)r.idx[i]], &r.xyz[3 * (size_t)i], 3 * sizeof(double));
+    }
+    hipStream_t st = m->ctx->stream;
+    hipError_t e = hipMemcpyAsync(m->d_tgt_idx, iv.data(), sizeof(int) * k, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(m->d_tgt_xyz, r.xyz.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        launch_scatter_targets(st, m->dev.pp, m->d_tgt_idx, m->d_tgt_xyz, k);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? MD_OK : MD_ERR_HIP;
 }
-extern "C" int md_server_add_mesh(md_server *, md_mesh *, int *) { set_error("server not available"); return MD_ERR_STATE; }
-extern "C" int md_server_submit(md_server *, int, int, const int *, const double *, int, uint64_t *) {
-    set_error("server not available"); return MD_ERR_STATE;
+
+void server_loop(md_server *S) {
+    hipSetDevice(S->ctx->device);
+    hipStream_t st = S->ctx->stream;
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> admitted_now;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            for (;;) {
+                if (S->stop) return;
+                // admission at the iteration boundary
+                for (auto it = S->queue.begin(); it != S->queue.end();) {
+                    auto &r = *it;
+                    bool slot = S->cfg.max_active <= 0 || (int)S->active.size() < S->cfg.max_active;
+                    if (slot && !S->busy[r->mesh_id]) {
+                        S->busy[r->mesh_id] = 1;
+                        r->admitted = true;
+                        S->active.push_back(r);
+                        admitted_now.push_back(r);
+                        it = S->queue.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+                if (!S->active.empty()) break;
+                S->cv_work.wait(lk);
+            }
+        }
+        for (auto &r : admitted_now) {
+            int rc = apply_targets(S->meshes[r->mesh_id], *r);
+            if (rc) { r->status = rc; r->error = "target upload failed"; r->left = 0; }
+        }
+        // plan over the active meshes (rebuilt only when the set changed)
+        std::vector<int> ids;
+        for (auto &r : S->active) ids.push_back(r->mesh_id);
+        if (ids != S->plan_meshes) {
+            std::vector<const MeshDev *> devs;
+            std::vector<const Factor *> facs;
+            for (int id : ids) { devs.push_back(&S->meshes[id]->dev); facs.push_back(&S->meshes[id]->fac); }
+            md_mesh *m0 = S->meshes[ids[0]];
+            hipError_t e = build_plan(S->plan, devs, facs, m0->fp64(), m0->cfg.refine_steps, st);
+            if (e != hipSuccess) { S->fatal = hipGetErrorString(e); }
+            S->plan_meshes = ids;
+        }
+        bool any_left = false;
+        for (auto &r : S->active) any_left |= r->left > 0;
+        hipError_t e = hipSuccess;
+        if (any_left && S->fatal.empty()) {
+            enqueue_iteration(S->plan, st, nullptr);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+        }
+        // retire finished requests
+        std::vector<std::shared_ptr<Request>> finished;
+        for (auto &r : S->active) {
+            if (r->left > 0) r->left--;
+            if (e != hipSuccess || !S->fatal.empty()) {
+                r->status = MD_ERR_HIP;
+                r->error = e != hipSuccess ? hipGetErrorString(e) : S->fatal;
+                r->left = 0;
+            }
+            if (r->left == 0) finished.push_back(r);
+        }
+        for (auto &r : finished) {
+            md_mesh *m = S->meshes[r->mesh_id];
+            m->iterated = true;
+            if (r->status == MD_OK) {
+                r->result.resize(3 * (size_t)m->n);
+                std::vector<d4> h(m->n);
+                hipError_t e2 = hipMemcpyAsync(h.data(), m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+                if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
+                if (e2 != hipSuccess) { r->status = MD_ERR_HIP; r->error = hipGetErrorString(e2); }
+                for (int iv = 0; iv < m->n; ++iv) {
+                    int v = m->topo.original_of[iv];
+                    r->result[3 * (size_t)v] = h[iv].x;
+                    r->result[3 * (size_t)v + 1] = h[iv].y;
+                    r->result[3 * (size_t)v + 2] = h[iv].z;
+                }
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            auto now = SClock::now();
+            for (auto &r : finished) {
+                r->done = true;
+                r->t_done = now;
+                S->busy[r->mesh_id] = 0;
+            }
+            S->active.erase(std::remove_if(S->active.begin(), S->active.end(),
+                                           [](const std::shared_ptr<Request> &r) { return r->done; }),
+                            S->active.end());
+        }
+        if (!finished.empty()) S->cv_done.notify_all();
+    }
 }
-extern "C" int md_server_poll(md_server *, uint64_t, int *, double *, double *) { set_error("server not available"); return MD_ERR_STATE; }
-extern "C" int md_server_wait(md_server *, uint64_t, double, double *, double *) { set_error("server not available"); return MD_ERR_STATE; }
-extern "C" int md_server_destroy(md_server *) { return MD_OK; }
+
+}  // namespace
+
+extern "C" int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_server **out) {
+    if (!ctx || !out) { set_error("md_server_create: invalid argument"); return MD_ERR_INVALID; }
+    std::unique_ptr<md_server> S(new md_server());
+    S->ctx = ctx;
+    if (cfg) S->cfg = *cfg;
+    S->worker = std::thread(server_loop, S.get());
+    *out = S.release();
+    return MD_OK;
+}
+
+extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
+    if (!S || !m || !mesh_id) { set_error("md_server_add_mesh: invalid argument"); return MD_ERR_INVALID; }
+    if (m->ctx != S->ctx) { set_error("md_server_add_mesh: mesh from another context"); return MD_ERR_INVALID; }
+    if (m->dirty || !m->analyzed || m->cfg.rest_mode != MD_REST_FIXED) {
+        set_error("md_server_add_mesh: the mesh must be prepared (md_mesh_prepare) with MD_REST_FIXED");
+        return MD_ERR_STATE;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->meshes.empty()) {
+        md_mesh *m0 = S->meshes[0];
+        if (m0->cfg.precision != m->cfg.precision || m0->cfg.refine_steps != m->cfg.refine_steps) {
+            set_error("md_server_add_mesh: meshes of one server share precision and refinement steps");
+            return MD_ERR_INVALID;
+        }
+    }
+    size_t bytes = m->arena.size();
+    if (S->cfg.hbm_budget_bytes > 0 && S->resident + bytes > S->cfg.hbm_budget_bytes) {
+        set_error("md_server_add_mesh: mesh needs " + std::to_string(bytes) + " device bytes, " +
+                  std::to_string(S->cfg.hbm_budget_bytes - S->resident) + " left in the HBM budget");
+        return MD_ERR_BUDGET;
+    }
+    S->resident += bytes;
+    S->meshes.push_back(m);
+    S->busy.push_back(0);
+    *mesh_id = (int)S->meshes.size() - 1;
+    return MD_OK;
+}
+
+extern "C" int md_server_submit(md_server *S, int mesh_id, int k, const int *idx, const double *xyz,
+                                int iterations, uint64_t *ticket) {
+    if (!S || !ticket || k < 0 || (k > 0 && (!idx || !xyz)) || iterations < 0) {
+        set_error("md_server_submit: invalid argument");
+        return MD_ERR_INVALID;
+    }
+    auto r = std::make_shared<Request>();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (mesh_id < 0 || mesh_id >= (int)S->meshes.size()) { set_error("md_server_submit: unknown mesh"); return MD_ERR_INVALID; }
+        md_mesh *m = S->meshes[mesh_id];
+        if (k > m->n - m->topo.nf) { set_error("md_server_submit: more targets than constrained vertices"); return MD_ERR_INVALID; }
+        for (int i = 0; i < k; ++i)
+            if (idx[i] < 0 || idx[i] >= m->n || !m->is_con[idx[i]]) {
+                set_error("md_server_submit: vertex " + std::to_string(idx[i]) + " is not a constrained vertex of the mesh");
+                return MD_ERR_INVALID;
+            }
+        r->ticket = S->next_ticket++;
+        r->mesh_id = mesh_id;
+        r->idx.assign(idx, idx + k);
+        r->xyz.assign(xyz, xyz + 3 * (size_t)k);
+        r->iters = r->left = iterations;
+        r->t_submit = SClock::now();
+        S->queue.push_back(r);
+        S->tickets[r->ticket] = r;
+        *ticket = r->ticket;
+    }
+    S->cv_work.notify_one();
+    return MD_OK;
+}
+
+static int finish_result(md_server *S, std::shared_ptr<Request> r, double *xyz, double *latency_ms) {
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->status != MD_OK) { set_error("request failed: " + r->error); }
+    else if (xyz) std::memcpy(xyz, r->result.data(), sizeof(double) * r->result.size());
+    S->tickets.erase(r->ticket);
+    return r->status;
+}
+
+extern "C" int md_server_poll(md_server *S, uint64_t ticket, int *done, double *xyz, double *latency_ms) {
+    if (!S || !done) { set_error("md_server_poll: invalid argument"); return MD_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->tickets.find(ticket);
+    if (it == S->tickets.end()) { set_error("md_server_poll: unknown ticket"); return MD_ERR_INVALID; }
+    *done = it->second->done ? 1 : 0;
+    if (!*done) return MD_OK;
+    return finish_result(S, it->second, xyz, latency_ms);
+}
+
+extern "C" int md_server_wait(md_server *S, uint64_t ticket, double timeout_ms, double *xyz, double *latency_ms) {
+    if (!S) { set_error("md_server_wait: invalid argument"); return MD_ERR_INVALID; }
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->tickets.find(ticket);
+    if (it == S->tickets.end()) { set_error("md_server_wait: unknown ticket"); return MD_ERR_INVALID; }
+    auto r = it->second;
+    bool ok = S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), [&] { return r->done; });
+    if (!ok) { set_error("md_server_wait: timed out"); return MD_ERR_TIMEOUT; }
+    return finish_result(S, r, xyz, latency_ms);
+}
+
+extern "C" int md_server_destroy(md_server *S) {
+    if (!S) return MD_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    hipSetDevice(S->ctx->device);
+    hipStreamSynchronize(S->ctx->stream);
+    delete S;
+    return MD_OK;
+}

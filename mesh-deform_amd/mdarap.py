@@ -36,7 +36,9 @@ class MeshStats(ctypes.Structure):
                 ("max_sn_rows", ctypes.c_int), ("slots", ctypes.c_int64), ("cotan_nnz", ctypes.c_int64),
                 ("panel_entries", ctypes.c_int64), ("factor_nnz", ctypes.c_int64),
                 ("fwd_items", ctypes.c_int64), ("bwd_items", ctypes.c_int64),
-                ("device_bytes", ctypes.c_int64), ("setup_ms_host", ctypes.c_double)]
+                ("device_bytes", ctypes.c_int64), ("setup_ms_host", ctypes.c_double),
+                ("bytes_local", ctypes.c_int64), ("bytes_residual", ctypes.c_int64),
+                ("bytes_forward", ctypes.c_int64), ("bytes_backward", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1,0 +1,134 @@
+"""Batched and served paths on the GPU.
+
+* The server's result for each drag request is bitwise the same whether the request runs
+  alone or overlapped with requests on other meshes (each launch covers many meshes, but a
+  mesh's arithmetic never depends on the others).
+* Served results match the FP64 oracle run in fixed-rest mode (the rest pose and factor of
+  the first deform are kept; each request moves the handles and warm-starts from the last
+  result), within the FP32 + refinement tolerance 1e-5.
+* HBM-budgeted residency: a mesh that does not fit the budget is rejected.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import meshgen
+import mdarap
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if mdarap.device_count() == 0:
+        pytest.fail("no GPU visible: the HIP path cannot run")
+    c = mdarap.Context(0)
+    yield c
+    c.close()
+
+
+def make_meshes(ctx, probs, precision=mdarap.PREC_FP32_REFINE):
+    out = []
+    for pr in probs:
+        idx, tgt = pr.constraints()
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=precision, rest_mode=mdarap.REST_FIXED))
+        m.set_constraints(idx, pr.xyz[idx])  # anchors and handles at rest first
+        m.prepare(pr.xyz)
+        out.append(m)
+    return out
+
+
+def request_stream(probs, count, seed):
+    """(mesh index, handle targets) drags: handles translated by a seeded offset."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(count):
+        k = int(rng.integers(len(probs)))
+        pr = probs[k]
+        off = rng.uniform(-0.2, 0.2, size=3)
+        reqs.append((k, pr.handles, pr.xyz[pr.handles] + off))
+    return reqs
+
+
+def test_served_alone_equals_in_mix_and_oracle(ctx):
+    say("case: server alone vs mix")
+    probs = meshgen.mixed_batch(5, 400, 4000, seed=11)
+    reqs = request_stream(probs, 14, seed=3)
+    iters = 5
+    # alone: one request at a time
+    meshes_a = make_meshes(ctx, probs)
+    srv = mdarap.Server(ctx)
+    ids = [srv.add_mesh(m) for m in meshes_a]
+    alone = []
+    for k, h, t in reqs:
+        tk = srv.submit(ids[k], h, t, iters)
+        pos, lat = srv.wait(tk, probs[k].n)
+        alone.append(pos)
+    srv.close()
+    # mix: everything submitted at once
+    meshes_b = make_meshes(ctx, probs)
+    srv = mdarap.Server(ctx)
+    ids = [srv.add_mesh(m) for m in meshes_b]
+    tickets = [srv.submit(ids[k], h, t, iters) for k, h, t in reqs]
+    mix = [srv.wait(tk, probs[k].n) for tk, (k, _, _) in zip(tickets, reqs)]
+    srv.close()
+    for i, ((pos, lat), ref) in enumerate(zip(mix, alone)):
+        assert np.array_equal(pos, ref), i
+        assert lat > 0
+    # oracle in fixed-rest mode replaying the same request order per mesh
+    orc = []
+    for pr in probs:
+        a = oracle.Arap(pr.xyz, pr.tri)
+        idx, _ = pr.constraints()
+        a.set_constraints(idx, pr.xyz[idx])
+        assert a.deform(0, rest_mode=1)
+        orc.append(a)
+    for (k, h, t), got in zip(reqs, alone):
+        a = orc[k]
+        a.set_constraints(h, t)
+        assert a.deform(iters, rest_mode=1)
+        err = meshgen.relative_error(got, a.mesh)
+        assert err < 1e-5, err
+
+
+def test_server_rejects_over_budget(ctx):
+    say("case: server budget")
+    pr = meshgen.sphere_problem(10, 16)
+    (m,) = make_meshes(ctx, [pr])
+    need = m.stats()["device_bytes"]
+    srv = mdarap.Server(ctx, hbm_budget_bytes=need // 2)
+    with pytest.raises(mdarap.MDError) as e:
+        srv.add_mesh(m)
+    assert e.value.code == mdarap.MD_ERR_BUDGET
+    srv.close()
+    srv = mdarap.Server(ctx, hbm_budget_bytes=need)
+    assert srv.add_mesh(m) == 0
+    # a request may only move constrained vertices
+    free_vertex = int(np.setdiff1d(np.arange(pr.n), pr.constraints()[0])[0])
+    with pytest.raises(mdarap.MDError) as e:
+        srv.submit(0, [free_vertex], [[0.0, 0.0, 0.0]], 1)
+    assert e.value.code == mdarap.MD_ERR_INVALID
+    srv.close()
+
+
+def test_server_max_active_serialises(ctx):
+    say("case: server max_active")
+    probs = meshgen.mixed_batch(3, 400, 2000, seed=5)
+    reqs = request_stream(probs, 6, seed=9)
+    res = []
+    for max_active in (1, 0):
+        meshes = make_meshes(ctx, probs)
+        srv = mdarap.Server(ctx, max_active=max_active)
+        ids = [srv.add_mesh(m) for m in meshes]
+        tks = [srv.submit(ids[k], h, t, 3) for k, h, t in reqs]
+        res.append([srv.wait(tk, probs[k].n)[0] for tk, (k, _, _) in zip(tks, reqs)])
+        srv.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
