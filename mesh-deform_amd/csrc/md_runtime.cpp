@@ -759,7 +759,7 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->bwd_items = s.bwd_items;
     out->device_bytes = (int64_t)m->arena.size();
     out->setup_ms_host = m->setup_ms_host;
-    // algorithmic bytes per iteration (see the byte model in the design notes)
+    // algorithmic bytes per iteration: each array element a phase must touch, counted once
     const Topology &t = m->topo;
     const Factor &f = m->fac;
     const int64_t es = m->fp64() ? 8 : 4;      // bytes per panel / edge scalar
