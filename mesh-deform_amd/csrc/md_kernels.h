@@ -52,6 +52,13 @@ struct MeshDev {
     const int *ft_ptr, *bt_ptr; // tiles per level
     const int4 *ft, *bt;        // (s, r0, q0, 0) / (s, q0, p0, 0)
     double *energy_part;        // one partial sum per 256 vertices
+    // Jacobi-PCG alternative to the triangular sweeps (solve type vectors, nf entries)
+    float *w32;                 // slots: cotan weights in FP32
+    void *diag;                 // nf: L_ii = sum_j w_ij (solve type scalar)
+    void *cg_x, *cg_p, *cg_ap;  // correction, direction, A * direction (V4)
+    void *cg_part;              // 3 x nblk partial dot products (V4): p.Ap, r.z (two buffers)
+    int nblk_free;              // 256-vertex blocks covering [0, nf)
+    double pcg_tol2;            // stop (freeze) when r.z <= tol^2 * (r.z)_0, per coordinate
 };
 
 // A set of meshes processed by one launch. Block b belongs to mesh k where
@@ -73,6 +80,12 @@ void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k);
+void launch_pcg_diag(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_pcg_init(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+void launch_pcg_update(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
+void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
+void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 
 // Threads per block and work per block of each kernel (used for the prefix tables).
 constexpr int kBlock = 256;

@@ -79,6 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_cotan_slot(LaunchSet L) {
         double ax = w * (pi.x - pj.x), ay = w * (pi.y - pj.y), az = w * (pi.z - pj.z);
         if (iv >= M.n) { w = 0.0; ax = ay = az = 0.0; }
         M.w64[g] = w;
+        M.w32[g] = (float)w;
         M.a64[g] = ax; M.a64[M.slots + g] = ay; M.a64[2 * M.slots + g] = az;
         M.a32[g] = (float)ax; M.a32[M.slots + g] = (float)ay; M.a32[2 * M.slots + g] = (float)az;
     }
@@ -426,6 +427,212 @@ __global__ __launch_bounds__(kBlock) void k_apply(LaunchSet L) {
     M.pp[i] = pv;
 }
 
+// ---- Jacobi-preconditioned CG on the correction system (alternative to the sweeps) -------
+//
+// Solves A d = r for the three coordinates at once (three independent CG recurrences, one
+// per coordinate), A = L of arap.h:325-332 over the free vertices, M = diag(A). d starts at 0:
+// the residual r = b - A p' already warm-starts from the previous iterate. Every block works
+// on 256 consecutive free vertices; dot products are per-block partials that each block of
+// the next kernel reduces in the same fixed order (deterministic, no atomics).
+
+template <typename T> __device__ __forceinline__ const T *wts(const MeshDev &M);
+template <> __device__ __forceinline__ const float *wts<float>(const MeshDev &M) { return M.w32; }
+template <> __device__ __forceinline__ const double *wts<double>(const MeshDev &M) { return M.w64; }
+
+// Sums the 3-component partials part[0 .. nb) in a fixed order; result broadcast via LDS.
+template <typename T, typename V>
+__device__ __forceinline__ void reduce_partials(const V *part, int nb, T (&out)[3], T (*red)[kBlock]) {
+    T a = 0, b = 0, c = 0;
+    for (int k = threadIdx.x; k < nb; k += kBlock) {
+        V v = part[k];
+        a += v.x; b += v.y; c += v.z;
+    }
+    red[0][threadIdx.x] = a; red[1][threadIdx.x] = b; red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+            red[2][threadIdx.x] += red[2][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    out[0] = red[0][0]; out[1] = red[1][0]; out[2] = red[2][0];
+    __syncthreads();
+}
+
+// Block-wide sum of a 3-vector into part[blk].
+template <typename T, typename V>
+__device__ __forceinline__ void block_partial(T a, T b, T c, V *part, int blk, T (*red)[kBlock]) {
+    red[0][threadIdx.x] = a; red[1][threadIdx.x] = b; red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+            red[2][threadIdx.x] += red[2][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        V v;
+        v.x = red[0][0]; v.y = red[1][0]; v.z = red[2][0]; v.w = 0;
+        part[blk] = v;
+    }
+}
+
+// L_ii = sum of every neighbour weight (arap.h:332), once per factorisation.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_diag(LaunchSet L) {
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (slice >= M.nslices_free) return;
+    int iv = slice * kSlice + lane;
+    if (iv >= M.nf) return;
+    int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
+    double d = 0.0;
+    for (int q = 0; q < wd; ++q) d += M.w64[(long long)(s0 + q) * kSlice + lane];
+    reinterpret_cast<T *>(M.diag)[iv] = (T)d;
+}
+
+// d = 0, p = M^-1 r, partial r.z into buffer 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    __shared__ T red[3][kBlock];
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int blk = blockIdx.x - L.prefix[k];
+    int i = blk * kBlock + threadIdx.x;
+    T a = 0, b = 0, c = 0;
+    if (i < M.nf) {
+        V r = reinterpret_cast<const V *>(M.r)[i];
+        T dinv = T(1) / reinterpret_cast<const T *>(M.diag)[i];
+        V z;
+        z.x = r.x * dinv; z.y = r.y * dinv; z.z = r.z * dinv; z.w = 0;
+        V zero;
+        zero.x = 0; zero.y = 0; zero.z = 0; zero.w = 0;
+        reinterpret_cast<V *>(M.cg_x)[i] = zero;
+        reinterpret_cast<V *>(M.cg_p)[i] = z;
+        a = r.x * z.x; b = r.y * z.y; c = r.z * z.z;
+    }
+    V *part = reinterpret_cast<V *>(M.cg_part);
+    block_partial<T, V>(a, b, c, part + M.nblk_free, blk, red);
+    if (threadIdx.x == 0) part[3 * M.nblk_free + blk] = part[M.nblk_free + blk];
+}
+
+// Ap = A p over the one-ring slots (constrained neighbours contribute 0), partial p.Ap.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    __shared__ T red[3][kBlock];
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int blk = blockIdx.x - L.prefix[k];
+    int slice = blk * kSlicesPerBlock + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    int iv = slice * kSlice + lane;
+    T a = 0, b = 0, c = 0;
+    if (slice < M.nslices_free && iv < M.nf) {
+        const V *P = reinterpret_cast<const V *>(M.cg_p);
+        const T *W = wts<T>(M);
+        int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
+        V pi = P[iv];
+        T dg = reinterpret_cast<const T *>(M.diag)[iv];
+        T yx = dg * pi.x, yy = dg * pi.y, yz = dg * pi.z;
+        for (int q = 0; q < wd; ++q) {
+            long long g = (long long)(s0 + q) * kSlice + lane;
+            int j = M.sell_col[g];
+            if (j != iv && j < M.nf) {
+                T w = W[g];
+                V pj = P[j];
+                yx -= w * pj.x; yy -= w * pj.y; yz -= w * pj.z;
+            }
+        }
+        V ap;
+        ap.x = yx; ap.y = yy; ap.z = yz; ap.w = 0;
+        reinterpret_cast<V *>(M.cg_ap)[iv] = ap;
+        a = pi.x * yx; b = pi.y * yy; c = pi.z * yz;
+    }
+    block_partial<T, V>(a, b, c, reinterpret_cast<V *>(M.cg_part), blk, red);
+}
+
+// alpha = (r.z) / (p.Ap); d += alpha p; r -= alpha Ap; partial of the new r.z into buffer
+// 1 + (1 - parity) (the old one is buffer 1 + parity).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(LaunchSet L, int parity) {
+    using V = typename V4T<T>::type;
+    __shared__ T red[3][kBlock];
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int blk = blockIdx.x - L.prefix[k];
+    V *part = reinterpret_cast<V *>(M.cg_part);
+    T rz[3], pap[3], rz0[3];
+    reduce_partials<T, V>(part + (1 + parity) * M.nblk_free, M.nblk_free, rz, red);
+    reduce_partials<T, V>(part, M.nblk_free, pap, red);
+    reduce_partials<T, V>(part + 3 * M.nblk_free, M.nblk_free, rz0, red);
+    T al[3];
+    for (int c = 0; c < 3; ++c) {
+        bool conv = (double)rz[c] <= M.pcg_tol2 * (double)rz0[c];
+        al[c] = (pap[c] != T(0) && !conv) ? rz[c] / pap[c] : T(0);
+    }
+    int i = blk * kBlock + threadIdx.x;
+    T a = 0, b = 0, c = 0;
+    if (i < M.nf) {
+        V p = reinterpret_cast<const V *>(M.cg_p)[i];
+        V ap = reinterpret_cast<const V *>(M.cg_ap)[i];
+        V x = reinterpret_cast<V *>(M.cg_x)[i];
+        V r = reinterpret_cast<V *>(M.r)[i];
+        x.x += al[0] * p.x; x.y += al[1] * p.y; x.z += al[2] * p.z;
+        r.x -= al[0] * ap.x; r.y -= al[1] * ap.y; r.z -= al[2] * ap.z;
+        reinterpret_cast<V *>(M.cg_x)[i] = x;
+        reinterpret_cast<V *>(M.r)[i] = r;
+        T dinv = T(1) / reinterpret_cast<const T *>(M.diag)[i];
+        a = r.x * r.x * dinv; b = r.y * r.y * dinv; c = r.z * r.z * dinv;
+    }
+    block_partial<T, V>(a, b, c, part + (2 - parity) * M.nblk_free, blk, red);
+}
+
+// beta = (r.z)_new / (r.z)_old; p = M^-1 r + beta p.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_direction(LaunchSet L, int parity) {
+    using V = typename V4T<T>::type;
+    __shared__ T red[3][kBlock];
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int blk = blockIdx.x - L.prefix[k];
+    const V *part = reinterpret_cast<const V *>(M.cg_part);
+    T rn[3], ro[3];
+    reduce_partials<T, V>(part + (2 - parity) * M.nblk_free, M.nblk_free, rn, red);
+    reduce_partials<T, V>(part + (1 + parity) * M.nblk_free, M.nblk_free, ro, red);
+    T be[3];
+    for (int c = 0; c < 3; ++c) be[c] = (ro[c] != T(0)) ? rn[c] / ro[c] : T(0);
+    int i = blk * kBlock + threadIdx.x;
+    if (i < M.nf) {
+        V r = reinterpret_cast<const V *>(M.r)[i];
+        V p = reinterpret_cast<V *>(M.cg_p)[i];
+        T dinv = T(1) / reinterpret_cast<const T *>(M.diag)[i];
+        p.x = r.x * dinv + be[0] * p.x; p.y = r.y * dinv + be[1] * p.y; p.z = r.z * dinv + be[2] * p.z;
+        reinterpret_cast<V *>(M.cg_p)[i] = p;
+    }
+}
+
+// p' += d for the free vertices.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_apply(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int i = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
+    if (i >= M.nf) return;
+    V x = reinterpret_cast<const V *>(M.cg_x)[i];
+    d4 pv = M.pp[i];
+    pv.x += (double)x.x; pv.y += (double)x.y; pv.z += (double)x.z;
+    M.pp[i] = pv;
+}
+
 // ---- ARAP energy -------------------------------------------------------------------------
 
 // E = sum_i sum_j w_ij |(p'_i - p'_j) - R_i (p_i - p_j)|^2 over all vertices, in FP64; one
@@ -478,6 +685,24 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
 }
 
 }  // namespace
+
+#define MD_LAUNCH_T(kern, blocks, s, ...)                                                        \
+    do {                                                                                        \
+        if ((blocks) <= 0) return;                                                              \
+        if (fp64) hipLaunchKernelGGL(kern<double>, dim3(blocks), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern<float>, dim3(blocks), dim3(kBlock), 0, s, __VA_ARGS__);     \
+    } while (0)
+
+void launch_pcg_diag(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_diag, blocks, s, L); }
+void launch_pcg_init(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_init, blocks, s, L); }
+void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_spmv, blocks, s, L); }
+void launch_pcg_update(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64) {
+    MD_LAUNCH_T(k_pcg_update, blocks, s, L, parity);
+}
+void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64) {
+    MD_LAUNCH_T(k_pcg_direction, blocks, s, L, parity);
+}
+void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_apply, blocks, s, L); }
 
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {
     if (k > 0) hipLaunchKernelGGL(k_scatter_targets, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pp, iv, xyz, k);

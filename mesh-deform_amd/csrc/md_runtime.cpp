@@ -116,6 +116,19 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
     for (int r = 0; r < plan.refine_steps; ++r) {
         launch_residual(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[2], stream);
+        if (plan.pcg) {
+            // Jacobi-PCG on the correction: init, K x (A p, update, direction), apply
+            launch_pcg_init(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
+            for (int k = 0; k < plan.pcg_iters; ++k) {
+                launch_pcg_spmv(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
+                launch_pcg_update(stream, plan.set(plan.off_apply), plan.blk_apply, k & 1, plan.fp64);
+                launch_pcg_direction(stream, plan.set(plan.off_apply), plan.blk_apply, k & 1, plan.fp64);
+            }
+            if (ev && r == 0) hipEventRecord(ev[3], stream);
+            launch_pcg_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
+            if (ev && r == 0) hipEventRecord(ev[4], stream);
+            continue;
+        }
         for (int l = 0; l < plan.nlevels; ++l)
             launch_fwd(stream, plan.set(plan.off_fwd[l]), plan.blk_fwd[l], l, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[3], stream);
@@ -241,8 +254,12 @@ extern "C" int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const m
         set_error("md_mesh_create: unknown precision");
         return MD_ERR_INVALID;
     }
-    if (m->cfg.solver != MD_SOLVER_LDLT) {
-        set_error("md_mesh_create: only the LDLT solver is available in this build");
+    if (m->cfg.solver != MD_SOLVER_LDLT && m->cfg.solver != MD_SOLVER_PCG) {
+        set_error("md_mesh_create: unknown solver");
+        return MD_ERR_INVALID;
+    }
+    if (m->cfg.solver == MD_SOLVER_PCG && (m->cfg.pcg_max_iters < 1 || !(m->cfg.pcg_tol >= 0.0))) {
+        set_error("md_mesh_create: PCG needs pcg_max_iters >= 1 and pcg_tol >= 0");
         return MD_ERR_INVALID;
     }
     m->cfg.refine_steps = std::max(1, m->cfg.refine_steps);
@@ -320,7 +337,7 @@ int analyze(md_mesh *m) {
         m->adj_ptr[i + 1] = (int)m->adj.size();
     }
     m->fac = Factor();
-    if (nf > 0 && nf < n) symbolic(nf, m->adj_ptr, m->adj, m->fac);
+    if (nf > 0 && nf < n && m->cfg.solver == MD_SOLVER_LDLT) symbolic(nf, m->adj_ptr, m->adj, m->fac);
     else m->fac.n = nf;
     const Factor &f = m->fac;
     // arena
@@ -364,6 +381,15 @@ int analyze(md_mesh *m) {
     size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
     int eblocks = std::max(1, cdiv(t.nslices, kSlicesPerBlock));
     size_t o_en = A.reserve(sizeof(double) * eblocks);
+    bool pcg = m->cfg.solver == MD_SOLVER_PCG;
+    int nblk = std::max(1, cdiv(nf, kBlock));
+    size_t ssz = fp64 ? sizeof(double) : sizeof(float);
+    size_t o_w32 = A.reserve(sizeof(float) * slots);
+    size_t o_dg = A.reserve(ssz * std::max(1, pcg ? nf : 1));
+    size_t o_cx = A.reserve(vsz * std::max(1, pcg ? nf : 1));
+    size_t o_cpp = A.reserve(vsz * std::max(1, pcg ? nf : 1));
+    size_t o_cap = A.reserve(vsz * std::max(1, pcg ? nf : 1));
+    size_t o_cpt = A.reserve(vsz * (pcg ? 4 * (size_t)nblk : 1));
     size_t o_ti = A.reserve(sizeof(int) * std::max(1, n - nf));
     size_t o_tx = A.reserve(sizeof(double) * 3 * std::max(1, n - nf));
     if (A.allocate() != hipSuccess) {
@@ -398,6 +424,11 @@ int analyze(md_mesh *m) {
     d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
     d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
     d.energy_part = (double *)A.at(o_en);
+    d.w32 = (float *)A.at(o_w32);
+    d.diag = A.at(o_dg);
+    d.cg_x = A.at(o_cx); d.cg_p = A.at(o_cpp); d.cg_ap = A.at(o_cap); d.cg_part = A.at(o_cpt);
+    d.nblk_free = nblk;
+    d.pcg_tol2 = m->cfg.pcg_tol * m->cfg.pcg_tol;
     m->d_tgt_idx = (int *)A.at(o_ti);
     m->d_tgt_xyz = (double *)A.at(o_tx);
     hipStream_t st = m->ctx->stream;
@@ -439,6 +470,8 @@ int analyze(md_mesh *m) {
     }
     std::vector<const MeshDev *> ms{&m->dev};
     std::vector<const Factor *> fs{&m->fac};
+    m->plan.pcg = m->cfg.solver == MD_SOLVER_PCG;
+    m->plan.pcg_iters = m->cfg.pcg_max_iters;
     e = build_plan(m->plan, ms, fs, fp64, m->cfg.refine_steps, st);
     if (e != hipSuccess) {
         set_error(std::string("launch plan upload failed: ") + hipGetErrorString(e));
@@ -488,7 +521,17 @@ hipError_t compute_cotan(md_mesh *m) {
 }
 
 // FP64 numeric factorization on the host from the device cotan weights; uploads panels.
+// With the PCG solver only the Jacobi diagonal is formed (on the device).
 int factorize(md_mesh *m) {
+    if (m->cfg.solver == MD_SOLVER_PCG) {
+        const Plan &P = m->plan;
+        launch_pcg_diag(m->ctx->stream, P.set(P.off_resid), P.blk_resid, m->fp64());
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
+        if (e != hipSuccess) { set_error(std::string("PCG diagonal: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+        m->factored = true;
+        return MD_OK;
+    }
     auto t0 = Clock::now();
     const Topology &t = m->topo;
     int nf = t.nf;

@@ -57,6 +57,8 @@ struct Plan {
     std::vector<MeshDev> h_meshes;
     bool fp64 = true;
     int refine_steps = 1;
+    bool pcg = false;
+    int pcg_iters = 0;
     hipGraphExec_t graph = nullptr;
     ~Plan();
     void drop_graph();

@@ -42,6 +42,13 @@ extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, m
         facs.push_back(&m->fac);
     }
     b->use_graph = meshes[0]->cfg.use_graph != 0;
+    for (int k = 0; k < count; ++k)
+        if (meshes[k]->cfg.solver != meshes[0]->cfg.solver || meshes[k]->cfg.pcg_max_iters != meshes[0]->cfg.pcg_max_iters) {
+            set_error("md_batch_create: meshes differ in solver settings");
+            return MD_ERR_INVALID;
+        }
+    b->plan.pcg = meshes[0]->cfg.solver == MD_SOLVER_PCG;
+    b->plan.pcg_iters = meshes[0]->cfg.pcg_max_iters;
     hipSetDevice(ctx->device);
     hipError_t e = build_plan(b->plan, devs, facs, prec == MD_PREC_FP64, refine, ctx->stream);
     if (e != hipSuccess) { set_error(std::string("md_batch_create: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
@@ -198,6 +205,8 @@ void server_loop(md_server *S) {
             std::vector<const Factor *> facs;
             for (int id : ids) { devs.push_back(&S->meshes[id]->dev); facs.push_back(&S->meshes[id]->fac); }
             md_mesh *m0 = S->meshes[ids[0]];
+            S->plan.pcg = m0->cfg.solver == MD_SOLVER_PCG;
+            S->plan.pcg_iters = m0->cfg.pcg_max_iters;
             hipError_t e = build_plan(S->plan, devs, facs, m0->fp64(), m0->cfg.refine_steps, st);
             if (e != hipSuccess) { S->fatal = hipGetErrorString(e); }
             S->plan_meshes = ids;
@@ -276,7 +285,8 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->meshes.empty()) {
         md_mesh *m0 = S->meshes[0];
-        if (m0->cfg.precision != m->cfg.precision || m0->cfg.refine_steps != m->cfg.refine_steps) {
+        if (m0->cfg.precision != m->cfg.precision || m0->cfg.refine_steps != m->cfg.refine_steps ||
+            m0->cfg.solver != m->cfg.solver || m0->cfg.pcg_max_iters != m->cfg.pcg_max_iters) {
             set_error("md_server_add_mesh: meshes of one server share precision and refinement steps");
             return MD_ERR_INVALID;
         }

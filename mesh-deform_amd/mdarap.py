@@ -128,7 +128,8 @@ def device_count():
     return c.value
 
 
-def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_graph=True, solver=SOLVER_LDLT):
+def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_graph=True, solver=SOLVER_LDLT,
+           pcg_iters=None, pcg_tol=None):
     c = Config()
     lib().md_config_default(ctypes.byref(c))
     c.precision = precision
@@ -136,6 +137,10 @@ def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_gr
     c.refine_steps = refine_steps
     c.use_graph = 1 if use_graph else 0
     c.solver = solver
+    if pcg_iters is not None:
+        c.pcg_max_iters = int(pcg_iters)
+    if pcg_tol is not None:
+        c.pcg_tol = float(pcg_tol)
     return c
 
 

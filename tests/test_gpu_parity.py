@@ -228,3 +228,18 @@ def test_batch_equals_individual(ctx):
     b.iterate(4)
     for m, ref in zip(bm, outs):
         np.testing.assert_array_equal(m.download(), ref)
+
+
+# ---- Jacobi-PCG alternative ------------------------------------------------------------------
+
+@pytest.mark.parametrize("precision,tol", [(mdarap.PREC_FP64, 1e-7), (mdarap.PREC_FP32_REFINE, 1e-5)])
+def test_pcg_matches_golden(ctx, golden_dir, precision, tol):
+    say("case: pcg", precision)
+    g = gold(golden_dir, "uv_sphere")
+    m, ok = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], int(g["iters"]), precision=precision,
+                    solver=mdarap.SOLVER_PCG, pcg_iters=400, pcg_tol=1e-12 if precision == mdarap.PREC_FP64 else 1e-7,
+                    refine_steps=2)
+    assert ok
+    err = meshgen.relative_error(m.mesh, g["pos"])
+    say("  rel err", err)
+    assert err < tol
