@@ -124,15 +124,44 @@ __device__ __forceinline__ void jrot(T (&a)[4][4], T (&v)[4][4]) {
 // unit quaternion (x, y, z, w). This is the rotation V diag(1,1,det(V U^T)) U^T of the
 // reference's JacobiSVD step (arap.h:376-382), computed as the dominant eigenvector of
 // Horn's symmetric 4x4 matrix (J. Opt. Soc. Am. A 4(4), 1987) by cyclic Jacobi sweeps.
+//
+// The sweeps start from the basis of the previous rotation (p = previous quaternion): the
+// left-multiplication matrix L(p) is orthogonal with first column p, so L^T N L is already
+// nearly diagonal when the rotation changed little and one or two sweeps converge. The
+// converged eigenvector does not depend on the start.
 template <typename T>
-__device__ __forceinline__ void best_rotation(const T (&m)[9], T &qx, T &qy, T &qz, T &qw) {
+__device__ __forceinline__ void best_rotation(const T (&m)[9], T px, T py, T pz, T pw, T &qx, T &qy, T &qz,
+                                              T &qw) {
     const T Sxx = m[0], Sxy = m[1], Sxz = m[2], Syx = m[3], Syy = m[4], Syz = m[5], Szx = m[6],
             Szy = m[7], Szz = m[8];
-    T a[4][4] = {{Sxx + Syy + Szz, Syz - Szy, Szx - Sxz, Sxy - Syx},
-                 {Syz - Szy, Sxx - Syy - Szz, Sxy + Syx, Szx + Sxz},
-                 {Szx - Sxz, Sxy + Syx, -Sxx + Syy - Szz, Syz + Szy},
-                 {Sxy - Syx, Szx + Sxz, Syz + Szy, -Sxx - Syy + Szz}};
-    T v[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    const T N[4][4] = {{Sxx + Syy + Szz, Syz - Szy, Szx - Sxz, Sxy - Syx},
+                       {Syz - Szy, Sxx - Syy - Szz, Sxy + Syx, Szx + Sxz},
+                       {Szx - Sxz, Sxy + Syx, -Sxx + Syy - Szz, Syz + Szy},
+                       {Sxy - Syx, Szx + Sxz, Syz + Szy, -Sxx - Syy + Szz}};
+    {
+        T pn = px * px + py * py + pz * pz + pw * pw;
+        T inv = T(1) / tsqrt<T>(pn > T(0) ? pn : T(1));
+        if (!(pn > T(0))) { px = 0; py = 0; pz = 0; pw = 1; }
+        px *= inv; py *= inv; pz *= inv; pw *= inv;
+    }
+    // L(p) for p = (w, x, y, z) in the (scalar, vector) order of N's rows
+    T v[4][4] = {{pw, -px, -py, -pz}, {px, pw, -pz, py}, {py, pz, pw, -px}, {pz, -py, px, pw}};
+    T a[4][4];
+    {
+        T t[4][4];  // t = N L
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) t[r][c] = N[r][0] * v[0][c] + N[r][1] * v[1][c] + N[r][2] * v[2][c] + N[r][3] * v[3][c];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = r; c < 4; ++c) {
+                T x = v[0][r] * t[0][c] + v[1][r] * t[1][c] + v[2][r] * t[2][c] + v[3][r] * t[3][c];
+                a[r][c] = x;
+                a[c][r] = x;
+            }
+    }
     constexpr int kSweeps = sizeof(T) == 4 ? 6 : 10;
     for (int sw = 0; sw < kSweeps; ++sw) {
         T off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[0][3]) + fabs(a[1][2]) + fabs(a[1][3]) + fabs(a[2][3]);
@@ -192,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_local(LaunchSet L) {
         c[6] += az * ex; c[7] += az * ey; c[8] += az * ez;
     }
     T qx, qy, qz, qw;
-    best_rotation<T>(c, qx, qy, qz, qw);
+    V prev = reinterpret_cast<const V *>(M.quat)[iv];
+    best_rotation<T>(c, prev.x, prev.y, prev.z, prev.w, qx, qy, qz, qw);
     V out;
     out.x = qx; out.y = qy; out.z = qz; out.w = qw;
     reinterpret_cast<V *>(M.quat)[iv] = out;
