@@ -124,12 +124,12 @@ def served_leg(ctx, count, seconds, rate, iters, seed):
         m = mdarap.Mesh(ctx, pr.xyz, pr.tri,
                         mdarap.config(precision=mdarap.PREC_FP32_REFINE, rest_mode=mdarap.REST_FIXED))
         m.set_constraints(idx, pr.xyz[idx])
-        m.prepare(pr.xyz)
         meshes.append(m)
         origin = np.eye(4)
         origin[:3, 3] = pr.xyz[pr.handles].mean(axis=0)  # handles pivot about their centroid
         utils.append(tj.DeformationUtil(pr.xyz, pr.handles, origin=origin))
         trajs.append(tj.random_trajectory(rng))
+    mdarap.prepare_many(ctx, meshes)
     setup_s = time.perf_counter() - t0
     srv = mdarap.Server(ctx)
     ids = [srv.add_mesh(m) for m in meshes]
@@ -182,8 +182,8 @@ def batched_leg(ctx, count, iters, seed):
         idx, tgt = pr.constraints()
         m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
         m.set_constraints(idx, tgt)
-        m.prepare(pr.xyz)
         meshes.append(m)
+    mdarap.prepare_many(ctx, meshes)
     # plain: each mesh iterated alone (its own graph), back to back
     t0 = time.perf_counter()
     for m in meshes:

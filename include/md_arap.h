@@ -115,6 +115,10 @@ int md_mesh_deform(md_mesh *mesh, int iterations, double *xyz, int *ok);
  * deform() (positions from xyz) without iterating; iterate runs iterations on the device
  * state without any host copy; download copies the current positions (original order). */
 int md_mesh_prepare(md_mesh *mesh, const double *xyz);
+/* md_mesh_prepare for count meshes at once (positions xyz[k]): the host analysis and
+ * factorization of different meshes run on up to nthreads threads (0 = all cores, at most
+ * 16); device work stays on the context's stream. */
+int md_meshes_prepare(md_ctx *ctx, int count, md_mesh *const *meshes, const double *const *xyz, int nthreads);
 int md_mesh_iterate(md_mesh *mesh, int iterations);
 int md_mesh_download(md_mesh *mesh, double *xyz);
 

@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <atomic>
 #include <mutex>
+#include <thread>
 
 namespace md {
 
@@ -314,8 +316,8 @@ using Clock = std::chrono::steady_clock;
 
 // Host analysis for the current constrained set: ordering, topology, symbolic factor and
 // the device arena with every static array uploaded. Positions come from m->rest.
-int analyze(md_mesh *m) {
-    auto t0 = Clock::now();
+// Host part of the analysis: ordering, topology, free-free pattern, symbolic factor.
+void analyze_host(md_mesh *m) {
     int n = m->n;
     std::vector<int> perm = order_free_vertices(n, m->F, m->tri.data(), m->rest.data(), m->is_con);
     m->topo = Topology();
@@ -339,6 +341,13 @@ int analyze(md_mesh *m) {
     m->fac = Factor();
     if (nf > 0 && nf < n && m->cfg.solver == MD_SOLVER_LDLT) symbolic(nf, m->adj_ptr, m->adj, m->fac);
     else m->fac.n = nf;
+}
+
+// Device part of the analysis: the arena with every static array uploaded, and the plan.
+int analyze_device(md_mesh *m) {
+    const int n = m->n;
+    const Topology &t = m->topo;
+    const int nf = t.nf;
     const Factor &f = m->fac;
     // arena
     bool fp64 = m->fp64();
@@ -478,8 +487,14 @@ int analyze(md_mesh *m) {
         return MD_ERR_HIP;
     }
     m->analyzed = true;
-    m->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     return MD_OK;
+}
+
+int analyze(md_mesh *m) {
+    auto t0 = Clock::now();
+    analyze_host(m);
+    m->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    return analyze_device(m);
 }
 
 // Uploads positions (original order, 3 per vertex) as d4 in internal order.
@@ -522,25 +537,35 @@ hipError_t compute_cotan(md_mesh *m) {
 
 // FP64 numeric factorization on the host from the device cotan weights; uploads panels.
 // With the PCG solver only the Jacobi diagonal is formed (on the device).
-int factorize(md_mesh *m) {
-    if (m->cfg.solver == MD_SOLVER_PCG) {
-        const Plan &P = m->plan;
-        launch_pcg_diag(m->ctx->stream, P.set(P.off_resid), P.blk_resid, m->fp64());
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
-        if (e != hipSuccess) { set_error(std::string("PCG diagonal: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
-        m->factored = true;
-        return MD_OK;
-    }
-    auto t0 = Clock::now();
-    const Topology &t = m->topo;
-    int nf = t.nf;
-    int64_t slots = t.total_slots();
-    std::vector<double> w(slots);
-    if (hipMemcpy(w.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+int factor_pcg(md_mesh *m) {
+    const Plan &P = m->plan;
+    launch_pcg_diag(m->ctx->stream, P.set(P.off_resid), P.blk_resid, m->fp64());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
+    if (e != hipSuccess) { set_error(std::string("PCG diagonal: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    m->factored = true;
+    return MD_OK;
+}
+
+// Cotan weights of the device back to the host (input of the host factorization).
+int weights_download(md_mesh *m) {
+    int64_t slots = m->topo.total_slots();
+    m->w_host.resize(slots);
+    if (hipMemcpy(m->w_host.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("download of cotan weights failed");
         return MD_ERR_HIP;
     }
+    return MD_OK;
+}
+
+// FP64 numeric factorization on the host and the two device panel layouts (column-major
+// G_s for the forward sweep, row-major [Dinv_s; G_s] for the backward sweep), rounded to the
+// solve precision. Thread-safe across meshes. Returns false with err set on failure.
+bool factor_host(md_mesh *m, std::string &err) {
+    auto t0 = Clock::now();
+    const Topology &t = m->topo;
+    int nf = t.nf;
+    const std::vector<double> &w = m->w_host;
     std::vector<double> diag(nf, 0.0), val(m->adj.size());
     for (int i = 0; i < nf; ++i) {
         int s = i / kSlice, l = i % kSlice;
@@ -552,69 +577,95 @@ int factorize(md_mesh *m) {
         diag[i] = acc;
     }
     for (size_t k = 0; k < m->adj.size(); ++k) val[k] = -w[m->adj_slot[k]];
-    std::string err;
     if (!numeric(m->adj_ptr, m->adj, val, diag, m->fac, err)) {
-        set_error(err);
         m->factored = false;
-        return MD_ERR_FACTOR;
+        return false;
     }
-    // device layouts: column-major G_s for the forward sweep, row-major [Dinv_s; G_s] for the
-    // backward sweep
     const Factor &f = m->fac;
     size_t npc = (size_t)f.pc_off[f.ns], npr = (size_t)f.pr_off[f.ns];
-    std::vector<double> pc(npc), pr(npr);
+    size_t es = m->fp64() ? sizeof(double) : sizeof(float);
+    m->pc_host.resize(npc * es);
+    m->pr_host.resize(npr * es);
     for (int s = 0; s < f.ns; ++s) {
-        int w = f.w[s], no = f.noff[s], mm = w + no;
+        int ww = f.w[s], no = f.noff[s], mm = ww + no;
         const double *P = f.panel.data() + f.poff[s];
-        double *C = pc.data() + f.pc_off[s];
-        double *Rm = pr.data() + f.pr_off[s];
-        for (int q = 0; q < w; ++q) {
-            for (int r = 0; r < no; ++r) C[(size_t)q * no + r] = P[(size_t)q * mm + w + r];
-            for (int p = 0; p < mm; ++p) Rm[(size_t)p * w + q] = P[(size_t)q * mm + p];
+        for (int q = 0; q < ww; ++q) {
+            for (int r = 0; r < no; ++r) {
+                size_t k = f.pc_off[s] + (size_t)q * no + r;
+                double v = P[(size_t)q * mm + ww + r];
+                if (m->fp64()) reinterpret_cast<double *>(m->pc_host.data())[k] = v;
+                else reinterpret_cast<float *>(m->pc_host.data())[k] = (float)v;
+            }
+            for (int p = 0; p < mm; ++p) {
+                size_t k = f.pr_off[s] + (size_t)p * ww + q;
+                double v = P[(size_t)q * mm + p];
+                if (m->fp64()) reinterpret_cast<double *>(m->pr_host.data())[k] = v;
+                else reinterpret_cast<float *>(m->pr_host.data())[k] = (float)v;
+            }
         }
     }
-    hipError_t e;
-    if (m->fp64()) {
-        e = hipMemcpy(const_cast<void *>(m->dev.pc), pc.data(), sizeof(double) * npc, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(const_cast<void *>(m->dev.pr), pr.data(), sizeof(double) * npr, hipMemcpyHostToDevice);
-    } else {
-        std::vector<float> c32(npc), r32(npr);
-        for (size_t k = 0; k < npc; ++k) c32[k] = (float)pc[k];
-        for (size_t k = 0; k < npr; ++k) r32[k] = (float)pr[k];
-        e = hipMemcpy(const_cast<void *>(m->dev.pc), c32.data(), sizeof(float) * npc, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(const_cast<void *>(m->dev.pr), r32.data(), sizeof(float) * npr, hipMemcpyHostToDevice);
-    }
+    m->fac.panel.clear();
+    m->fac.panel.shrink_to_fit();
+    m->setup_ms_host += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    return true;
+}
+
+int factor_upload(md_mesh *m) {
+    hipError_t e = hipMemcpy(const_cast<void *>(m->dev.pc), m->pc_host.data(), m->pc_host.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(const_cast<void *>(m->dev.pr), m->pr_host.data(), m->pr_host.size(), hipMemcpyHostToDevice);
+    std::vector<char>().swap(m->pc_host);
+    std::vector<char>().swap(m->pr_host);
     if (e != hipSuccess) {
         set_error("panel upload failed");
         return MD_ERR_HIP;
     }
     m->factored = true;
-    m->setup_ms_host += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     return MD_OK;
 }
 
-// The dirty branch of deform (arap.h:102-120). Returns MD_OK with *early = true when no
-// vertex is constrained (arap.h:113-114).
-int setup(md_mesh *m, const double *xyz, bool *early) {
-    *early = false;
-    bool full = m->cfg.rest_mode == MD_REST_REFERENCE || !m->initialized;
-    if (full) m->rest.assign(xyz, xyz + 3 * (size_t)m->n);
-    bool need_analysis = !m->analyzed || m->con_set_changed;
-    if (need_analysis) {
-        m->factored = false;
-        int rc = analyze(m);
+int factorize(md_mesh *m) {
+    if (m->cfg.solver == MD_SOLVER_PCG) return factor_pcg(m);
+    int rc = weights_download(m);
+    if (rc) return rc;
+    std::string err;
+    if (!factor_host(m, err)) {
+        set_error(err);
+        return MD_ERR_FACTOR;
+    }
+    return factor_upload(m);
+}
+
+// The dirty branch of deform (arap.h:102-120), in phases so that the host work of many
+// meshes can run in parallel (md_meshes_prepare) while device work stays on one stream.
+struct Prep {
+    const double *xyz = nullptr;
+    bool full = false, need_analysis = false, need_factor = false, early = false;
+};
+
+// Decisions and the rest pose (arap.h:102-108,162-168).
+void prep_begin(md_mesh *m, const double *xyz, Prep &p) {
+    p.xyz = xyz;
+    p.full = m->cfg.rest_mode == MD_REST_REFERENCE || !m->initialized;
+    if (p.full) m->rest.assign(xyz, xyz + 3 * (size_t)m->n);
+    p.need_analysis = !m->analyzed || m->con_set_changed;
+    if (p.need_analysis) m->factored = false;
+}
+
+// Device work before the factorization: arena, weights, initial guess, identity rotations.
+int prep_device_a(md_mesh *m, Prep &p) {
+    if (p.need_analysis) {
+        int rc = analyze_device(m);
         if (rc) return rc;
     }
     hipError_t e;
-    if (full || need_analysis) {
+    if (p.full || p.need_analysis) {
         e = upload_positions(m, m->rest.data(), m->dev.p);
         if (e == hipSuccess) e = compute_cotan(m);
         if (e != hipSuccess) { set_error(std::string("cotan weights: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     }
     // initial guess: the current mesh with the targets pasted (arap.h:162-168, 277-281)
-    std::vector<double> init(xyz, xyz + 3 * (size_t)m->n);
+    std::vector<double> init(p.xyz, p.xyz + 3 * (size_t)m->n);
     for (int v = 0; v < m->n; ++v)
         if (m->is_con[v]) std::memcpy(&init[3 * (size_t)v], &m->con[3 * (size_t)v], 3 * sizeof(double));
     e = upload_positions(m, init.data(), m->dev.pp);
@@ -623,14 +674,56 @@ int setup(md_mesh *m, const double *xyz, bool *early) {
     launch_identity_quat(m->ctx->stream, m->plan.set(m->plan.off_ident), m->plan.blk_ident, m->fp64());
     m->initialized = true;
     m->iterated = false;
-    if (m->ncon == 0) { *early = true; return MD_OK; }
-    if (m->topo.nf > 0 && (full || need_analysis || !m->factored)) {
-        int rc = factorize(m);
+    if (m->ncon == 0) { p.early = true; return MD_OK; }  // arap.h:113-114
+    p.need_factor = m->topo.nf > 0 && (p.full || p.need_analysis || !m->factored);
+    if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT) return weights_download(m);
+    return MD_OK;
+}
+
+// Device work after the factorization.
+int prep_device_b(md_mesh *m, Prep &p) {
+    if (p.need_factor) {
+        int rc = m->cfg.solver == MD_SOLVER_PCG ? factor_pcg(m) : factor_upload(m);
         if (rc) return rc;
     }
     m->dirty = false;
     m->con_set_changed = false;
     return MD_OK;
+}
+
+// Returns MD_OK with *early = true when no vertex is constrained (arap.h:113-114).
+int setup(md_mesh *m, const double *xyz, bool *early) {
+    Prep p;
+    prep_begin(m, xyz, p);
+    if (p.need_analysis) {
+        auto t0 = Clock::now();
+        analyze_host(m);
+        m->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    }
+    int rc = prep_device_a(m, p);
+    *early = p.early;
+    if (rc || p.early) return rc;
+    if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT) {
+        std::string err;
+        if (!factor_host(m, err)) { set_error(err); return MD_ERR_FACTOR; }
+    }
+    return prep_device_b(m, p);
+}
+
+// Runs fn(k) for k in [0, count) on up to nthreads threads.
+template <class Fn> void parallel_for(int count, int nthreads, Fn fn) {
+    int nt = std::max(1, std::min(nthreads, count));
+    if (nt == 1) {
+        for (int k = 0; k < count; ++k) fn(k);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (int k = next++; k < count; k = next++) fn(k);
+        });
+    for (auto &x : th) x.join();
 }
 
 int iterate(md_mesh *m, int iterations) {
@@ -652,6 +745,59 @@ extern "C" int md_mesh_prepare(md_mesh *m, const double *xyz) {
     int rc = setup(m, xyz, &early);
     if (rc) return rc;
     if (early) { set_error("md_mesh_prepare: no constrained vertex"); return MD_ERR_STATE; }
+    return MD_OK;
+}
+
+extern "C" int md_meshes_prepare(md_ctx *ctx, int count, md_mesh *const *meshes, const double *const *xyz,
+                                 int nthreads) {
+    if (!ctx || count < 0 || (count > 0 && (!meshes || !xyz))) {
+        set_error("md_meshes_prepare: invalid argument");
+        return MD_ERR_INVALID;
+    }
+    for (int k = 0; k < count; ++k)
+        if (!meshes[k] || meshes[k]->ctx != ctx || !xyz[k]) {
+            set_error("md_meshes_prepare: mesh " + std::to_string(k) + " is null or from another context");
+            return MD_ERR_INVALID;
+        }
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    nthreads = std::min(nthreads, 16);
+    hipSetDevice(ctx->device);
+    std::vector<md_mesh *> todo;
+    std::vector<Prep> preps;
+    for (int k = 0; k < count; ++k)
+        if (meshes[k]->dirty) {
+            todo.push_back(meshes[k]);
+            preps.emplace_back();
+            prep_begin(meshes[k], xyz[k], preps.back());
+        }
+    int nt = (int)todo.size();
+    parallel_for(nt, nthreads, [&](int k) {
+        if (preps[k].need_analysis) {
+            auto t0 = Clock::now();
+            analyze_host(todo[k]);
+            todo[k]->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+        }
+    });
+    for (int k = 0; k < nt; ++k) {
+        int rc = prep_device_a(todo[k], preps[k]);
+        if (rc) return rc;
+        if (preps[k].early) {
+            set_error("md_meshes_prepare: mesh without constrained vertices");
+            return MD_ERR_STATE;
+        }
+    }
+    std::vector<std::string> errs(nt);
+    std::vector<char> bad(nt, 0);
+    parallel_for(nt, nthreads, [&](int k) {
+        if (preps[k].need_factor && todo[k]->cfg.solver == MD_SOLVER_LDLT)
+            bad[k] = !factor_host(todo[k], errs[k]);
+    });
+    for (int k = 0; k < nt; ++k)
+        if (bad[k]) { set_error(errs[k]); return MD_ERR_FACTOR; }
+    for (int k = 0; k < nt; ++k) {
+        int rc = prep_device_b(todo[k], preps[k]);
+        if (rc) return rc;
+    }
     return MD_OK;
 }
 

@@ -102,6 +102,8 @@ struct md_mesh {
     std::vector<int64_t> adj_slot; // slot of each adj entry
     std::vector<int64_t> diag_slot_ptr; // unused placeholder for symmetry with adj
     double setup_ms_host = 0.0;
+    std::vector<double> w_host;    // cotan weights on the host (factorization input)
+    std::vector<char> pc_host, pr_host;  // device panel layouts staged on the host
     int *d_tgt_idx = nullptr;      // staging for moved targets (ncon entries)
     double *d_tgt_xyz = nullptr;
     bool fp64() const { return cfg.precision == MD_PREC_FP64; }

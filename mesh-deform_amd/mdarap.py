@@ -72,6 +72,7 @@ _SIGS = [
     ("md_mesh_set_constraints", ctypes.c_int, [_vp, ctypes.c_int, _i32p, _f64p]),
     ("md_mesh_deform", ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.POINTER(ctypes.c_int)]),
     ("md_mesh_prepare", ctypes.c_int, [_vp, _f64p]),
+    ("md_meshes_prepare", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int]),
     ("md_mesh_iterate", ctypes.c_int, [_vp, ctypes.c_int]),
     ("md_mesh_download", ctypes.c_int, [_vp, _f64p]),
     ("md_mesh_cotan_nnz", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
@@ -258,6 +259,15 @@ class Mesh:
         ph = np.zeros(4, np.float64)
         check(lib().md_mesh_time_iterations(self._h, int(warmup), int(iters), ctypes.byref(tot), ph))
         return tot.value, ph
+
+
+def prepare_many(ctx, meshes, positions=None, nthreads=0):
+    """md_meshes_prepare: the dirty setup of many meshes, host work on parallel threads."""
+    bufs = [np.ascontiguousarray(m.mesh if positions is None else positions[k], np.float64).ravel().copy()
+            for k, m in enumerate(meshes)]
+    hs = (_vp * len(meshes))(*[m.handle for m in meshes])
+    ps = (_vp * len(meshes))(*[b.ctypes.data for b in bufs])
+    check(lib().md_meshes_prepare(ctx.handle, len(meshes), hs, ps, int(nthreads)))
 
 
 class Batch:

@@ -132,3 +132,26 @@ def test_server_max_active_serialises(ctx):
         srv.close()
     for a, b in zip(*res):
         assert np.array_equal(a, b)
+
+
+def test_prepare_many_equals_prepare(ctx):
+    say("case: prepare_many")
+    probs = meshgen.mixed_batch(5, 500, 5000, seed=21)
+    outs = []
+    for parallel in (False, True):
+        meshes = []
+        for pr in probs:
+            idx, tgt = pr.constraints()
+            m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+            m.set_constraints(idx, tgt)
+            meshes.append(m)
+        if parallel:
+            mdarap.prepare_many(ctx, meshes, nthreads=4)
+        else:
+            for m in meshes:
+                m.prepare()
+        for m in meshes:
+            m.iterate(3)
+        outs.append([m.download() for m in meshes])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
