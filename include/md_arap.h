@@ -129,6 +129,10 @@ int md_mesh_cotan_csr(md_mesh *mesh, int *rowptr, int *col, double *val);
 
 /* Rotations of the last local step, row-major 3x3 per vertex in original order. */
 int md_mesh_rotations(md_mesh *mesh, double *R);
+/* Unit vertex normals of the current positions (3n, original order): the normalised sum of
+ * the incident faces' area-weighted normals, for rendering clients that update normals after
+ * every deform (examples/osg_viewer.cpp:48). */
+int md_mesh_normals(md_mesh *mesh, double *nrm);
 /* ARAP energy sum_i sum_j w_ij |(p'_i - p'_j) - R_i (p_i - p_j)|^2 of the current state. */
 int md_mesh_energy(md_mesh *mesh, double *energy);
 int md_mesh_get_stats(md_mesh *mesh, md_mesh_stats *out);

@@ -80,6 +80,7 @@ void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k);
+void launch_normals(hipStream_t s, LaunchSet Lf, int blocks_face, LaunchSet Ls, int blocks_slice, double *fn, d4 *out);
 void launch_pcg_diag(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_pcg_init(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64);

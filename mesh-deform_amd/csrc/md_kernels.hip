@@ -663,6 +663,50 @@ __global__ __launch_bounds__(kBlock) void k_pcg_apply(LaunchSet L) {
     M.pp[i] = pv;
 }
 
+// ---- vertex normals of the deformed mesh (for rendering clients) --------------------------
+
+// Area-weighted face normal of every face from p' (cross product of two edges, unnormalised),
+// three doubles per face into fn.
+__global__ __launch_bounds__(kBlock) void k_face_normals(LaunchSet L, double *fn) {
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int f = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
+    if (f >= M.F) return;
+    d4 a = M.pp[M.tri[3 * f]], b = M.pp[M.tri[3 * f + 1]], c = M.pp[M.tri[3 * f + 2]];
+    double ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
+    double vx = c.x - a.x, vy = c.y - a.y, vz = c.z - a.z;
+    fn[3 * f] = uy * vz - uz * vy;
+    fn[3 * f + 1] = uz * vx - ux * vz;
+    fn[3 * f + 2] = ux * vy - uy * vx;
+}
+
+// Per vertex: every incident face appears on exactly two of its one-ring slots (the two face
+// edges at the vertex), so half the sum over the slots' face lists is the sum of incident
+// face normals; normalised (zero stays zero). Output in internal order.
+__global__ __launch_bounds__(kBlock) void k_vertex_normals(LaunchSet L, const double *fn, d4 *out) {
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (slice >= M.nslices) return;
+    int iv = slice * kSlice + lane;
+    if (iv >= M.n) return;
+    int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
+    double x = 0, y = 0, z = 0;
+    for (int q = 0; q < wd; ++q) {
+        long long g = (long long)(s0 + q) * kSlice + lane;
+        for (int e = M.slot_fe_ptr[g]; e < M.slot_fe_ptr[g + 1]; ++e) {
+            int f = M.slot_fe[e] / 3;
+            x += fn[3 * f]; y += fn[3 * f + 1]; z += fn[3 * f + 2];
+        }
+    }
+    double nrm = sqrt(x * x + y * y + z * z);
+    double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    d4 o;
+    o.x = x * inv; o.y = y * inv; o.z = z * inv; o.w = 0.0;
+    out[iv] = o;
+}
+
 // ---- ARAP energy -------------------------------------------------------------------------
 
 // E = sum_i sum_j w_ij |(p'_i - p'_j) - R_i (p_i - p_j)|^2 over all vertices, in FP64; one
@@ -733,6 +777,11 @@ void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bo
     MD_LAUNCH_T(k_pcg_direction, blocks, s, L, parity);
 }
 void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_apply, blocks, s, L); }
+
+void launch_normals(hipStream_t s, LaunchSet Lf, int blocks_face, LaunchSet Ls, int blocks_slice, double *fn, d4 *out) {
+    if (blocks_face > 0) hipLaunchKernelGGL(k_face_normals, dim3(blocks_face), dim3(kBlock), 0, s, Lf, fn);
+    if (blocks_slice > 0) hipLaunchKernelGGL(k_vertex_normals, dim3(blocks_slice), dim3(kBlock), 0, s, Ls, (const double *)fn, out);
+}
 
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {
     if (k > 0) hipLaunchKernelGGL(k_scatter_targets, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pp, iv, xyz, k);

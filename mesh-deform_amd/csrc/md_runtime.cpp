@@ -908,6 +908,34 @@ extern "C" int md_mesh_rotations(md_mesh *m, double *R) {
     return MD_OK;
 }
 
+extern "C" int md_mesh_normals(md_mesh *m, double *nrm) {
+    if (!m || !nrm) { set_error("md_mesh_normals: null pointer"); return MD_ERR_INVALID; }
+    if (!m->analyzed) { set_error("md_mesh_normals: call deform first"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    hipStream_t st = m->ctx->stream;
+    double *fn = nullptr;
+    d4 *out = nullptr;
+    MD_HIP(hipMalloc((void **)&fn, sizeof(double) * 3 * std::max(1, m->F)));
+    hipError_t e = hipMalloc((void **)&out, sizeof(d4) * std::max(1, m->n));
+    if (e == hipSuccess) {
+        const Plan &P = m->plan;
+        launch_normals(st, P.set(P.off_face), P.blk_face, P.set(P.off_slice), P.blk_slice, fn, out);
+        e = hipGetLastError();
+    }
+    std::vector<d4> h(m->n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), out, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipStreamSynchronize(st);
+    if (fn) hipFree(fn);
+    if (out) hipFree(out);
+    if (e != hipSuccess) { set_error(std::string("md_mesh_normals: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    for (int iv = 0; iv < m->n; ++iv) {
+        double *o = nrm + 3 * (size_t)m->topo.original_of[iv];
+        o[0] = h[iv].x; o[1] = h[iv].y; o[2] = h[iv].z;
+    }
+    return MD_OK;
+}
+
 extern "C" int md_mesh_energy(md_mesh *m, double *energy) {
     if (!m || !energy) { set_error("md_mesh_energy: null pointer"); return MD_ERR_INVALID; }
     if (!m->analyzed) { set_error("md_mesh_energy: call deform first"); return MD_ERR_STATE; }

@@ -78,6 +78,7 @@ _SIGS = [
     ("md_mesh_cotan_nnz", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     ("md_mesh_cotan_csr", ctypes.c_int, [_vp, _i32p, _i32p, _f64p]),
     ("md_mesh_rotations", ctypes.c_int, [_vp, _f64p]),
+    ("md_mesh_normals", ctypes.c_int, [_vp, _f64p]),
     ("md_mesh_energy", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     ("md_mesh_get_stats", ctypes.c_int, [_vp, ctypes.POINTER(MeshStats)]),
     ("md_mesh_time_iterations", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -243,6 +244,11 @@ class Mesh:
         R = np.zeros(9 * self.n, np.float64)
         check(lib().md_mesh_rotations(self._h, R))
         return R.reshape(self.n, 3, 3)
+
+    def normals(self):
+        out = np.zeros(3 * self.n, np.float64)
+        check(lib().md_mesh_normals(self._h, out))
+        return out.reshape(-1, 3)
 
     def energy(self):
         e = ctypes.c_double(0)

@@ -243,3 +243,23 @@ def test_pcg_matches_golden(ctx, golden_dir, precision, tol):
     err = meshgen.relative_error(m.mesh, g["pos"])
     say("  rel err", err)
     assert err < tol
+
+
+def test_vertex_normals(ctx):
+    """Normals of the deformed mesh: unit, radial on an undeformed sphere, and equal to the
+    normalised sum of incident area-weighted face normals computed independently."""
+    say("case: normals")
+    xyz, tri = meshgen.icosphere(3)
+    m = mdarap.Mesh(ctx, xyz, tri)
+    m.set_constraint(0, xyz[0])
+    assert m.deform(1)  # one constraint at rest: the sphere stays put
+    nrm = m.normals()
+    np.testing.assert_allclose(np.linalg.norm(nrm, axis=1), 1.0, rtol=1e-12)
+    assert np.min(np.einsum("ij,ij->i", nrm, xyz / np.linalg.norm(xyz, axis=1, keepdims=True))) > 0.995
+    pos = m.mesh
+    fn = np.cross(pos[tri[:, 1]] - pos[tri[:, 0]], pos[tri[:, 2]] - pos[tri[:, 0]])
+    acc = np.zeros_like(pos)
+    for k in range(3):
+        np.add.at(acc, tri[:, k], fn)
+    acc /= np.linalg.norm(acc, axis=1, keepdims=True)
+    np.testing.assert_allclose(nrm, acc, atol=1e-12)
