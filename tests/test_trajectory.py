@@ -104,3 +104,30 @@ def test_deformation_util_conjugates_by_origin():
     np.testing.assert_allclose(du.targets(T), exp, atol=1e-12)
     # identity motion keeps the handles in place
     np.testing.assert_allclose(du.targets(np.eye(4)), xyz[handles], atol=1e-12)
+
+
+def test_cpp_header_matches_python_restatement():
+    """include/deform/trajectory.h and trajectory.py are independent restatements of
+    trajectory.h:61-73; on the reference's four poses they agree, and the C++ float path
+    passes the reference's endpoint test."""
+    import os
+    import subprocess
+
+    cpp = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp")
+    subprocess.check_call(["make", "-s", "-C", cpp, "test_trajectory"])
+    out = subprocess.run([os.path.join(cpp, "test_trajectory")], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.strip().splitlines()
+    assert lines[0] == "endpoints 1 1"
+    path = tj.TrajectorySE3()
+    pose0 = np.eye(4)
+    path.add_key_pose(pose0)
+    pose1 = translation(0, 0, 1) @ pose0
+    path.add_key_pose(pose1)
+    pose2 = translation(0, 0, 1) @ rotation_x(math.pi / 4) @ pose1
+    path.add_key_pose(pose2)
+    path.add_key_pose(translation(0, 0, 1) @ pose2)
+    for line in lines[1:]:
+        vals = [float(v) for v in line.split()[1:]]
+        t, T = vals[0], np.array(vals[1:]).reshape(4, 4)
+        np.testing.assert_allclose(T, path(t), atol=1e-12)
