@@ -20,6 +20,16 @@ thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
 
+void ctx_retain(md_ctx *c) { c->refs.fetch_add(1); }
+
+void ctx_release(md_ctx *c) {
+    if (c->refs.fetch_sub(1) != 1) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
 #define MD_HIP(call)                                                                    \
     do {                                                                                \
         hipError_t e_ = (call);                                                         \
@@ -219,10 +229,7 @@ extern "C" int md_ctx_create(int device, md_ctx **out) {
 
 extern "C" int md_ctx_destroy(md_ctx *ctx) {
     if (!ctx) return MD_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_release(ctx);
     return MD_OK;
 }
 
@@ -270,15 +277,18 @@ extern "C" int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const m
     m->tri.assign(tri, tri + 3 * (size_t)F);
     m->is_con.assign(n, 0);
     m->con.assign(3 * (size_t)n, 0.0);
+    ctx_retain(ctx);
     *out = m.release();
     return MD_OK;
 }
 
 extern "C" int md_mesh_destroy(md_mesh *m) {
     if (!m) return MD_OK;
-    hipSetDevice(m->ctx->device);
-    hipStreamSynchronize(m->ctx->stream);
+    md_ctx *ctx = m->ctx;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     delete m;
+    ctx_release(ctx);
     return MD_OK;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -80,7 +81,15 @@ struct md_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string arch;
+    // the caller's handle plus one reference per mesh, batch and server created from it:
+    // the stream is destroyed with the last holder, whatever the order of destruction
+    std::atomic<int> refs{1};
 };
+
+namespace md {
+void ctx_retain(md_ctx *c);
+void ctx_release(md_ctx *c);
+}  // namespace md
 
 struct md_mesh {
     md_ctx *ctx = nullptr;

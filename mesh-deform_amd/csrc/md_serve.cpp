@@ -52,6 +52,7 @@ extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, m
     hipSetDevice(ctx->device);
     hipError_t e = build_plan(b->plan, devs, facs, prec == MD_PREC_FP64, refine, ctx->stream);
     if (e != hipSuccess) { set_error(std::string("md_batch_create: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    ctx_retain(ctx);
     *out = b.release();
     return MD_OK;
 }
@@ -90,9 +91,11 @@ extern "C" int md_batch_time_iterations(md_batch *b, int warmup, int iterations,
 
 extern "C" int md_batch_destroy(md_batch *b) {
     if (!b) return MD_OK;
-    hipSetDevice(b->ctx->device);
-    hipStreamSynchronize(b->ctx->stream);
+    md_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     delete b;
+    ctx_release(ctx);
     return MD_OK;
 }
 
@@ -270,6 +273,7 @@ extern "C" int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_ser
     std::unique_ptr<md_server> S(new md_server());
     S->ctx = ctx;
     if (cfg) S->cfg = *cfg;
+    ctx_retain(ctx);
     S->worker = std::thread(server_loop, S.get());
     *out = S.release();
     return MD_OK;
@@ -372,8 +376,10 @@ extern "C" int md_server_destroy(md_server *S) {
     }
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
-    hipSetDevice(S->ctx->device);
-    hipStreamSynchronize(S->ctx->stream);
+    md_ctx *ctx = S->ctx;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     delete S;
+    ctx_release(ctx);
     return MD_OK;
 }

@@ -263,3 +263,27 @@ def test_vertex_normals(ctx):
         np.add.at(acc, tri[:, k], fn)
     acc /= np.linalg.norm(acc, axis=1, keepdims=True)
     np.testing.assert_allclose(nrm, acc, atol=1e-12)
+
+
+def test_context_outlives_its_meshes_regression():
+    """A context destroyed before its meshes, batches or servers stays valid until the last
+    of them is destroyed (each holds a reference)."""
+    say("case: context lifetime")
+    if mdarap.device_count() == 0:
+        pytest.fail("no GPU visible")
+    c = mdarap.Context(0)
+    pr = meshgen.sphere_problem(6, 8)
+    idx, tgt = pr.constraints()
+    m = mdarap.Mesh(c, pr.xyz, pr.tri, mdarap.config(rest_mode=mdarap.REST_FIXED))
+    m.set_constraints(idx, tgt)
+    m.prepare()
+    b = mdarap.Batch(c, [m])
+    s = mdarap.Server(c)
+    s.add_mesh(m)
+    c.close()          # the caller's handle goes first
+    b.iterate(1)       # the objects still work
+    s.close()
+    b.close()
+    m.iterate(1)
+    assert np.all(np.isfinite(m.download()))
+    m.close()
