@@ -210,15 +210,32 @@ __global__ __launch_bounds__(kBlock) void k_local(LaunchSet L) {
     int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
     d4 pi = M.pp[iv];
     T c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int q = 0; q < wd; ++q) {
-        long long g = (long long)(s0 + q) * kSlice + lane;
-        int j = M.sell_col[g];
-        T ax = A[g], ay = A[M.slots + g], az = A[2 * M.slots + g];
-        d4 pj = M.pp[j];
-        T ex = (T)(pi.x - pj.x), ey = (T)(pi.y - pj.y), ez = (T)(pi.z - pj.z);
-        c[0] += ax * ex; c[1] += ax * ey; c[2] += ax * ez;
-        c[3] += ay * ex; c[4] += ay * ey; c[5] += ay * ez;
-        c[6] += az * ex; c[7] += az * ey; c[8] += az * ez;
+    // 8 slots at a time: ids and edges first, then the neighbour gathers, then arithmetic
+    // (slots past the slice width are clamped; they are padding with zero edges anyway)
+    constexpr int kU = 8;
+    for (int q0 = 0; q0 < wd; q0 += kU) {
+        int jj[kU];
+        T ex_[kU], ey_[kU], ez_[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            long long g = (long long)(s0 + min(q0 + u, wd - 1)) * kSlice + lane;
+            jj[u] = M.sell_col[g];
+            bool ok = q0 + u < wd;
+            ex_[u] = ok ? A[g] : T(0);
+            ey_[u] = ok ? A[M.slots + g] : T(0);
+            ez_[u] = ok ? A[2 * M.slots + g] : T(0);
+        }
+        d4 pj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) pj[u] = M.pp[jj[u]];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            T ax = ex_[u], ay = ey_[u], az = ez_[u];
+            T ex = (T)(pi.x - pj[u].x), ey = (T)(pi.y - pj[u].y), ez = (T)(pi.z - pj[u].z);
+            c[0] += ax * ex; c[1] += ax * ey; c[2] += ax * ez;
+            c[3] += ay * ex; c[4] += ay * ey; c[5] += ay * ez;
+            c[6] += az * ex; c[7] += az * ey; c[8] += az * ez;
+        }
     }
     T qx, qy, qz, qw;
     V prev = reinterpret_cast<const V *>(M.quat)[iv];
@@ -266,20 +283,37 @@ __global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
     T sax = 0, say = 0, saz = 0;   // sum_j a_ij
     T rx = 0, ry = 0, rz = 0;      // sum_j R_j a_ij
     double lx = 0, ly = 0, lz = 0; // sum_j w_ij (p'_i - p'_j)
-    for (int q = 0; q < wd; ++q) {
-        long long g = (long long)(s0 + q) * kSlice + lane;
-        int j = M.sell_col[g];
-        T ax = A[g], ay = A[M.slots + g], az = A[2 * M.slots + g];
-        double w = M.w64[g];
-        V qj = Q[j];
-        d4 pj = M.pp[j];
-        T Rj[9];
-        quat_to_mat<T>(qj.x, qj.y, qj.z, qj.w, Rj);
-        sax += ax; say += ay; saz += az;
-        rx += Rj[0] * ax + Rj[1] * ay + Rj[2] * az;
-        ry += Rj[3] * ax + Rj[4] * ay + Rj[5] * az;
-        rz += Rj[6] * ax + Rj[7] * ay + Rj[8] * az;
-        lx += w * (pi.x - pj.x); ly += w * (pi.y - pj.y); lz += w * (pi.z - pj.z);
+    constexpr int kU = 8;
+    for (int q0 = 0; q0 < wd; q0 += kU) {
+        int jj[kU];
+        T ax_[kU], ay_[kU], az_[kU];
+        double w_[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            long long g = (long long)(s0 + min(q0 + u, wd - 1)) * kSlice + lane;
+            jj[u] = M.sell_col[g];
+            bool ok = q0 + u < wd;
+            ax_[u] = ok ? A[g] : T(0);
+            ay_[u] = ok ? A[M.slots + g] : T(0);
+            az_[u] = ok ? A[2 * M.slots + g] : T(0);
+            w_[u] = ok ? M.w64[g] : 0.0;
+        }
+        V qj_[kU];
+        d4 pj_[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) { qj_[u] = Q[jj[u]]; pj_[u] = M.pp[jj[u]]; }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            T ax = ax_[u], ay = ay_[u], az = az_[u];
+            T Rj[9];
+            quat_to_mat<T>(qj_[u].x, qj_[u].y, qj_[u].z, qj_[u].w, Rj);
+            sax += ax; say += ay; saz += az;
+            rx += Rj[0] * ax + Rj[1] * ay + Rj[2] * az;
+            ry += Rj[3] * ax + Rj[4] * ay + Rj[5] * az;
+            rz += Rj[6] * ax + Rj[7] * ay + Rj[8] * az;
+            double w = w_[u];
+            lx += w * (pi.x - pj_[u].x); ly += w * (pi.y - pj_[u].y); lz += w * (pi.z - pj_[u].z);
+        }
     }
     T Ri[9];
     quat_to_mat<T>(qi.x, qi.y, qi.z, qi.w, Ri);
@@ -300,15 +334,45 @@ template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
     return reinterpret_cast<const T *>(p);
 }
 
-// Sum of the kc partial update vectors of a child at entry e (fixed order).
+// Sum of cnt partial vectors U[k * stride + e], k = 0 .. cnt-1, in that order. The loads are
+// issued 16 at a time (clamped addresses, masked values) so their latencies overlap.
 template <typename V>
 __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e) {
-    V a = U[e];
-    for (int k = 1; k < cnt; ++k) {
-        V b = U[(long long)k * stride + e];
-        a.x += b.x; a.y += b.y; a.z += b.z;
+    using T = decltype(V{}.x);
+    V a;
+    a.x = 0; a.y = 0; a.z = 0; a.w = 0;
+    for (int k0 = 0; k0 < cnt; k0 += 16) {
+        V b[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) b[u] = U[(long long)min(k0 + u, cnt - 1) * stride + e];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            T m = (k0 + u < cnt) ? T(1) : T(0);
+            a.x += m * b[u].x; a.y += m * b[u].y; a.z += m * b[u].z;
+        }
     }
     return a;
+}
+
+// acc += sum_{i < n} P[i * stride] * v[i] for n <= N (N = 16, 32 or 64): all N loads are
+// issued together (clamped addresses, masked values).
+template <int N, typename T, typename V>
+__device__ __forceinline__ void tile_dot(const T *P, long long stride, const V *v, int n, T &ax, T &ay, T &az) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        int ii = min(i, n - 1);
+        T g = P[(long long)ii * stride];
+        g = i < n ? g : T(0);
+        V x = v[ii];
+        ax += g * x.x; ay += g * x.y; az += g * x.z;
+    }
+}
+
+template <typename T, typename V>
+__device__ __forceinline__ void tile_dot_any(const T *P, long long stride, const V *v, int n, T &ax, T &ay, T &az) {
+    if (n <= 16) tile_dot<16>(P, stride, v, n, ax, ay, az);
+    else if (n <= 32) tile_dot<32>(P, stride, v, n, ax, ay, az);
+    else tile_dot<64>(P, stride, v, n, ax, ay, az);
 }
 
 // Forward tile (s, r0, q0), one wavefront:
@@ -321,8 +385,8 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
 template <typename T>
 __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     using V = typename V4T<T>::type;
-    __shared__ T vd[3][kTile];
-    __shared__ T vo[3][kTile];
+    __shared__ V vd[kTile];
+    __shared__ V vo[kTile];
     int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
     const int4 tl = M.ft[M.ft_ptr[level] + (blockIdx.x - L.prefix[k])];
@@ -336,8 +400,11 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     {
         V b;
         if (lane < qn) b = R[S.c0 + q0 + lane]; else { b.x = 0; b.y = 0; b.z = 0; }
-        vd[0][lane] = b.x; vd[1][lane] = b.y; vd[2][lane] = b.z;
-        vo[0][lane] = 0; vo[1][lane] = 0; vo[2][lane] = 0;
+        b.w = 0;
+        vd[lane] = b;
+        V zz;
+        zz.x = 0; zz.y = 0; zz.z = 0; zz.w = 0;
+        vo[lane] = zz;
     }
     __syncthreads();
     const int lo = w + r0, hi = w + r0 + rn;
@@ -351,7 +418,8 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
             int pos = rel[e];
             if (pos >= q0 + qn) continue;  // the block's tail past column w holds off rows
             V u = sum_partials<V>(Uc, C.kc, C.noff, e);
-            vd[0][pos - q0] += u.x; vd[1][pos - q0] += u.y; vd[2][pos - q0] += u.z;
+            V &d = vd[pos - q0];
+            d.x += u.x; d.y += u.y; d.z += u.z;
         }
         if (kcb == 0 && rn > 0) {
             e0 = bp[lo / kTile];
@@ -360,34 +428,24 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
                 int pos = rel[e];
                 if (pos >= lo && pos < hi) {
                     V u = sum_partials<V>(Uc, C.kc, C.noff, e);
-                    vo[0][pos - lo] += u.x; vo[1][pos - lo] += u.y; vo[2][pos - lo] += u.z;
+                    V &d = vo[pos - lo];
+                    d.x += u.x; d.y += u.y; d.z += u.z;
                 }
             }
         }
         __syncthreads();
     }
-    if (r0 == 0 && lane < qn) {
-        V zz;
-        zz.x = vd[0][lane]; zz.y = vd[1][lane]; zz.z = vd[2][lane]; zz.w = 0;
-        reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = zz;
-    }
+    if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
     if (rn == 0) return;
     // column-major G_s: entry (row r, column q) at q * no + r
     const T *G = tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1);
     T ax = 0, ay = 0, az = 0;
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) {
-        int ii = min(i, qn - 1);
-        T g = G[(long long)ii * no];
-        T m = i < qn ? T(1) : T(0);
-        g *= m;
-        ax += g * vd[0][ii]; ay += g * vd[1][ii]; az += g * vd[2][ii];
-    }
+    tile_dot_any<T, V>(G, no, vd, qn, ax, ay, az);
     if (lane < rn) {
-        V out;
-        out.x = (kcb == 0 ? vo[0][lane] : T(0)) - ax;
-        out.y = (kcb == 0 ? vo[1][lane] : T(0)) - ay;
-        out.z = (kcb == 0 ? vo[2][lane] : T(0)) - az;
+        V o = vo[lane], out;
+        out.x = (kcb == 0 ? o.x : T(0)) - ax;
+        out.y = (kcb == 0 ? o.y : T(0)) - ay;
+        out.z = (kcb == 0 ? o.z : T(0)) - az;
         out.w = 0;
         reinterpret_cast<V *>(M.upart)[S.uoff + (long long)kcb * no + r0 + lane] = out;
     }
@@ -400,7 +458,7 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
 template <typename T>
 __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
     using V = typename V4T<T>::type;
-    __shared__ T tv[3][kTile];
+    __shared__ V tv[kTile];
     int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
     const int4 tl = M.bt[M.bt_ptr[level] + (blockIdx.x - L.prefix[k])];
@@ -422,19 +480,13 @@ __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
                 t.x = -a.x; t.y = -a.y; t.z = -a.z;
             }
         }
-        tv[0][lane] = t.x; tv[1][lane] = t.y; tv[2][lane] = t.z;
+        t.w = 0;
+        tv[lane] = t;
     }
     __syncthreads();
     const T *P = tptr<T>(M.pr) + S.pr_off + (long long)p0 * w + q0 + min(lane, qn - 1);
     T ax = 0, ay = 0, az = 0;
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) {
-        int ii = min(i, pn - 1);
-        T g = P[(long long)ii * w];
-        T m1 = i < pn ? T(1) : T(0);
-        g *= m1;
-        ax += g * tv[0][ii]; ay += g * tv[1][ii]; az += g * tv[2][ii];
-    }
+    tile_dot_any<T, V>(P, w, tv, pn, ax, ay, az);
     if (lane < qn) {
         V out;
         out.x = ax; out.y = ay; out.z = az; out.w = 0;
