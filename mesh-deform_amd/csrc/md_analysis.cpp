@@ -459,18 +459,32 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
         f.uoff[t + 1] = f.uoff[t] + (int64_t)f.kc[t] * f.noff[t];
         f.xoff[t + 1] = f.xoff[t] + (int64_t)f.kr[t] * f.w[t];
     }
-    f.bp_off.assign(nn, 0);
-    f.bp.clear();
-    for (int t = 0; t < nn; ++t) {
-        int p = f.parent[t];
-        f.bp_off[t] = (int)f.bp.size();
-        if (p < 0) { f.bp.push_back(0); continue; }
-        int nb = cdiv(f.w[p] + f.noff[p], kTile);
-        const int* rl = f.rel.data() + f.roff[t];
-        int e = 0, cn = f.noff[t];
-        for (int b = 0; b <= nb; ++b) {
-            while (e < cn && rl[e] < b * kTile) ++e;
-            f.bp.push_back(e);
+    // contributions of the children's update entries to each position of a front, in child
+    // order: the forward gather pulls them per position (no write conflicts)
+    f.cp_off.assign(nn, 0);
+    f.cptr.clear();
+    f.cent.clear();
+    {
+        std::vector<std::vector<std::pair<int, int>>> lists;  // per position: (child, entry)
+        for (int t = 0; t < nn; ++t) {
+            int m = f.w[t] + f.noff[t];
+            lists.assign(m, {});
+            for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
+                int c = f.child[k];
+                for (int e = 0; e < f.noff[c]; ++e) lists[f.rel[f.roff[c] + e]].emplace_back(c, e);
+            }
+            f.cp_off[t] = (int)f.cptr.size();
+            for (int p = 0; p < m; ++p) {
+                f.cptr.push_back((int)(f.cent.size() / 4));
+                for (const auto& ce : lists[p]) {
+                    int c = ce.first, e = ce.second;
+                    f.cent.push_back((int)(f.uoff[c] + e));
+                    f.cent.push_back(f.noff[c]);
+                    f.cent.push_back(f.kc[c]);
+                    f.cent.push_back(0);
+                }
+            }
+            f.cptr.push_back((int)(f.cent.size() / 4));
         }
     }
     f.rowinfo.assign(4 * (size_t)n, 0);
@@ -704,36 +718,31 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
             double vd[3][kTile] = {}, vo[3][kTile] = {};
             for (int i = 0; i < qn; ++i)
                 for (int c = 0; c < 3; ++c) vd[c][i] = b[(size_t)c * n + c0 + q0 + i];
-            int lo = w + r0, hi = w + r0 + rn;
-            for (int ci = f.child_ptr[s]; ci < f.child_ptr[s + 1]; ++ci) {
-                int ch = f.child[ci];
-                const int* bp = f.bp.data() + f.bp_off[ch];
-                int nb = (f.w[s] + f.noff[s] + kTile - 1) / kTile;
-                MD_CHECK(kcb + 1 <= nb, "bucket index (columns)");
-                for (int e = bp[kcb]; e < bp[kcb + 1]; ++e) {
-                    MD_CHECK(e >= 0 && e < f.noff[ch], "child entry");
-                    int pos = f.rel[f.roff[ch] + e];
-                    MD_CHECK(pos >= q0 && pos < q0 + kTile, "child position outside column block");
-                    if (pos >= q0 + qn) continue;
+            auto pull = [&](int pos, double* acc3) -> bool {
+                int64_t c0p = f.cp_off[s];
+                if (c0p + pos + 1 >= (int64_t)f.cptr.size()) return false;
+                for (int q = f.cptr[c0p + pos]; q < f.cptr[c0p + pos + 1]; ++q) {
+                    if (q < 0 || 4 * (size_t)q + 3 >= f.cent.size()) return false;
+                    const int* ce = f.cent.data() + 4 * (size_t)q;
                     for (int c = 0; c < 3; ++c) {
                         double v;
-                        MD_CHECK(sum_part(up, f.uoff[ch], f.kc[ch], f.noff[ch], e, c, v), "update partial");
-                        vd[c][pos - q0] += v;
+                        if (!sum_part(up, ce[0], ce[2], ce[1], 0, c, v)) return false;
+                        acc3[c] += v;
                     }
                 }
-                if (kcb == 0 && rn > 0) {
-                    MD_CHECK((hi - 1) / kTile + 1 <= nb, "bucket index (rows)");
-                    for (int e = bp[lo / kTile]; e < bp[(hi - 1) / kTile + 1]; ++e) {
-                        int pos = f.rel[f.roff[ch] + e];
-                        if (pos < lo || pos >= hi) continue;
-                        for (int c = 0; c < 3; ++c) {
-                            double v;
-                            MD_CHECK(sum_part(up, f.uoff[ch], f.kc[ch], f.noff[ch], e, c, v), "update partial");
-                            vo[c][pos - lo] += v;
-                        }
-                    }
-                }
+                return true;
+            };
+            for (int i = 0; i < qn; ++i) {
+                double a[3] = {0, 0, 0};
+                MD_CHECK(pull(q0 + i, a), "column contributions");
+                for (int c = 0; c < 3; ++c) vd[c][i] += a[c];
             }
+            if (kcb == 0)
+                for (int i = 0; i < rn; ++i) {
+                    double a[3] = {0, 0, 0};
+                    MD_CHECK(pull(w + r0 + i, a), "row contributions");
+                    for (int c = 0; c < 3; ++c) vo[c][i] += a[c];
+                }
             if (r0 == 0)
                 for (int i = 0; i < qn; ++i)
                     for (int c = 0; c < 3; ++c) z[3 * (size_t)(c0 + q0 + i) + c] = vd[c][i];

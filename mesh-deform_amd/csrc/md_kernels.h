@@ -16,7 +16,7 @@ template <> struct V4T<double> { using type = d4; };
 // Per-supernode record of the block LDL^T sweeps (see Factor in md_analysis.h).
 struct SnDev {
     int c0, w, noff, roff;     // columns [c0, c0+w), off rows offrows[roff .. roff+noff)
-    int kc, kr, bp_off, pad;   // partial counts of u_s / x_s, bucket pointers as a child
+    int kc, kr, cp_off, pad;   // partial counts of u_s / x_s, contribution pointers
     long long pc_off, pr_off;  // column-major G_s, row-major [Dinv_s; G_s]
     long long uoff, xoff;      // partial update / partial solution entries
 };
@@ -44,8 +44,8 @@ struct MeshDev {
     void *upart, *xpart;        // partial update / solution vectors (V4)
     const SnDev *sn;
     const int *child_ptr, *child;
-    const int *rel;             // per off row entry: position in the parent's rows
-    const int *bp;              // bucket pointers into rel
+    const int *cptr;            // per front position: range of its contributions
+    const int4 *cent;           // (child partial base, stride, count, 0) per contribution
     const int4 *rowinfo;        // per row: (first partial, stride, count, 0)
     const int4 *xloc;           // per off row entry: rowinfo of that row
     const void *pc, *pr;        // panels (float or double), forward and backward layouts

@@ -334,11 +334,13 @@ template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
     return reinterpret_cast<const T *>(p);
 }
 
-// Sum of cnt partial vectors U[k * stride + e], k = 0 .. cnt-1, in that order. The loads are
-// issued 16 at a time (clamped addresses, masked values) so their latencies overlap.
+// Sum of cnt partial vectors U[k * stride + e], k = 0 .. cnt-1, in that order. One partial is
+// one load; more are loaded 16 at a time (clamped addresses, masked values) so their
+// latencies overlap.
 template <typename V>
 __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e) {
     using T = decltype(V{}.x);
+    if (cnt == 1) return U[e];
     V a;
     a.x = 0; a.y = 0; a.z = 0; a.w = 0;
     for (int k0 = 0; k0 < cnt; k0 += 16) {
@@ -350,6 +352,21 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
             T m = (k0 + u < cnt) ? T(1) : T(0);
             a.x += m * b[u].x; a.y += m * b[u].y; a.z += m * b[u].z;
         }
+    }
+    return a;
+}
+
+// Children's contributions to position pos of supernode S's front, summed in child order.
+template <typename V>
+__device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S, int pos) {
+    const V *U = reinterpret_cast<const V *>(M.upart);
+    int q0 = M.cptr[S.cp_off + pos], q1 = M.cptr[S.cp_off + pos + 1];
+    V a;
+    a.x = 0; a.y = 0; a.z = 0; a.w = 0;
+    for (int q = q0; q < q1; ++q) {
+        int4 ce = M.cent[q];
+        V u = sum_partials<V>(U + ce.x, ce.z, ce.y, 0);
+        a.x += u.x; a.y += u.y; a.z += u.z;
     }
     return a;
 }
@@ -380,8 +397,8 @@ __device__ __forceinline__ void tile_dot_any(const T *P, long long stride, const
 //   v_o[r0 .. r0+64) = children's updates landing on those off rows (first column tile),
 //   z_s[q0 ..] = v_d (first row tile),
 //   partial q0/64 of u_s on rows [r0, r0+64) = (q0 == 0 ? v_o : 0) - G_s[rows, cols] v_d.
-// Children's updates arrive as kc partial vectors; bucket pointers give the contiguous range
-// of each child's entries that lands in a 64-position block of this supernode's rows.
+// Children's updates arrive as kc partial vectors; each lane pulls the contributions listed
+// for its position (host-built lists in child order), so no two lanes write the same place.
 template <typename T>
 __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     using V = typename V4T<T>::type;
@@ -396,45 +413,21 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     const int qn = min(kTile, w - q0), rn = max(0, min(kTile, no - r0));
     const int kcb = q0 / kTile;
     const V *R = reinterpret_cast<const V *>(M.r);
-    const V *U = reinterpret_cast<const V *>(M.upart);
     {
         V b;
-        if (lane < qn) b = R[S.c0 + q0 + lane]; else { b.x = 0; b.y = 0; b.z = 0; }
-        b.w = 0;
+        b.x = 0; b.y = 0; b.z = 0; b.w = 0;
+        if (lane < qn) {
+            b = R[S.c0 + q0 + lane];
+            V c = pull_contributions<V>(M, S, q0 + lane);
+            b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
+        }
         vd[lane] = b;
-        V zz;
-        zz.x = 0; zz.y = 0; zz.z = 0; zz.w = 0;
-        vo[lane] = zz;
+        V o;
+        o.x = 0; o.y = 0; o.z = 0; o.w = 0;
+        if (kcb == 0 && lane < rn) o = pull_contributions<V>(M, S, w + r0 + lane);
+        vo[lane] = o;
     }
     __syncthreads();
-    const int lo = w + r0, hi = w + r0 + rn;
-    for (int ci = M.child_ptr[tl.x]; ci < M.child_ptr[tl.x + 1]; ++ci) {
-        const SnDev C = M.sn[M.child[ci]];
-        const int *rel = M.rel + C.roff;
-        const int *bp = M.bp + C.bp_off;
-        const V *Uc = U + C.uoff;
-        int e0 = bp[kcb], e1 = bp[kcb + 1];
-        for (int e = e0 + lane; e < e1; e += kTile) {
-            int pos = rel[e];
-            if (pos >= q0 + qn) continue;  // the block's tail past column w holds off rows
-            V u = sum_partials<V>(Uc, C.kc, C.noff, e);
-            V &d = vd[pos - q0];
-            d.x += u.x; d.y += u.y; d.z += u.z;
-        }
-        if (kcb == 0 && rn > 0) {
-            e0 = bp[lo / kTile];
-            e1 = bp[(hi - 1) / kTile + 1];
-            for (int e = e0 + lane; e < e1; e += kTile) {
-                int pos = rel[e];
-                if (pos >= lo && pos < hi) {
-                    V u = sum_partials<V>(Uc, C.kc, C.noff, e);
-                    V &d = vo[pos - lo];
-                    d.x += u.x; d.y += u.y; d.z += u.z;
-                }
-            }
-        }
-        __syncthreads();
-    }
     if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
     if (rn == 0) return;
     // column-major G_s: entry (row r, column q) at q * no + r

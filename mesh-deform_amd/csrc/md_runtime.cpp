@@ -387,8 +387,8 @@ int analyze_device(md_mesh *m) {
     size_t o_sn = A.reserve(sizeof(SnDev) * std::max(1, ns));
     size_t o_cp = A.reserve(sizeof(int) * (ns + 1));
     size_t o_ch = A.reserve(sizeof(int) * std::max<size_t>(1, f.child.size()));
-    size_t o_rel = A.reserve(sizeof(int) * std::max<size_t>(1, f.rel.size()));
-    size_t o_bp = A.reserve(sizeof(int) * std::max<size_t>(1, f.bp.size()));
+    size_t o_cptr = A.reserve(sizeof(int) * std::max<size_t>(1, f.cptr.size()));
+    size_t o_cen = A.reserve(sizeof(int) * std::max<size_t>(4, f.cent.size()));
     size_t o_ri = A.reserve(sizeof(int) * std::max<size_t>(4, f.rowinfo.size()));
     size_t o_xl = A.reserve(sizeof(int) * std::max<size_t>(4, f.xloc.size()));
     size_t o_pc = A.reserve(psz * std::max<int64_t>(1, last(f.pc_off)));
@@ -437,7 +437,7 @@ int analyze_device(md_mesh *m) {
     d.upart = A.at(o_up); d.xpart = A.at(o_xp);
     d.sn = (const SnDev *)A.at(o_sn);
     d.child_ptr = (const int *)A.at(o_cp); d.child = (const int *)A.at(o_ch);
-    d.rel = (const int *)A.at(o_rel); d.bp = (const int *)A.at(o_bp);
+    d.cptr = (const int *)A.at(o_cptr); d.cent = (const int4 *)A.at(o_cen);
     d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
     d.pc = A.at(o_pc); d.pr = A.at(o_pr);
     d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
@@ -459,7 +459,7 @@ int analyze_device(md_mesh *m) {
     for (int k = 0; k < ns; ++k) {
         SnDev &x = snh[k];
         x.c0 = f.c0[k]; x.w = f.w[k]; x.noff = f.noff[k]; x.roff = f.roff[k];
-        x.kc = f.kc[k]; x.kr = f.kr[k]; x.bp_off = f.bp_off[k]; x.pad = 0;
+        x.kc = f.kc[k]; x.kr = f.kr[k]; x.cp_off = f.cp_off[k]; x.pad = 0;
         x.pc_off = f.pc_off[k]; x.pr_off = f.pr_off[k]; x.uoff = f.uoff[k]; x.xoff = f.xoff[k];
     }
     hipError_t e = hipSuccess;
@@ -473,8 +473,8 @@ int analyze_device(md_mesh *m) {
         chk(up(d.sn, snh.data(), sizeof(SnDev) * ns));
         chk(up(d.child_ptr, f.child_ptr.data(), sizeof(int) * (ns + 1)));
         chk(up(d.child, f.child.data(), sizeof(int) * f.child.size()));
-        chk(up(d.rel, f.rel.data(), sizeof(int) * f.rel.size()));
-        chk(up(d.bp, f.bp.data(), sizeof(int) * f.bp.size()));
+        chk(up(d.cptr, f.cptr.data(), sizeof(int) * f.cptr.size()));
+        chk(up(d.cent, f.cent.data(), sizeof(int) * f.cent.size()));
         chk(up(d.rowinfo, f.rowinfo.data(), sizeof(int) * f.rowinfo.size()));
         chk(up(d.xloc, f.xloc.data(), sizeof(int) * f.xloc.size()));
         chk(up(d.ft_ptr, f.ft_ptr.data(), sizeof(int) * (nl + 1)));
