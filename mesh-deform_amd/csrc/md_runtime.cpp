@@ -60,7 +60,7 @@ static int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
                       const std::vector<const Factor *> &factors, bool fp64, int refine_steps,
-                      hipStream_t stream) {
+                      hipStream_t stream, bool sync) {
     plan.drop_graph();
     plan.fp64 = fp64;
     plan.refine_steps = std::max(1, refine_steps);
@@ -118,7 +118,7 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     if (!plan.h_prefix.empty())
         MD_HIP_E(hipMemcpyAsync(plan.d_prefix, plan.h_prefix.data(), sizeof(int) * plan.h_prefix.size(),
                                 hipMemcpyHostToDevice, stream));
-    return hipStreamSynchronize(stream);
+    return sync ? hipStreamSynchronize(stream) : hipSuccess;
 }
 
 void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {

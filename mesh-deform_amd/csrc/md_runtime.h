@@ -67,9 +67,11 @@ struct Plan {
 };
 
 // Builds (or rebuilds) a plan for the given meshes and uploads its tables on `stream`.
+// With sync = false the uploads are only stream-ordered; the plan's host tables must then
+// stay unchanged until the stream has passed them.
 hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
                       const std::vector<const Factor *> &factors, bool fp64, int refine_steps,
-                      hipStream_t stream);
+                      hipStream_t stream, bool sync = true);
 // Enqueues one ARAP iteration (local step + refine_steps x (residual, sweeps)).
 void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *phase_events);
 // Runs `iters` iterations, from a captured graph when use_graph is set.

@@ -122,7 +122,9 @@ struct Request {
     bool admitted = false, done = false;
     int status = MD_OK;
     std::string error;
-    std::vector<double> result;
+    std::vector<int> iv;        // internal ids of the moved targets (upload source)
+    d4 *pinned = nullptr;       // positions in internal order, copied at completion
+    int n = 0;
 };
 }  // namespace
 
@@ -143,29 +145,44 @@ struct md_server {
     Plan plan;
     std::vector<int> plan_meshes;
     std::string fatal;
+    std::vector<std::vector<d4 *>> pinned_free;  // per mesh: reusable pinned result buffers
+    std::vector<d4 *> pinned_all;
+
+    d4 *take_pinned(int mesh_id, int n) {  // caller holds mu
+        auto &fl = pinned_free[mesh_id];
+        if (!fl.empty()) {
+            d4 *p = fl.back();
+            fl.pop_back();
+            return p;
+        }
+        void *p = nullptr;
+        if (hipHostMalloc(&p, sizeof(d4) * (size_t)std::max(1, n), hipHostMallocDefault) != hipSuccess) return nullptr;
+        pinned_all.push_back((d4 *)p);
+        return (d4 *)p;
+    }
 };
 
 namespace {
 
-int apply_targets(md_mesh *m, const Request &r) {
-    // paste the new targets into p' (fixed rest pose: the previous result is the initial
-    // guess) with one upload and one scatter launch
+// Pastes the request's targets into p' (fixed rest pose: the previous result is the initial
+// guess): one upload and one scatter launch, stream-ordered, no host synchronisation. The
+// sources live in the request until it completes.
+int apply_targets(md_mesh *m, Request &r) {
     int k = (int)r.idx.size();
     if (k == 0) return MD_OK;
-    std::vector<int> iv(k);
+    r.iv.resize(k);
     for (int i = 0; i < k; ++i) {
-        iv[i] = m->topo.internal_of[r.idx[i]];
+        r.iv[i] = m->topo.internal_of[r.idx[i]];
         std::memcpy(&m->con[3 * (size_t)r.idx[i]], &r.xyz[3 * (size_t)i], 3 * sizeof(double));
     }
     hipStream_t st = m->ctx->stream;
-    hipError_t e = hipMemcpyAsync(m->d_tgt_idx, iv.data(), sizeof(int) * k, hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(m->d_tgt_idx, r.iv.data(), sizeof(int) * k, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(m->d_tgt_xyz, r.xyz.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         launch_scatter_targets(st, m->dev.pp, m->d_tgt_idx, m->d_tgt_xyz, k);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? MD_OK : MD_ERR_HIP;
 }
 
@@ -195,12 +212,19 @@ void server_loop(md_server *S) {
                 if (!S->active.empty()) break;
                 S->cv_work.wait(lk);
             }
+            // pinned result buffers for the requests that finish after this iteration
+            for (auto &r : S->active)
+                if (r->left <= 1 && !r->pinned && r->status == MD_OK) {
+                    r->n = S->meshes[r->mesh_id]->n;
+                    r->pinned = S->take_pinned(r->mesh_id, r->n);
+                    if (!r->pinned) { r->status = MD_ERR_NOMEM; r->error = "pinned result buffer"; r->left = 0; }
+                }
         }
         for (auto &r : admitted_now) {
             int rc = apply_targets(S->meshes[r->mesh_id], *r);
             if (rc) { r->status = rc; r->error = "target upload failed"; r->left = 0; }
         }
-        // plan over the active meshes (rebuilt only when the set changed)
+        // plan over the active meshes (rebuilt only when the set changed; stream-ordered)
         std::vector<int> ids;
         for (auto &r : S->active) ids.push_back(r->mesh_id);
         if (ids != S->plan_meshes) {
@@ -210,7 +234,7 @@ void server_loop(md_server *S) {
             md_mesh *m0 = S->meshes[ids[0]];
             S->plan.pcg = m0->cfg.solver == MD_SOLVER_PCG;
             S->plan.pcg_iters = m0->cfg.pcg_max_iters;
-            hipError_t e = build_plan(S->plan, devs, facs, m0->fp64(), m0->cfg.refine_steps, st);
+            hipError_t e = build_plan(S->plan, devs, facs, m0->fp64(), m0->cfg.refine_steps, st, false);
             if (e != hipSuccess) { S->fatal = hipGetErrorString(e); }
             S->plan_meshes = ids;
         }
@@ -220,9 +244,14 @@ void server_loop(md_server *S) {
         if (any_left && S->fatal.empty()) {
             enqueue_iteration(S->plan, st, nullptr);
             e = hipGetLastError();
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
         }
-        // retire finished requests
+        // results of the requests finishing now: device-to-host into their pinned buffers
+        for (auto &r : S->active)
+            if (e == hipSuccess && r->left <= 1 && r->pinned && r->status == MD_OK) {
+                md_mesh *m = S->meshes[r->mesh_id];
+                e = hipMemcpyAsync(r->pinned, m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+            }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         std::vector<std::shared_ptr<Request>> finished;
         for (auto &r : S->active) {
             if (r->left > 0) r->left--;
@@ -233,27 +262,11 @@ void server_loop(md_server *S) {
             }
             if (r->left == 0) finished.push_back(r);
         }
-        for (auto &r : finished) {
-            md_mesh *m = S->meshes[r->mesh_id];
-            m->iterated = true;
-            if (r->status == MD_OK) {
-                r->result.resize(3 * (size_t)m->n);
-                std::vector<d4> h(m->n);
-                hipError_t e2 = hipMemcpyAsync(h.data(), m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
-                if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
-                if (e2 != hipSuccess) { r->status = MD_ERR_HIP; r->error = hipGetErrorString(e2); }
-                for (int iv = 0; iv < m->n; ++iv) {
-                    int v = m->topo.original_of[iv];
-                    r->result[3 * (size_t)v] = h[iv].x;
-                    r->result[3 * (size_t)v + 1] = h[iv].y;
-                    r->result[3 * (size_t)v + 2] = h[iv].z;
-                }
-            }
-        }
         {
             std::lock_guard<std::mutex> lk(S->mu);
             auto now = SClock::now();
             for (auto &r : finished) {
+                S->meshes[r->mesh_id]->iterated = true;
                 r->done = true;
                 r->t_done = now;
                 S->busy[r->mesh_id] = 0;
@@ -304,6 +317,7 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
     S->resident += bytes;
     S->meshes.push_back(m);
     S->busy.push_back(0);
+    S->pinned_free.emplace_back();
     *mesh_id = (int)S->meshes.size() - 1;
     return MD_OK;
 }
@@ -339,33 +353,63 @@ extern "C" int md_server_submit(md_server *S, int mesh_id, int k, const int *idx
     return MD_OK;
 }
 
-static int finish_result(md_server *S, std::shared_ptr<Request> r, double *xyz, double *latency_ms) {
+// Called with S->mu held: takes the finished request out of the ticket map. The permutation
+// of its positions into the original vertex order runs afterwards without the lock.
+static std::shared_ptr<Request> take_finished(md_server *S, uint64_t ticket) {
+    auto it = S->tickets.find(ticket);
+    std::shared_ptr<Request> r = it->second;
+    S->tickets.erase(it);
+    return r;
+}
+
+static int deliver(md_server *S, std::shared_ptr<Request> r, double *xyz, double *latency_ms) {
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
-    if (r->status != MD_OK) { set_error("request failed: " + r->error); }
-    else if (xyz) std::memcpy(xyz, r->result.data(), sizeof(double) * r->result.size());
-    S->tickets.erase(r->ticket);
-    return r->status;
+    int rc = r->status;
+    if (rc != MD_OK) set_error("request failed: " + r->error);
+    else if (xyz && r->pinned) {
+        md_mesh *m = S->meshes[r->mesh_id];
+        for (int iv = 0; iv < m->n; ++iv) {
+            int v = m->topo.original_of[iv];
+            xyz[3 * (size_t)v] = r->pinned[iv].x;
+            xyz[3 * (size_t)v + 1] = r->pinned[iv].y;
+            xyz[3 * (size_t)v + 2] = r->pinned[iv].z;
+        }
+    }
+    if (r->pinned) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->pinned_free[r->mesh_id].push_back(r->pinned);
+        r->pinned = nullptr;
+    }
+    return rc;
 }
 
 extern "C" int md_server_poll(md_server *S, uint64_t ticket, int *done, double *xyz, double *latency_ms) {
     if (!S || !done) { set_error("md_server_poll: invalid argument"); return MD_ERR_INVALID; }
-    std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->tickets.find(ticket);
-    if (it == S->tickets.end()) { set_error("md_server_poll: unknown ticket"); return MD_ERR_INVALID; }
-    *done = it->second->done ? 1 : 0;
-    if (!*done) return MD_OK;
-    return finish_result(S, it->second, xyz, latency_ms);
+    std::shared_ptr<Request> r;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->tickets.find(ticket);
+        if (it == S->tickets.end()) { set_error("md_server_poll: unknown ticket"); return MD_ERR_INVALID; }
+        *done = it->second->done ? 1 : 0;
+        if (!*done) return MD_OK;
+        r = take_finished(S, ticket);
+    }
+    return deliver(S, r, xyz, latency_ms);
 }
 
 extern "C" int md_server_wait(md_server *S, uint64_t ticket, double timeout_ms, double *xyz, double *latency_ms) {
     if (!S) { set_error("md_server_wait: invalid argument"); return MD_ERR_INVALID; }
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->tickets.find(ticket);
-    if (it == S->tickets.end()) { set_error("md_server_wait: unknown ticket"); return MD_ERR_INVALID; }
-    auto r = it->second;
-    bool ok = S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), [&] { return r->done; });
-    if (!ok) { set_error("md_server_wait: timed out"); return MD_ERR_TIMEOUT; }
-    return finish_result(S, r, xyz, latency_ms);
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->tickets.find(ticket);
+        if (it == S->tickets.end()) { set_error("md_server_wait: unknown ticket"); return MD_ERR_INVALID; }
+        r = it->second;
+        bool ok = S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), [&] { return r->done; });
+        if (!ok) { set_error("md_server_wait: timed out"); return MD_ERR_TIMEOUT; }
+        take_finished(S, ticket);
+    }
+    return deliver(S, r, xyz, latency_ms);
 }
 
 extern "C" int md_server_destroy(md_server *S) {
@@ -379,6 +423,7 @@ extern "C" int md_server_destroy(md_server *S) {
     md_ctx *ctx = S->ctx;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    for (d4 *p : S->pinned_all) hipHostFree(p);
     delete S;
     ctx_release(ctx);
     return MD_OK;
