@@ -199,6 +199,103 @@ def batched_leg(ctx, count, iters, seed):
                 speedup=(total_iters / (ms / 1e3)) / (total_iters / plain_s))
 
 
+def canonical_bytes(pr, st):
+    """Algorithmic bytes of one ARAP iteration by the canonical per-pass counts (FP32 arrays,
+    int32 indices; each array once per pass): local step + rotated-edge pass
+    (120 + 16 d) n + 36 nf with d = 2E/n, the FP64 residual of the refinement step
+    12 nnz(A) + 124 nf, and one direct sweep pair with the FP32 factor 16 nL + 68 nf
+    (nL = strict nnz of the factor). An iteration here runs one residual and one sweep
+    pair (the residual form folds the right-hand side into the refinement), so
+    B_iter = B_local+rhs + B_ref with B_ref = 12 nnz(A) + 124 nf + B_solve."""
+    n, nf = pr.n, st["nf"]
+    e = np.sort(np.concatenate([pr.tri[:, [0, 1]], pr.tri[:, [1, 2]], pr.tri[:, [2, 0]]]), axis=1)
+    e = np.unique(e, axis=0)
+    con = np.zeros(n, bool)
+    con[pr.constraints()[0]] = True
+    e_ff = int(np.count_nonzero(~con[e[:, 0]] & ~con[e[:, 1]]))
+    nnz_a = nf + 2 * e_ff
+    n_l = st["factor_nnz"]
+    d = 2.0 * e.shape[0] / n
+    b_local = (120 + 16 * d) * n + 36 * nf
+    b_solve = 16 * n_l + 68 * nf
+    b_ref = 12 * nnz_a + 124 * nf + b_solve
+    return dict(B_iter=int(b_local + b_ref), B_local_rhs=int(b_local), B_solve=int(b_solve), B_ref=int(b_ref),
+                n=n, nf=nf, E=int(e.shape[0]), nnzA_full=int(nnz_a), nL=int(n_l), mean_valence=round(d, 4))
+
+
+def traffic_probe(ctx, iters):
+    """Workload for rocprofv3 --pmc passes: a 256 MiB device-to-device elementwise copy
+    (calibration: it reads and writes exactly 256 MiB), then `iters` ARAP iterations of the
+    plain leg's mesh and configuration."""
+    import torch
+
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    pr = meshgen.noisy_icosphere_problem(7, 0.01, 1234)
+    idx, tgt = pr.constraints()
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+    m.set_constraints(idx, tgt)
+    m.prepare(pr.xyz)
+    m.iterate(2)
+    ctx.synchronize()
+    log("traffic probe: calibration copies done, running %d iterations" % iters)
+    m.iterate(iters)
+    ctx.synchronize()
+    m.close()
+    print(json.dumps({"probe": "traffic", "copy_bytes": nbytes, "copies": 3, "iterations": iters,
+                      "warmup_iterations": 2}), flush=True)
+
+
+def traffic_summary(fetch_csv, write_csv, iters):
+    """HBM bytes per ARAP iteration from two rocprofv3 --pmc passes of --traffic-probe:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), summed over the library's kernels of the
+    last `iters` iterations, each scaled by bytes / counter of the 256 MiB calibration copy."""
+    import csv
+
+    def load(path, counter):
+        rows = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                d = int(r["Dispatch_Id"])
+                rows[d] = (r["Kernel_Name"], rows.get(d, ("", 0.0))[1] + float(r["Counter_Value"]))
+        return [rows[d] for d in sorted(rows)]
+
+    out = {}
+    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        disp = load(path, name)
+        calib = [v for k, v in disp if "md::" not in k]
+        ours = [(k, v) for k, v in disp if "md::" in k]
+        copy_kib = max(calib)  # the 256 MiB copy: the largest dispatch outside the library
+        scale = (256 << 20) / (copy_kib * 1024.0)
+        per_iter = None
+        # the library's dispatches of the timed iterations: the last `iters` local steps start them
+        starts = [i for i, (k, _) in enumerate(ours) if "k_local" in k]
+        first = starts[-iters]
+        per_iter = sum(v for _, v in ours[first:]) * 1024.0 * scale / iters
+        out[name] = dict(copy_counter_kib=copy_kib, scale=scale, bytes_per_iter=per_iter,
+                         dispatches_per_iter=(len(ours) - first) / iters)
+    out["traffic_bytes_per_iter"] = out["FETCH_SIZE"]["bytes_per_iter"] + out["WRITE_SIZE"]["bytes_per_iter"]
+    return out
+
+
+def read_traffic(path):
+    """traffic_bytes_per_iter from a committed traffic summary (a line 'traffic_bytes_per_iter: N')."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_iter:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,7 +310,16 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--batched-meshes", type=int, default=256)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run the workload of the rocprofv3 --pmc traffic passes")
+    ap.add_argument("--probe-iters", type=int, default=5)
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two --pmc counter_collection.csv files of --traffic-probe (no GPU)")
+    ap.add_argument("--traffic-summary", default=os.path.join(ROOT, "profiles", "traffic_icosphere164k_fp32_refine.md"))
     args = ap.parse_args()
+    if args.traffic_from:
+        print(json.dumps(traffic_summary(args.traffic_from[0], args.traffic_from[1], args.probe_iters), indent=1))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (meshes would be "
               "sharded across GPUs by device footprint). Run with --gpus 1.", file=sys.stderr)
@@ -222,6 +328,10 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     ctx = mdarap.Context(0)
+    if args.traffic_probe:
+        traffic_probe(ctx, args.probe_iters)
+        ctx.close()
+        return
     pr = meshgen.noisy_icosphere_problem(7, 0.01, 1234)
     log("plain leg: %s n=%d F=%d" % (pr.name, pr.n, pr.F))
     m, st, setup_s, wall, ev_ms, phase = plain_leg(ctx, pr, args.steps, args.warmup)
@@ -232,7 +342,9 @@ def main():
     solve_ms = ph_step[2] + ph_step[3]
     solve_bytes = st["bytes_forward"] + st["bytes_backward"]
     iter_bytes = st["bytes_local"] + st["bytes_residual"] + solve_bytes
-    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
+    cb = canonical_bytes(pr, st)
+    achieved = cb["B_iter"] / (ev_step_ms * 1e-3) / 1e9
+    traffic = read_traffic(args.traffic_summary)
     log("plain: %.1f iters/s (%.4f ms/iter wall, %.4f ms event); phases ms %s" %
         (value, ms_step, ev_step_ms, np.round(ph_step, 4).tolist()))
     out = {
@@ -248,11 +360,15 @@ def main():
                    "levels": st["levels"], "panel_entries": st["panel_entries"],
                    "factor_nnz": st["factor_nnz"], "setup_s": round(setup_s, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "triangular sweeps (k_fwd + k_bwd + k_apply) per iteration",
-                     "algorithmic_bytes": int(solve_bytes), "ms_per_launch_group": round(solve_ms, 5),
-                     "iteration_bytes": int(iter_bytes),
-                     "iteration_achieved_GBs": round(iter_bytes / (ev_step_ms * 1e-3) / 1e9, 1)},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None if traffic is None else int(traffic),
+                     "kernel": "one ARAP iteration = one hipGraph launch (k_local, k_residual, k_sub_fwd, "
+                               "k_fwd per level, k_bwd per level, k_sub_bwd, k_apply), HIP events on the "
+                               "library stream",
+                     "algorithmic_bytes": cb["B_iter"], "ms_per_launch": round(ev_step_ms, 5),
+                     "canonical": cb,
+                     "counted_bytes": {"iteration": int(iter_bytes), "sweeps": int(solve_bytes),
+                                       "sweeps_GBs": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 1)}},
         "phases_ms_per_iter": {"local": round(ph_step[0], 5), "residual": round(ph_step[1], 5),
                                "forward": round(ph_step[2], 5), "backward": round(ph_step[3], 5)},
     }

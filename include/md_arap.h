@@ -60,6 +60,9 @@ typedef struct {
     int pcg_max_iters; /* MD_SOLVER_PCG: CG iterations per solve, default 200 */
     double pcg_tol;    /* MD_SOLVER_PCG: relative residual target, default 1e-7 */
     int use_graph;     /* replay iterations from a captured hipGraph, default 1 */
+    int subtree_budget;/* MD_SOLVER_LDLT: LDS vector entries of one bottom subtree of the
+                          factor swept by a single wavefront (fronts + columns of its
+                          supernodes); 0 sweeps every supernode by level launches; default 512 */
 } md_config;
 
 typedef struct {
@@ -143,11 +146,28 @@ int md_mesh_get_stats(md_mesh *mesh, md_mesh_stats *out);
 int md_mesh_time_iterations(md_mesh *mesh, int warmup, int iterations, double *total_ms,
                             double *phase_ms4);
 
+/* Autotuner of the sweep layout (MD_SOLVER_LDLT, prepared mesh): for each of the `count`
+ * candidate bottom-subtree budgets (md_config.subtree_budget, each in [0, 2048]) the factor
+ * is re-laid out on the device and `iterations` iterations are timed with HIP events; the
+ * fastest candidate is kept (also in the mesh's config, for later re-preparations). The
+ * positions and rotations are restored after every trial, so the mesh state is unchanged.
+ * best_budget and best_ms (per iteration) may be NULL. Batches built from the mesh before
+ * the call must be rebuilt; a mesh registered with a server is MD_ERR_STATE. */
+int md_mesh_autotune(md_mesh *mesh, const int *budgets, int count, int iterations, int *best_budget,
+                     double *best_ms);
+
 /* ---- batches: many prepared meshes iterated in the same launches -------------------- */
 typedef struct md_batch md_batch;
+/* A batch launches the device arrays its meshes have at creation: a mesh re-prepared with a
+ * new constraint set, or re-laid out by md_mesh_autotune, makes the batch's calls return
+ * MD_ERR_STATE until it is rebuilt. */
 int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, md_batch **out);
 int md_batch_iterate(md_batch *batch, int iterations);
 int md_batch_time_iterations(md_batch *batch, int warmup, int iterations, double *total_ms);
+/* md_mesh_autotune for the whole batch: one budget for all its meshes, chosen by timing
+ * batched iterations. */
+int md_batch_autotune(md_batch *batch, const int *budgets, int count, int iterations, int *best_budget,
+                      double *best_ms);
 int md_batch_destroy(md_batch *batch);
 
 /* ---- serving: resident meshes, continuous batching of drag requests ----------------- */
@@ -159,7 +179,8 @@ typedef struct {
 
 int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_server **out);
 /* Registers a prepared mesh (md_mesh_prepare done, MD_REST_FIXED); fails with
- * MD_ERR_BUDGET when it does not fit the budget. The server takes no ownership. */
+ * MD_ERR_BUDGET when it does not fit the budget. The server takes no ownership: the mesh
+ * must outlive the server. */
 int md_server_add_mesh(md_server *srv, md_mesh *mesh, int *mesh_id);
 /* Queues a drag: move the k constrained vertices idx to xyz, then run `iterations`
  * warm-started iterations. Indices must already be constrained on that mesh. */
@@ -174,10 +195,11 @@ int md_server_destroy(md_server *srv);
 /* ---- host-only diagnostics (no GPU needed) ------------------------------------------ */
 /* Factors the SPD matrix given as full CSR (n rows, ascending columns), with coordinates
  * xyz (3 per row) for the ordering, by the same host analysis and FP64 block-LDL^T the
- * library uploads, then solves nrhs right-hand sides (b[r*n + i]) on the host. */
+ * library uploads, then solves nrhs right-hand sides (b[r*n + i]) on the host by an
+ * emulation of the device sweeps (subtree_budget as in md_config; negative: the default). */
 int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
                          const double *xyz, int nrhs, const double *b, double *x,
-                         int64_t *panel_entries, int *levels);
+                         int64_t *panel_entries, int *levels, int subtree_budget);
 
 #ifdef __cplusplus
 }

@@ -295,7 +295,7 @@ void build_topology(int n, int F, const int* tri, const std::vector<int>& perm,
 int g_amalg_small = 16;
 double g_amalg_ratio = 0.3;
 
-void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, Factor& f) {
+void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, Factor& f, int sub_budget) {
     f = Factor();
     f.n = n;
     std::vector<int> ident(n);
@@ -445,15 +445,58 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     }
     f.poff.assign(nn + 1, 0);
     for (int t = 0; t < nn; ++t) f.poff[t + 1] = f.poff[t] + (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
-    // device layout: tiles, partial vectors, bucket pointers, row locators
+    device_layout(f, sub_budget);
+}
+
+void device_layout(Factor& f, int sub_budget) {
+    const int n = f.n, nn = f.ns;
     auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
-    f.kc.resize(nn); f.kr.resize(nn);
+    // bottom subtrees: maximal subtrees of small supernodes within the LDS budget
+    std::vector<int> sub_root(nn, -1), size(nn, 1);
+    std::vector<int64_t> lds(nn, 0);
+    std::vector<char> ok(nn, 0);
+    int64_t panel_total = 0;
+    for (int t = 0; t < nn; ++t) panel_total += (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
+    if (sub_budget > 0 && panel_total < (int64_t)INT32_MAX / 2) {
+        for (int t = 0; t < nn; ++t) {
+            int m = f.w[t] + f.noff[t];
+            bool good = f.w[t] <= kSubMaxWidth && m <= kSubMaxRows;
+            lds[t] = m + f.w[t];
+            for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
+                int c = f.child[k];
+                good = good && ok[c];
+                lds[t] += lds[c];
+                size[t] += size[c];
+            }
+            ok[t] = good && lds[t] <= sub_budget;
+        }
+        for (int t = nn - 1; t >= 0; --t) {
+            int p = f.parent[t];
+            if (!ok[t]) continue;
+            sub_root[t] = (p >= 0 && ok[p]) ? sub_root[p] : t;
+        }
+    }
+    f.level.assign(nn, -1);
+    f.nlevels = 0;
+    for (int t = 0; t < nn; ++t) {
+        if (sub_root[t] >= 0) continue;
+        int l = 0;
+        for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
+            int c = f.child[k];
+            if (sub_root[c] < 0) l = std::max(l, f.level[c] + 1);
+        }
+        f.level[t] = l;
+        f.nlevels = std::max(f.nlevels, l + 1);
+    }
+    // device layout: tiles, partial vectors, bucket pointers, row locators
     f.pc_off.assign(nn + 1, 0); f.pr_off.assign(nn + 1, 0);
     f.uoff.assign(nn + 1, 0); f.xoff.assign(nn + 1, 0);
+    f.kc.resize(nn); f.kr.resize(nn);
     for (int t = 0; t < nn; ++t) {
         int m = f.w[t] + f.noff[t];
-        f.kc[t] = cdiv(f.w[t], kTile);
-        f.kr[t] = cdiv(m, kTile);
+        bool in_sub = sub_root[t] >= 0;
+        f.kc[t] = in_sub ? (sub_root[t] == t ? 1 : 0) : cdiv(f.w[t], kTile);
+        f.kr[t] = in_sub ? 1 : cdiv(m, kTile);
         f.pc_off[t + 1] = f.pc_off[t] + (int64_t)f.noff[t] * f.w[t];
         f.pr_off[t + 1] = f.pr_off[t] + (int64_t)m * f.w[t];
         f.uoff[t + 1] = f.uoff[t] + (int64_t)f.kc[t] * f.noff[t];
@@ -467,6 +510,7 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     {
         std::vector<std::vector<std::pair<int, int>>> lists;  // per position: (child, entry)
         for (int t = 0; t < nn; ++t) {
+            if (sub_root[t] >= 0) { f.cp_off[t] = 0; continue; }  // swept by its subtree
             int m = f.w[t] + f.noff[t];
             lists.assign(m, {});
             for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
@@ -501,9 +545,10 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     f.ft_ptr.assign(f.nlevels + 1, 0);
     f.bt_ptr.assign(f.nlevels + 1, 0);
     for (int t = 0; t < nn; ++t) {
+        if (f.level[t] < 0) continue;
         int m = f.w[t] + f.noff[t];
-        f.ft_ptr[f.height[t] + 1] += std::max(1, cdiv(f.noff[t], kTile)) * f.kc[t];
-        f.bt_ptr[f.height[t] + 1] += f.kc[t] * cdiv(m, kTile);
+        f.ft_ptr[f.level[t] + 1] += std::max(1, cdiv(f.noff[t], kTile)) * f.kc[t];
+        f.bt_ptr[f.level[t] + 1] += f.kc[t] * cdiv(m, kTile);
     }
     for (int l = 0; l < f.nlevels; ++l) {
         f.ft_ptr[l + 1] += f.ft_ptr[l];
@@ -513,7 +558,8 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     f.bt.assign(4 * (size_t)f.bt_ptr[f.nlevels], 0);
     std::vector<int> ff(f.ft_ptr.begin(), f.ft_ptr.end() - 1), bf(f.bt_ptr.begin(), f.bt_ptr.end() - 1);
     for (int t = 0; t < nn; ++t) {
-        int l = f.height[t], m = f.w[t] + f.noff[t];
+        int l = f.level[t], m = f.w[t] + f.noff[t];
+        if (l < 0) continue;
         int rbs = std::max(1, cdiv(f.noff[t], kTile));
         for (int rb = 0; rb < rbs; ++rb)
             for (int cb = 0; cb < f.kc[t]; ++cb) {
@@ -525,6 +571,73 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
                 int* tt = &f.bt[4 * (size_t)bf[l]++];
                 tt[0] = t; tt[1] = cb * kTile; tt[2] = pb * kTile;
             }
+    }
+    // bottom subtree items (the subtree of root r is the id range [r - size + 1, r])
+    f.nsub = 0; f.sub_lds = 0;
+    f.sub.clear(); f.sub_h.clear();
+    f.sfa.clear(); f.sfc.clear(); f.sfb.clear(); f.sba.clear(); f.sbb.clear();
+    std::vector<int> fb(nn, 0);
+    for (int r = 0; r < nn; ++r) {
+        if (sub_root[r] != r) continue;
+        const int first = r - size[r] + 1, nh = f.height[r] + 1;
+        const int col0 = f.c0[first], col1 = f.c0[r] + f.w[r];
+        int fronts = 0;
+        for (int t = first; t <= r; ++t) { fb[t] = fronts; fronts += f.w[t] + f.noff[t]; }
+        const int xs0 = fronts, entries = fronts + (col1 - col0);
+        f.sub.push_back((int)(f.sub_h.size() / 4));
+        f.sub.push_back(nh);
+        f.sub.push_back(entries);
+        f.sub.push_back(r);
+        f.sub_lds = std::max(f.sub_lds, entries);
+        ++f.nsub;
+        std::vector<std::vector<int>> at(nh);
+        for (int t = first; t <= r; ++t) at[f.height[t]].push_back(t);
+        std::vector<std::vector<int>> lists;
+        for (int h = 0; h <= nh; ++h) {
+            f.sub_h.push_back((int)(f.sfa.size() / 4));
+            f.sub_h.push_back((int)(f.sfb.size() / 8));
+            f.sub_h.push_back((int)(f.sba.size() / 4));
+            f.sub_h.push_back((int)(f.sbb.size() / 8));
+            if (h == nh) break;
+            for (int t : at[h]) {
+                const int w = f.w[t], no = f.noff[t], m = w + no;
+                // children's update entries landing on each position, in child order
+                lists.assign(m, {});
+                for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
+                    int c = f.child[k];
+                    for (int e = 0; e < f.noff[c]; ++e)
+                        lists[f.rel[f.roff[c] + e]].push_back(fb[c] + f.w[c] + e);
+                }
+                for (int p = 0; p < m; ++p) {
+                    f.sfa.push_back(p < w ? f.c0[t] + p : -1);
+                    f.sfa.push_back(fb[t] + p);
+                    f.sfa.push_back((int)f.sfc.size());
+                    f.sfc.insert(f.sfc.end(), lists[p].begin(), lists[p].end());
+                    f.sfa.push_back((int)f.sfc.size());
+                }
+                for (int i = 0; i < no; ++i) {
+                    int v[8] = {(int)(f.pc_off[t] + i), no, w, fb[t], fb[t] + w + i,
+                                t == r ? (int)(f.uoff[t] + i) : -1, 0, 0};
+                    f.sfb.insert(f.sfb.end(), v, v + 8);
+                }
+                for (int p = 0; p < m; ++p) {
+                    int v[4] = {-1, -1, -1, fb[t] + p};
+                    if (p < w) {
+                        v[0] = f.c0[t] + p;
+                    } else {
+                        int e = f.roff[t] + p - w, row = f.offrows[e];
+                        if (row >= col0 && row < col1) v[1] = xs0 + row - col0;
+                        else v[2] = e;
+                    }
+                    f.sba.insert(f.sba.end(), v, v + 4);
+                }
+                for (int q = 0; q < w; ++q) {
+                    int v[8] = {(int)(f.pr_off[t] + q), w, m, fb[t], xs0 + f.c0[t] + q - col0,
+                                (int)(f.xoff[t] + q), 0, 0};
+                    f.sbb.insert(f.sbb.end(), v, v + 8);
+                }
+            }
+        }
     }
 }
 
@@ -708,10 +821,87 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
         }
         return true;
     };
+    // bottom subtrees: LDS entries carry the phase that wrote them; a read must see an entry
+    // written in an earlier phase (a same-phase read would race on the device)
+    std::vector<double> lds;
+    std::vector<int> stamp;
+    auto sub_range = [&](int i, int h, int comp, int& a, int& b) -> bool {
+        int hoff = f.sub[4 * (size_t)i];
+        size_t e0 = 4 * (size_t)(hoff + h) + comp, e1 = 4 * (size_t)(hoff + h + 1) + comp;
+        if (hoff < 0 || e1 >= f.sub_h.size()) return false;
+        a = f.sub_h[e0];
+        b = f.sub_h[e1];
+        return a >= 0 && a <= b;
+    };
+    auto lds_read = [&](int e, int entries, int phase, int c, double& v) -> bool {
+        if (e < 0 || e >= entries || stamp[e] <= 0 || stamp[e] >= phase) return false;
+        v = lds[3 * (size_t)e + c];
+        return true;
+    };
+    auto lds_claim = [&](int e, int entries, int phase) -> bool {
+        if (e < 0 || e >= entries || stamp[e] == phase) return false;
+        stamp[e] = phase;
+        return true;
+    };
+    MD_CHECK((int)f.sub.size() == 4 * f.nsub, "subtree table");
+    for (int i = 0; i < f.nsub; ++i) {
+        const int nh = f.sub[4 * (size_t)i + 1], entries = f.sub[4 * (size_t)i + 2];
+        MD_CHECK(entries > 0 && entries <= f.sub_lds && nh > 0, "subtree extent");
+        lds.assign(3 * (size_t)entries, 0.0);
+        stamp.assign(entries, 0);
+        int phase = 0;
+        for (int h = 0; h < nh; ++h) {
+            int a0, a1, b0, b1;
+            MD_CHECK(sub_range(i, h, 0, a0, a1) && sub_range(i, h, 1, b0, b1), "subtree forward range");
+            MD_CHECK(4 * (size_t)a1 <= f.sfa.size() && 8 * (size_t)b1 <= f.sfb.size(), "subtree forward items");
+            ++phase;
+            for (int t = a0; t < a1; ++t) {
+                const int* it = f.sfa.data() + 4 * (size_t)t;
+                MD_CHECK(it[0] < n && it[2] >= 0 && it[2] <= it[3] && (size_t)it[3] <= f.sfc.size(),
+                         "subtree position item");
+                double v[3] = {0, 0, 0};
+                for (int c = 0; c < 3; ++c) {
+                    if (it[0] >= 0) v[c] = b[(size_t)c * n + it[0]];
+                    for (int q = it[2]; q < it[3]; ++q) {
+                        double u;
+                        MD_CHECK(lds_read(f.sfc[q], entries, phase, c, u), "subtree child update read");
+                        v[c] += u;
+                    }
+                }
+                MD_CHECK(lds_claim(it[1], entries, phase), "subtree front write");
+                for (int c = 0; c < 3; ++c) {
+                    lds[3 * (size_t)it[1] + c] = v[c];
+                    if (it[0] >= 0) z[3 * (size_t)it[0] + c] = v[c];
+                }
+            }
+            ++phase;
+            for (int t = b0; t < b1; ++t) {
+                const int* it = f.sfb.data() + 8 * (size_t)t;
+                const int w = it[2], stride = it[1];
+                MD_CHECK(w > 0 && w <= kTile && it[0] >= 0 &&
+                         (int64_t)it[0] + (int64_t)(w - 1) * stride < (int64_t)pc.size(), "subtree G range");
+                double acc[3] = {0, 0, 0}, o[3];
+                for (int q = 0; q < w; ++q)
+                    for (int c = 0; c < 3; ++c) {
+                        double v;
+                        MD_CHECK(lds_read(it[3] + q, entries, phase, c, v), "subtree front read");
+                        acc[c] += pc[it[0] + (size_t)q * stride] * v;
+                    }
+                for (int c = 0; c < 3; ++c) MD_CHECK(lds_read(it[4], entries, phase, c, o[c]), "subtree off-row read");
+                MD_CHECK(lds_claim(it[4], entries, phase), "subtree update write");
+                MD_CHECK(it[5] < nu, "subtree update output");
+                for (int c = 0; c < 3; ++c) {
+                    double u = o[c] - acc[c];
+                    lds[3 * (size_t)it[4] + c] = u;
+                    if (it[5] >= 0) up[3 * (size_t)it[5] + c] = u;
+                }
+            }
+        }
+    }
     for (int l = 0; l < f.nlevels; ++l) {
         for (int t = f.ft_ptr[l]; t < f.ft_ptr[l + 1]; ++t) {
             int s = f.ft[4 * (size_t)t], r0 = f.ft[4 * (size_t)t + 1], q0 = f.ft[4 * (size_t)t + 2];
-            MD_CHECK(s >= 0 && s < f.ns && f.height[s] == l, "forward tile supernode");
+            MD_CHECK(s >= 0 && s < f.ns && f.level[s] == l, "forward tile supernode");
             int w = f.w[s], no = f.noff[s], c0 = f.c0[s];
             int qn = std::min(kTile, w - q0), rn = std::max(0, std::min(kTile, no - r0)), kcb = q0 / kTile;
             MD_CHECK(qn > 0 && q0 % kTile == 0 && r0 % kTile == 0, "forward tile origin");
@@ -762,7 +952,7 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
     for (int l = f.nlevels - 1; l >= 0; --l) {
         for (int t = f.bt_ptr[l]; t < f.bt_ptr[l + 1]; ++t) {
             int s = f.bt[4 * (size_t)t], q0 = f.bt[4 * (size_t)t + 1], p0 = f.bt[4 * (size_t)t + 2];
-            MD_CHECK(s >= 0 && s < f.ns && f.height[s] == l, "backward tile supernode");
+            MD_CHECK(s >= 0 && s < f.ns && f.level[s] == l, "backward tile supernode");
             int w = f.w[s], mm = w + f.noff[s];
             int qn = std::min(kTile, w - q0), pn = std::min(kTile, mm - p0);
             MD_CHECK(qn > 0 && pn > 0, "backward tile extent");
@@ -792,6 +982,58 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                 int64_t o = f.xoff[s] + (int64_t)(p0 / kTile) * w + q0 + lane;
                 MD_CHECK(o < f.xoff[s + 1], "solution output");
                 for (int c = 0; c < 3; ++c) xp[3 * o + c] = acc[c];
+            }
+        }
+    }
+    for (int i = 0; i < f.nsub; ++i) {
+        const int nh = f.sub[4 * (size_t)i + 1], entries = f.sub[4 * (size_t)i + 2];
+        lds.assign(3 * (size_t)entries, 0.0);
+        stamp.assign(entries, 0);
+        int phase = 0;
+        for (int h = nh - 1; h >= 0; --h) {
+            int a0, a1, b0, b1;
+            MD_CHECK(sub_range(i, h, 2, a0, a1) && sub_range(i, h, 3, b0, b1), "subtree backward range");
+            MD_CHECK(4 * (size_t)a1 <= f.sba.size() && 8 * (size_t)b1 <= f.sbb.size(), "subtree backward items");
+            ++phase;
+            for (int t = a0; t < a1; ++t) {
+                const int* it = f.sba.data() + 4 * (size_t)t;
+                MD_CHECK((it[0] >= 0) + (it[1] >= 0) + (it[2] >= 0) == 1 && it[0] < n, "subtree source item");
+                double v[3];
+                for (int c = 0; c < 3; ++c) {
+                    if (it[0] >= 0) {
+                        v[c] = z[3 * (size_t)it[0] + c];
+                    } else if (it[1] >= 0) {
+                        MD_CHECK(lds_read(it[1], entries, phase, c, v[c]), "subtree solution read");
+                        v[c] = -v[c];
+                    } else {
+                        MD_CHECK((size_t)it[2] < f.offrows.size(), "subtree off row entry");
+                        const int* xl = f.xloc.data() + 4 * (size_t)it[2];
+                        MD_CHECK(sum_part(xp, xl[0], xl[2], xl[1], 0, c, v[c]), "subtree outer solution");
+                        v[c] = -v[c];
+                    }
+                }
+                MD_CHECK(lds_claim(it[3], entries, phase), "subtree rhs write");
+                for (int c = 0; c < 3; ++c) lds[3 * (size_t)it[3] + c] = v[c];
+            }
+            ++phase;
+            for (int t = b0; t < b1; ++t) {
+                const int* it = f.sbb.data() + 8 * (size_t)t;
+                const int w = it[1], mm = it[2];
+                MD_CHECK(w > 0 && mm >= w && mm <= kSubMaxRows && it[0] >= 0 &&
+                         (int64_t)it[0] + (int64_t)(mm - 1) * w < (int64_t)pr.size(), "subtree P range");
+                double acc[3] = {0, 0, 0};
+                for (int p = 0; p < mm; ++p)
+                    for (int c = 0; c < 3; ++c) {
+                        double v;
+                        MD_CHECK(lds_read(it[3] + p, entries, phase, c, v), "subtree rhs read");
+                        acc[c] += pr[it[0] + (size_t)p * w] * v;
+                    }
+                MD_CHECK(lds_claim(it[4], entries, phase), "subtree solution write");
+                MD_CHECK(it[5] >= 0 && it[5] < nx, "subtree solution output");
+                for (int c = 0; c < 3; ++c) {
+                    lds[3 * (size_t)it[4] + c] = acc[c];
+                    xp[3 * (size_t)it[5] + c] = acc[c];
+                }
             }
         }
     }

@@ -19,6 +19,13 @@ constexpr int kMaxSnWidth = 1024;
 constexpr int kTile = 64;
 // Vertices per one-ring slice (one wavefront).
 constexpr int kSlice = 64;
+// Default LDS budget (vector entries) of one bottom subtree swept by a single wavefront.
+constexpr int kSubBudgetDefault = 512;
+// Largest budget: 2048 FP64 vector entries (32 bytes each) are 64 KiB of LDS.
+constexpr int kSubBudgetMax = 2048;
+// Largest supernode a bottom subtree may hold: columns, and rows of the front.
+constexpr int kSubMaxWidth = 64;
+constexpr int kSubMaxRows = 128;
 
 // One-ring topology in the order the kernels use ("internal" vertex ids):
 //   internal ids [0, nf) are the free vertices in solve (elimination) order,
@@ -56,6 +63,19 @@ struct Topology {
 //   backward tile (s, q0, p0): rows [p0, p0+64) of [Dinv_s; G_s]^T [z_s; -x_off], read
 //     from a row-major copy; it writes partial kb = p0/64 of x_s, so x_s is stored as
 //     kr_s = ceil(m/64) partials that every reader (descendants, the final update) sums.
+//
+// Bottom subtrees: a subtree whose supernodes are all small (w <= 64, m <= 128) and whose
+// fronts and columns fit an LDS budget is swept by one wavefront, height by height, with
+// every front in LDS (no level launches, no global round trips inside the subtree). Its
+// supernodes keep one partial of x_s (kr = 1); only the subtree root stores u_s (kc = 1,
+// the others kc = 0). The host resolves every index into flat item lists:
+//   forward, per height:  position items (front init + children's updates from LDS, z),
+//                         off-row items (u = v_o - G v_d in place; the root also to global);
+//   backward, per height: position items (t = z, or -x from LDS / from global partials),
+//                         column items (x_s = [Dinv; G]^T t, to LDS and to global).
+// Level schedule: supernodes outside subtrees get level = 1 + the largest level of their
+// children outside subtrees (0 if none); subtree sweeps run before the forward levels and
+// after the backward levels.
 struct Factor {
     int n = 0;                       // dimension (free vertices)
     int ns = 0;                      // supernodes
@@ -83,6 +103,17 @@ struct Factor {
     // tiles grouped by height level: level l occupies [ft_ptr[l], ft_ptr[l+1]) (4 ints each)
     std::vector<int> ft_ptr, ft;     // forward (s, r0, q0, 0)
     std::vector<int> bt_ptr, bt;     // backward (s, q0, p0, 0)
+    std::vector<int> level;          // device level of supernodes outside subtrees, -1 inside
+    // bottom subtrees
+    int nsub = 0, sub_lds = 0;       // subtrees; LDS vector entries of the largest
+    std::vector<int> sub;            // 4 per subtree: (height table offset, heights, LDS entries, root)
+    std::vector<int> sub_h;          // 4 per (subtree, height) + a terminal per subtree:
+                                     //   first forward position / off-row / backward position / column item
+    std::vector<int> sfa;            // 4 per forward position item: (b and z index or -1, LDS dst, contrib begin, end)
+    std::vector<int> sfc;            // LDS index of each child update entry
+    std::vector<int> sfb;            // 8 per off-row item: (G base, G stride, w, front base, LDS dst, u dst or -1, 0, 0)
+    std::vector<int> sba;            // 4 per backward position item: (z index, LDS x index, xloc entry: one >= 0; LDS dst)
+    std::vector<int> sbb;            // 8 per column item: (P base, P stride, m, front base, LDS x dst, x dst, 0, 0)
     int64_t factor_nnz() const { return poff.empty() ? 0 : poff.back(); }
 };
 
@@ -108,7 +139,14 @@ std::vector<int> order_graph(int m, const int* rowptr, const int* col, const dou
 
 // Symbolic supernodal analysis of the free-free graph in solve order.
 // adj_ptr/adj (CSR over solve indices, no diagonal needed) describe A's pattern.
-void symbolic(int n, const std::vector<int>& adj_ptr, const std::vector<int>& adj, Factor& f);
+// sub_budget: LDS vector entries of one bottom subtree (0: no subtree sweeps).
+void symbolic(int n, const std::vector<int>& adj_ptr, const std::vector<int>& adj, Factor& f,
+              int sub_budget = kSubBudgetDefault);
+
+// Device layout of a symbolic factor (partials, contribution lists, tiles, bottom subtrees)
+// for the given subtree budget; symbolic() calls it, and it may be called again to re-lay
+// the same structure out with another budget.
+void device_layout(Factor& f, int sub_budget);
 
 // FP64 multifrontal numeric factorization. A is given by its diagonal (diag[i]) and the
 // strictly-off-diagonal entries adj/adj_val (CSR in solve order, symmetric pattern).

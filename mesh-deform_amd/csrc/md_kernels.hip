@@ -12,6 +12,7 @@ namespace md {
 namespace {
 
 __device__ __forceinline__ int find_mesh(const LaunchSet &L, int b) {
+    if (L.bmesh) return L.bmesh[b];
     int lo = 0, hi = L.nactive - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -170,11 +171,18 @@ __device__ __forceinline__ void best_rotation(const T (&m)[9], T px, T py, T pz,
         jrot<T, 0, 1>(a, v); jrot<T, 0, 2>(a, v); jrot<T, 0, 3>(a, v);
         jrot<T, 1, 2>(a, v); jrot<T, 1, 3>(a, v); jrot<T, 2, 3>(a, v);
     }
+    // column of the largest eigenvalue, by selects (an index into v would put v in scratch)
     T best = a[0][0];
     T q0 = v[0][0], q1 = v[1][0], q2 = v[2][0], q3 = v[3][0];
-    if (a[1][1] > best) { best = a[1][1]; q0 = v[0][1]; q1 = v[1][1]; q2 = v[2][1]; q3 = v[3][1]; }
-    if (a[2][2] > best) { best = a[2][2]; q0 = v[0][2]; q1 = v[1][2]; q2 = v[2][2]; q3 = v[3][2]; }
-    if (a[3][3] > best) { best = a[3][3]; q0 = v[0][3]; q1 = v[1][3]; q2 = v[2][3]; q3 = v[3][3]; }
+#pragma unroll
+    for (int c = 1; c < 4; ++c) {
+        const bool b = a[c][c] > best;
+        best = b ? a[c][c] : best;
+        q0 = b ? v[0][c] : q0;
+        q1 = b ? v[1][c] : q1;
+        q2 = b ? v[2][c] : q2;
+        q3 = b ? v[3][c] : q3;
+    }
     T nrm = T(1) / tsqrt<T>(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
     // canonical sign: scalar part >= 0
     if (q0 < T(0)) nrm = -nrm;
@@ -335,20 +343,22 @@ template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
 }
 
 // Sum of cnt partial vectors U[k * stride + e], k = 0 .. cnt-1, in that order. One partial is
-// one load; more are loaded 16 at a time (clamped addresses, masked values) so their
+// one load; more are loaded 8 at a time (clamped addresses, masked values) so their
 // latencies overlap.
+constexpr int kPartialBatch = 8;
+
 template <typename V>
 __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e) {
     using T = decltype(V{}.x);
     if (cnt == 1) return U[e];
     V a;
     a.x = 0; a.y = 0; a.z = 0; a.w = 0;
-    for (int k0 = 0; k0 < cnt; k0 += 16) {
-        V b[16];
+    for (int k0 = 0; k0 < cnt; k0 += kPartialBatch) {
+        V b[kPartialBatch];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) b[u] = U[(long long)min(k0 + u, cnt - 1) * stride + e];
+        for (int u = 0; u < kPartialBatch; ++u) b[u] = U[(long long)min(k0 + u, cnt - 1) * stride + e];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        for (int u = 0; u < kPartialBatch; ++u) {
             T m = (k0 + u < cnt) ? T(1) : T(0);
             a.x += m * b[u].x; a.y += m * b[u].y; a.z += m * b[u].z;
         }
@@ -404,7 +414,7 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     using V = typename V4T<T>::type;
     __shared__ V vd[kTile];
     __shared__ V vo[kTile];
-    int k = find_mesh(L, blockIdx.x);
+    const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
     const int4 tl = M.ft[M.ft_ptr[level] + (blockIdx.x - L.prefix[k])];
     const SnDev S = M.sn[tl.x];
@@ -452,7 +462,7 @@ template <typename T>
 __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
     using V = typename V4T<T>::type;
     __shared__ V tv[kTile];
-    int k = find_mesh(L, blockIdx.x);
+    const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
     const int4 tl = M.bt[M.bt_ptr[level] + (blockIdx.x - L.prefix[k])];
     const SnDev S = M.sn[tl.x];
@@ -484,6 +494,113 @@ __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
         V out;
         out.x = ax; out.y = ay; out.z = az; out.w = 0;
         reinterpret_cast<V *>(M.xpart)[S.xoff + (long long)(p0 / kTile) * w + q0 + lane] = out;
+    }
+}
+
+// Bottom subtree, forward sweep, one wavefront per subtree. Heights bottom-up, each in two
+// phases with every front in LDS:
+//   positions: front[p] = (p < w ? r[c0 + p] : 0) + children's update entries; z = front[p < w],
+//   off rows:  front[w + i] -= G_s[i, 0..w) front[0 .. w)  (u_s in place; the root also stores
+//              u_s for its parent outside the subtree).
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    extern __shared__ __align__(32) unsigned char sub_smem[];
+    V *F = reinterpret_cast<V *>(sub_smem);
+    const int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    const int4 sd = M.sub[blockIdx.x - L.prefix[k]];
+    const int lane = threadIdx.x;
+    const V *R = reinterpret_cast<const V *>(M.r);
+    V *Z = reinterpret_cast<V *>(M.z);
+    V *U = reinterpret_cast<V *>(M.upart);
+    const T *G = tptr<T>(M.pc);
+    for (int h = 0; h < sd.y; ++h) {
+        const int4 h0 = M.sub_h[sd.x + h], h1 = M.sub_h[sd.x + h + 1];
+        for (int t = h0.x + lane; t < h1.x; t += kTile) {
+            const int4 it = M.sfa[t];
+            V v;
+            v.x = 0; v.y = 0; v.z = 0; v.w = 0;
+            if (it.x >= 0) v = R[it.x];
+            for (int c0 = it.z; c0 < it.w; c0 += 4) {
+                int e[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) e[u] = M.sfc[min(c0 + u, it.w - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (c0 + u < it.w) {
+                        const V a = F[e[u]];
+                        v.x += a.x; v.y += a.y; v.z += a.z;
+                    }
+                }
+            }
+            v.w = 0;
+            F[it.y] = v;
+            if (it.x >= 0) Z[it.x] = v;
+        }
+        __syncthreads();
+        for (int t = h0.y + lane; t < h1.y; t += kTile) {
+            const int4 a = M.sfb[2 * t], b = M.sfb[2 * t + 1];
+            T ax = 0, ay = 0, az = 0;
+            tile_dot_any<T, V>(G + a.x, a.y, F + a.w, a.z, ax, ay, az);
+            V o = F[b.x];
+            o.x -= ax; o.y -= ay; o.z -= az;
+            F[b.x] = o;
+            if (b.y >= 0) U[b.y] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// Bottom subtree, backward sweep, one wavefront per subtree. Heights top-down:
+//   positions: t[p] = z[c0 + p] (p < w), else -x of the off row, from LDS when the row lies
+//              in the subtree, else the sum of its owner's partials,
+//   columns:   x_s[q] = sum_p [Dinv_s; G_s][p, q] t[p], to LDS and to the single partial.
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    extern __shared__ __align__(32) unsigned char sub_smem[];
+    V *F = reinterpret_cast<V *>(sub_smem);
+    const int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    const int4 sd = M.sub[blockIdx.x - L.prefix[k]];
+    const int lane = threadIdx.x;
+    const V *Zs = reinterpret_cast<const V *>(M.z);
+    V *X = reinterpret_cast<V *>(M.xpart);
+    const T *P = tptr<T>(M.pr);
+    for (int h = sd.y - 1; h >= 0; --h) {
+        const int4 h0 = M.sub_h[sd.x + h], h1 = M.sub_h[sd.x + h + 1];
+        for (int t = h0.z + lane; t < h1.z; t += kTile) {
+            const int4 it = M.sba[t];
+            V v;
+            if (it.x >= 0) {
+                v = Zs[it.x];
+            } else {
+                V a;
+                if (it.y >= 0) {
+                    a = F[it.y];
+                } else {
+                    const int4 xl = M.xloc[it.z];
+                    a = sum_partials<V>(X + xl.x, xl.z, xl.y, 0);
+                }
+                v.x = -a.x; v.y = -a.y; v.z = -a.z;
+            }
+            v.w = 0;
+            F[it.w] = v;
+        }
+        __syncthreads();
+        for (int t = h0.w + lane; t < h1.w; t += kTile) {
+            const int4 a = M.sbb[2 * t], b = M.sbb[2 * t + 1];
+            T ax = 0, ay = 0, az = 0;
+            tile_dot_any<T, V>(P + a.x, a.y, F + a.w, min(a.z, kTile), ax, ay, az);
+            if (a.z > kTile)
+                tile_dot_any<T, V>(P + a.x + (long long)kTile * a.y, a.y, F + a.w + kTile, a.z - kTile, ax, ay, az);
+            V o;
+            o.x = ax; o.y = ay; o.z = az; o.w = 0;
+            F[b.x] = o;
+            X[b.y] = o;
+        }
+        __syncthreads();
     }
 }
 
@@ -862,6 +979,16 @@ void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
     if (blocks <= 0) return;
     if (fp64) hipLaunchKernelGGL(k_bwd<double>, dim3(blocks), dim3(kTile), 0, s, L, level);
     else hipLaunchKernelGGL(k_bwd<float>, dim3(blocks), dim3(kTile), 0, s, L, level);
+}
+void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int lds_entries, bool fp64) {
+    if (blocks <= 0) return;
+    if (fp64) hipLaunchKernelGGL(k_sub_fwd<double>, dim3(blocks), dim3(kTile), sizeof(d4) * lds_entries, s, L);
+    else hipLaunchKernelGGL(k_sub_fwd<float>, dim3(blocks), dim3(kTile), sizeof(f4) * lds_entries, s, L);
+}
+void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int lds_entries, bool fp64) {
+    if (blocks <= 0) return;
+    if (fp64) hipLaunchKernelGGL(k_sub_bwd<double>, dim3(blocks), dim3(kTile), sizeof(d4) * lds_entries, s, L);
+    else hipLaunchKernelGGL(k_sub_bwd<float>, dim3(blocks), dim3(kTile), sizeof(f4) * lds_entries, s, L);
 }
 void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
     if (blocks <= 0) return;

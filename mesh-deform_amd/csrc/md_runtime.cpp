@@ -49,6 +49,7 @@ Plan::~Plan() {
     drop_graph();
     if (d_meshes) hipFree(d_meshes);
     if (d_prefix) hipFree(d_prefix);
+    if (d_bmesh) hipFree(d_bmesh);
 }
 
 void Plan::drop_graph() {
@@ -60,7 +61,7 @@ static int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
                       const std::vector<const Factor *> &factors, bool fp64, int refine_steps,
-                      hipStream_t stream, bool sync) {
+                      hipStream_t stream, bool sync, bool block_maps) {
     plan.drop_graph();
     plan.fp64 = fp64;
     plan.refine_steps = std::max(1, refine_steps);
@@ -69,7 +70,7 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     plan.nlevels = 0;
     for (const Factor *f : factors) plan.nlevels = std::max(plan.nlevels, f->nlevels);
     int nl = plan.nlevels;
-    int ntables = 5 + 2 * nl;
+    int ntables = 6 + 2 * nl;
     plan.h_prefix.assign((size_t)ntables * (na + 1), 0);
     plan.off_fwd.assign(nl, 0);
     plan.off_bwd.assign(nl, 0);
@@ -89,15 +90,39 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
           [&](int k) { return cdiv(meshes[k]->nslices_free, kSlicesPerBlock); });
     table(3, plan.off_ident, plan.blk_ident, [&](int k) { return cdiv(meshes[k]->n, kBlock); });
     table(4, plan.off_apply, plan.blk_apply, [&](int k) { return cdiv(meshes[k]->nf, kBlock); });
+    table(5, plan.off_sub, plan.blk_sub, [&](int k) { return factors[k]->nsub; });
+    plan.sub_lds = 0;
+    for (const Factor *f : factors) plan.sub_lds = std::max(plan.sub_lds, f->sub_lds);
     for (int l = 0; l < nl; ++l) {
-        table(5 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
+        table(6 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
             const Factor *f = factors[k];
             return l < f->nlevels ? f->ft_ptr[l + 1] - f->ft_ptr[l] : 0;
         });
-        table(6 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
+        table(7 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
             const Factor *f = factors[k];
             return l < f->nlevels ? f->bt_ptr[l + 1] - f->bt_ptr[l] : 0;
         });
+    }
+    // block -> mesh maps: one entry per block of every table, so kernels of large batches
+    // find their mesh with one load instead of a binary search over the prefix table
+    plan.bmesh_off.assign(ntables, (size_t)-1);
+    plan.h_bmesh.clear();
+    if (block_maps && na > 1) {
+        for (int t = 0; t < ntables; ++t) {
+            const int *p = plan.h_prefix.data() + (size_t)t * (na + 1);
+            plan.bmesh_off[t] = plan.h_bmesh.size();
+            for (int k = 0; k < na; ++k)
+                for (int b = p[k]; b < p[k + 1]; ++b) plan.h_bmesh.push_back(k);
+        }
+        if (plan.h_bmesh.size() > plan.cap_bmesh) {
+            if (plan.d_bmesh) hipFree(plan.d_bmesh);
+            plan.d_bmesh = nullptr;
+            MD_HIP_E(hipMalloc(&plan.d_bmesh, sizeof(int) * std::max<size_t>(1, plan.h_bmesh.size())));
+            plan.cap_bmesh = plan.h_bmesh.size();
+        }
+        if (!plan.h_bmesh.empty())
+            MD_HIP_E(hipMemcpyAsync(plan.d_bmesh, plan.h_bmesh.data(), sizeof(int) * plan.h_bmesh.size(),
+                                    hipMemcpyHostToDevice, stream));
     }
     plan.h_meshes.resize(na);
     for (int k = 0; k < na; ++k) plan.h_meshes[k] = *meshes[k];
@@ -141,11 +166,13 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
             if (ev && r == 0) hipEventRecord(ev[4], stream);
             continue;
         }
+        launch_sub_fwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_lds, plan.fp64);
         for (int l = 0; l < plan.nlevels; ++l)
             launch_fwd(stream, plan.set(plan.off_fwd[l]), plan.blk_fwd[l], l, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[3], stream);
         for (int l = plan.nlevels - 1; l >= 0; --l)
             launch_bwd(stream, plan.set(plan.off_bwd[l]), plan.blk_bwd[l], l, plan.fp64);
+        launch_sub_bwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_lds, plan.fp64);
         launch_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[4], stream);
     }
@@ -188,6 +215,7 @@ extern "C" void md_config_default(md_config *cfg) {
     cfg->pcg_max_iters = 200;
     cfg->pcg_tol = 1e-7;
     cfg->use_graph = 1;
+    cfg->subtree_budget = kSubBudgetDefault;
 }
 
 extern "C" const char *md_last_error(void) { return g_err.c_str(); }
@@ -271,6 +299,10 @@ extern "C" int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const m
         set_error("md_mesh_create: PCG needs pcg_max_iters >= 1 and pcg_tol >= 0");
         return MD_ERR_INVALID;
     }
+    if (m->cfg.subtree_budget < 0 || m->cfg.subtree_budget > kSubBudgetMax) {
+        set_error("md_mesh_create: subtree_budget must lie in [0, " + std::to_string(kSubBudgetMax) + "]");
+        return MD_ERR_INVALID;
+    }
     m->cfg.refine_steps = std::max(1, m->cfg.refine_steps);
     m->n = n;
     m->F = F;
@@ -349,9 +381,12 @@ void analyze_host(md_mesh *m) {
         m->adj_ptr[i + 1] = (int)m->adj.size();
     }
     m->fac = Factor();
-    if (nf > 0 && nf < n && m->cfg.solver == MD_SOLVER_LDLT) symbolic(nf, m->adj_ptr, m->adj, m->fac);
+    if (nf > 0 && nf < n && m->cfg.solver == MD_SOLVER_LDLT)
+        symbolic(nf, m->adj_ptr, m->adj, m->fac, m->cfg.subtree_budget);
     else m->fac.n = nf;
 }
+
+int upload_layout(md_mesh *m);
 
 // Device part of the analysis: the arena with every static array uploaded, and the plan.
 int analyze_device(md_mesh *m) {
@@ -382,22 +417,10 @@ int analyze_device(md_mesh *m) {
     size_t o_z = A.reserve(vsz * std::max(1, nf));
     int ns = f.ns;
     auto last = [](const std::vector<int64_t> &v) { return v.empty() ? (int64_t)0 : v.back(); };
-    size_t o_up = A.reserve(vsz * std::max<int64_t>(1, last(f.uoff)));
-    size_t o_xp = A.reserve(vsz * std::max<int64_t>(1, last(f.xoff)));
-    size_t o_sn = A.reserve(sizeof(SnDev) * std::max(1, ns));
     size_t o_cp = A.reserve(sizeof(int) * (ns + 1));
     size_t o_ch = A.reserve(sizeof(int) * std::max<size_t>(1, f.child.size()));
-    size_t o_cptr = A.reserve(sizeof(int) * std::max<size_t>(1, f.cptr.size()));
-    size_t o_cen = A.reserve(sizeof(int) * std::max<size_t>(4, f.cent.size()));
-    size_t o_ri = A.reserve(sizeof(int) * std::max<size_t>(4, f.rowinfo.size()));
-    size_t o_xl = A.reserve(sizeof(int) * std::max<size_t>(4, f.xloc.size()));
     size_t o_pc = A.reserve(psz * std::max<int64_t>(1, last(f.pc_off)));
     size_t o_pr = A.reserve(psz * std::max<int64_t>(1, last(f.pr_off)));
-    int nl = f.nlevels;
-    size_t o_ftp = A.reserve(sizeof(int) * (nl + 1));
-    size_t o_btp = A.reserve(sizeof(int) * (nl + 1));
-    size_t o_ft = A.reserve(sizeof(int) * std::max<size_t>(4, f.ft.size()));
-    size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
     int eblocks = std::max(1, cdiv(t.nslices, kSlicesPerBlock));
     size_t o_en = A.reserve(sizeof(double) * eblocks);
     bool pcg = m->cfg.solver == MD_SOLVER_PCG;
@@ -434,14 +457,8 @@ int analyze_device(md_mesh *m) {
     d.pp = (d4 *)A.at(o_pp);
     d.quat = A.at(o_q);
     d.r = A.at(o_r); d.z = A.at(o_z);
-    d.upart = A.at(o_up); d.xpart = A.at(o_xp);
-    d.sn = (const SnDev *)A.at(o_sn);
     d.child_ptr = (const int *)A.at(o_cp); d.child = (const int *)A.at(o_ch);
-    d.cptr = (const int *)A.at(o_cptr); d.cent = (const int4 *)A.at(o_cen);
-    d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
     d.pc = A.at(o_pc); d.pr = A.at(o_pr);
-    d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
-    d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
     d.energy_part = (double *)A.at(o_en);
     d.w32 = (float *)A.at(o_w32);
     d.diag = A.at(o_dg);
@@ -450,6 +467,79 @@ int analyze_device(md_mesh *m) {
     d.pcg_tol2 = m->cfg.pcg_tol * m->cfg.pcg_tol;
     m->d_tgt_idx = (int *)A.at(o_ti);
     m->d_tgt_xyz = (double *)A.at(o_tx);
+    hipStream_t st = m->ctx->stream;
+    auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
+        if (bytes == 0) return hipSuccess;
+        return hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, st);
+    };
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(up(d.slice_ptr, t.slice_ptr.data(), sizeof(int) * (t.nslices + 1)));
+    chk(up(d.sell_col, t.sell_col.data(), sizeof(int) * slots));
+    chk(up(d.tri, t.tri_internal.data(), sizeof(int) * 3 * (size_t)m->F));
+    chk(up(d.slot_fe_ptr, t.slot_fe_ptr.data(), sizeof(int) * (slots + 1)));
+    chk(up(d.slot_fe, t.slot_fe.data(), sizeof(int) * t.slot_fe.size()));
+    if (ns > 0) {
+        chk(up(d.child_ptr, f.child_ptr.data(), sizeof(int) * (ns + 1)));
+        chk(up(d.child, f.child.data(), sizeof(int) * f.child.size()));
+    }
+    chk(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+        set_error(std::string("upload of the mesh analysis failed: ") + hipGetErrorString(e));
+        return MD_ERR_HIP;
+    }
+    int rc = upload_layout(m);
+    if (rc) return rc;
+    ++m->gen;
+    m->analyzed = true;
+    return MD_OK;
+}
+
+// The sweep layout of the factor (device_layout) in its own arena, and the mesh's plan.
+// Called after the analysis and again when the layout is re-chosen (relayout); the panels
+// and every other device array are untouched.
+int upload_layout(md_mesh *m) {
+    const Factor &f = m->fac;
+    const int ns = f.ns, nl = f.nlevels;
+    const bool fp64 = m->fp64();
+    const size_t vsz = fp64 ? sizeof(d4) : sizeof(f4);
+    auto last = [](const std::vector<int64_t> &v) { return v.empty() ? (int64_t)0 : v.back(); };
+    Arena &A = m->larena;
+    A.reset();
+    size_t o_up = A.reserve(vsz * std::max<int64_t>(1, last(f.uoff)));
+    size_t o_xp = A.reserve(vsz * std::max<int64_t>(1, last(f.xoff)));
+    size_t o_sn = A.reserve(sizeof(SnDev) * std::max(1, ns));
+    size_t o_cptr = A.reserve(sizeof(int) * std::max<size_t>(1, f.cptr.size()));
+    size_t o_cen = A.reserve(sizeof(int) * std::max<size_t>(4, f.cent.size()));
+    size_t o_ri = A.reserve(sizeof(int) * std::max<size_t>(4, f.rowinfo.size()));
+    size_t o_xl = A.reserve(sizeof(int) * std::max<size_t>(4, f.xloc.size()));
+    size_t o_ftp = A.reserve(sizeof(int) * (nl + 1));
+    size_t o_btp = A.reserve(sizeof(int) * (nl + 1));
+    size_t o_ft = A.reserve(sizeof(int) * std::max<size_t>(4, f.ft.size()));
+    size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
+    size_t o_sub = A.reserve(sizeof(int) * std::max<size_t>(4, f.sub.size()));
+    size_t o_subh = A.reserve(sizeof(int) * std::max<size_t>(4, f.sub_h.size()));
+    size_t o_sfa = A.reserve(sizeof(int) * std::max<size_t>(4, f.sfa.size()));
+    size_t o_sfc = A.reserve(sizeof(int) * std::max<size_t>(4, f.sfc.size()));
+    size_t o_sfb = A.reserve(sizeof(int) * std::max<size_t>(8, f.sfb.size()));
+    size_t o_sba = A.reserve(sizeof(int) * std::max<size_t>(4, f.sba.size()));
+    size_t o_sbb = A.reserve(sizeof(int) * std::max<size_t>(8, f.sbb.size()));
+    if (A.allocate() != hipSuccess) {
+        set_error("device allocation of " + std::to_string(A.size()) + " bytes (sweep layout) failed");
+        A.reset();
+        return MD_ERR_NOMEM;
+    }
+    MeshDev &d = m->dev;
+    d.upart = A.at(o_up); d.xpart = A.at(o_xp);
+    d.sn = (const SnDev *)A.at(o_sn);
+    d.cptr = (const int *)A.at(o_cptr); d.cent = (const int4 *)A.at(o_cen);
+    d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
+    d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
+    d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
+    d.sub = (const int4 *)A.at(o_sub); d.sub_h = (const int4 *)A.at(o_subh);
+    d.sfa = (const int4 *)A.at(o_sfa); d.sfc = (const int *)A.at(o_sfc);
+    d.sfb = (const int4 *)A.at(o_sfb); d.sba = (const int4 *)A.at(o_sba);
+    d.sbb = (const int4 *)A.at(o_sbb);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -464,15 +554,8 @@ int analyze_device(md_mesh *m) {
     }
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-    chk(up(d.slice_ptr, t.slice_ptr.data(), sizeof(int) * (t.nslices + 1)));
-    chk(up(d.sell_col, t.sell_col.data(), sizeof(int) * slots));
-    chk(up(d.tri, t.tri_internal.data(), sizeof(int) * 3 * (size_t)m->F));
-    chk(up(d.slot_fe_ptr, t.slot_fe_ptr.data(), sizeof(int) * (slots + 1)));
-    chk(up(d.slot_fe, t.slot_fe.data(), sizeof(int) * t.slot_fe.size()));
     if (ns > 0) {
         chk(up(d.sn, snh.data(), sizeof(SnDev) * ns));
-        chk(up(d.child_ptr, f.child_ptr.data(), sizeof(int) * (ns + 1)));
-        chk(up(d.child, f.child.data(), sizeof(int) * f.child.size()));
         chk(up(d.cptr, f.cptr.data(), sizeof(int) * f.cptr.size()));
         chk(up(d.cent, f.cent.data(), sizeof(int) * f.cent.size()));
         chk(up(d.rowinfo, f.rowinfo.data(), sizeof(int) * f.rowinfo.size()));
@@ -481,10 +564,17 @@ int analyze_device(md_mesh *m) {
         chk(up(d.bt_ptr, f.bt_ptr.data(), sizeof(int) * (nl + 1)));
         chk(up(d.ft, f.ft.data(), sizeof(int) * f.ft.size()));
         chk(up(d.bt, f.bt.data(), sizeof(int) * f.bt.size()));
+        chk(up(d.sub, f.sub.data(), sizeof(int) * f.sub.size()));
+        chk(up(d.sub_h, f.sub_h.data(), sizeof(int) * f.sub_h.size()));
+        chk(up(d.sfa, f.sfa.data(), sizeof(int) * f.sfa.size()));
+        chk(up(d.sfc, f.sfc.data(), sizeof(int) * f.sfc.size()));
+        chk(up(d.sfb, f.sfb.data(), sizeof(int) * f.sfb.size()));
+        chk(up(d.sba, f.sba.data(), sizeof(int) * f.sba.size()));
+        chk(up(d.sbb, f.sbb.data(), sizeof(int) * f.sbb.size()));
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
-        set_error(std::string("upload of the mesh analysis failed: ") + hipGetErrorString(e));
+        set_error(std::string("upload of the sweep layout failed: ") + hipGetErrorString(e));
         return MD_ERR_HIP;
     }
     std::vector<const MeshDev *> ms{&m->dev};
@@ -496,15 +586,7 @@ int analyze_device(md_mesh *m) {
         set_error(std::string("launch plan upload failed: ") + hipGetErrorString(e));
         return MD_ERR_HIP;
     }
-    m->analyzed = true;
     return MD_OK;
-}
-
-int analyze(md_mesh *m) {
-    auto t0 = Clock::now();
-    analyze_host(m);
-    m->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-    return analyze_device(m);
 }
 
 // Uploads positions (original order, 3 per vertex) as d4 in internal order.
@@ -634,18 +716,6 @@ int factor_upload(md_mesh *m) {
     return MD_OK;
 }
 
-int factorize(md_mesh *m) {
-    if (m->cfg.solver == MD_SOLVER_PCG) return factor_pcg(m);
-    int rc = weights_download(m);
-    if (rc) return rc;
-    std::string err;
-    if (!factor_host(m, err)) {
-        set_error(err);
-        return MD_ERR_FACTOR;
-    }
-    return factor_upload(m);
-}
-
 // The dirty branch of deform (arap.h:102-120), in phases so that the host work of many
 // meshes can run in parallel (md_meshes_prepare) while device work stays on one stream.
 struct Prep {
@@ -746,6 +816,115 @@ int iterate(md_mesh *m, int iterations) {
 }
 
 }  // namespace
+
+namespace md {
+
+int relayout(md_mesh *m, int budget) {
+    hipError_t e = hipStreamSynchronize(m->ctx->stream);  // the old layout may still be in use
+    if (e != hipSuccess) { set_error(std::string("relayout: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    m->cfg.subtree_budget = budget;
+    device_layout(m->fac, budget);
+    int rc = upload_layout(m);
+    ++m->gen;
+    return rc;
+}
+
+StateSnapshot::~StateSnapshot() {
+    for (void *p : buf)
+        if (p) hipFree(p);
+}
+
+hipError_t snapshot_state(const std::vector<md_mesh *> &meshes, hipStream_t st, StateSnapshot &s) {
+    for (md_mesh *m : meshes) {
+        size_t qb = (m->fp64() ? sizeof(d4) : sizeof(f4)) * (size_t)m->n, pb = sizeof(d4) * (size_t)m->n;
+        void *p = nullptr;
+        MD_HIP_E(hipMalloc(&p, pb + qb));
+        s.buf.push_back(p);
+        MD_HIP_E(hipMemcpyAsync(p, m->dev.pp, pb, hipMemcpyDeviceToDevice, st));
+        MD_HIP_E(hipMemcpyAsync(static_cast<char *>(p) + pb, m->dev.quat, qb, hipMemcpyDeviceToDevice, st));
+    }
+    return hipStreamSynchronize(st);
+}
+
+hipError_t restore_state(const std::vector<md_mesh *> &meshes, hipStream_t st, const StateSnapshot &s) {
+    for (size_t k = 0; k < meshes.size(); ++k) {
+        md_mesh *m = meshes[k];
+        size_t qb = (m->fp64() ? sizeof(d4) : sizeof(f4)) * (size_t)m->n, pb = sizeof(d4) * (size_t)m->n;
+        MD_HIP_E(hipMemcpyAsync(m->dev.pp, s.buf[k], pb, hipMemcpyDeviceToDevice, st));
+        MD_HIP_E(hipMemcpyAsync(m->dev.quat, static_cast<const char *>(s.buf[k]) + pb, qb, hipMemcpyDeviceToDevice, st));
+    }
+    return hipStreamSynchronize(st);
+}
+
+hipError_t time_plan(Plan &plan, hipStream_t st, int iters, bool use_graph, double &ms) {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, st);
+    if (e == hipSuccess) e = run_iterations(plan, st, iters, use_graph);
+    if (e == hipSuccess) e = hipEventRecord(b, st);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    ms = t;
+    return e;
+}
+
+}  // namespace md
+
+namespace {
+
+// Checks the candidate budgets of an autotune call.
+int check_budgets(const char *who, const int *budgets, int count, int iterations) {
+    if (!budgets || count <= 0 || iterations <= 0) {
+        set_error(std::string(who) + ": need at least one candidate and one iteration");
+        return MD_ERR_INVALID;
+    }
+    for (int i = 0; i < count; ++i)
+        if (budgets[i] < 0 || budgets[i] > kSubBudgetMax) {
+            set_error(std::string(who) + ": budget " + std::to_string(budgets[i]) + " outside [0, " +
+                      std::to_string(kSubBudgetMax) + "]");
+            return MD_ERR_INVALID;
+        }
+    return MD_OK;
+}
+
+}  // namespace
+
+extern "C" int md_mesh_autotune(md_mesh *m, const int *budgets, int count, int iterations, int *best_budget,
+                                double *best_ms) {
+    if (!m) { set_error("md_mesh_autotune: null mesh"); return MD_ERR_INVALID; }
+    int rc = check_budgets("md_mesh_autotune", budgets, count, iterations);
+    if (rc) return rc;
+    if (m->dirty || !m->analyzed || !m->factored) { set_error("md_mesh_autotune: mesh is not prepared"); return MD_ERR_STATE; }
+    if (m->cfg.solver != MD_SOLVER_LDLT) { set_error("md_mesh_autotune: only the LDLT solver has a sweep layout"); return MD_ERR_INVALID; }
+    if (m->servers > 0) { set_error("md_mesh_autotune: the mesh is registered with a server"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    hipStream_t st = m->ctx->stream;
+    std::vector<md_mesh *> ms{m};
+    StateSnapshot snap;
+    MD_HIP(snapshot_state(ms, st, snap));
+    const bool g = m->cfg.use_graph != 0;
+    int best = m->cfg.subtree_budget;
+    double best_t = 1e300;
+    for (int i = 0; i < count; ++i) {
+        rc = relayout(m, budgets[i]);
+        if (rc) return rc;
+        double t = 0.0;
+        MD_HIP(run_iterations(m->plan, st, 1, g));  // first replay captures the graph
+        MD_HIP(restore_state(ms, st, snap));
+        MD_HIP(time_plan(m->plan, st, iterations, g, t));
+        MD_HIP(restore_state(ms, st, snap));
+        if (t < best_t) { best_t = t; best = budgets[i]; }
+    }
+    rc = relayout(m, best);
+    if (rc) return rc;
+    if (best_budget) *best_budget = best;
+    if (best_ms) *best_ms = best_t / iterations;
+    return MD_OK;
+}
 
 extern "C" int md_mesh_prepare(md_mesh *m, const double *xyz) {
     if (!m || !xyz) { set_error("md_mesh_prepare: null pointer"); return MD_ERR_INVALID; }
@@ -984,7 +1163,7 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->factor_nnz = s.strict_nnz_equiv;
     out->fwd_items = s.fwd_items;
     out->bwd_items = s.bwd_items;
-    out->device_bytes = (int64_t)m->arena.size();
+    out->device_bytes = (int64_t)(m->arena.size() + m->larena.size());
     out->setup_ms_host = m->setup_ms_host;
     // algorithmic bytes per iteration: each array element a phase must touch, counted once
     const Topology &t = m->topo;
@@ -1006,13 +1185,22 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
         xp += (int64_t)f.kr[k] * f.w[k];
         offr += f.noff[k];
     }
-    for (size_t e = 0; e < f.offrows.size(); ++e) xloc_reads += f.xloc[4 * e + 2];
+    // solution partials gathered from HBM: off rows of level-swept supernodes, and off rows
+    // of subtree supernodes that lie outside their subtree (the others are read from LDS)
+    for (int k = 0; k < f.ns; ++k)
+        if (f.level[k] >= 0)
+            for (int e = f.roff[k]; e < f.roff[k + 1]; ++e) xloc_reads += f.xloc[4 * (size_t)e + 2];
+    for (size_t i = 0; i < f.sba.size(); i += 4)
+        if (f.sba[i + 2] >= 0) xloc_reads += f.xloc[4 * (size_t)f.sba[i + 2] + 2];
+    // index lists of the bottom-subtree sweeps
+    const int64_t sub_fwd_idx = 4 * (int64_t)(f.sfa.size() + f.sfc.size() + f.sfb.size());
+    const int64_t sub_bwd_idx = 4 * (int64_t)(f.sba.size() + f.sbb.size());
     // forward: G panel, r in, z out, update partials out and read back by the parent, rel
-    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + offr * 4;
+    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + offr * 4 + sub_fwd_idx;
     // backward + update: Dinv and G panels, z in, x partials of ancestors gathered (with
     // their locators), x partials out, final sum and the p' update
     out->bytes_backward = (dw + gw) * es + (int64_t)t.nf * vs + xloc_reads * vs + offr * 16 + 2 * xp * vs +
-                          (int64_t)t.nf * (16 + 2 * ps);
+                          (int64_t)t.nf * (16 + 2 * ps) + sub_bwd_idx;
     return MD_OK;
 }
 
@@ -1064,7 +1252,7 @@ extern "C" int md_mesh_time_iterations(md_mesh *m, int warmup, int iterations, d
 
 extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
                                     const double *xyz, int nrhs, const double *b, double *x,
-                                    int64_t *panel_entries, int *levels) {
+                                    int64_t *panel_entries, int *levels, int subtree_budget) {
     if (n <= 0 || !rowptr || !col || !val || !xyz || nrhs < 0 || (nrhs > 0 && (!b || !x))) {
         set_error("md_host_factor_solve: invalid argument");
         return MD_ERR_INVALID;
@@ -1084,7 +1272,7 @@ extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, co
         ptr[k + 1] = (int)adj.size();
     }
     Factor f;
-    symbolic(n, ptr, adj, f);
+    symbolic(n, ptr, adj, f, subtree_budget < 0 ? kSubBudgetDefault : std::min(subtree_budget, kSubBudgetMax));
     std::string err;
     if (!numeric(ptr, adj, av, diag, f, err)) { set_error(err); return MD_ERR_FACTOR; }
     std::vector<double> bp((size_t)n * nrhs), xp((size_t)n * nrhs);

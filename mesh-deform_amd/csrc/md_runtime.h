@@ -47,13 +47,17 @@ private:
 struct Plan {
     int nactive = 0;
     int nlevels = 0;
-    int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0, blk_apply = 0;
+    int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0, blk_apply = 0, blk_sub = 0;
     std::vector<int> blk_fwd, blk_bwd;
-    int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0, off_apply = 0;
+    int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0, off_apply = 0, off_sub = 0;
+    int sub_lds = 0;                   // LDS vector entries of the largest bottom subtree
     std::vector<int> off_fwd, off_bwd;
     MeshDev *d_meshes = nullptr;
     int *d_prefix = nullptr;
-    size_t cap_meshes = 0, cap_prefix = 0;
+    int *d_bmesh = nullptr;            // block -> mesh maps of every table (static batches)
+    size_t cap_meshes = 0, cap_prefix = 0, cap_bmesh = 0;
+    std::vector<int> h_bmesh;
+    std::vector<size_t> bmesh_off;     // per table: offset into d_bmesh (SIZE_MAX: no map)
     std::vector<int> h_prefix;
     std::vector<MeshDev> h_meshes;
     bool fp64 = true;
@@ -63,7 +67,11 @@ struct Plan {
     hipGraphExec_t graph = nullptr;
     ~Plan();
     void drop_graph();
-    LaunchSet set(int off) const { return LaunchSet{d_meshes, d_prefix + off, nactive}; }
+    LaunchSet set(int off) const {
+        size_t t = (size_t)off / (size_t)(nactive + 1);
+        const int *bm = (d_bmesh && t < bmesh_off.size() && bmesh_off[t] != (size_t)-1) ? d_bmesh + bmesh_off[t] : nullptr;
+        return LaunchSet{d_meshes, d_prefix + off, nactive, bm};
+    }
 };
 
 // Builds (or rebuilds) a plan for the given meshes and uploads its tables on `stream`.
@@ -71,11 +79,26 @@ struct Plan {
 // stay unchanged until the stream has passed them.
 hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
                       const std::vector<const Factor *> &factors, bool fp64, int refine_steps,
-                      hipStream_t stream, bool sync = true);
+                      hipStream_t stream, bool sync = true, bool block_maps = false);
 // Enqueues one ARAP iteration (local step + refine_steps x (residual, sweeps)).
 void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *phase_events);
 // Runs `iters` iterations, from a captured graph when use_graph is set.
 hipError_t run_iterations(Plan &plan, hipStream_t stream, int iters, bool use_graph);
+
+// Sweep layout of a prepared LDLT mesh with another bottom-subtree budget (device_layout +
+// a new layout arena + the mesh's plan). Invalidates plans built from the old layout.
+int relayout(md_mesh *m, int budget);
+
+// Device copies of the iteration state (p' and rotations) of meshes, for trial runs.
+struct StateSnapshot {
+    std::vector<void *> buf;
+    ~StateSnapshot();
+};
+hipError_t snapshot_state(const std::vector<md_mesh *> &meshes, hipStream_t st, StateSnapshot &s);
+hipError_t restore_state(const std::vector<md_mesh *> &meshes, hipStream_t st, const StateSnapshot &s);
+
+// Event-timed replay of `iters` iterations of a plan (ms for all of them).
+hipError_t time_plan(Plan &plan, hipStream_t st, int iters, bool use_graph, double &ms);
 
 }  // namespace md
 
@@ -106,12 +129,14 @@ struct md_mesh {
     std::vector<double> rest;      // rest positions, original order
     md::Topology topo;
     md::Factor fac;
-    md::Arena arena;
+    md::Arena arena;               // everything that does not depend on the sweep layout
+    md::Arena larena;              // sweep layout: partials, contribution lists, tiles, subtrees
     md::MeshDev dev{};
+    uint64_t gen = 0;              // bumped whenever device pointers in dev change
+    std::atomic<int> servers{0};   // servers this mesh is registered with (its layout is fixed)
     md::Plan plan;                 // single-mesh launch plan
     std::vector<int> adj_ptr, adj; // free-free pattern in solve order
     std::vector<int64_t> adj_slot; // slot of each adj entry
-    std::vector<int64_t> diag_slot_ptr; // unused placeholder for symmetry with adj
     double setup_ms_host = 0.0;
     std::vector<double> w_host;    // cotan weights on the host (factorization input)
     std::vector<char> pc_host, pr_host;  // device panel layouts staged on the host

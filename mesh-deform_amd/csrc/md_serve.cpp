@@ -17,17 +17,45 @@ using namespace md;
 struct md_batch {
     md_ctx *ctx = nullptr;
     std::vector<md_mesh *> meshes;
+    std::vector<uint64_t> gens;  // device layout generation of each mesh when the plan was built
     Plan plan;
     bool use_graph = true;
 };
+
+namespace {
+
+hipError_t batch_plan(md_batch *b) {
+    std::vector<const MeshDev *> devs;
+    std::vector<const Factor *> facs;
+    b->gens.clear();
+    for (md_mesh *m : b->meshes) {
+        devs.push_back(&m->dev);
+        facs.push_back(&m->fac);
+        b->gens.push_back(m->gen);
+    }
+    const md_config &c = b->meshes[0]->cfg;
+    return build_plan(b->plan, devs, facs, c.precision == MD_PREC_FP64, c.refine_steps, b->ctx->stream, true, true);
+}
+
+// A mesh re-prepared with a new constraint set, or given another sweep layout, since the
+// plan was built has moved its device arrays: launching the old plan would read freed memory.
+int batch_current(md_batch *b, const char *who) {
+    for (size_t k = 0; k < b->meshes.size(); ++k)
+        if (b->meshes[k]->gen != b->gens[k] || b->meshes[k]->dirty) {
+            set_error(std::string(who) + ": mesh " + std::to_string(k) +
+                      " was re-prepared or re-laid out after the batch was built");
+            return MD_ERR_STATE;
+        }
+    return MD_OK;
+}
+
+}  // namespace
 
 extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, md_batch **out) {
     if (!ctx || !out || count <= 0 || !meshes) { set_error("md_batch_create: invalid argument"); return MD_ERR_INVALID; }
     *out = nullptr;
     std::unique_ptr<md_batch> b(new md_batch());
     b->ctx = ctx;
-    std::vector<const MeshDev *> devs;
-    std::vector<const Factor *> facs;
     int prec = meshes[0]->cfg.precision, refine = meshes[0]->cfg.refine_steps;
     for (int k = 0; k < count; ++k) {
         md_mesh *m = meshes[k];
@@ -38,8 +66,6 @@ extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, m
             return MD_ERR_INVALID;
         }
         b->meshes.push_back(m);
-        devs.push_back(&m->dev);
-        facs.push_back(&m->fac);
     }
     b->use_graph = meshes[0]->cfg.use_graph != 0;
     for (int k = 0; k < count; ++k)
@@ -50,7 +76,7 @@ extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, m
     b->plan.pcg = meshes[0]->cfg.solver == MD_SOLVER_PCG;
     b->plan.pcg_iters = meshes[0]->cfg.pcg_max_iters;
     hipSetDevice(ctx->device);
-    hipError_t e = build_plan(b->plan, devs, facs, prec == MD_PREC_FP64, refine, ctx->stream);
+    hipError_t e = batch_plan(b.get());
     if (e != hipSuccess) { set_error(std::string("md_batch_create: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     ctx_retain(ctx);
     *out = b.release();
@@ -59,6 +85,7 @@ extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, m
 
 extern "C" int md_batch_iterate(md_batch *b, int iterations) {
     if (!b || iterations < 0) { set_error("md_batch_iterate: invalid argument"); return MD_ERR_INVALID; }
+    if (int rc = batch_current(b, "md_batch_iterate")) return rc;
     hipSetDevice(b->ctx->device);
     hipError_t e = run_iterations(b->plan, b->ctx->stream, iterations, b->use_graph);
     if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
@@ -69,6 +96,7 @@ extern "C" int md_batch_iterate(md_batch *b, int iterations) {
 
 extern "C" int md_batch_time_iterations(md_batch *b, int warmup, int iterations, double *total_ms) {
     if (!b || iterations < 0 || warmup < 0) { set_error("md_batch_time_iterations: invalid argument"); return MD_ERR_INVALID; }
+    if (int rc = batch_current(b, "md_batch_time_iterations")) return rc;
     hipSetDevice(b->ctx->device);
     hipStream_t st = b->ctx->stream;
     hipError_t e = run_iterations(b->plan, st, warmup, b->use_graph);
@@ -86,6 +114,56 @@ extern "C" int md_batch_time_iterations(md_batch *b, int warmup, int iterations,
     if (c) hipEventDestroy(c);
     if (e != hipSuccess) { set_error(std::string("md_batch_time_iterations: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     if (total_ms) *total_ms = ms;
+    return MD_OK;
+}
+
+extern "C" int md_batch_autotune(md_batch *b, const int *budgets, int count, int iterations, int *best_budget,
+                                 double *best_ms) {
+    if (!b) { set_error("md_batch_autotune: null batch"); return MD_ERR_INVALID; }
+    if (!budgets || count <= 0 || iterations <= 0) {
+        set_error("md_batch_autotune: need at least one candidate and one iteration");
+        return MD_ERR_INVALID;
+    }
+    for (int i = 0; i < count; ++i)
+        if (budgets[i] < 0 || budgets[i] > kSubBudgetMax) {
+            set_error("md_batch_autotune: budget " + std::to_string(budgets[i]) + " outside [0, " +
+                      std::to_string(kSubBudgetMax) + "]");
+            return MD_ERR_INVALID;
+        }
+    if (int rc = batch_current(b, "md_batch_autotune")) return rc;
+    if (b->meshes[0]->cfg.solver != MD_SOLVER_LDLT) {
+        set_error("md_batch_autotune: only the LDLT solver has a sweep layout");
+        return MD_ERR_INVALID;
+    }
+    for (md_mesh *m : b->meshes)
+        if (m->servers > 0) { set_error("md_batch_autotune: a mesh is registered with a server"); return MD_ERR_STATE; }
+    hipSetDevice(b->ctx->device);
+    hipStream_t st = b->ctx->stream;
+    StateSnapshot snap;
+    hipError_t e = snapshot_state(b->meshes, st, snap);
+    if (e != hipSuccess) { set_error(std::string("md_batch_autotune: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    int best = b->meshes[0]->cfg.subtree_budget;
+    double best_t = 1e300;
+    auto use = [&](int budget) -> int {
+        for (md_mesh *m : b->meshes)
+            if (int rc = relayout(m, budget)) return rc;
+        hipError_t x = batch_plan(b);
+        if (x != hipSuccess) { set_error(std::string("md_batch_autotune: ") + hipGetErrorString(x)); return MD_ERR_HIP; }
+        return MD_OK;
+    };
+    for (int i = 0; i < count; ++i) {
+        if (int rc = use(budgets[i])) return rc;
+        double t = 0.0;
+        e = run_iterations(b->plan, st, 1, b->use_graph);  // first replay captures the graph
+        if (e == hipSuccess) e = restore_state(b->meshes, st, snap);
+        if (e == hipSuccess) e = time_plan(b->plan, st, iterations, b->use_graph, t);
+        if (e == hipSuccess) e = restore_state(b->meshes, st, snap);
+        if (e != hipSuccess) { set_error(std::string("md_batch_autotune: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+        if (t < best_t) { best_t = t; best = budgets[i]; }
+    }
+    if (int rc = use(best)) return rc;
+    if (best_budget) *best_budget = best;
+    if (best_ms) *best_ms = best_t / iterations;
     return MD_OK;
 }
 
@@ -308,7 +386,7 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
             return MD_ERR_INVALID;
         }
     }
-    size_t bytes = m->arena.size();
+    size_t bytes = m->arena.size() + m->larena.size();
     if (S->cfg.hbm_budget_bytes > 0 && S->resident + bytes > S->cfg.hbm_budget_bytes) {
         set_error("md_server_add_mesh: mesh needs " + std::to_string(bytes) + " device bytes, " +
                   std::to_string(S->cfg.hbm_budget_bytes - S->resident) + " left in the HBM budget");
@@ -316,6 +394,7 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
     }
     S->resident += bytes;
     S->meshes.push_back(m);
+    ++m->servers;
     S->busy.push_back(0);
     S->pinned_free.emplace_back();
     *mesh_id = (int)S->meshes.size() - 1;
@@ -424,6 +503,7 @@ extern "C" int md_server_destroy(md_server *S) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (d4 *p : S->pinned_all) hipHostFree(p);
+    for (md_mesh *m : S->meshes) --m->servers;
     delete S;
     ctx_release(ctx);
     return MD_OK;

@@ -19,6 +19,9 @@ PREC_FP64, PREC_FP32_REFINE = 0, 1
 SOLVER_LDLT, SOLVER_PCG = 0, 1
 REST_REFERENCE, REST_FIXED = 0, 1
 
+# candidate bottom-subtree budgets of the sweep-layout autotuner (LDS vector entries; 0: none)
+AUTOTUNE_BUDGETS = (0, 256, 512, 1024, 2048)
+
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _vp = ctypes.c_void_p
@@ -27,7 +30,7 @@ _vp = ctypes.c_void_p
 class Config(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("solver", ctypes.c_int), ("refine_steps", ctypes.c_int),
                 ("rest_mode", ctypes.c_int), ("pcg_max_iters", ctypes.c_int), ("pcg_tol", ctypes.c_double),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("subtree_budget", ctypes.c_int)]
 
 
 class MeshStats(ctypes.Structure):
@@ -86,6 +89,10 @@ _SIGS = [
     ("md_batch_create", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("md_batch_iterate", ctypes.c_int, [_vp, ctypes.c_int]),
     ("md_batch_time_iterations", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    ("md_mesh_autotune", ctypes.c_int, [_vp, _i32p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_double)]),
+    ("md_batch_autotune", ctypes.c_int, [_vp, _i32p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_double)]),
     ("md_batch_destroy", ctypes.c_int, [_vp]),
     ("md_server_create", ctypes.c_int, [_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
     ("md_server_add_mesh", ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int)]),
@@ -96,7 +103,7 @@ _SIGS = [
     ("md_server_wait", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_double, _vp, ctypes.POINTER(ctypes.c_double)]),
     ("md_server_destroy", ctypes.c_int, [_vp]),
     ("md_host_factor_solve", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _f64p, _f64p,
-                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
 ]
 
 
@@ -131,7 +138,7 @@ def device_count():
 
 
 def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_graph=True, solver=SOLVER_LDLT,
-           pcg_iters=None, pcg_tol=None):
+           pcg_iters=None, pcg_tol=None, subtree_budget=None):
     c = Config()
     lib().md_config_default(ctypes.byref(c))
     c.precision = precision
@@ -143,6 +150,8 @@ def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_gr
         c.pcg_max_iters = int(pcg_iters)
     if pcg_tol is not None:
         c.pcg_tol = float(pcg_tol)
+    if subtree_budget is not None:
+        c.subtree_budget = int(subtree_budget)
     return c
 
 
@@ -266,6 +275,18 @@ class Mesh:
         check(lib().md_mesh_time_iterations(self._h, int(warmup), int(iters), ctypes.byref(tot), ph))
         return tot.value, ph
 
+    def autotune(self, budgets=AUTOTUNE_BUDGETS, iters=5):
+        """md_mesh_autotune: keeps the fastest sweep layout; returns (budget, ms per iteration)."""
+        return _autotune(lib().md_mesh_autotune, self._h, budgets, iters)
+
+
+def _autotune(fn, h, budgets, iters):
+    b = np.ascontiguousarray(budgets, np.int32)
+    best = ctypes.c_int(0)
+    ms = ctypes.c_double(0)
+    check(fn(h, b, len(b), int(iters), ctypes.byref(best), ctypes.byref(ms)))
+    return best.value, ms.value
+
 
 def prepare_many(ctx, meshes, positions=None, nthreads=0):
     """md_meshes_prepare: the dirty setup of many meshes, host work on parallel threads."""
@@ -292,6 +313,10 @@ class Batch:
         tot = ctypes.c_double(0)
         check(lib().md_batch_time_iterations(self._h, int(warmup), int(iters), ctypes.byref(tot)))
         return tot.value
+
+    def autotune(self, budgets=AUTOTUNE_BUDGETS, iters=3):
+        """md_batch_autotune: one sweep layout for every mesh; returns (budget, ms per iteration)."""
+        return _autotune(lib().md_batch_autotune, self._h, budgets, iters)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -359,8 +384,9 @@ class Server:
             pass
 
 
-def host_factor_solve(rp, ci, v, xyz, b):
-    """Host-only: the library's ordering + symbolic + FP64 block LDL^T, solved on the CPU."""
+def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1):
+    """Host-only: the library's ordering + symbolic + FP64 block LDL^T, solved on the CPU by an
+    emulation of the device sweeps (subtree_budget as in Config; -1: the default)."""
     rp = np.ascontiguousarray(rp, np.int32)
     ci = np.ascontiguousarray(ci, np.int32)
     v = np.ascontiguousarray(v, np.float64)
@@ -371,5 +397,5 @@ def host_factor_solve(rp, ci, v, xyz, b):
     lv = ctypes.c_int(0)
     check(lib().md_host_factor_solve(n, rp, ci, v, np.ascontiguousarray(xyz, np.float64).ravel(), b2.shape[0],
                                      np.ascontiguousarray(b2).ravel(), x.reshape(-1), ctypes.byref(pe),
-                                     ctypes.byref(lv)))
+                                     ctypes.byref(lv), int(subtree_budget)))
     return (x[0] if np.ndim(b) == 1 else x), pe.value, lv.value

@@ -112,6 +112,22 @@ def test_deform_fp32_refine_matches_golden(ctx, golden_dir, name):
     assert err < POS_TOL_FP32
 
 
+@pytest.mark.parametrize("name", ["sphere_obj", "bar_obj", "uv_sphere"])
+@pytest.mark.parametrize("budget", [0, 64, 2048])
+def test_deform_sweep_layouts_match_golden(ctx, golden_dir, name, budget):
+    """Every sweep layout (level tiles only, small and large bottom subtrees swept in LDS)
+    gives the oracle's positions: 1e-9 in FP64, 1e-5 in FP32 + refinement."""
+    say("case: sweep layout", name, "budget", budget)
+    g = gold(golden_dir, name)
+    for prec, tol in ((mdarap.PREC_FP64, POS_TOL_FP64), (mdarap.PREC_FP32_REFINE, POS_TOL_FP32)):
+        m, ok = run_gpu(ctx, g["xyz"], g["tri"], g["con_idx"], g["con_xyz"], int(g["iters"]), precision=prec,
+                        subtree_budget=budget)
+        assert ok
+        err = meshgen.relative_error(m.mesh, g["pos"])
+        say("  precision", prec, "levels", m.stats()["levels"], "rel err", err)
+        assert err < tol
+
+
 def test_energy_trace_monotone(ctx):
     say("case: energy trace")
     pr = meshgen.sphere_problem()
@@ -228,6 +244,66 @@ def test_batch_equals_individual(ctx):
     b.iterate(4)
     for m, ref in zip(bm, outs):
         np.testing.assert_array_equal(m.download(), ref)
+
+
+def _prepared(ctx, probs, budget=None):
+    out = []
+    for pr in probs:
+        idx, tgt = pr.constraints()
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget))
+        m.set_constraints(idx, tgt)
+        m.prepare(pr.xyz)
+        out.append(m)
+    return out
+
+
+def test_mesh_autotune_keeps_state_and_results(ctx):
+    """The autotuner leaves the mesh state as it found it and keeps a candidate layout; the
+    iterations after it give what a mesh created with that layout gives, bitwise."""
+    say("case: mesh autotune")
+    pr = meshgen.noisy_icosphere_problem(5)
+    (m,) = _prepared(ctx, [pr])
+    before = m.download()
+    best, ms = m.autotune((0, 128, 2048), iters=2)
+    say("  best budget", best, "ms/iter", ms)
+    assert best in (0, 128, 2048) and ms > 0
+    np.testing.assert_array_equal(m.download(), before)
+    m.iterate(5)
+    (ref,) = _prepared(ctx, [pr], budget=best)
+    ref.iterate(5)
+    np.testing.assert_array_equal(m.download(), ref.download())
+    # tuning between iterations: the warm-start rotations are restored too
+    m2, ref2 = _prepared(ctx, [pr, pr], budget=best)
+    m2.iterate(2)
+    ref2.iterate(2)
+    m2.autotune((0, 2048, best), iters=2)
+    m2.autotune((best,), iters=1)
+    m2.iterate(3)
+    ref2.iterate(3)
+    np.testing.assert_array_equal(m2.download(), ref2.download())
+
+
+def test_batch_autotune_and_stale_batch(ctx):
+    """Batch autotune: one layout for all meshes, results bitwise those of meshes created with
+    it; a batch whose mesh was re-laid out afterwards refuses to launch."""
+    say("case: batch autotune")
+    probs = meshgen.mixed_batch(5, 500, 8000, seed=13)
+    meshes = _prepared(ctx, probs)
+    b = mdarap.Batch(ctx, meshes)
+    best, ms = b.autotune((0, 512, 2048), iters=2)
+    say("  best budget", best, "ms/iter", ms)
+    assert best in (0, 512, 2048)
+    b.iterate(4)
+    refs = _prepared(ctx, probs, budget=best)
+    for r in refs:
+        r.iterate(4)
+    for m, r in zip(meshes, refs):
+        np.testing.assert_array_equal(m.download(), r.download())
+    meshes[0].autotune((0,), iters=1)
+    with pytest.raises(mdarap.MDError) as e:
+        b.iterate(1)
+    assert e.value.code == mdarap.MD_ERR_STATE
+    b.close()
 
 
 # ---- Jacobi-PCG alternative ------------------------------------------------------------------

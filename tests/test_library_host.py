@@ -55,15 +55,22 @@ def _problem_system(pr):
 
 @pytest.mark.parametrize("make", [meshgen.sphere_problem, lambda: meshgen.bar_problem(80, 8, 8),
                                   lambda: meshgen.noisy_icosphere_problem(4)])
-def test_host_block_ldlt_matches_scipy(make):
+@pytest.mark.parametrize("budget", [0, 64, 512, 2048])
+def test_host_block_ldlt_matches_scipy(make, budget):
+    """The device sweeps (level tiles, and bottom subtrees under each LDS budget), emulated
+    on the host with every index bounds-checked and LDS phases race-checked, solve L x = b."""
     pr = make()
     L, fx = _problem_system(pr)
     b = np.random.default_rng(1).standard_normal((3, L.shape[0]))
-    x, panel_entries, levels = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b)
+    x, panel_entries, levels = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, subtree_budget=budget)
     ref = spla.spsolve(L.tocsc(), b.T).T
     np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
     assert np.abs(L @ x.T - b.T).max() < 1e-10 * np.abs(b).max()
-    assert levels >= 1 and panel_entries >= L.nnz // 2
+    assert panel_entries >= L.nnz // 2
+    assert levels >= 1 if budget == 0 else levels >= 0
+    if budget == 0:
+        _, _, lv_default = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b)
+        assert lv_default < levels  # the default budget takes the bottom levels into subtrees
 
 
 def test_host_block_ldlt_dense_small():
@@ -74,9 +81,11 @@ def test_host_block_ldlt_dense_small():
     A = A @ A.T + n * np.eye(n)
     S = sp.csr_matrix(A)
     b = rng.standard_normal(n)
-    x, pe, lv = mdarap.host_factor_solve(S.indptr, S.indices, S.data, rng.standard_normal((n, 3)), b)
-    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-11)
-    assert lv == 1 and pe == n * n
+    xyz = rng.standard_normal((n, 3))
+    for budget, levels in ((0, 1), (-1, 0)):  # one level tile, or one subtree (no level)
+        x, pe, lv = mdarap.host_factor_solve(S.indptr, S.indices, S.data, xyz, b, subtree_budget=budget)
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-11)
+        assert lv == levels and pe == n * n
 
 
 def test_host_block_ldlt_rejects_singular():
