@@ -67,6 +67,9 @@ def plain_leg(ctx, pr, steps, warmup):
     t0 = time.perf_counter()
     m.prepare(pr.xyz)
     setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    budget, _ = m.autotune()  # sweep layout: the fastest bottom-subtree budget (state restored)
+    tune_s = time.perf_counter() - t0
     m.iterate(warmup)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -77,6 +80,8 @@ def plain_leg(ctx, pr, steps, warmup):
     # the phases (local, residual, forward sweeps, backward sweeps + update)
     ev_total_ms, phase_ms = m.time_iterations(0, steps)
     st = m.stats()
+    st["subtree_budget"] = budget
+    st["autotune_s"] = tune_s
     return m, st, setup_s, wall, ev_total_ms, phase_ms
 
 
@@ -130,6 +135,8 @@ def served_leg(ctx, count, seconds, rate, iters, seed):
         utils.append(tj.DeformationUtil(pr.xyz, pr.handles, origin=origin))
         trajs.append(tj.random_trajectory(rng))
     mdarap.prepare_many(ctx, meshes)
+    for m in meshes:
+        m.autotune(iters=3)
     setup_s = time.perf_counter() - t0
     srv = mdarap.Server(ctx)
     ids = [srv.add_mesh(m) for m in meshes]
@@ -183,20 +190,34 @@ def batched_leg(ctx, count, iters, seed):
         m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
         m.set_constraints(idx, tgt)
         meshes.append(m)
+    t0 = time.perf_counter()
     mdarap.prepare_many(ctx, meshes)
-    # plain: each mesh iterated alone (its own graph), back to back
+    setup_s = time.perf_counter() - t0
+    # plain: each mesh with its own tuned sweep layout, iterated alone (its own graph), back
+    # to back
+    t0 = time.perf_counter()
+    for m in meshes:
+        m.autotune(iters=3)
+    tune_plain_s = time.perf_counter() - t0
+    for m in meshes:
+        m.iterate(1)  # graph capture
+    ctx.synchronize()
     t0 = time.perf_counter()
     for m in meshes:
         m.iterate(iters)
     plain_s = time.perf_counter() - t0
     b = mdarap.Batch(ctx, meshes)
+    t0 = time.perf_counter()
+    budget, _ = b.autotune()
+    tune_batch_s = time.perf_counter() - t0
     b.iterate(2)  # warm-up (graph capture)
     ms = b.time_iterations(0, iters)
     b.close()
     total_iters = count * iters
     return dict(meshes=count, total_vertices=int(sum(p.n for p in probs)), iterations=iters,
                 batched_iters_per_s=total_iters / (ms / 1e3), plain_iters_per_s=total_iters / plain_s,
-                speedup=(total_iters / (ms / 1e3)) / (total_iters / plain_s))
+                speedup=(total_iters / (ms / 1e3)) / (total_iters / plain_s), batch_subtree_budget=budget,
+                setup_s=setup_s, autotune_plain_s=tune_plain_s, autotune_batch_s=tune_batch_s)
 
 
 def canonical_bytes(pr, st):
@@ -358,7 +379,8 @@ def main():
                                "refinement per iteration",
                    "n": pr.n, "faces": pr.F, "free": st["nf"], "supernodes": st["supernodes"],
                    "levels": st["levels"], "panel_entries": st["panel_entries"],
-                   "factor_nnz": st["factor_nnz"], "setup_s": round(setup_s, 3)},
+                   "factor_nnz": st["factor_nnz"], "setup_s": round(setup_s, 3),
+                   "subtree_budget": st["subtree_budget"], "autotune_s": round(st["autotune_s"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None if traffic is None else int(traffic),

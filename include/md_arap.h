@@ -60,9 +60,10 @@ typedef struct {
     int pcg_max_iters; /* MD_SOLVER_PCG: CG iterations per solve, default 200 */
     double pcg_tol;    /* MD_SOLVER_PCG: relative residual target, default 1e-7 */
     int use_graph;     /* replay iterations from a captured hipGraph, default 1 */
-    int subtree_budget;/* MD_SOLVER_LDLT: LDS vector entries of one bottom subtree of the
-                          factor swept by a single wavefront (fronts + columns of its
-                          supernodes); 0 sweeps every supernode by level launches; default 512 */
+    int subtree_budget;/* MD_SOLVER_LDLT: LDS vector entries (at most 1024) of one bottom
+                          subtree of the factor swept by a single wavefront (fronts + columns of
+                          its supernodes); 0 sweeps every supernode by level launches;
+                          default 512 */
 } md_config;
 
 typedef struct {
@@ -147,7 +148,7 @@ int md_mesh_time_iterations(md_mesh *mesh, int warmup, int iterations, double *t
                             double *phase_ms4);
 
 /* Autotuner of the sweep layout (MD_SOLVER_LDLT, prepared mesh): for each of the `count`
- * candidate bottom-subtree budgets (md_config.subtree_budget, each in [0, 2048]) the factor
+ * candidate bottom-subtree budgets (md_config.subtree_budget, each in [0, 1024]) the factor
  * is re-laid out on the device and `iterations` iterations are timed with HIP events; the
  * fastest candidate is kept (also in the mesh's config, for later re-preparations). The
  * positions and rotations are restored after every trial, so the mesh state is unchanged.

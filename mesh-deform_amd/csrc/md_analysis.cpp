@@ -455,9 +455,13 @@ void device_layout(Factor& f, int sub_budget) {
     std::vector<int> sub_root(nn, -1), size(nn, 1);
     std::vector<int64_t> lds(nn, 0);
     std::vector<char> ok(nn, 0);
-    int64_t panel_total = 0;
-    for (int t = 0; t < nn; ++t) panel_total += (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
-    if (sub_budget > 0 && panel_total < (int64_t)INT32_MAX / 2) {
+    // subtree records hold int offsets, and a partial count (< 2^15) with a stride (< 2^16)
+    int64_t panel_total = 0, max_kr = 0;
+    for (int t = 0; t < nn; ++t) {
+        panel_total += (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
+        max_kr = std::max<int64_t>(max_kr, cdiv(f.w[t] + f.noff[t], kTile));
+    }
+    if (sub_budget > 0 && panel_total < (int64_t)INT32_MAX / 2 && max_kr < (1 << 15)) {
         for (int t = 0; t < nn; ++t) {
             int m = f.w[t] + f.noff[t];
             bool good = f.w[t] <= kSubMaxWidth && m <= kSubMaxRows;
@@ -468,7 +472,7 @@ void device_layout(Factor& f, int sub_budget) {
                 lds[t] += lds[c];
                 size[t] += size[c];
             }
-            ok[t] = good && lds[t] <= sub_budget;
+            ok[t] = good && lds[t] <= sub_budget && size[t] <= kSubMaxNodes;
         }
         for (int t = nn - 1; t >= 0; --t) {
             int p = f.parent[t];
@@ -572,35 +576,47 @@ void device_layout(Factor& f, int sub_budget) {
                 tt[0] = t; tt[1] = cb * kTile; tt[2] = pb * kTile;
             }
     }
-    // bottom subtree items (the subtree of root r is the id range [r - size + 1, r])
-    f.nsub = 0; f.sub_lds = 0;
-    f.sub.clear(); f.sub_h.clear();
-    f.sfa.clear(); f.sfc.clear(); f.sfb.clear(); f.sba.clear(); f.sbb.clear();
-    std::vector<int> fb(nn, 0);
+    // bottom subtrees (the subtree of root r is the id range [r - size + 1, r]): per-supernode
+    // records by height, contribution entries, backward off-row sources
+    f.nsub = 0; f.sub_lds_fwd = 0; f.sub_lds_bwd = 0; f.sub_nodes = 0;
+    f.sub.clear(); f.sub_h.clear(); f.snode.clear(); f.scp.clear(); f.sfc.clear(); f.sbo.clear();
+    std::vector<int> fb(nn, 0), fbb(nn, 0);
     for (int r = 0; r < nn; ++r) {
         if (sub_root[r] != r) continue;
         const int first = r - size[r] + 1, nh = f.height[r] + 1;
         const int col0 = f.c0[first], col1 = f.c0[r] + f.w[r];
-        int fronts = 0;
-        for (int t = first; t <= r; ++t) { fb[t] = fronts; fronts += f.w[t] + f.noff[t]; }
-        const int xs0 = fronts, entries = fronts + (col1 - col0);
-        f.sub.push_back((int)(f.sub_h.size() / 4));
-        f.sub.push_back(nh);
-        f.sub.push_back(entries);
-        f.sub.push_back(r);
-        f.sub_lds = std::max(f.sub_lds, entries);
-        ++f.nsub;
         std::vector<std::vector<int>> at(nh);
         for (int t = first; t <= r; ++t) at[f.height[t]].push_back(t);
+        // forward: every front; backward: the columns, then the fronts of one height
+        int fronts = 0, height_fronts = 0;
+        for (int t = first; t <= r; ++t) { fb[t] = fronts; fronts += f.w[t] + f.noff[t]; }
+        const int ncols = col1 - col0;
+        for (int h = 0; h < nh; ++h) {
+            int off = ncols;
+            for (int t : at[h]) { fbb[t] = off; off += f.w[t] + f.noff[t]; }
+            height_fronts = std::max(height_fronts, off - ncols);
+        }
+        const int rec0 = (int)(f.snode.size() / 16);
+        f.sub.push_back(rec0);
+        f.sub.push_back(rec0 + size[r]);
+        f.sub.push_back((int)f.sub_h.size());
+        f.sub.push_back(nh);
+        f.sub_lds_fwd = std::max(f.sub_lds_fwd, fronts);
+        f.sub_lds_bwd = std::max(f.sub_lds_bwd, ncols + height_fronts);
+        f.sub_nodes = std::max(f.sub_nodes, size[r]);
+        ++f.nsub;
         std::vector<std::vector<int>> lists;
         for (int h = 0; h <= nh; ++h) {
-            f.sub_h.push_back((int)(f.sfa.size() / 4));
-            f.sub_h.push_back((int)(f.sfb.size() / 8));
-            f.sub_h.push_back((int)(f.sba.size() / 4));
-            f.sub_h.push_back((int)(f.sbb.size() / 8));
+            f.sub_h.push_back((int)(f.snode.size() / 16));
             if (h == nh) break;
+            int pa = 0, pb = 0, pq = 0;
             for (int t : at[h]) {
                 const int w = f.w[t], no = f.noff[t], m = w + no;
+                int rec[16] = {f.c0[t], w, no, fb[t], (int)f.pc_off[t], (int)f.pr_off[t], (int)(f.scp.size() / 2),
+                               (int)(f.sbo.size() / 2), t == r ? (int)f.uoff[t] : -1, (int)f.xoff[t],
+                               f.c0[t] - col0, pa, pb, pq, fbb[t], 0};
+                f.snode.insert(f.snode.end(), rec, rec + 16);
+                pa += m; pb += no; pq += w;
                 // children's update entries landing on each position, in child order
                 lists.assign(m, {});
                 for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
@@ -609,32 +625,26 @@ void device_layout(Factor& f, int sub_budget) {
                         lists[f.rel[f.roff[c] + e]].push_back(fb[c] + f.w[c] + e);
                 }
                 for (int p = 0; p < m; ++p) {
-                    f.sfa.push_back(p < w ? f.c0[t] + p : -1);
-                    f.sfa.push_back(fb[t] + p);
-                    f.sfa.push_back((int)f.sfc.size());
-                    f.sfc.insert(f.sfc.end(), lists[p].begin(), lists[p].end());
-                    f.sfa.push_back((int)f.sfc.size());
-                }
-                for (int i = 0; i < no; ++i) {
-                    int v[8] = {(int)(f.pc_off[t] + i), no, w, fb[t], fb[t] + w + i,
-                                t == r ? (int)(f.uoff[t] + i) : -1, 0, 0};
-                    f.sfb.insert(f.sfb.end(), v, v + 8);
-                }
-                for (int p = 0; p < m; ++p) {
-                    int v[4] = {-1, -1, -1, fb[t] + p};
-                    if (p < w) {
-                        v[0] = f.c0[t] + p;
+                    const std::vector<int>& L = lists[p];
+                    f.scp.push_back(L.size() > 0 ? L[0] : -1);
+                    if (L.size() <= 2) {
+                        f.scp.push_back(L.size() > 1 ? L[1] : -1);
                     } else {
-                        int e = f.roff[t] + p - w, row = f.offrows[e];
-                        if (row >= col0 && row < col1) v[1] = xs0 + row - col0;
-                        else v[2] = e;
+                        f.scp.push_back(-2 - (int)f.sfc.size());
+                        f.sfc.insert(f.sfc.end(), L.begin() + 1, L.end());
+                        f.sfc.push_back(-1);
                     }
-                    f.sba.insert(f.sba.end(), v, v + 4);
                 }
-                for (int q = 0; q < w; ++q) {
-                    int v[8] = {(int)(f.pr_off[t] + q), w, m, fb[t], xs0 + f.c0[t] + q - col0,
-                                (int)(f.xoff[t] + q), 0, 0};
-                    f.sbb.insert(f.sbb.end(), v, v + 8);
+                for (int e = f.roff[t]; e < f.roff[t + 1]; ++e) {
+                    int row = f.offrows[e];
+                    if (row >= col0 && row < col1) {
+                        f.sbo.push_back(row - col0);
+                        f.sbo.push_back(-1);
+                    } else {  // the owner's partials of x at that row
+                        const int* ri = f.rowinfo.data() + 4 * (size_t)row;
+                        f.sbo.push_back(ri[0]);
+                        f.sbo.push_back((ri[2] << 16) | ri[1]);
+                    }
                 }
             }
         }
@@ -821,18 +831,12 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
         }
         return true;
     };
-    // bottom subtrees: LDS entries carry the phase that wrote them; a read must see an entry
-    // written in an earlier phase (a same-phase read would race on the device)
+    // bottom subtrees, as the kernels run them: records of the subtree, lanes mapped to
+    // (supernode, item) by binary search over the prefixes of a height. LDS entries carry
+    // the phase that wrote them; a read must see an entry written in an earlier phase (a
+    // same-phase read would race on the device).
     std::vector<double> lds;
     std::vector<int> stamp;
-    auto sub_range = [&](int i, int h, int comp, int& a, int& b) -> bool {
-        int hoff = f.sub[4 * (size_t)i];
-        size_t e0 = 4 * (size_t)(hoff + h) + comp, e1 = 4 * (size_t)(hoff + h + 1) + comp;
-        if (hoff < 0 || e1 >= f.sub_h.size()) return false;
-        a = f.sub_h[e0];
-        b = f.sub_h[e1];
-        return a >= 0 && a <= b;
-    };
     auto lds_read = [&](int e, int entries, int phase, int c, double& v) -> bool {
         if (e < 0 || e >= entries || stamp[e] <= 0 || stamp[e] >= phase) return false;
         v = lds[3 * (size_t)e + c];
@@ -843,57 +847,113 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
         stamp[e] = phase;
         return true;
     };
+    // records [a, b) of one height; field: 11 positions, 12 off rows, 13 columns; count of
+    // the items of record j: m, noff or w
+    auto rec = [&](int j) { return f.snode.data() + 16 * (size_t)j; };
+    auto items = [&](int j, int field) {
+        const int* R = rec(j);
+        return field == 11 ? R[1] + R[2] : field == 12 ? R[2] : R[1];
+    };
+    auto find = [&](int a, int b, int field, int t) {
+        int lo = a, hi = b - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (rec(mid)[field] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    auto sub_heights = [&](int i, int h, int& a, int& b) -> bool {
+        size_t e = (size_t)f.sub[4 * (size_t)i + 2] + h;
+        if (e + 1 >= f.sub_h.size()) return false;
+        a = f.sub_h[e];
+        b = f.sub_h[e + 1];
+        return a >= f.sub[4 * (size_t)i] && a < b && b <= f.sub[4 * (size_t)i + 1] && 16 * (size_t)b <= f.snode.size();
+    };
+    // LDS entries one subtree uses in each sweep
+    auto sub_entries = [&](int i, bool fwd) {
+        int e = 0;
+        for (int j = f.sub[4 * (size_t)i]; j < f.sub[4 * (size_t)i + 1]; ++j) {
+            const int* R = rec(j);
+            if (fwd) e = std::max(e, R[3] + R[1] + R[2]);
+            else e = std::max(e, std::max(R[10] + R[1], R[14] + R[1] + R[2]));
+        }
+        return e;
+    };
     MD_CHECK((int)f.sub.size() == 4 * f.nsub, "subtree table");
     for (int i = 0; i < f.nsub; ++i) {
-        const int nh = f.sub[4 * (size_t)i + 1], entries = f.sub[4 * (size_t)i + 2];
-        MD_CHECK(entries > 0 && entries <= f.sub_lds && nh > 0, "subtree extent");
+        const int nh = f.sub[4 * (size_t)i + 3], entries = sub_entries(i, true);
+        MD_CHECK(entries > 0 && entries <= f.sub_lds_fwd && nh > 0, "subtree extent");
+        MD_CHECK(f.sub[4 * (size_t)i + 1] - f.sub[4 * (size_t)i] <= f.sub_nodes, "subtree records");
         lds.assign(3 * (size_t)entries, 0.0);
         stamp.assign(entries, 0);
         int phase = 0;
         for (int h = 0; h < nh; ++h) {
-            int a0, a1, b0, b1;
-            MD_CHECK(sub_range(i, h, 0, a0, a1) && sub_range(i, h, 1, b0, b1), "subtree forward range");
-            MD_CHECK(4 * (size_t)a1 <= f.sfa.size() && 8 * (size_t)b1 <= f.sfb.size(), "subtree forward items");
+            int ra, rb;
+            MD_CHECK(sub_heights(i, h, ra, rb), "subtree height range");
             ++phase;
-            for (int t = a0; t < a1; ++t) {
-                const int* it = f.sfa.data() + 4 * (size_t)t;
-                MD_CHECK(it[0] < n && it[2] >= 0 && it[2] <= it[3] && (size_t)it[3] <= f.sfc.size(),
-                         "subtree position item");
+            const int npos = rec(rb - 1)[11] + items(rb - 1, 11);
+            for (int t = 0; t < npos; ++t) {
+                const int j = find(ra, rb, 11, t);
+                const int* R = rec(j);
+                const int p = t - R[11], w = R[1];
+                MD_CHECK(p >= 0 && p < w + R[2], "subtree position map");
+                MD_CHECK(R[6] >= 0 && 2 * ((size_t)R[6] + p) + 1 < f.scp.size(), "subtree contribution entry");
+                // child entries: the first two inline, the rest from a -1-terminated list
+                std::vector<int> src;
+                const int e0 = f.scp[2 * (R[6] + p)], e1 = f.scp[2 * (R[6] + p) + 1];
+                if (e0 >= 0) src.push_back(e0);
+                if (e1 >= 0) {
+                    src.push_back(e1);
+                } else if (e1 <= -2) {
+                    MD_CHECK(e0 >= 0, "subtree overflow without a first entry");
+                    for (size_t c = (size_t)(-2 - e1);; ++c) {
+                        MD_CHECK(c < f.sfc.size(), "subtree overflow list");
+                        if (f.sfc[c] < 0) break;
+                        src.push_back(f.sfc[c]);
+                    }
+                }
                 double v[3] = {0, 0, 0};
                 for (int c = 0; c < 3; ++c) {
-                    if (it[0] >= 0) v[c] = b[(size_t)c * n + it[0]];
-                    for (int q = it[2]; q < it[3]; ++q) {
+                    if (p < w) {
+                        MD_CHECK(R[0] + p < n, "subtree rhs index");
+                        v[c] = b[(size_t)c * n + R[0] + p];
+                    }
+                    for (int e : src) {
                         double u;
-                        MD_CHECK(lds_read(f.sfc[q], entries, phase, c, u), "subtree child update read");
+                        MD_CHECK(lds_read(e, entries, phase, c, u), "subtree child update read");
                         v[c] += u;
                     }
                 }
-                MD_CHECK(lds_claim(it[1], entries, phase), "subtree front write");
+                MD_CHECK(lds_claim(R[3] + p, entries, phase), "subtree front write");
                 for (int c = 0; c < 3; ++c) {
-                    lds[3 * (size_t)it[1] + c] = v[c];
-                    if (it[0] >= 0) z[3 * (size_t)it[0] + c] = v[c];
+                    lds[3 * (size_t)(R[3] + p) + c] = v[c];
+                    if (p < w) z[3 * (size_t)(R[0] + p) + c] = v[c];
                 }
             }
             ++phase;
-            for (int t = b0; t < b1; ++t) {
-                const int* it = f.sfb.data() + 8 * (size_t)t;
-                const int w = it[2], stride = it[1];
-                MD_CHECK(w > 0 && w <= kTile && it[0] >= 0 &&
-                         (int64_t)it[0] + (int64_t)(w - 1) * stride < (int64_t)pc.size(), "subtree G range");
+            const int nrow = rec(rb - 1)[12] + items(rb - 1, 12);
+            for (int t = 0; t < nrow; ++t) {
+                const int j = find(ra, rb, 12, t);
+                const int* R = rec(j);
+                const int i_ = t - R[12], w = R[1], no = R[2];
+                MD_CHECK(i_ >= 0 && i_ < no && w <= kTile, "subtree off-row map");
+                MD_CHECK(R[4] >= 0 && (int64_t)R[4] + i_ + (int64_t)(w - 1) * no < (int64_t)pc.size(), "subtree G range");
                 double acc[3] = {0, 0, 0}, o[3];
                 for (int q = 0; q < w; ++q)
                     for (int c = 0; c < 3; ++c) {
                         double v;
-                        MD_CHECK(lds_read(it[3] + q, entries, phase, c, v), "subtree front read");
-                        acc[c] += pc[it[0] + (size_t)q * stride] * v;
+                        MD_CHECK(lds_read(R[3] + q, entries, phase, c, v), "subtree front read");
+                        acc[c] += pc[R[4] + i_ + (size_t)q * no] * v;
                     }
-                for (int c = 0; c < 3; ++c) MD_CHECK(lds_read(it[4], entries, phase, c, o[c]), "subtree off-row read");
-                MD_CHECK(lds_claim(it[4], entries, phase), "subtree update write");
-                MD_CHECK(it[5] < nu, "subtree update output");
+                const int dst = R[3] + w + i_;
+                for (int c = 0; c < 3; ++c) MD_CHECK(lds_read(dst, entries, phase, c, o[c]), "subtree off-row read");
+                MD_CHECK(lds_claim(dst, entries, phase), "subtree update write");
+                MD_CHECK(R[8] < 0 || R[8] + i_ < nu, "subtree update output");
                 for (int c = 0; c < 3; ++c) {
                     double u = o[c] - acc[c];
-                    lds[3 * (size_t)it[4] + c] = u;
-                    if (it[5] >= 0) up[3 * (size_t)it[5] + c] = u;
+                    lds[3 * (size_t)dst + c] = u;
+                    if (R[8] >= 0) up[3 * (size_t)(R[8] + i_) + c] = u;
                 }
             }
         }
@@ -986,53 +1046,60 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
         }
     }
     for (int i = 0; i < f.nsub; ++i) {
-        const int nh = f.sub[4 * (size_t)i + 1], entries = f.sub[4 * (size_t)i + 2];
+        const int nh = f.sub[4 * (size_t)i + 3], entries = sub_entries(i, false);
+        MD_CHECK(entries > 0 && entries <= f.sub_lds_bwd, "subtree extent (backward)");
         lds.assign(3 * (size_t)entries, 0.0);
         stamp.assign(entries, 0);
         int phase = 0;
         for (int h = nh - 1; h >= 0; --h) {
-            int a0, a1, b0, b1;
-            MD_CHECK(sub_range(i, h, 2, a0, a1) && sub_range(i, h, 3, b0, b1), "subtree backward range");
-            MD_CHECK(4 * (size_t)a1 <= f.sba.size() && 8 * (size_t)b1 <= f.sbb.size(), "subtree backward items");
+            int ra, rb;
+            MD_CHECK(sub_heights(i, h, ra, rb), "subtree height range");
             ++phase;
-            for (int t = a0; t < a1; ++t) {
-                const int* it = f.sba.data() + 4 * (size_t)t;
-                MD_CHECK((it[0] >= 0) + (it[1] >= 0) + (it[2] >= 0) == 1 && it[0] < n, "subtree source item");
+            const int npos = rec(rb - 1)[11] + items(rb - 1, 11);
+            for (int t = 0; t < npos; ++t) {
+                const int j = find(ra, rb, 11, t);
+                const int* R = rec(j);
+                const int p = t - R[11], w = R[1];
                 double v[3];
-                for (int c = 0; c < 3; ++c) {
-                    if (it[0] >= 0) {
-                        v[c] = z[3 * (size_t)it[0] + c];
-                    } else if (it[1] >= 0) {
-                        MD_CHECK(lds_read(it[1], entries, phase, c, v[c]), "subtree solution read");
-                        v[c] = -v[c];
-                    } else {
-                        MD_CHECK((size_t)it[2] < f.offrows.size(), "subtree off row entry");
-                        const int* xl = f.xloc.data() + 4 * (size_t)it[2];
-                        MD_CHECK(sum_part(xp, xl[0], xl[2], xl[1], 0, c, v[c]), "subtree outer solution");
+                if (p < w) {
+                    for (int c = 0; c < 3; ++c) v[c] = z[3 * (size_t)(R[0] + p) + c];
+                } else {
+                    const size_t e = (size_t)R[7] + p - w;
+                    MD_CHECK(R[7] >= 0 && 2 * e + 1 < f.sbo.size(), "subtree off-row source");
+                    const int src = f.sbo[2 * e], info = f.sbo[2 * e + 1];
+                    for (int c = 0; c < 3; ++c) {
+                        if (info < 0) {
+                            MD_CHECK(lds_read(src, entries, phase, c, v[c]), "subtree solution read");
+                        } else {
+                            MD_CHECK((info >> 16) >= 1, "subtree outer partial count");
+                            MD_CHECK(sum_part(xp, src, info >> 16, info & 0xffff, 0, c, v[c]), "subtree outer solution");
+                        }
                         v[c] = -v[c];
                     }
                 }
-                MD_CHECK(lds_claim(it[3], entries, phase), "subtree rhs write");
-                for (int c = 0; c < 3; ++c) lds[3 * (size_t)it[3] + c] = v[c];
+                MD_CHECK(lds_claim(R[14] + p, entries, phase), "subtree rhs write");
+                for (int c = 0; c < 3; ++c) lds[3 * (size_t)(R[14] + p) + c] = v[c];
             }
             ++phase;
-            for (int t = b0; t < b1; ++t) {
-                const int* it = f.sbb.data() + 8 * (size_t)t;
-                const int w = it[1], mm = it[2];
-                MD_CHECK(w > 0 && mm >= w && mm <= kSubMaxRows && it[0] >= 0 &&
-                         (int64_t)it[0] + (int64_t)(mm - 1) * w < (int64_t)pr.size(), "subtree P range");
+            const int ncol = rec(rb - 1)[13] + items(rb - 1, 13);
+            for (int t = 0; t < ncol; ++t) {
+                const int j = find(ra, rb, 13, t);
+                const int* R = rec(j);
+                const int q = t - R[13], w = R[1], mm = w + R[2];
+                MD_CHECK(q >= 0 && q < w && mm <= kSubMaxRows && R[5] >= 0 &&
+                         (int64_t)R[5] + q + (int64_t)(mm - 1) * w < (int64_t)pr.size(), "subtree P range");
                 double acc[3] = {0, 0, 0};
                 for (int p = 0; p < mm; ++p)
                     for (int c = 0; c < 3; ++c) {
                         double v;
-                        MD_CHECK(lds_read(it[3] + p, entries, phase, c, v), "subtree rhs read");
-                        acc[c] += pr[it[0] + (size_t)p * w] * v;
+                        MD_CHECK(lds_read(R[14] + p, entries, phase, c, v), "subtree rhs read");
+                        acc[c] += pr[R[5] + q + (size_t)p * w] * v;
                     }
-                MD_CHECK(lds_claim(it[4], entries, phase), "subtree solution write");
-                MD_CHECK(it[5] >= 0 && it[5] < nx, "subtree solution output");
+                MD_CHECK(lds_claim(R[10] + q, entries, phase), "subtree solution write");
+                MD_CHECK(R[9] >= 0 && R[9] + q < nx, "subtree solution output");
                 for (int c = 0; c < 3; ++c) {
-                    lds[3 * (size_t)it[4] + c] = acc[c];
-                    xp[3 * (size_t)it[5] + c] = acc[c];
+                    lds[3 * (size_t)(R[10] + q) + c] = acc[c];
+                    xp[3 * (size_t)(R[9] + q) + c] = acc[c];
                 }
             }
         }

@@ -21,8 +21,10 @@ constexpr int kTile = 64;
 constexpr int kSlice = 64;
 // Default LDS budget (vector entries) of one bottom subtree swept by a single wavefront.
 constexpr int kSubBudgetDefault = 512;
-// Largest budget: 2048 FP64 vector entries (32 bytes each) are 64 KiB of LDS.
-constexpr int kSubBudgetMax = 2048;
+// Largest budget: 1024 FP64 vector entries (32 bytes each) are 32 KiB of LDS; with at most
+// kSubMaxNodes 64-byte supernode records a subtree stays within 36 KiB.
+constexpr int kSubBudgetMax = 1024;
+constexpr int kSubMaxNodes = 64;
 // Largest supernode a bottom subtree may hold: columns, and rows of the front.
 constexpr int kSubMaxWidth = 64;
 constexpr int kSubMaxRows = 128;
@@ -68,11 +70,19 @@ struct Topology {
 // fronts and columns fit an LDS budget is swept by one wavefront, height by height, with
 // every front in LDS (no level launches, no global round trips inside the subtree). Its
 // supernodes keep one partial of x_s (kr = 1); only the subtree root stores u_s (kc = 1,
-// the others kc = 0). The host resolves every index into flat item lists:
-//   forward, per height:  position items (front init + children's updates from LDS, z),
-//                         off-row items (u = v_o - G v_d in place; the root also to global);
-//   backward, per height: position items (t = z, or -x from LDS / from global partials),
-//                         column items (x_s = [Dinv; G]^T t, to LDS and to global).
+// the others kc = 0). Each supernode of a subtree has a 16-int record (SubNode layout:
+// c0, w, noff, forward front base, G offset, [Dinv; G] offset, contribution base, off-row
+// source base, u output or -1, x output, LDS column base of x, position / off-row / column
+// prefix within its height, backward front base, 0). Records of a subtree are contiguous
+// and ordered by height; the wavefront copies them to LDS and maps lane items to supernodes
+// by binary search over the prefixes. LDS holds, forward, the fronts of all the subtree's
+// supernodes; backward, the solution columns of the subtree and the fronts of one height.
+// Per item only this remains in HBM:
+//   forward positions: the LDS indices of the first two children's update entries landing
+//     there (-1: none); a second index <= -2 points (-2 - i) at sfc[i ..], a -1-terminated
+//     list of the rest;
+//   backward off-row positions: (LDS index of x, -1) when the row lies in the subtree, else
+//     (first partial of x, count << 16 | stride) of the row's owner.
 // Level schedule: supernodes outside subtrees get level = 1 + the largest level of their
 // children outside subtrees (0 if none); subtree sweeps run before the forward levels and
 // after the backward levels.
@@ -105,15 +115,15 @@ struct Factor {
     std::vector<int> bt_ptr, bt;     // backward (s, q0, p0, 0)
     std::vector<int> level;          // device level of supernodes outside subtrees, -1 inside
     // bottom subtrees
-    int nsub = 0, sub_lds = 0;       // subtrees; LDS vector entries of the largest
-    std::vector<int> sub;            // 4 per subtree: (height table offset, heights, LDS entries, root)
-    std::vector<int> sub_h;          // 4 per (subtree, height) + a terminal per subtree:
-                                     //   first forward position / off-row / backward position / column item
-    std::vector<int> sfa;            // 4 per forward position item: (b and z index or -1, LDS dst, contrib begin, end)
-    std::vector<int> sfc;            // LDS index of each child update entry
-    std::vector<int> sfb;            // 8 per off-row item: (G base, G stride, w, front base, LDS dst, u dst or -1, 0, 0)
-    std::vector<int> sba;            // 4 per backward position item: (z index, LDS x index, xloc entry: one >= 0; LDS dst)
-    std::vector<int> sbb;            // 8 per column item: (P base, P stride, m, front base, LDS x dst, x dst, 0, 0)
+    int nsub = 0;                    // subtrees
+    int sub_lds_fwd = 0, sub_lds_bwd = 0;  // LDS vector entries of the largest, per sweep
+    int sub_nodes = 0;               // supernodes of the largest subtree
+    std::vector<int> sub;            // 4 per subtree: (first record, end record, height table offset, heights)
+    std::vector<int> sub_h;          // per (subtree, height) + a terminal per subtree: first record
+    std::vector<int> snode;          // 16 per subtree supernode (SubNode), by subtree then height
+    std::vector<int> scp;            // 2 per forward position: first two child entries (see above)
+    std::vector<int> sfc;            // -1-terminated overflow lists of child entries
+    std::vector<int> sbo;            // 2 per off-row position of a subtree supernode (backward source)
     int64_t factor_nnz() const { return poff.empty() ? 0 : poff.back(); }
 };
 

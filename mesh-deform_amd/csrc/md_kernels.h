@@ -51,14 +51,13 @@ struct MeshDev {
     const void *pc, *pr;        // panels (float or double), forward and backward layouts
     const int *ft_ptr, *bt_ptr; // tiles per level
     const int4 *ft, *bt;        // (s, r0, q0, 0) / (s, q0, p0, 0)
-    // bottom subtrees, one wavefront each (item layouts in md_analysis.h, Factor)
-    const int4 *sub;            // (height table offset, heights, LDS entries, root)
-    const int4 *sub_h;          // item starts per (subtree, height): fwd pos, fwd row, bwd pos, bwd col
-    const int4 *sfa;            // forward position items
-    const int *sfc;             // LDS index of each child update entry
-    const int4 *sfb;            // forward off-row items, two int4 each
-    const int4 *sba;            // backward position items
-    const int4 *sbb;            // backward column items, two int4 each
+    // bottom subtrees, one wavefront each (layouts in md_analysis.h, Factor)
+    const int4 *sub;            // (first record, end record, height table offset, heights)
+    const int *sub_h;           // first record of each (subtree, height), + a terminal per subtree
+    const int4 *snode;          // supernode records, four int4 each (SubNode)
+    const int2 *scp;            // forward positions: first two child update entries (LDS), overflow
+    const int *sfc;             // -1-terminated overflow lists of child update entries
+    const int2 *sbo;            // backward off-row sources
     double *energy_part;        // one partial sum per 256 vertices
     // Jacobi-PCG alternative to the triangular sweeps (solve type vectors, nf entries)
     float *w32;                 // slots: cotan weights in FP32
@@ -86,9 +85,10 @@ void launch_residual(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
-// Bottom-subtree sweeps: one 64-lane block per subtree, lds_entries vector entries of LDS.
-void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int lds_entries, bool fp64);
-void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int lds_entries, bool fp64);
+// Bottom-subtree sweeps: one 64-lane block per subtree; LDS for `nodes` supernode records
+// and `lds_entries` vector entries.
+void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64);
+void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k);

@@ -382,12 +382,13 @@ __device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S
 }
 
 // acc += sum_{i < n} P[i * stride] * v[i] for n <= N (N = 16, 32 or 64): all N loads are
-// issued together (clamped addresses, masked values).
+// issued together (addresses clamped to [0, n-1], or to 0 when n <= 0; values masked), so
+// P[0] and v[0] must be readable.
 template <int N, typename T, typename V>
 __device__ __forceinline__ void tile_dot(const T *P, long long stride, const V *v, int n, T &ax, T &ay, T &az) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        int ii = min(i, n - 1);
+        int ii = max(0, min(i, n - 1));
         T g = P[(long long)ii * stride];
         g = i < n ? g : T(0);
         V x = v[ii];
@@ -497,16 +498,55 @@ __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
     }
 }
 
-// Bottom subtree, forward sweep, one wavefront per subtree. Heights bottom-up, each in two
-// phases with every front in LDS:
+// acc += sum_{i < n} P[i * stride] * v[i] (n <= 64; n <= 0 adds nothing but still reads P[0]
+// and v[0]) with the unrolled width chosen per wavefront (the widest active lane's class,
+// 16 / 32 / 64), so lanes of different widths do not diverge.
+template <typename T, typename V>
+__device__ __forceinline__ void wave_dot(const T *P, long long stride, const V *v, int n, T &ax, T &ay, T &az) {
+    if (__any(n > 32)) tile_dot<64>(P, stride, v, n, ax, ay, az);
+    else if (__any(n > 16)) tile_dot<32>(P, stride, v, n, ax, ay, az);
+    else tile_dot<16>(P, stride, v, n, ax, ay, az);
+}
+
+// Record of one supernode of a bottom subtree (16 ints, as the host lays it out).
+struct SubNode {
+    int c0, w, noff, fb;     // columns, off rows, forward front base in LDS
+    int pc, pr, cp, ob;      // G offset, [Dinv; G] offset, contribution entries, off-row sources
+    int uo, xo, xs, pa;      // u output (root) or -1, x output, LDS column base of x, position prefix
+    int pb, pq, fbb, pad;    // off-row prefix, column prefix, backward front base in LDS
+};
+
+// Last record j in [a, b) whose prefix (field F) is <= t.
+template <int SubNode::*F>
+__device__ __forceinline__ int sub_find(const SubNode *N, int a, int b, int t) {
+    int lo = a, hi = b - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (N[mid].*F <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Copies the records [sd.x, sd.y) of a subtree to LDS.
+__device__ __forceinline__ void sub_records(const MeshDev &M, const int4 &sd, SubNode *N) {
+    int4 *dst = reinterpret_cast<int4 *>(N);
+    for (int i = threadIdx.x; i < 4 * (sd.y - sd.x); i += kTile) dst[i] = M.snode[4 * sd.x + i];
+    __syncthreads();
+}
+
+// Bottom subtree, forward sweep, one wavefront per subtree (sd = first record, end record,
+// height table offset, heights). Heights bottom-up, each in two phases with every front in
+// LDS:
 //   positions: front[p] = (p < w ? r[c0 + p] : 0) + children's update entries; z = front[p < w],
 //   off rows:  front[w + i] -= G_s[i, 0..w) front[0 .. w)  (u_s in place; the root also stores
 //              u_s for its parent outside the subtree).
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L) {
+__global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L, int node_cap) {
     using V = typename V4T<T>::type;
     extern __shared__ __align__(32) unsigned char sub_smem[];
-    V *F = reinterpret_cast<V *>(sub_smem);
+    SubNode *N = reinterpret_cast<SubNode *>(sub_smem);
+    V *F = reinterpret_cast<V *>(sub_smem + sizeof(SubNode) * node_cap);
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
     const int4 sd = M.sub[blockIdx.x - L.prefix[k]];
@@ -515,52 +555,69 @@ __global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L) {
     V *Z = reinterpret_cast<V *>(M.z);
     V *U = reinterpret_cast<V *>(M.upart);
     const T *G = tptr<T>(M.pc);
-    for (int h = 0; h < sd.y; ++h) {
-        const int4 h0 = M.sub_h[sd.x + h], h1 = M.sub_h[sd.x + h + 1];
-        for (int t = h0.x + lane; t < h1.x; t += kTile) {
-            const int4 it = M.sfa[t];
+    sub_records(M, sd, N);
+    for (int h = 0; h < sd.w; ++h) {
+        const int a = M.sub_h[sd.z + h] - sd.x, b = M.sub_h[sd.z + h + 1] - sd.x;
+        const int npos = N[b - 1].pa + N[b - 1].w + N[b - 1].noff;
+        for (int t = lane; t < npos; t += kTile) {
+            const SubNode &S = N[sub_find<&SubNode::pa>(N, a, b, t)];
+            const int p = t - S.pa;
             V v;
             v.x = 0; v.y = 0; v.z = 0; v.w = 0;
-            if (it.x >= 0) v = R[it.x];
-            for (int c0 = it.z; c0 < it.w; c0 += 4) {
-                int e[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) e[u] = M.sfc[min(c0 + u, it.w - 1)];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (c0 + u < it.w) {
-                        const V a = F[e[u]];
-                        v.x += a.x; v.y += a.y; v.z += a.z;
-                    }
+            if (p < S.w) v = R[S.c0 + p];
+            // children's update entries: two inline, the rest from a -1-terminated list
+            const int2 e = M.scp[S.cp + p];
+            if (e.x >= 0) {
+                const V u = F[e.x];
+                v.x += u.x; v.y += u.y; v.z += u.z;
+            }
+            if (e.y >= 0) {
+                const V u = F[e.y];
+                v.x += u.x; v.y += u.y; v.z += u.z;
+            } else if (e.y <= -2) {
+                for (int c = -2 - e.y;; ++c) {
+                    const int q = M.sfc[c];
+                    if (q < 0) break;
+                    const V u = F[q];
+                    v.x += u.x; v.y += u.y; v.z += u.z;
                 }
             }
             v.w = 0;
-            F[it.y] = v;
-            if (it.x >= 0) Z[it.x] = v;
+            F[S.fb + p] = v;
+            if (p < S.w) Z[S.c0 + p] = v;
         }
         __syncthreads();
-        for (int t = h0.y + lane; t < h1.y; t += kTile) {
-            const int4 a = M.sfb[2 * t], b = M.sfb[2 * t + 1];
+        const int nrow = N[b - 1].pb + N[b - 1].noff;
+        for (int t0 = 0; t0 < nrow; t0 += kTile) {
+            const int t = t0 + lane;
+            const bool on = t < nrow;
+            const SubNode &S = N[sub_find<&SubNode::pb>(N, a, b, on ? t : nrow - 1)];
+            const int i = (on ? t : nrow - 1) - S.pb;
             T ax = 0, ay = 0, az = 0;
-            tile_dot_any<T, V>(G + a.x, a.y, F + a.w, a.z, ax, ay, az);
-            V o = F[b.x];
-            o.x -= ax; o.y -= ay; o.z -= az;
-            F[b.x] = o;
-            if (b.y >= 0) U[b.y] = o;
+            // column-major G_s: entry (row i, column q) at q * noff + i
+            wave_dot<T, V>(G + S.pc + i, S.noff, F + S.fb, on ? S.w : 0, ax, ay, az);
+            if (on) {
+                V o = F[S.fb + S.w + i];
+                o.x -= ax; o.y -= ay; o.z -= az;
+                F[S.fb + S.w + i] = o;
+                if (S.uo >= 0) U[S.uo + i] = o;
+            }
         }
         __syncthreads();
     }
 }
 
-// Bottom subtree, backward sweep, one wavefront per subtree. Heights top-down:
+// Bottom subtree, backward sweep, one wavefront per subtree. Heights top-down, with the
+// subtree's solution columns and the fronts of one height in LDS:
 //   positions: t[p] = z[c0 + p] (p < w), else -x of the off row, from LDS when the row lies
 //              in the subtree, else the sum of its owner's partials,
 //   columns:   x_s[q] = sum_p [Dinv_s; G_s][p, q] t[p], to LDS and to the single partial.
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L) {
+__global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
     using V = typename V4T<T>::type;
     extern __shared__ __align__(32) unsigned char sub_smem[];
-    V *F = reinterpret_cast<V *>(sub_smem);
+    SubNode *N = reinterpret_cast<SubNode *>(sub_smem);
+    V *F = reinterpret_cast<V *>(sub_smem + sizeof(SubNode) * node_cap);
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
     const int4 sd = M.sub[blockIdx.x - L.prefix[k]];
@@ -568,37 +625,56 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L) {
     const V *Zs = reinterpret_cast<const V *>(M.z);
     V *X = reinterpret_cast<V *>(M.xpart);
     const T *P = tptr<T>(M.pr);
-    for (int h = sd.y - 1; h >= 0; --h) {
-        const int4 h0 = M.sub_h[sd.x + h], h1 = M.sub_h[sd.x + h + 1];
-        for (int t = h0.z + lane; t < h1.z; t += kTile) {
-            const int4 it = M.sba[t];
+    sub_records(M, sd, N);
+    for (int h = sd.w - 1; h >= 0; --h) {
+        const int a = M.sub_h[sd.z + h] - sd.x, b = M.sub_h[sd.z + h + 1] - sd.x;
+        const int npos = N[b - 1].pa + N[b - 1].w + N[b - 1].noff;
+        for (int t = lane; t < npos; t += kTile) {
+            const SubNode &S = N[sub_find<&SubNode::pa>(N, a, b, t)];
+            const int p = t - S.pa;
             V v;
-            if (it.x >= 0) {
-                v = Zs[it.x];
+            if (p < S.w) {
+                v = Zs[S.c0 + p];
             } else {
-                V a;
-                if (it.y >= 0) {
-                    a = F[it.y];
+                const int2 d = M.sbo[S.ob + p - S.w];
+                V g;
+                if (d.y < 0) {
+                    g = F[d.x];
                 } else {
-                    const int4 xl = M.xloc[it.z];
-                    a = sum_partials<V>(X + xl.x, xl.z, xl.y, 0);
+                    g = X[d.x];
+                    const int cnt = d.y >> 16, stride = d.y & 0xffff;
+                    for (int c = 1; c < cnt; ++c) {
+                        const V e = X[d.x + (long long)c * stride];
+                        g.x += e.x; g.y += e.y; g.z += e.z;
+                    }
                 }
-                v.x = -a.x; v.y = -a.y; v.z = -a.z;
+                v.x = -g.x; v.y = -g.y; v.z = -g.z;
             }
             v.w = 0;
-            F[it.w] = v;
+            F[S.fbb + p] = v;
         }
         __syncthreads();
-        for (int t = h0.w + lane; t < h1.w; t += kTile) {
-            const int4 a = M.sbb[2 * t], b = M.sbb[2 * t + 1];
+        const int ncol = N[b - 1].pq + N[b - 1].w;
+        for (int t0 = 0; t0 < ncol; t0 += kTile) {
+            const int t = t0 + lane;
+            const bool on = t < ncol;
+            const SubNode &S = N[sub_find<&SubNode::pq>(N, a, b, on ? t : ncol - 1)];
+            const int q = (on ? t : ncol - 1) - S.pq, m = S.w + S.noff;
             T ax = 0, ay = 0, az = 0;
-            tile_dot_any<T, V>(P + a.x, a.y, F + a.w, min(a.z, kTile), ax, ay, az);
-            if (a.z > kTile)
-                tile_dot_any<T, V>(P + a.x + (long long)kTile * a.y, a.y, F + a.w + kTile, a.z - kTile, ax, ay, az);
-            V o;
-            o.x = ax; o.y = ay; o.z = az; o.w = 0;
-            F[b.x] = o;
-            X[b.y] = o;
+            // row-major [Dinv_s; G_s]: entry (row p, column q) at p * w + q
+            wave_dot<T, V>(P + S.pr + q, S.w, F + S.fbb, on ? min(m, kTile) : 0, ax, ay, az);
+            if (__any(on && m > kTile)) {
+                // rows 64 .. m-1; lanes with m <= 64 keep their own base and add nothing
+                const bool two = on && m > kTile;
+                wave_dot<T, V>(P + S.pr + q + (two ? (long long)kTile * S.w : 0), S.w, F + S.fbb + (two ? kTile : 0),
+                               two ? m - kTile : 0, ax, ay, az);
+            }
+            if (on) {
+                V o;
+                o.x = ax; o.y = ay; o.z = az; o.w = 0;
+                F[S.xs + q] = o;
+                X[S.xo + q] = o;
+            }
         }
         __syncthreads();
     }
@@ -980,15 +1056,17 @@ void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
     if (fp64) hipLaunchKernelGGL(k_bwd<double>, dim3(blocks), dim3(kTile), 0, s, L, level);
     else hipLaunchKernelGGL(k_bwd<float>, dim3(blocks), dim3(kTile), 0, s, L, level);
 }
-void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int lds_entries, bool fp64) {
+void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64) {
     if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_sub_fwd<double>, dim3(blocks), dim3(kTile), sizeof(d4) * lds_entries, s, L);
-    else hipLaunchKernelGGL(k_sub_fwd<float>, dim3(blocks), dim3(kTile), sizeof(f4) * lds_entries, s, L);
+    const size_t rec = sizeof(SubNode) * (size_t)nodes;
+    if (fp64) hipLaunchKernelGGL(k_sub_fwd<double>, dim3(blocks), dim3(kTile), rec + sizeof(d4) * lds_entries, s, L, nodes);
+    else hipLaunchKernelGGL(k_sub_fwd<float>, dim3(blocks), dim3(kTile), rec + sizeof(f4) * lds_entries, s, L, nodes);
 }
-void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int lds_entries, bool fp64) {
+void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64) {
     if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_sub_bwd<double>, dim3(blocks), dim3(kTile), sizeof(d4) * lds_entries, s, L);
-    else hipLaunchKernelGGL(k_sub_bwd<float>, dim3(blocks), dim3(kTile), sizeof(f4) * lds_entries, s, L);
+    const size_t rec = sizeof(SubNode) * (size_t)nodes;
+    if (fp64) hipLaunchKernelGGL(k_sub_bwd<double>, dim3(blocks), dim3(kTile), rec + sizeof(d4) * lds_entries, s, L, nodes);
+    else hipLaunchKernelGGL(k_sub_bwd<float>, dim3(blocks), dim3(kTile), rec + sizeof(f4) * lds_entries, s, L, nodes);
 }
 void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
     if (blocks <= 0) return;

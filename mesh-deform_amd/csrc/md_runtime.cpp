@@ -91,8 +91,14 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     table(3, plan.off_ident, plan.blk_ident, [&](int k) { return cdiv(meshes[k]->n, kBlock); });
     table(4, plan.off_apply, plan.blk_apply, [&](int k) { return cdiv(meshes[k]->nf, kBlock); });
     table(5, plan.off_sub, plan.blk_sub, [&](int k) { return factors[k]->nsub; });
-    plan.sub_lds = 0;
-    for (const Factor *f : factors) plan.sub_lds = std::max(plan.sub_lds, f->sub_lds);
+    plan.sub_lds_fwd = 0;
+    plan.sub_lds_bwd = 0;
+    plan.sub_nodes = 0;
+    for (const Factor *f : factors) {
+        plan.sub_lds_fwd = std::max(plan.sub_lds_fwd, f->sub_lds_fwd);
+        plan.sub_lds_bwd = std::max(plan.sub_lds_bwd, f->sub_lds_bwd);
+        plan.sub_nodes = std::max(plan.sub_nodes, f->sub_nodes);
+    }
     for (int l = 0; l < nl; ++l) {
         table(6 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
             const Factor *f = factors[k];
@@ -166,13 +172,13 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
             if (ev && r == 0) hipEventRecord(ev[4], stream);
             continue;
         }
-        launch_sub_fwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_lds, plan.fp64);
+        launch_sub_fwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_fwd, plan.fp64);
         for (int l = 0; l < plan.nlevels; ++l)
             launch_fwd(stream, plan.set(plan.off_fwd[l]), plan.blk_fwd[l], l, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[3], stream);
         for (int l = plan.nlevels - 1; l >= 0; --l)
             launch_bwd(stream, plan.set(plan.off_bwd[l]), plan.blk_bwd[l], l, plan.fp64);
-        launch_sub_bwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_lds, plan.fp64);
+        launch_sub_bwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_bwd, plan.fp64);
         launch_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[4], stream);
     }
@@ -518,12 +524,11 @@ int upload_layout(md_mesh *m) {
     size_t o_ft = A.reserve(sizeof(int) * std::max<size_t>(4, f.ft.size()));
     size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
     size_t o_sub = A.reserve(sizeof(int) * std::max<size_t>(4, f.sub.size()));
-    size_t o_subh = A.reserve(sizeof(int) * std::max<size_t>(4, f.sub_h.size()));
-    size_t o_sfa = A.reserve(sizeof(int) * std::max<size_t>(4, f.sfa.size()));
-    size_t o_sfc = A.reserve(sizeof(int) * std::max<size_t>(4, f.sfc.size()));
-    size_t o_sfb = A.reserve(sizeof(int) * std::max<size_t>(8, f.sfb.size()));
-    size_t o_sba = A.reserve(sizeof(int) * std::max<size_t>(4, f.sba.size()));
-    size_t o_sbb = A.reserve(sizeof(int) * std::max<size_t>(8, f.sbb.size()));
+    size_t o_subh = A.reserve(sizeof(int) * std::max<size_t>(1, f.sub_h.size()));
+    size_t o_snode = A.reserve(sizeof(int) * std::max<size_t>(16, f.snode.size()));
+    size_t o_scp = A.reserve(sizeof(int) * std::max<size_t>(2, f.scp.size()));
+    size_t o_sfc = A.reserve(sizeof(int) * std::max<size_t>(1, f.sfc.size()));
+    size_t o_sbo = A.reserve(sizeof(int) * std::max<size_t>(2, f.sbo.size()));
     if (A.allocate() != hipSuccess) {
         set_error("device allocation of " + std::to_string(A.size()) + " bytes (sweep layout) failed");
         A.reset();
@@ -536,10 +541,9 @@ int upload_layout(md_mesh *m) {
     d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
     d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
     d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
-    d.sub = (const int4 *)A.at(o_sub); d.sub_h = (const int4 *)A.at(o_subh);
-    d.sfa = (const int4 *)A.at(o_sfa); d.sfc = (const int *)A.at(o_sfc);
-    d.sfb = (const int4 *)A.at(o_sfb); d.sba = (const int4 *)A.at(o_sba);
-    d.sbb = (const int4 *)A.at(o_sbb);
+    d.sub = (const int4 *)A.at(o_sub); d.sub_h = (const int *)A.at(o_subh);
+    d.snode = (const int4 *)A.at(o_snode); d.scp = (const int2 *)A.at(o_scp);
+    d.sfc = (const int *)A.at(o_sfc); d.sbo = (const int2 *)A.at(o_sbo);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -566,11 +570,10 @@ int upload_layout(md_mesh *m) {
         chk(up(d.bt, f.bt.data(), sizeof(int) * f.bt.size()));
         chk(up(d.sub, f.sub.data(), sizeof(int) * f.sub.size()));
         chk(up(d.sub_h, f.sub_h.data(), sizeof(int) * f.sub_h.size()));
-        chk(up(d.sfa, f.sfa.data(), sizeof(int) * f.sfa.size()));
+        chk(up(d.snode, f.snode.data(), sizeof(int) * f.snode.size()));
+        chk(up(d.scp, f.scp.data(), sizeof(int) * f.scp.size()));
         chk(up(d.sfc, f.sfc.data(), sizeof(int) * f.sfc.size()));
-        chk(up(d.sfb, f.sfb.data(), sizeof(int) * f.sfb.size()));
-        chk(up(d.sba, f.sba.data(), sizeof(int) * f.sba.size()));
-        chk(up(d.sbb, f.sbb.data(), sizeof(int) * f.sbb.size()));
+        chk(up(d.sbo, f.sbo.data(), sizeof(int) * f.sbo.size()));
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
@@ -1190,11 +1193,12 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     for (int k = 0; k < f.ns; ++k)
         if (f.level[k] >= 0)
             for (int e = f.roff[k]; e < f.roff[k + 1]; ++e) xloc_reads += f.xloc[4 * (size_t)e + 2];
-    for (size_t i = 0; i < f.sba.size(); i += 4)
-        if (f.sba[i + 2] >= 0) xloc_reads += f.xloc[4 * (size_t)f.sba[i + 2] + 2];
-    // index lists of the bottom-subtree sweeps
-    const int64_t sub_fwd_idx = 4 * (int64_t)(f.sfa.size() + f.sfc.size() + f.sfb.size());
-    const int64_t sub_bwd_idx = 4 * (int64_t)(f.sba.size() + f.sbb.size());
+    for (size_t i = 0; i < f.sbo.size(); i += 2)
+        if (f.sbo[i + 1] >= 0) xloc_reads += f.sbo[i + 1] >> 16;
+    // index data of the bottom-subtree sweeps: records (both), contribution lists (forward),
+    // off-row sources (backward)
+    const int64_t sub_fwd_idx = 4 * (int64_t)(f.snode.size() + f.scp.size() + f.sfc.size() + f.sub_h.size());  // scp: 2 ints per position
+    const int64_t sub_bwd_idx = 4 * (int64_t)(f.snode.size() + f.sbo.size() + f.sub_h.size());
     // forward: G panel, r in, z out, update partials out and read back by the parent, rel
     out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + offr * 4 + sub_fwd_idx;
     // backward + update: Dinv and G panels, z in, x partials of ancestors gathered (with

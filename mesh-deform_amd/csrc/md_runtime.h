@@ -50,7 +50,8 @@ struct Plan {
     int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0, blk_apply = 0, blk_sub = 0;
     std::vector<int> blk_fwd, blk_bwd;
     int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0, off_apply = 0, off_sub = 0;
-    int sub_lds = 0;                   // LDS vector entries of the largest bottom subtree
+    int sub_lds_fwd = 0, sub_lds_bwd = 0;  // LDS vector entries of the largest bottom subtree
+    int sub_nodes = 0;                 // supernode records of the largest bottom subtree
     std::vector<int> off_fwd, off_bwd;
     MeshDev *d_meshes = nullptr;
     int *d_prefix = nullptr;

@@ -20,7 +20,7 @@ SOLVER_LDLT, SOLVER_PCG = 0, 1
 REST_REFERENCE, REST_FIXED = 0, 1
 
 # candidate bottom-subtree budgets of the sweep-layout autotuner (LDS vector entries; 0: none)
-AUTOTUNE_BUDGETS = (0, 256, 512, 1024, 2048)
+AUTOTUNE_BUDGETS = (0, 256, 512, 1024)
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
