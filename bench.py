@@ -60,15 +60,16 @@ def cpu_model():
     return platform.processor()
 
 
-def plain_leg(ctx, pr, steps, warmup):
+def plain_leg(ctx, pr, steps, warmup, tune=True, budget=None):
     idx, tgt = pr.constraints()
-    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget))
     m.set_constraints(idx, tgt)
     t0 = time.perf_counter()
     m.prepare(pr.xyz)
     setup_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    budget, _ = m.autotune()  # sweep layout: the fastest bottom-subtree budget (state restored)
+    # sweep layout: the fastest bottom-subtree budget (state restored)
+    budget = m.autotune()[0] if tune and budget is None else m.stats()["subtree_budget"]
     tune_s = time.perf_counter() - t0
     m.iterate(warmup)
     ctx.synchronize()
@@ -182,7 +183,7 @@ def served_leg(ctx, count, seconds, rate, iters, seed):
                 submit_lag_p99_ms=float(np.percentile(lag, 99) * 1e3))
 
 
-def batched_leg(ctx, count, iters, seed):
+def batched_leg(ctx, count, iters, seed, tune=True, baseline=True):
     probs = meshgen.mixed_batch(count, 1000, 200000, seed=seed)
     meshes = []
     for pr in probs:
@@ -196,27 +197,32 @@ def batched_leg(ctx, count, iters, seed):
     # plain: each mesh with its own tuned sweep layout, iterated alone (its own graph), back
     # to back
     t0 = time.perf_counter()
-    for m in meshes:
-        m.autotune(iters=3)
+    if tune and baseline:
+        for m in meshes:
+            m.autotune(iters=3)
     tune_plain_s = time.perf_counter() - t0
-    for m in meshes:
-        m.iterate(1)  # graph capture
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for m in meshes:
-        m.iterate(iters)
-    plain_s = time.perf_counter() - t0
+    plain_s = None
+    if baseline:
+        for m in meshes:
+            m.iterate(1)  # graph capture
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for m in meshes:
+            m.iterate(iters)
+        plain_s = time.perf_counter() - t0
     b = mdarap.Batch(ctx, meshes)
     t0 = time.perf_counter()
-    budget, _ = b.autotune()
+    budget = b.autotune()[0] if tune else meshes[0].stats()["subtree_budget"]
     tune_batch_s = time.perf_counter() - t0
     b.iterate(2)  # warm-up (graph capture)
     ms = b.time_iterations(0, iters)
     b.close()
     total_iters = count * iters
     return dict(meshes=count, total_vertices=int(sum(p.n for p in probs)), iterations=iters,
-                batched_iters_per_s=total_iters / (ms / 1e3), plain_iters_per_s=total_iters / plain_s,
-                speedup=(total_iters / (ms / 1e3)) / (total_iters / plain_s), batch_subtree_budget=budget,
+                batched_iters_per_s=total_iters / (ms / 1e3),
+                plain_iters_per_s=None if plain_s is None else total_iters / plain_s,
+                speedup=None if plain_s is None else (total_iters / (ms / 1e3)) / (total_iters / plain_s),
+                batch_subtree_budget=budget,
                 setup_s=setup_s, autotune_plain_s=tune_plain_s, autotune_batch_s=tune_batch_s)
 
 
@@ -244,31 +250,28 @@ def canonical_bytes(pr, st):
                 n=n, nf=nf, E=int(e.shape[0]), nnzA_full=int(nnz_a), nL=int(n_l), mean_valence=round(d, 4))
 
 
-def traffic_probe(ctx, iters):
-    """Workload for rocprofv3 --pmc passes: a 256 MiB device-to-device elementwise copy
-    (calibration: it reads and writes exactly 256 MiB), then `iters` ARAP iterations of the
-    plain leg's mesh and configuration."""
-    import torch
-
+def traffic_probe(ctx, iters, budget=None):
+    """Workload for rocprofv3 --pmc passes: the library's plain copy kernel over 256 MiB
+    (calibration: each dispatch reads and writes exactly 256 MiB), then `iters` ARAP
+    iterations of the plain leg's mesh and configuration."""
     nbytes = 256 << 20
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        torch.neg(a, out=b)
-    torch.cuda.synchronize()
+    gbps = mdarap.copy_probe(ctx, nbytes, 3)
     pr = meshgen.noisy_icosphere_problem(7, 0.01, 1234)
     idx, tgt = pr.constraints()
-    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget))
     m.set_constraints(idx, tgt)
     m.prepare(pr.xyz)
+    if budget is None:
+        m.autotune()
     m.iterate(2)
     ctx.synchronize()
     log("traffic probe: calibration copies done, running %d iterations" % iters)
     m.iterate(iters)
     ctx.synchronize()
+    used = m.stats()["subtree_budget"]
     m.close()
-    print(json.dumps({"probe": "traffic", "copy_bytes": nbytes, "copies": 3, "iterations": iters,
-                      "warmup_iterations": 2}), flush=True)
+    print(json.dumps({"probe": "traffic", "copy_bytes": nbytes, "copies": 4, "copy_GBs": round(gbps, 1),
+                      "iterations": iters, "warmup_iterations": 2, "subtree_budget": used}), flush=True)
 
 
 def traffic_summary(fetch_csv, write_csv, iters):
@@ -290,9 +293,9 @@ def traffic_summary(fetch_csv, write_csv, iters):
     out = {}
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         disp = load(path, name)
-        calib = [v for k, v in disp if "md::" not in k]
-        ours = [(k, v) for k, v in disp if "md::" in k]
-        copy_kib = max(calib)  # the 256 MiB copy: the largest dispatch outside the library
+        calib = [v for k, v in disp if "k_copy_probe" in k]
+        ours = [(k, v) for k, v in disp if "md::" in k and "k_copy_probe" not in k]
+        copy_kib = float(np.median(calib))  # each copy dispatch moves 256 MiB each way
         scale = (256 << 20) / (copy_kib * 1024.0)
         per_iter = None
         # the library's dispatches of the timed iterations: the last `iters` local steps start them
@@ -331,6 +334,13 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--batched-meshes", type=int, default=256)
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (for profiling)")
+    ap.add_argument("--no-plain-baseline", action="store_true",
+                    help="batched leg without the one-by-one baseline (profiling runs: one graph per mesh)")
+    ap.add_argument("--subtree-budget", type=int, default=None,
+                    help="fixed sweep layout of the plain leg and the traffic probe (no autotune)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default sweep layout (profiling runs: the tuner re-captures graphs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the workload of the rocprofv3 --pmc traffic passes")
     ap.add_argument("--probe-iters", type=int, default=5)
@@ -350,12 +360,19 @@ def main():
         sys.exit(1)
     ctx = mdarap.Context(0)
     if args.traffic_probe:
-        traffic_probe(ctx, args.probe_iters)
+        traffic_probe(ctx, args.probe_iters, args.subtree_budget)
+        ctx.close()
+        return
+    if args.batched_only:
+        log("batched leg: %d meshes" % args.batched_meshes)
+        print(json.dumps({"batched": batched_leg(ctx, args.batched_meshes, 20, 4242, not args.no_autotune,
+                                                 not args.no_plain_baseline)}), flush=True)
         ctx.close()
         return
     pr = meshgen.noisy_icosphere_problem(7, 0.01, 1234)
     log("plain leg: %s n=%d F=%d" % (pr.name, pr.n, pr.F))
-    m, st, setup_s, wall, ev_ms, phase = plain_leg(ctx, pr, args.steps, args.warmup)
+    m, st, setup_s, wall, ev_ms, phase = plain_leg(ctx, pr, args.steps, args.warmup, not args.no_autotune,
+                                                   args.subtree_budget)
     value = args.steps / wall
     ms_step = wall * 1e3 / args.steps
     ev_step_ms = ev_ms / args.steps
@@ -413,7 +430,7 @@ def main():
         out["served"]["plain_iters_per_s_at_164k"] = round(value, 2)
     if args.batched:
         log("batched leg: %d meshes" % args.batched_meshes)
-        out["batched"] = batched_leg(ctx, args.batched_meshes, 20, 4242)
+        out["batched"] = batched_leg(ctx, args.batched_meshes, 20, 4242, not args.no_autotune)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -81,6 +81,8 @@ typedef struct {
     /* algorithmic bytes of one ARAP iteration with one refinement step, per phase: every
      * array element the phase must read or write counted once */
     int64_t bytes_local, bytes_residual, bytes_forward, bytes_backward;
+    int subtree_budget;      /* sweep layout in use (md_config.subtree_budget or the autotuned one) */
+    int subtrees;            /* bottom subtrees swept by one wavefront each */
 } md_mesh_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -192,6 +194,13 @@ int md_server_submit(md_server *srv, int mesh_id, int k, const int *idx, const d
 int md_server_poll(md_server *srv, uint64_t ticket, int *done, double *xyz, double *latency_ms);
 int md_server_wait(md_server *srv, uint64_t ticket, double timeout_ms, double *xyz, double *latency_ms);
 int md_server_destroy(md_server *srv);
+
+/* ---- device diagnostics ------------------------------------------------------------- */
+/* Copies bytes (rounded down to 16) between two fresh device buffers `repeats` times with a
+ * plain copy kernel and returns the achieved rate (read + write bytes per second, GB/s):
+ * the known-size reference for HBM traffic counters and the achievable copy bandwidth.
+ * No counterpart in the reference. */
+int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbps);
 
 /* ---- host-only diagnostics (no GPU needed) ------------------------------------------ */
 /* Factors the SPD matrix given as full CSR (n rows, ascending columns), with coordinates

@@ -99,6 +99,8 @@ void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_pcg_update(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
 void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
 void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+// Plain copy of n16 16-byte elements (counter calibration, copy bandwidth).
+void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16);
 
 // Threads per block and work per block of each kernel (used for the prefix tables).
 constexpr int kBlock = 256;

@@ -637,17 +637,8 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
                 v = Zs[S.c0 + p];
             } else {
                 const int2 d = M.sbo[S.ob + p - S.w];
-                V g;
-                if (d.y < 0) {
-                    g = F[d.x];
-                } else {
-                    g = X[d.x];
-                    const int cnt = d.y >> 16, stride = d.y & 0xffff;
-                    for (int c = 1; c < cnt; ++c) {
-                        const V e = X[d.x + (long long)c * stride];
-                        g.x += e.x; g.y += e.y; g.z += e.z;
-                    }
-                }
+                // x of the row: from LDS, or the owner's partials loaded 8 at a time
+                const V g = d.y < 0 ? F[d.x] : sum_partials<V>(X + d.x, d.y >> 16, d.y & 0xffff, 0);
                 v.x = -g.x; v.y = -g.y; v.z = -g.z;
             }
             v.w = 0;
@@ -996,7 +987,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
     pp[iv[t]] = v;
 }
 
+// Plain device copy of n 16-byte elements (grid-stride): the known-size reference kernel for
+// HBM counters and the achievable copy bandwidth.
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ src, f4 *__restrict__ dst, long long n) {
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16) {
+    if (n16 <= 0) return;
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(kBlock), 0, s, static_cast<const f4 *>(src), static_cast<f4 *>(dst), n16);
+}
 
 #define MD_LAUNCH_T(kern, blocks, s, ...)                                                        \
     do {                                                                                        \

@@ -1150,7 +1150,9 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     std::memset(out, 0, sizeof(*out));
     out->n = m->n;
     out->F = m->F;
+    out->subtree_budget = m->cfg.subtree_budget;
     if (!m->analyzed) return MD_OK;
+    out->subtrees = m->fac.nsub;
     AnalysisStats s = stats(m->topo, m->fac);
     out->nf = s.nf;
     out->nslices = s.nslices;
@@ -1247,6 +1249,34 @@ extern "C" int md_mesh_time_iterations(md_mesh *m, int warmup, int iterations, d
     hipEventDestroy(a);
     hipEventDestroy(b);
     m->iterated = true;
+    return MD_OK;
+}
+
+extern "C" int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbps) {
+    if (!ctx || bytes < 16 || repeats < 1) { set_error("md_copy_probe: invalid argument"); return MD_ERR_INVALID; }
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const long long n16 = (long long)(bytes / 16);
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, (size_t)n16 * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, (size_t)n16 * 16);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, (size_t)n16 * 16, st);
+    if (e == hipSuccess) launch_copy_probe(st, a, b, n16);  // warm-up
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int i = 0; e == hipSuccess && i < repeats; ++i) launch_copy_probe(st, a, b, n16);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) { set_error(std::string("md_copy_probe: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    if (gbps) *gbps = 2.0 * 16.0 * (double)n16 * repeats / (ms * 1e-3) / 1e9;
     return MD_OK;
 }
 

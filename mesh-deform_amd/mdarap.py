@@ -41,7 +41,8 @@ class MeshStats(ctypes.Structure):
                 ("fwd_items", ctypes.c_int64), ("bwd_items", ctypes.c_int64),
                 ("device_bytes", ctypes.c_int64), ("setup_ms_host", ctypes.c_double),
                 ("bytes_local", ctypes.c_int64), ("bytes_residual", ctypes.c_int64),
-                ("bytes_forward", ctypes.c_int64), ("bytes_backward", ctypes.c_int64)]
+                ("bytes_forward", ctypes.c_int64), ("bytes_backward", ctypes.c_int64),
+                ("subtree_budget", ctypes.c_int), ("subtrees", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -102,6 +103,7 @@ _SIGS = [
                                       ctypes.POINTER(ctypes.c_double)]),
     ("md_server_wait", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_double, _vp, ctypes.POINTER(ctypes.c_double)]),
     ("md_server_destroy", ctypes.c_int, [_vp]),
+    ("md_copy_probe", ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("md_host_factor_solve", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _f64p, _f64p,
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
 ]
@@ -382,6 +384,13 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def copy_probe(ctx, nbytes, repeats=3):
+    """md_copy_probe: plain device copy of nbytes, repeats times; achieved GB/s (read + write)."""
+    g = ctypes.c_double(0)
+    check(lib().md_copy_probe(ctx.handle, int(nbytes), int(repeats), ctypes.byref(g)))
+    return g.value
 
 
 def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1):
