@@ -507,10 +507,11 @@ void device_layout(Factor& f, int sub_budget) {
         f.xoff[t + 1] = f.xoff[t] + (int64_t)f.kr[t] * f.w[t];
     }
     // contributions of the children's update entries to each position of a front, in child
-    // order: the forward gather pulls them per position (no write conflicts)
+    // order: the forward gather pulls them per position (no write conflicts). The first one is
+    // inline in the position's record, the rest in a list ended by a zero count.
     f.cp_off.assign(nn, 0);
-    f.cptr.clear();
-    f.cent.clear();
+    f.cpos.clear();
+    f.cmore.clear();
     {
         std::vector<std::vector<std::pair<int, int>>> lists;  // per position: (child, entry)
         for (int t = 0; t < nn; ++t) {
@@ -521,18 +522,27 @@ void device_layout(Factor& f, int sub_budget) {
                 int c = f.child[k];
                 for (int e = 0; e < f.noff[c]; ++e) lists[f.rel[f.roff[c] + e]].emplace_back(c, e);
             }
-            f.cp_off[t] = (int)f.cptr.size();
+            f.cp_off[t] = (int)(f.cpos.size() / 4);
             for (int p = 0; p < m; ++p) {
-                f.cptr.push_back((int)(f.cent.size() / 4));
-                for (const auto& ce : lists[p]) {
-                    int c = ce.first, e = ce.second;
-                    f.cent.push_back((int)(f.uoff[c] + e));
-                    f.cent.push_back(f.noff[c]);
-                    f.cent.push_back(f.kc[c]);
-                    f.cent.push_back(0);
+                const auto& L = lists[p];
+                if (L.empty()) {
+                    int rec[4] = {-1, 0, 0, -1};
+                    f.cpos.insert(f.cpos.end(), rec, rec + 4);
+                    continue;
+                }
+                int rec[4] = {(int)(f.uoff[L[0].first] + L[0].second), f.noff[L[0].first], f.kc[L[0].first],
+                              L.size() > 1 ? (int)(f.cmore.size() / 4) : -1};
+                f.cpos.insert(f.cpos.end(), rec, rec + 4);
+                if (L.size() > 1) {
+                    for (size_t i = 1; i < L.size(); ++i) {
+                        int c = L[i].first, e = L[i].second;
+                        int ent[4] = {(int)(f.uoff[c] + e), f.noff[c], f.kc[c], 0};
+                        f.cmore.insert(f.cmore.end(), ent, ent + 4);
+                    }
+                    int end[4] = {0, 0, 0, 0};
+                    f.cmore.insert(f.cmore.end(), end, end + 4);
                 }
             }
-            f.cptr.push_back((int)(f.cent.size() / 4));
         }
     }
     f.rowinfo.assign(4 * (size_t)n, 0);
@@ -969,15 +979,25 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
             for (int i = 0; i < qn; ++i)
                 for (int c = 0; c < 3; ++c) vd[c][i] = b[(size_t)c * n + c0 + q0 + i];
             auto pull = [&](int pos, double* acc3) -> bool {
-                int64_t c0p = f.cp_off[s];
-                if (c0p + pos + 1 >= (int64_t)f.cptr.size()) return false;
-                for (int q = f.cptr[c0p + pos]; q < f.cptr[c0p + pos + 1]; ++q) {
-                    if (q < 0 || 4 * (size_t)q + 3 >= f.cent.size()) return false;
-                    const int* ce = f.cent.data() + 4 * (size_t)q;
+                size_t r = 4 * ((size_t)f.cp_off[s] + pos);
+                if (r + 3 >= f.cpos.size()) return false;
+                const int* rec = f.cpos.data() + r;
+                auto add = [&](const int* ce) -> bool {
                     for (int c = 0; c < 3; ++c) {
                         double v;
                         if (!sum_part(up, ce[0], ce[2], ce[1], 0, c, v)) return false;
                         acc3[c] += v;
+                    }
+                    return true;
+                };
+                if (rec[2] > 0 && !add(rec)) return false;
+                if (rec[3] >= 0) {
+                    if (rec[2] <= 0) return false;  // overflow entries follow a first one
+                    for (size_t q = (size_t)rec[3];; ++q) {
+                        if (4 * q + 3 >= f.cmore.size()) return false;
+                        const int* ce = f.cmore.data() + 4 * q;
+                        if (ce[2] == 0) break;
+                        if (!add(ce)) return false;
                     }
                 }
                 return true;

@@ -105,9 +105,10 @@ struct Factor {
     std::vector<int64_t> pr_off;     // ns + 1: row-major [Dinv_s; G_s] (m x w)
     std::vector<int64_t> uoff;       // ns + 1: kc * noff partial update entries
     std::vector<int64_t> xoff;       // ns + 1: kr * w partial solution entries
-    std::vector<int> cp_off;         // per supernode: offset of its m + 1 contribution pointers
-    std::vector<int> cptr;           // position p of s's front: contributions [cptr[cp_off+p], ..+1)
-    std::vector<int> cent;           // 4 per contribution: (child uoff + e, child noff, child kc, 0)
+    std::vector<int> cp_off;         // per supernode: its first position record
+    std::vector<int> cpos;           // 4 per front position: first contribution (child uoff + e,
+                                     //   child noff, child kc; count 0: none), more: cmore index or -1
+    std::vector<int> cmore;          // 4 per further contribution (same fields, 0), lists end with count 0
     std::vector<int> rowinfo;        // 4 per row: first partial, stride, count, 0
     std::vector<int> xloc;           // 4 per off row entry: rowinfo of that row
     // tiles grouped by height level: level l occupies [ft_ptr[l], ft_ptr[l+1]) (4 ints each)

@@ -21,6 +21,12 @@ struct SnDev {
     long long uoff, xoff;      // partial update / partial solution entries
 };
 
+// One level tile of a sweep: the supernode's record and the tile origin, so a tile is one load.
+struct TileDev {
+    SnDev s;
+    int a, b, pad0, pad1;      // forward (r0, q0); backward (q0, p0)
+};
+
 // Device view of one mesh. All pointers are device pointers into the mesh's arena.
 // Internal vertex ids: [0, nf) free vertices in solve order, [nf, n) constrained ones.
 struct MeshDev {
@@ -42,15 +48,14 @@ struct MeshDev {
     void *quat;                 // rotations as quaternions (x, y, z, w=scalar): f4 or d4
     void *r, *z;                // residual and forward result (V4 of the solve type), nf each
     void *upart, *xpart;        // partial update / solution vectors (V4)
-    const SnDev *sn;
     const int *child_ptr, *child;
-    const int *cptr;            // per front position: range of its contributions
-    const int4 *cent;           // (child partial base, stride, count, 0) per contribution
+    const int4 *cpos;           // per front position: first contribution (child partial base,
+                                //   stride, count; count 0: none), index of more in cmore or -1
+    const int4 *cmore;          // further contributions, lists ended by a zero count
     const int4 *rowinfo;        // per row: (first partial, stride, count, 0)
     const int4 *xloc;           // per off row entry: rowinfo of that row
     const void *pc, *pr;        // panels (float or double), forward and backward layouts
-    const int *ft_ptr, *bt_ptr; // tiles per level
-    const int4 *ft, *bt;        // (s, r0, q0, 0) / (s, q0, p0, 0)
+    const TileDev *ftile, *btile;  // level tiles, level by level (bases in the launch tables)
     // bottom subtrees, one wavefront each (layouts in md_analysis.h, Factor)
     const int4 *sub;            // (first record, end record, height table offset, heights)
     const int *sub_h;           // first record of each (subtree, height), + a terminal per subtree
@@ -75,6 +80,7 @@ struct LaunchSet {
     const int *prefix;
     int nactive;
     const int *bmesh;  // optional: mesh of every block (static batches), else a binary search
+    const int *tbase;  // per mesh: first tile of the launch's level (level sweeps), else 0
 };
 
 // Launchers (grid = total blocks of the launch, from the host-side prefix table).

@@ -21,9 +21,18 @@ __device__ __forceinline__ int find_mesh(const LaunchSet &L, int b) {
     return lo;
 }
 
-template <typename T> __device__ __forceinline__ T tsqrt(T x);
-template <> __device__ __forceinline__ float tsqrt<float>(float x) { return sqrtf(x); }
-template <> __device__ __forceinline__ double tsqrt<double>(double x) { return sqrt(x); }
+// Reciprocal, square root and reciprocal square root of the local step's Jacobi rotations:
+// single hardware instructions in FP32 (about 1 ulp; the rotation is renormalised at the
+// end), correctly rounded in FP64.
+template <typename T> __device__ __forceinline__ T jrcp(T x);
+template <> __device__ __forceinline__ float jrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double jrcp<double>(double x) { return 1.0 / x; }
+template <typename T> __device__ __forceinline__ T jsqrt(T x);
+template <> __device__ __forceinline__ float jsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <> __device__ __forceinline__ double jsqrt<double>(double x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T jrsqrt(T x);
+template <> __device__ __forceinline__ float jrsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double jrsqrt<double>(double x) { return 1.0 / sqrt(x); }
 
 // std::max<Scalar>(a, b) = (a < b) ? b : a; arap.h:199-218 pass the clamp first, so NaN
 // yields the clamp.
@@ -98,9 +107,9 @@ __device__ __forceinline__ void jrot(T (&a)[4][4], T (&v)[4][4]) {
         if (fabs(apq) <= (sizeof(T) == 4 ? T(1e-30) : T(1e-300))) return;
         if (scale > T(0)) return;
     }
-    T theta = (a[Q][Q] - a[P][P]) / (T(2) * apq);
-    T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + tsqrt<T>(theta * theta + T(1)));
-    T c = T(1) / tsqrt<T>(t * t + T(1));
+    T theta = (a[Q][Q] - a[P][P]) * jrcp<T>(T(2) * apq);
+    T t = (theta >= T(0) ? T(1) : T(-1)) * jrcp<T>(fabs(theta) + jsqrt<T>(theta * theta + T(1)));
+    T c = jrsqrt<T>(t * t + T(1));
     T s = t * c;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -141,7 +150,7 @@ __device__ __forceinline__ void best_rotation(const T (&m)[9], T px, T py, T pz,
                        {Sxy - Syx, Szx + Sxz, Syz + Szy, -Sxx - Syy + Szz}};
     {
         T pn = px * px + py * py + pz * pz + pw * pw;
-        T inv = T(1) / tsqrt<T>(pn > T(0) ? pn : T(1));
+        T inv = jrsqrt<T>(pn > T(0) ? pn : T(1));
         if (!(pn > T(0))) { px = 0; py = 0; pz = 0; pw = 1; }
         px *= inv; py *= inv; pz *= inv; pw *= inv;
     }
@@ -183,7 +192,7 @@ __device__ __forceinline__ void best_rotation(const T (&m)[9], T px, T py, T pz,
         q2 = b ? v[2][c] : q2;
         q3 = b ? v[3][c] : q3;
     }
-    T nrm = T(1) / tsqrt<T>(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+    T nrm = jrsqrt<T>(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
     // canonical sign: scalar part >= 0
     if (q0 < T(0)) nrm = -nrm;
     qw = q0 * nrm; qx = q1 * nrm; qy = q2 * nrm; qz = q3 * nrm;
@@ -366,18 +375,22 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
     return a;
 }
 
-// Children's contributions to position pos of supernode S's front, summed in child order.
+// Children's contributions to position pos of supernode S's front, summed in child order:
+// the first from the position's record, the rest from its overflow list.
 template <typename V>
 __device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S, int pos) {
     const V *U = reinterpret_cast<const V *>(M.upart);
-    int q0 = M.cptr[S.cp_off + pos], q1 = M.cptr[S.cp_off + pos + 1];
+    const int4 r = M.cpos[S.cp_off + pos];
     V a;
     a.x = 0; a.y = 0; a.z = 0; a.w = 0;
-    for (int q = q0; q < q1; ++q) {
-        int4 ce = M.cent[q];
-        V u = sum_partials<V>(U + ce.x, ce.z, ce.y, 0);
-        a.x += u.x; a.y += u.y; a.z += u.z;
-    }
+    if (r.z > 0) a = sum_partials<V>(U + r.x, r.z, r.y, 0);
+    if (r.w >= 0)
+        for (int q = r.w;; ++q) {
+            const int4 ce = M.cmore[q];
+            if (ce.z == 0) break;
+            const V u = sum_partials<V>(U + ce.x, ce.z, ce.y, 0);
+            a.x += u.x; a.y += u.y; a.z += u.z;
+        }
     return a;
 }
 
@@ -417,9 +430,9 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     __shared__ V vo[kTile];
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    const int4 tl = M.ft[M.ft_ptr[level] + (blockIdx.x - L.prefix[k])];
-    const SnDev S = M.sn[tl.x];
-    const int r0 = tl.y, q0 = tl.z, lane = threadIdx.x;
+    const TileDev tl = M.ftile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
+    const SnDev &S = tl.s;
+    const int r0 = tl.a, q0 = tl.b, lane = threadIdx.x;
     const int w = S.w, no = S.noff;
     const int qn = min(kTile, w - q0), rn = max(0, min(kTile, no - r0));
     const int kcb = q0 / kTile;
@@ -465,9 +478,9 @@ __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
     __shared__ V tv[kTile];
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    const int4 tl = M.bt[M.bt_ptr[level] + (blockIdx.x - L.prefix[k])];
-    const SnDev S = M.sn[tl.x];
-    const int q0 = tl.y, p0 = tl.z, lane = threadIdx.x;
+    const TileDev tl = M.btile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
+    const SnDev &S = tl.s;
+    const int q0 = tl.a, p0 = tl.b, lane = threadIdx.x;
     const int w = S.w, m = S.w + S.noff;
     const int qn = min(kTile, w - q0), pn = min(kTile, m - p0);
     const V *X = reinterpret_cast<const V *>(M.xpart);

@@ -49,6 +49,7 @@ Plan::~Plan() {
     drop_graph();
     if (d_meshes) hipFree(d_meshes);
     if (d_prefix) hipFree(d_prefix);
+    if (d_tbase) hipFree(d_tbase);
     if (d_bmesh) hipFree(d_bmesh);
 }
 
@@ -72,6 +73,7 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     int nl = plan.nlevels;
     int ntables = 6 + 2 * nl;
     plan.h_prefix.assign((size_t)ntables * (na + 1), 0);
+    plan.h_tbase.assign((size_t)ntables * (na + 1), 0);
     plan.off_fwd.assign(nl, 0);
     plan.off_bwd.assign(nl, 0);
     plan.blk_fwd.assign(nl, 0);
@@ -108,6 +110,11 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
             const Factor *f = factors[k];
             return l < f->nlevels ? f->bt_ptr[l + 1] - f->bt_ptr[l] : 0;
         });
+        for (int k = 0; k < na; ++k) {
+            const Factor *f = factors[k];
+            plan.h_tbase[(size_t)plan.off_fwd[l] + k] = l < f->nlevels ? f->ft_ptr[l] : 0;
+            plan.h_tbase[(size_t)plan.off_bwd[l] + k] = l < f->nlevels ? f->bt_ptr[l] : 0;
+        }
     }
     // block -> mesh maps: one entry per block of every table, so kernels of large batches
     // find their mesh with one load instead of a binary search over the prefix table
@@ -140,15 +147,21 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     }
     if (plan.h_prefix.size() > plan.cap_prefix) {
         if (plan.d_prefix) hipFree(plan.d_prefix);
+        if (plan.d_tbase) hipFree(plan.d_tbase);
         plan.d_prefix = nullptr;
+        plan.d_tbase = nullptr;
         MD_HIP_E(hipMalloc(&plan.d_prefix, sizeof(int) * plan.h_prefix.size()));
+        MD_HIP_E(hipMalloc(&plan.d_tbase, sizeof(int) * plan.h_prefix.size()));
         plan.cap_prefix = plan.h_prefix.size();
     }
     if (na > 0) MD_HIP_E(hipMemcpyAsync(plan.d_meshes, plan.h_meshes.data(), sizeof(MeshDev) * na,
                                         hipMemcpyHostToDevice, stream));
-    if (!plan.h_prefix.empty())
+    if (!plan.h_prefix.empty()) {
         MD_HIP_E(hipMemcpyAsync(plan.d_prefix, plan.h_prefix.data(), sizeof(int) * plan.h_prefix.size(),
                                 hipMemcpyHostToDevice, stream));
+        MD_HIP_E(hipMemcpyAsync(plan.d_tbase, plan.h_tbase.data(), sizeof(int) * plan.h_tbase.size(),
+                                hipMemcpyHostToDevice, stream));
+    }
     return sync ? hipStreamSynchronize(stream) : hipSuccess;
 }
 
@@ -514,15 +527,13 @@ int upload_layout(md_mesh *m) {
     A.reset();
     size_t o_up = A.reserve(vsz * std::max<int64_t>(1, last(f.uoff)));
     size_t o_xp = A.reserve(vsz * std::max<int64_t>(1, last(f.xoff)));
-    size_t o_sn = A.reserve(sizeof(SnDev) * std::max(1, ns));
-    size_t o_cptr = A.reserve(sizeof(int) * std::max<size_t>(1, f.cptr.size()));
-    size_t o_cen = A.reserve(sizeof(int) * std::max<size_t>(4, f.cent.size()));
+    const size_t nft = f.ft.size() / 4, nbt = f.bt.size() / 4;
+    size_t o_cpos = A.reserve(sizeof(int) * std::max<size_t>(4, f.cpos.size()));
+    size_t o_cmore = A.reserve(sizeof(int) * std::max<size_t>(4, f.cmore.size()));
     size_t o_ri = A.reserve(sizeof(int) * std::max<size_t>(4, f.rowinfo.size()));
     size_t o_xl = A.reserve(sizeof(int) * std::max<size_t>(4, f.xloc.size()));
-    size_t o_ftp = A.reserve(sizeof(int) * (nl + 1));
-    size_t o_btp = A.reserve(sizeof(int) * (nl + 1));
-    size_t o_ft = A.reserve(sizeof(int) * std::max<size_t>(4, f.ft.size()));
-    size_t o_bt = A.reserve(sizeof(int) * std::max<size_t>(4, f.bt.size()));
+    size_t o_ftile = A.reserve(sizeof(TileDev) * std::max<size_t>(1, nft));
+    size_t o_btile = A.reserve(sizeof(TileDev) * std::max<size_t>(1, nbt));
     size_t o_sub = A.reserve(sizeof(int) * std::max<size_t>(4, f.sub.size()));
     size_t o_subh = A.reserve(sizeof(int) * std::max<size_t>(1, f.sub_h.size()));
     size_t o_snode = A.reserve(sizeof(int) * std::max<size_t>(16, f.snode.size()));
@@ -536,11 +547,9 @@ int upload_layout(md_mesh *m) {
     }
     MeshDev &d = m->dev;
     d.upart = A.at(o_up); d.xpart = A.at(o_xp);
-    d.sn = (const SnDev *)A.at(o_sn);
-    d.cptr = (const int *)A.at(o_cptr); d.cent = (const int4 *)A.at(o_cen);
+    d.cpos = (const int4 *)A.at(o_cpos); d.cmore = (const int4 *)A.at(o_cmore);
     d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
-    d.ft_ptr = (const int *)A.at(o_ftp); d.bt_ptr = (const int *)A.at(o_btp);
-    d.ft = (const int4 *)A.at(o_ft); d.bt = (const int4 *)A.at(o_bt);
+    d.ftile = (const TileDev *)A.at(o_ftile); d.btile = (const TileDev *)A.at(o_btile);
     d.sub = (const int4 *)A.at(o_sub); d.sub_h = (const int *)A.at(o_subh);
     d.snode = (const int4 *)A.at(o_snode); d.scp = (const int2 *)A.at(o_scp);
     d.sfc = (const int *)A.at(o_sfc); d.sbo = (const int2 *)A.at(o_sbo);
@@ -549,25 +558,27 @@ int upload_layout(md_mesh *m) {
         if (bytes == 0) return hipSuccess;
         return hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, st);
     };
-    std::vector<SnDev> snh(ns);
-    for (int k = 0; k < ns; ++k) {
-        SnDev &x = snh[k];
+    auto sn_of = [&](int k) {
+        SnDev x;
         x.c0 = f.c0[k]; x.w = f.w[k]; x.noff = f.noff[k]; x.roff = f.roff[k];
         x.kc = f.kc[k]; x.kr = f.kr[k]; x.cp_off = f.cp_off[k]; x.pad = 0;
         x.pc_off = f.pc_off[k]; x.pr_off = f.pr_off[k]; x.uoff = f.uoff[k]; x.xoff = f.xoff[k];
-    }
+        return x;
+    };
+    std::vector<TileDev> fth(nft), bth(nbt);
+    for (size_t i = 0; i < nft; ++i)
+        fth[i] = TileDev{sn_of(f.ft[4 * i]), f.ft[4 * i + 1], f.ft[4 * i + 2], 0, 0};
+    for (size_t i = 0; i < nbt; ++i)
+        bth[i] = TileDev{sn_of(f.bt[4 * i]), f.bt[4 * i + 1], f.bt[4 * i + 2], 0, 0};
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
     if (ns > 0) {
-        chk(up(d.sn, snh.data(), sizeof(SnDev) * ns));
-        chk(up(d.cptr, f.cptr.data(), sizeof(int) * f.cptr.size()));
-        chk(up(d.cent, f.cent.data(), sizeof(int) * f.cent.size()));
+        chk(up(d.cpos, f.cpos.data(), sizeof(int) * f.cpos.size()));
+        chk(up(d.cmore, f.cmore.data(), sizeof(int) * f.cmore.size()));
         chk(up(d.rowinfo, f.rowinfo.data(), sizeof(int) * f.rowinfo.size()));
         chk(up(d.xloc, f.xloc.data(), sizeof(int) * f.xloc.size()));
-        chk(up(d.ft_ptr, f.ft_ptr.data(), sizeof(int) * (nl + 1)));
-        chk(up(d.bt_ptr, f.bt_ptr.data(), sizeof(int) * (nl + 1)));
-        chk(up(d.ft, f.ft.data(), sizeof(int) * f.ft.size()));
-        chk(up(d.bt, f.bt.data(), sizeof(int) * f.bt.size()));
+        chk(up(d.ftile, fth.data(), sizeof(TileDev) * nft));
+        chk(up(d.btile, bth.data(), sizeof(TileDev) * nbt));
         chk(up(d.sub, f.sub.data(), sizeof(int) * f.sub.size()));
         chk(up(d.sub_h, f.sub_h.data(), sizeof(int) * f.sub_h.size()));
         chk(up(d.snode, f.snode.data(), sizeof(int) * f.snode.size()));

@@ -55,11 +55,12 @@ struct Plan {
     std::vector<int> off_fwd, off_bwd;
     MeshDev *d_meshes = nullptr;
     int *d_prefix = nullptr;
+    int *d_tbase = nullptr;            // per table and mesh: first tile of the level (level sweeps)
     int *d_bmesh = nullptr;            // block -> mesh maps of every table (static batches)
     size_t cap_meshes = 0, cap_prefix = 0, cap_bmesh = 0;
     std::vector<int> h_bmesh;
     std::vector<size_t> bmesh_off;     // per table: offset into d_bmesh (SIZE_MAX: no map)
-    std::vector<int> h_prefix;
+    std::vector<int> h_prefix, h_tbase;
     std::vector<MeshDev> h_meshes;
     bool fp64 = true;
     int refine_steps = 1;
@@ -71,7 +72,7 @@ struct Plan {
     LaunchSet set(int off) const {
         size_t t = (size_t)off / (size_t)(nactive + 1);
         const int *bm = (d_bmesh && t < bmesh_off.size() && bmesh_off[t] != (size_t)-1) ? d_bmesh + bmesh_off[t] : nullptr;
-        return LaunchSet{d_meshes, d_prefix + off, nactive, bm};
+        return LaunchSet{d_meshes, d_prefix + off, nactive, bm, d_tbase + off};
     }
 };
 
