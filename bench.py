@@ -60,16 +60,22 @@ def cpu_model():
     return platform.processor()
 
 
-def plain_leg(ctx, pr, steps, warmup, tune=True, budget=None):
+def plain_leg(ctx, pr, steps, warmup, tune=True, budget=None, fixed_ratio=None):
     idx, tgt = pr.constraints()
-    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget))
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget,
+                                                        amalg_ratio=fixed_ratio))
     m.set_constraints(idx, tgt)
     t0 = time.perf_counter()
     m.prepare(pr.xyz)
     setup_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    # sweep layout: the fastest bottom-subtree budget (state restored)
-    budget = m.autotune()[0] if tune and budget is None else m.stats()["subtree_budget"]
+    # factor and sweep layout: the fastest (amalgamation ratio, subtree budget), measured on
+    # this mesh; the mesh is left freshly prepared with them
+    ratio = fixed_ratio
+    if tune and budget is None and fixed_ratio is None:
+        ratio, budget, _ = m.autotune_factor()
+    else:
+        budget = m.stats()["subtree_budget"]
     tune_s = time.perf_counter() - t0
     m.iterate(warmup)
     ctx.synchronize()
@@ -82,11 +88,12 @@ def plain_leg(ctx, pr, steps, warmup, tune=True, budget=None):
     ev_total_ms, phase_ms = m.time_iterations(0, steps)
     st = m.stats()
     st["subtree_budget"] = budget
+    st["amalg_ratio"] = ratio if ratio is not None else mdarap.config().amalg_ratio
     st["autotune_s"] = tune_s
     return m, st, setup_s, wall, ev_total_ms, phase_ms
 
 
-def check_leg(ctx, pr, iters, cpu_seconds_cap):
+def check_leg(ctx, pr, iters, cpu_seconds_cap, budget, ratio):
     import oracle
 
     idx, tgt = pr.constraints()
@@ -106,7 +113,8 @@ def check_leg(ctx, pr, iters, cpu_seconds_cap):
     cpu_iter_s = (one + rest_s) / n_it
     errs = {}
     for name, prec in (("fp32_refine", mdarap.PREC_FP32_REFINE), ("fp64", mdarap.PREC_FP64)):
-        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=prec))
+        # the plain leg's factor and sweep layout
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=prec, subtree_budget=budget, amalg_ratio=ratio))
         m.set_constraints(idx, tgt)
         assert m.deform(n_it)
         errs[name] = meshgen.relative_error(m.mesh, a.mesh)
@@ -231,7 +239,8 @@ def canonical_bytes(pr, st):
     int32 indices; each array once per pass): local step + rotated-edge pass
     (120 + 16 d) n + 36 nf with d = 2E/n, the FP64 residual of the refinement step
     12 nnz(A) + 124 nf, and one direct sweep pair with the FP32 factor 16 nL + 68 nf
-    (nL = strict nnz of the factor). An iteration here runs one residual and one sweep
+    (nL = strict nnz of the exact factor: the explicit zeros that relaxed amalgamation
+    stores are not counted). An iteration here runs one residual and one sweep
     pair (the residual form folds the right-hand side into the refinement), so
     B_iter = B_local+rhs + B_ref with B_ref = 12 nnz(A) + 124 nf + B_solve."""
     n, nf = pr.n, st["nf"]
@@ -241,7 +250,7 @@ def canonical_bytes(pr, st):
     con[pr.constraints()[0]] = True
     e_ff = int(np.count_nonzero(~con[e[:, 0]] & ~con[e[:, 1]]))
     nnz_a = nf + 2 * e_ff
-    n_l = st["factor_nnz"]
+    n_l = st["factor_nnz_exact"]
     d = 2.0 * e.shape[0] / n
     b_local = (120 + 16 * d) * n + 36 * nf
     b_solve = 16 * n_l + 68 * nf
@@ -250,7 +259,7 @@ def canonical_bytes(pr, st):
                 n=n, nf=nf, E=int(e.shape[0]), nnzA_full=int(nnz_a), nL=int(n_l), mean_valence=round(d, 4))
 
 
-def traffic_probe(ctx, iters, budget=None):
+def traffic_probe(ctx, iters, budget=None, ratio=None):
     """Workload for rocprofv3 --pmc passes: the library's plain copy kernel over 256 MiB
     (calibration: each dispatch reads and writes exactly 256 MiB), then `iters` ARAP
     iterations of the plain leg's mesh and configuration."""
@@ -258,11 +267,12 @@ def traffic_probe(ctx, iters, budget=None):
     gbps = mdarap.copy_probe(ctx, nbytes, 3)
     pr = meshgen.noisy_icosphere_problem(7, 0.01, 1234)
     idx, tgt = pr.constraints()
-    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget))
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, subtree_budget=budget,
+                                                        amalg_ratio=ratio))
     m.set_constraints(idx, tgt)
     m.prepare(pr.xyz)
-    if budget is None:
-        m.autotune()
+    if budget is None and ratio is None:
+        m.autotune_factor()
     m.iterate(2)
     ctx.synchronize()
     log("traffic probe: calibration copies done, running %d iterations" % iters)
@@ -271,7 +281,8 @@ def traffic_probe(ctx, iters, budget=None):
     used = m.stats()["subtree_budget"]
     m.close()
     print(json.dumps({"probe": "traffic", "copy_bytes": nbytes, "copies": 4, "copy_GBs": round(gbps, 1),
-                      "iterations": iters, "warmup_iterations": 2, "subtree_budget": used}), flush=True)
+                      "iterations": iters, "warmup_iterations": 2, "subtree_budget": used,
+                      "amalg_ratio": ratio}), flush=True)
 
 
 def traffic_summary(fetch_csv, write_csv, iters):
@@ -308,16 +319,25 @@ def traffic_summary(fetch_csv, write_csv, iters):
     return out
 
 
-def read_traffic(path):
-    """traffic_bytes_per_iter from a committed traffic summary (a line 'traffic_bytes_per_iter: N')."""
+def read_traffic(path, budget, ratio):
+    """traffic_bytes_per_iter from a committed traffic summary (lines 'traffic_bytes_per_iter: N'
+    and 'traffic_config: subtree_budget=B amalg_ratio=R'), when it was measured on the
+    factor and sweep layout the plain leg runs."""
+    val, cfg = None, None
     try:
         with open(path) as f:
             for line in f:
-                if line.strip().startswith("traffic_bytes_per_iter:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+                line = line.strip()
+                if line.startswith("traffic_bytes_per_iter:"):
+                    val = float(line.split(":", 1)[1])
+                elif line.startswith("traffic_config:"):
+                    kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
+                    cfg = (int(kv["subtree_budget"]), float(kv["amalg_ratio"]))
+    except (OSError, ValueError, KeyError):
+        return None
+    if cfg is None or cfg[0] != budget or abs(cfg[1] - ratio) > 1e-9:
+        return None
+    return val
 
 
 def main():
@@ -339,6 +359,8 @@ def main():
                     help="batched leg without the one-by-one baseline (profiling runs: one graph per mesh)")
     ap.add_argument("--subtree-budget", type=int, default=None,
                     help="fixed sweep layout of the plain leg and the traffic probe (no autotune)")
+    ap.add_argument("--amalg-ratio", type=float, default=None,
+                    help="fixed factor amalgamation of the plain leg and the traffic probe (no autotune)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default sweep layout (profiling runs: the tuner re-captures graphs)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -360,7 +382,7 @@ def main():
         sys.exit(1)
     ctx = mdarap.Context(0)
     if args.traffic_probe:
-        traffic_probe(ctx, args.probe_iters, args.subtree_budget)
+        traffic_probe(ctx, args.probe_iters, args.subtree_budget, args.amalg_ratio)
         ctx.close()
         return
     if args.batched_only:
@@ -372,7 +394,7 @@ def main():
     pr = meshgen.noisy_icosphere_problem(7, 0.01, 1234)
     log("plain leg: %s n=%d F=%d" % (pr.name, pr.n, pr.F))
     m, st, setup_s, wall, ev_ms, phase = plain_leg(ctx, pr, args.steps, args.warmup, not args.no_autotune,
-                                                   args.subtree_budget)
+                                                   args.subtree_budget, args.amalg_ratio)
     value = args.steps / wall
     ms_step = wall * 1e3 / args.steps
     ev_step_ms = ev_ms / args.steps
@@ -382,7 +404,7 @@ def main():
     iter_bytes = st["bytes_local"] + st["bytes_residual"] + solve_bytes
     cb = canonical_bytes(pr, st)
     achieved = cb["B_iter"] / (ev_step_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.traffic_summary)
+    traffic = read_traffic(args.traffic_summary, st["subtree_budget"], st["amalg_ratio"])
     log("plain: %.1f iters/s (%.4f ms/iter wall, %.4f ms event); phases ms %s" %
         (value, ms_step, ev_step_ms, np.round(ph_step, 4).tolist()))
     out = {
@@ -397,7 +419,8 @@ def main():
                    "n": pr.n, "faces": pr.F, "free": st["nf"], "supernodes": st["supernodes"],
                    "levels": st["levels"], "panel_entries": st["panel_entries"],
                    "factor_nnz": st["factor_nnz"], "setup_s": round(setup_s, 3),
-                   "subtree_budget": st["subtree_budget"], "autotune_s": round(st["autotune_s"], 3)},
+                   "subtree_budget": st["subtree_budget"], "amalg_ratio": st["amalg_ratio"],
+                   "autotune_s": round(st["autotune_s"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None if traffic is None else int(traffic),
@@ -414,7 +437,7 @@ def main():
     m.close()
     if not args.no_check:
         log("check + cpu leg (oracle, bounded to ~%.0f s)" % args.cpu_seconds)
-        ck = check_leg(ctx, pr, args.check_iters, args.cpu_seconds)
+        ck = check_leg(ctx, pr, args.check_iters, args.cpu_seconds, st["subtree_budget"], st["amalg_ratio"])
         out["rel_err_vs_fp64_oracle"] = {k: float("%.3e" % v) for k, v in ck["errors"].items()}
         out["rel_err_iters"] = ck["iters"]
         out["cpu_baseline"] = {"value": round(1.0 / ck["iter_s"], 3), "unit": "iters/s", "cores": 1,

@@ -64,6 +64,10 @@ typedef struct {
                           subtree of the factor swept by a single wavefront (fronts + columns of
                           its supernodes); 0 sweeps every supernode by level launches;
                           default 512 */
+    double amalg_ratio;/* MD_SOLVER_LDLT: relaxed supernode amalgamation, the share of explicit
+                          zeros a merge may leave (half of it for merges wider than 48
+                          columns), in [0, 1]; larger gives fewer, wider supernodes and fewer
+                          sweep levels for more panel bytes; default 0.3 */
 } md_config;
 
 typedef struct {
@@ -74,7 +78,8 @@ typedef struct {
     int64_t slots;           /* one-ring slots incl. padding (64 * slice widths) */
     int64_t cotan_nnz;       /* nonzeros of the cotan weight matrix (2E) */
     int64_t panel_entries;   /* stored factor entries (Dinv + G blocks) */
-    int64_t factor_nnz;      /* strictly-lower entries of the equivalent LDL^T factor */
+    int64_t factor_nnz;      /* strictly-lower entries of the supernodal factor as stored
+                                (explicit zeros of relaxed amalgamation included) */
     int64_t fwd_items, bwd_items;
     int64_t device_bytes;    /* device memory held by the mesh */
     double setup_ms_host;    /* last analysis + factorization time on the host */
@@ -83,6 +88,7 @@ typedef struct {
     int64_t bytes_local, bytes_residual, bytes_forward, bytes_backward;
     int subtree_budget;      /* sweep layout in use (md_config.subtree_budget or the autotuned one) */
     int subtrees;            /* bottom subtrees swept by one wavefront each */
+    int64_t factor_nnz_exact;/* strictly-lower nonzeros of the exact factor (no explicit zeros) */
 } md_mesh_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -158,6 +164,14 @@ int md_mesh_time_iterations(md_mesh *mesh, int warmup, int iterations, double *t
  * the call must be rebuilt; a mesh registered with a server is MD_ERR_STATE. */
 int md_mesh_autotune(md_mesh *mesh, const int *budgets, int count, int iterations, int *best_budget,
                      double *best_ms);
+/* Autotuner of the factor layout as well: for each of the nratios amalgamation ratios
+ * (md_config.amalg_ratio) the dirty setup is re-run from the rest pose (analysis and FP64
+ * factorization) and md_mesh_autotune chooses the subtree budget; the fastest pair is kept and
+ * the mesh is left freshly prepared with it. Call it right after md_mesh_prepare, before any
+ * iteration (MD_ERR_STATE otherwise). Costs one setup per ratio. */
+int md_mesh_autotune_factor(md_mesh *mesh, const double *ratios, int nratios, const int *budgets,
+                            int nbudgets, int iterations, double *best_ratio, int *best_budget,
+                            double *best_ms);
 
 /* ---- batches: many prepared meshes iterated in the same launches -------------------- */
 typedef struct md_batch md_batch;

@@ -292,10 +292,11 @@ void build_topology(int n, int F, const int* tri, const std::vector<int>& perm,
 
 // ---- symbolic ---------------------------------------------------------------------------
 
-int g_amalg_small = 16;
-double g_amalg_ratio = 0.3;
+// Supernodes of at most this many columns always merge with their parent.
+constexpr int kAmalgSmall = 16;
 
-void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, Factor& f, int sub_budget) {
+void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, Factor& f, int sub_budget,
+              double amalg_ratio) {
     f = Factor();
     f.n = n;
     std::vector<int> ident(n);
@@ -315,6 +316,8 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
             }
         }
     }
+    f.nnz_exact = 0;
+    for (int j = 0; j < n; ++j) f.nnz_exact += cc[j] - 1;
     std::vector<int> nchild(n, 0);
     for (int j = 0; j < n; ++j)
         if (par[j] != -1) nchild[par[j]]++;
@@ -372,8 +375,8 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
             if (wm > kMaxSnWidth) break;
             // relaxed amalgamation: tiny supernodes always merge, wider ones only while the
             // share of explicit zeros stays small
-            bool ok = wm <= g_amalg_small || (wm <= 16 && frac < 0.5) ||
-                      (wm <= 48 && frac < g_amalg_ratio) || frac < 0.5 * g_amalg_ratio;
+            bool ok = wm <= kAmalgSmall || (wm <= 16 && frac < 0.5) ||
+                      (wm <= 48 && frac < amalg_ratio) || frac < 0.5 * amalg_ratio;
             if (!ok) break;
             zeros[p] = z;
             // merge c into p
@@ -1149,6 +1152,7 @@ AnalysisStats stats(const Topology& t, const Factor& f) {
     for (int k = 0; k < f.ns; ++k)
         strict += (int64_t)f.w[k] * (f.w[k] - 1) / 2 + (int64_t)f.w[k] * f.noff[k];
     s.strict_nnz_equiv = strict;
+    s.nnz_exact = f.nnz_exact;
     s.fwd_items = f.ft_ptr.empty() ? 0 : f.ft_ptr.back();
     s.bwd_items = f.bt_ptr.empty() ? 0 : f.bt_ptr.back();
     s.max_width = 0;

@@ -21,6 +21,9 @@ constexpr int kTile = 64;
 constexpr int kSlice = 64;
 // Default LDS budget (vector entries) of one bottom subtree swept by a single wavefront.
 constexpr int kSubBudgetDefault = 512;
+// Default relaxed-amalgamation limit: a child merges into its parent while the explicit zeros
+// stay below this share of the merged panel (half of it for merges wider than 48 columns).
+constexpr double kAmalgRatioDefault = 0.3;
 // Largest budget: 1024 FP64 vector entries (32 bytes each) are 32 KiB of LDS; with at most
 // kSubMaxNodes 64-byte supernode records a subtree stays within 36 KiB.
 constexpr int kSubBudgetMax = 1024;
@@ -125,12 +128,13 @@ struct Factor {
     std::vector<int> scp;            // 2 per forward position: first two child entries (see above)
     std::vector<int> sfc;            // -1-terminated overflow lists of child entries
     std::vector<int> sbo;            // 2 per off-row position of a subtree supernode (backward source)
+    int64_t nnz_exact = 0;           // strictly-lower nonzeros of the exact factor (no amalgamation zeros)
     int64_t factor_nnz() const { return poff.empty() ? 0 : poff.back(); }
 };
 
 struct AnalysisStats {
     int n, nf, nslices, ns, nlevels;
-    int64_t slots, panel_entries, strict_nnz_equiv, fwd_items, bwd_items;
+    int64_t slots, panel_entries, strict_nnz_equiv, nnz_exact, fwd_items, bwd_items;
     int max_width, max_rows;
 };
 
@@ -150,9 +154,10 @@ std::vector<int> order_graph(int m, const int* rowptr, const int* col, const dou
 
 // Symbolic supernodal analysis of the free-free graph in solve order.
 // adj_ptr/adj (CSR over solve indices, no diagonal needed) describe A's pattern.
-// sub_budget: LDS vector entries of one bottom subtree (0: no subtree sweeps).
+// sub_budget: LDS vector entries of one bottom subtree (0: no subtree sweeps); amalg_ratio:
+// relaxed-amalgamation limit (more merges: fewer, wider supernodes and levels, more zeros).
 void symbolic(int n, const std::vector<int>& adj_ptr, const std::vector<int>& adj, Factor& f,
-              int sub_budget = kSubBudgetDefault);
+              int sub_budget = kSubBudgetDefault, double amalg_ratio = kAmalgRatioDefault);
 
 // Device layout of a symbolic factor (partials, contribution lists, tiles, bottom subtrees)
 // for the given subtree budget; symbolic() calls it, and it may be called again to re-lay

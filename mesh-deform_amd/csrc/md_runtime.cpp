@@ -235,6 +235,7 @@ extern "C" void md_config_default(md_config *cfg) {
     cfg->pcg_tol = 1e-7;
     cfg->use_graph = 1;
     cfg->subtree_budget = kSubBudgetDefault;
+    cfg->amalg_ratio = kAmalgRatioDefault;
 }
 
 extern "C" const char *md_last_error(void) { return g_err.c_str(); }
@@ -316,6 +317,10 @@ extern "C" int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const m
     }
     if (m->cfg.solver == MD_SOLVER_PCG && (m->cfg.pcg_max_iters < 1 || !(m->cfg.pcg_tol >= 0.0))) {
         set_error("md_mesh_create: PCG needs pcg_max_iters >= 1 and pcg_tol >= 0");
+        return MD_ERR_INVALID;
+    }
+    if (!(m->cfg.amalg_ratio >= 0.0 && m->cfg.amalg_ratio <= 1.0)) {
+        set_error("md_mesh_create: amalg_ratio must lie in [0, 1]");
         return MD_ERR_INVALID;
     }
     if (m->cfg.subtree_budget < 0 || m->cfg.subtree_budget > kSubBudgetMax) {
@@ -401,7 +406,7 @@ void analyze_host(md_mesh *m) {
     }
     m->fac = Factor();
     if (nf > 0 && nf < n && m->cfg.solver == MD_SOLVER_LDLT)
-        symbolic(nf, m->adj_ptr, m->adj, m->fac, m->cfg.subtree_budget);
+        symbolic(nf, m->adj_ptr, m->adj, m->fac, m->cfg.subtree_budget, m->cfg.amalg_ratio);
     else m->fac.n = nf;
 }
 
@@ -519,7 +524,7 @@ int analyze_device(md_mesh *m) {
 // and every other device array are untouched.
 int upload_layout(md_mesh *m) {
     const Factor &f = m->fac;
-    const int ns = f.ns, nl = f.nlevels;
+    const int ns = f.ns;
     const bool fp64 = m->fp64();
     const size_t vsz = fp64 ? sizeof(d4) : sizeof(f4);
     auto last = [](const std::vector<int64_t> &v) { return v.empty() ? (int64_t)0 : v.back(); };
@@ -940,6 +945,55 @@ extern "C" int md_mesh_autotune(md_mesh *m, const int *budgets, int count, int i
     return MD_OK;
 }
 
+extern "C" int md_mesh_autotune_factor(md_mesh *m, const double *ratios, int nratios, const int *budgets,
+                                       int nbudgets, int iterations, double *best_ratio, int *best_budget,
+                                       double *best_ms) {
+    if (!m || !ratios || nratios <= 0) { set_error("md_mesh_autotune_factor: need at least one ratio"); return MD_ERR_INVALID; }
+    int rc = check_budgets("md_mesh_autotune_factor", budgets, nbudgets, iterations);
+    if (rc) return rc;
+    for (int i = 0; i < nratios; ++i)
+        if (!(ratios[i] >= 0.0 && ratios[i] <= 1.0)) {
+            set_error("md_mesh_autotune_factor: ratios must lie in [0, 1]");
+            return MD_ERR_INVALID;
+        }
+    if (m->dirty || !m->analyzed || !m->factored || m->iterated) {
+        set_error("md_mesh_autotune_factor: call it on a freshly prepared mesh (md_mesh_prepare, no iteration yet)");
+        return MD_ERR_STATE;
+    }
+    if (m->cfg.solver != MD_SOLVER_LDLT) { set_error("md_mesh_autotune_factor: only the LDLT solver has a factor layout"); return MD_ERR_INVALID; }
+    if (m->servers > 0) { set_error("md_mesh_autotune_factor: the mesh is registered with a server"); return MD_ERR_STATE; }
+    hipSetDevice(m->ctx->device);
+    // every candidate re-runs the dirty setup from the rest pose: the state it leaves is the
+    // freshly prepared one, whatever the candidate
+    const std::vector<double> rest = m->rest;
+    auto reprepare = [&](double ratio) -> int {
+        m->cfg.amalg_ratio = ratio;
+        m->dirty = true;
+        m->con_set_changed = true;
+        m->initialized = false;
+        bool early = false;
+        int r = setup(m, rest.data(), &early);
+        if (r == MD_OK && early) { set_error("md_mesh_autotune_factor: no constrained vertex"); r = MD_ERR_STATE; }
+        if (r == MD_OK) ++m->gen;
+        return r;
+    };
+    double bt = 1e300, br = m->cfg.amalg_ratio;
+    int bb = m->cfg.subtree_budget;
+    for (int i = 0; i < nratios; ++i) {
+        if ((rc = reprepare(ratios[i]))) return rc;
+        int b = 0;
+        double t = 0.0;
+        if ((rc = md_mesh_autotune(m, budgets, nbudgets, iterations, &b, &t))) return rc;
+        if (t < bt) { bt = t; br = ratios[i]; bb = b; }
+    }
+    m->cfg.subtree_budget = bb;
+    if ((rc = reprepare(br))) return rc;
+    if (best_ratio) *best_ratio = br;
+    if (best_budget) *best_budget = bb;
+    if (best_ms) *best_ms = bt;
+    return MD_OK;
+}
+
 extern "C" int md_mesh_prepare(md_mesh *m, const double *xyz) {
     if (!m || !xyz) { set_error("md_mesh_prepare: null pointer"); return MD_ERR_INVALID; }
     hipSetDevice(m->ctx->device);
@@ -1177,6 +1231,7 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->cotan_nnz = nnz;
     out->panel_entries = s.panel_entries;
     out->factor_nnz = s.strict_nnz_equiv;
+    out->factor_nnz_exact = s.nnz_exact;
     out->fwd_items = s.fwd_items;
     out->bwd_items = s.bwd_items;
     out->device_bytes = (int64_t)(m->arena.size() + m->larena.size());

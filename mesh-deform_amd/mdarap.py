@@ -21,6 +21,8 @@ REST_REFERENCE, REST_FIXED = 0, 1
 
 # candidate bottom-subtree budgets of the sweep-layout autotuner (LDS vector entries; 0: none)
 AUTOTUNE_BUDGETS = (0, 256, 512, 1024)
+# candidate relaxed-amalgamation ratios of the factor-layout autotuner
+AUTOTUNE_RATIOS = (0.3, 0.5, 0.7, 1.0)
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -30,7 +32,7 @@ _vp = ctypes.c_void_p
 class Config(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("solver", ctypes.c_int), ("refine_steps", ctypes.c_int),
                 ("rest_mode", ctypes.c_int), ("pcg_max_iters", ctypes.c_int), ("pcg_tol", ctypes.c_double),
-                ("use_graph", ctypes.c_int), ("subtree_budget", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("subtree_budget", ctypes.c_int), ("amalg_ratio", ctypes.c_double)]
 
 
 class MeshStats(ctypes.Structure):
@@ -42,7 +44,7 @@ class MeshStats(ctypes.Structure):
                 ("device_bytes", ctypes.c_int64), ("setup_ms_host", ctypes.c_double),
                 ("bytes_local", ctypes.c_int64), ("bytes_residual", ctypes.c_int64),
                 ("bytes_forward", ctypes.c_int64), ("bytes_backward", ctypes.c_int64),
-                ("subtree_budget", ctypes.c_int), ("subtrees", ctypes.c_int)]
+                ("subtree_budget", ctypes.c_int), ("subtrees", ctypes.c_int), ("factor_nnz_exact", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -92,6 +94,9 @@ _SIGS = [
     ("md_batch_time_iterations", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("md_mesh_autotune", ctypes.c_int, [_vp, _i32p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_double)]),
+    ("md_mesh_autotune_factor", ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_double)]),
     ("md_batch_autotune", ctypes.c_int, [_vp, _i32p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_double)]),
     ("md_batch_destroy", ctypes.c_int, [_vp]),
@@ -140,7 +145,7 @@ def device_count():
 
 
 def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_graph=True, solver=SOLVER_LDLT,
-           pcg_iters=None, pcg_tol=None, subtree_budget=None):
+           pcg_iters=None, pcg_tol=None, subtree_budget=None, amalg_ratio=None):
     c = Config()
     lib().md_config_default(ctypes.byref(c))
     c.precision = precision
@@ -154,6 +159,8 @@ def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_gr
         c.pcg_tol = float(pcg_tol)
     if subtree_budget is not None:
         c.subtree_budget = int(subtree_budget)
+    if amalg_ratio is not None:
+        c.amalg_ratio = float(amalg_ratio)
     return c
 
 
@@ -280,6 +287,16 @@ class Mesh:
     def autotune(self, budgets=AUTOTUNE_BUDGETS, iters=5):
         """md_mesh_autotune: keeps the fastest sweep layout; returns (budget, ms per iteration)."""
         return _autotune(lib().md_mesh_autotune, self._h, budgets, iters)
+
+    def autotune_factor(self, ratios=AUTOTUNE_RATIOS, budgets=AUTOTUNE_BUDGETS, iters=5):
+        """md_mesh_autotune_factor (freshly prepared mesh): fastest (amalgamation ratio, subtree
+        budget); returns (ratio, budget, ms per iteration)."""
+        r = np.ascontiguousarray(ratios, np.float64)
+        b = np.ascontiguousarray(budgets, np.int32)
+        br, bb, ms = ctypes.c_double(0), ctypes.c_int(0), ctypes.c_double(0)
+        check(lib().md_mesh_autotune_factor(self._h, r, len(r), b, len(b), int(iters), ctypes.byref(br),
+                                            ctypes.byref(bb), ctypes.byref(ms)))
+        return br.value, bb.value, ms.value
 
 
 def _autotune(fn, h, budgets, iters):

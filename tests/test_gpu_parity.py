@@ -283,6 +283,42 @@ def test_mesh_autotune_keeps_state_and_results(ctx):
     np.testing.assert_array_equal(m2.download(), ref2.download())
 
 
+def test_factor_autotune_equals_fresh_mesh(ctx):
+    """The factor-layout autotuner leaves the freshly prepared state of the chosen (ratio,
+    budget); iterations then match a mesh created with that configuration bitwise, and the
+    oracle within the FP32 + refinement tolerance."""
+    say("case: factor autotune")
+    pr = meshgen.noisy_icosphere_problem(4)
+    idx, tgt = pr.constraints()
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+    m.set_constraints(idx, tgt)
+    m.prepare(pr.xyz)
+    fresh = m.download()
+    ratio, budget, ms = m.autotune_factor((0.3, 1.0), (0, 256), iters=2)
+    say("  best ratio", ratio, "budget", budget, "ms/iter", ms)
+    assert ratio in (0.3, 1.0) and budget in (0, 256) and ms > 0
+    st = m.stats()
+    assert st["subtree_budget"] == budget
+    np.testing.assert_array_equal(m.download(), fresh)
+    with pytest.raises(mdarap.MDError) as e:  # only before the first iteration
+        m.iterate(1)
+        m.autotune_factor((0.3,), (0,), iters=1)
+    assert e.value.code == mdarap.MD_ERR_STATE
+    ref = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, amalg_ratio=ratio,
+                                                          subtree_budget=budget))
+    ref.set_constraints(idx, tgt)
+    ref.prepare(pr.xyz)
+    ref.iterate(1)
+    m.iterate(4)
+    ref.iterate(4)
+    got = m.download()
+    np.testing.assert_array_equal(got, ref.download())
+    a = oracle.Arap(pr.xyz, pr.tri)
+    a.set_constraints(idx, tgt)
+    assert a.deform(5)
+    assert meshgen.relative_error(got, a.mesh) < POS_TOL_FP32
+
+
 def test_batch_autotune_and_stale_batch(ctx):
     """Batch autotune: one layout for all meshes, results bitwise those of meshes created with
     it; a batch whose mesh was re-laid out afterwards refuses to launch."""
