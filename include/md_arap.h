@@ -68,6 +68,9 @@ typedef struct {
                           zeros a merge may leave (half of it for merges wider than 48
                           columns), in [0, 1]; larger gives fewer, wider supernodes and fewer
                           sweep levels for more panel bytes; default 0.3 */
+    int factor_device; /* MD_SOLVER_LDLT: 1 (default) runs the FP64 numeric factorization on the
+                          device (multifrontal, one height of the supernodal tree per launch
+                          pair); 0 runs it on the host */
 } md_config;
 
 typedef struct {
@@ -89,6 +92,7 @@ typedef struct {
     int subtree_budget;      /* sweep layout in use (md_config.subtree_budget or the autotuned one) */
     int subtrees;            /* bottom subtrees swept by one wavefront each */
     int64_t factor_nnz_exact;/* strictly-lower nonzeros of the exact factor (no explicit zeros) */
+    double factor_ms;        /* wall time of the last numeric factorization (device or host) */
 } md_mesh_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -220,10 +224,13 @@ int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbps);
 /* Factors the SPD matrix given as full CSR (n rows, ascending columns), with coordinates
  * xyz (3 per row) for the ordering, by the same host analysis and FP64 block-LDL^T the
  * library uploads, then solves nrhs right-hand sides (b[r*n + i]) on the host by an
- * emulation of the device sweeps (subtree_budget as in md_config; negative: the default). */
+ * emulation of the device sweeps (subtree_budget as in md_config; negative: the default).
+ * device_factor_emulation != 0 factors by a host emulation of the device factorization (the
+ * same steps and index math as its kernels, every index checked) instead. */
 int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
                          const double *xyz, int nrhs, const double *b, double *x,
-                         int64_t *panel_entries, int *levels, int subtree_budget);
+                         int64_t *panel_entries, int *levels, int subtree_budget,
+                         int device_factor_emulation);
 
 #ifdef __cplusplus
 }

@@ -448,6 +448,25 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
     }
     f.poff.assign(nn + 1, 0);
     for (int t = 0; t < nn; ++t) f.poff[t + 1] = f.poff[t] + (int64_t)(f.w[t] + f.noff[t]) * f.w[t];
+    // device numeric factorization: heights, workspace offsets, column owners
+    f.hptr.assign(f.nlevels + 1, 0);
+    for (int t = 0; t < nn; ++t) f.hptr[f.height[t] + 1]++;
+    for (int h = 0; h < f.nlevels; ++h) f.hptr[h + 1] += f.hptr[h];
+    f.hlist.assign(nn, 0);
+    {
+        std::vector<int> fill(f.hptr.begin(), f.hptr.end() - 1);
+        for (int t = 0; t < nn; ++t) f.hlist[fill[f.height[t]]++] = t;
+    }
+    f.woff_s.assign(nn + 1, 0); f.woff_t.assign(nn + 1, 0); f.woff_u.assign(nn + 1, 0);
+    for (int t = 0; t < nn; ++t) {
+        const int64_t w = f.w[t], no = f.noff[t];
+        f.woff_s[t + 1] = f.woff_s[t] + no * w;
+        f.woff_t[t + 1] = f.woff_t[t] + w * w;
+        f.woff_u[t + 1] = f.woff_u[t] + no * no;
+    }
+    f.col_sn.assign(n, 0);
+    for (int t = 0; t < nn; ++t)
+        for (int k = 0; k < f.w[t]; ++k) f.col_sn[f.c0[t] + k] = t;
     device_layout(f, sub_budget);
 }
 
@@ -774,6 +793,128 @@ bool numeric(const std::vector<int>& ptr, const std::vector<int>& adj, const std
         for (int k = 0; k < w; ++k) pos[c0 + k] = -1;
         for (int k = 0; k < no; ++k) pos[rows[k]] = -1;
     }
+    return true;
+}
+
+// The device numeric factorization run on the host, step for step as the kernels run it
+// (one height at a time; per supernode: zeroed workspace, original entries, children's update
+// matrices in child order, left-looking Cholesky of the pivot block, its inverse, D^{-1},
+// G = F_os D^{-1}, the update matrix), with every index checked against its block. Leaves the
+// FP64 panels in f.panel.
+bool numeric_device_emulation(const std::vector<int>& ptr, const std::vector<int>& adj,
+                              const std::vector<double>& val, const std::vector<double>& diag, Factor& f,
+                              std::string& err) {
+    const int n = f.n, ns = f.ns;
+    std::vector<double> P(f.poff[ns], 0.0), S(f.woff_s[ns], 0.0), T(f.woff_t[ns], 0.0), U(f.woff_u[ns], 0.0);
+    double amax = 0.0;
+    for (int i = 0; i < n; ++i) amax = std::max(amax, std::fabs(diag[i]));
+#define MD_FCHECK(cond, what)                                  \
+    if (!(cond)) {                                             \
+        err = std::string("factor emulation: ") + what;        \
+        return false;                                          \
+    }
+    const int nh = (int)f.hptr.size() - 1;
+    for (int h = 0; h < nh; ++h)
+        for (int hi = f.hptr[h]; hi < f.hptr[h + 1]; ++hi) {
+            const int t = f.hlist[hi];
+            const int c0 = f.c0[t], w = f.w[t], no = f.noff[t], m = w + no;
+            const int64_t po = f.poff[t], so = f.woff_s[t], to = f.woff_t[t], uo = f.woff_u[t];
+            auto Pp = [&](int r, int c) -> double& { return P[po + (int64_t)c * m + r]; };
+            auto Sp = [&](int r, int c) -> double& { return S[so + (int64_t)c * no + r]; };
+            auto Tp = [&](int r, int c) -> double& { return T[to + (int64_t)r * w + c]; };  // row-major
+            auto Up = [&](int r, int c) -> double& { return U[uo + (int64_t)c * no + r]; };
+            const int* rows = f.offrows.data() + f.roff[t];
+            // original entries, one column at a time
+            for (int k = 0; k < w; ++k) {
+                const int j = c0 + k;
+                MD_FCHECK(f.col_sn[j] == t, "column owner");
+                Pp(k, k) = diag[j];
+                for (int e = ptr[j]; e < ptr[j + 1]; ++e) {
+                    const int i = adj[e];
+                    if (i <= j) continue;
+                    if (i < c0 + w) {
+                        Pp(i - c0, k) += val[e];
+                    } else {
+                        const int r = (int)(std::lower_bound(rows, rows + no, i) - rows);
+                        MD_FCHECK(r < no && rows[r] == i, "off row of an original entry");
+                        Sp(r, k) += val[e];
+                    }
+                }
+            }
+            // children's update matrices, in child order
+            for (int q = f.child_ptr[t]; q < f.child_ptr[t + 1]; ++q) {
+                const int c = f.child[q], nc = f.noff[c];
+                const int* rel = f.rel.data() + f.roff[c];
+                for (int64_t idx = 0; idx < (int64_t)nc * nc; ++idx) {
+                    const int b = (int)(idx / nc), a = (int)(idx % nc);
+                    if (a < b) continue;
+                    const double v = U[f.woff_u[c] + (int64_t)b * nc + a];
+                    const int ra = rel[a], rb = rel[b];
+                    MD_FCHECK(ra >= rb && ra < m && rb >= 0, "update position");
+                    if (rb < w) {
+                        if (ra < w) Pp(ra, rb) += v;
+                        else Sp(ra - w, rb) += v;
+                    } else {
+                        Up(ra - w, rb - w) += v;
+                    }
+                }
+            }
+            // right-looking Cholesky of the pivot block (lower, in place)
+            for (int k = 0; k < w; ++k) {
+                const double d0 = Pp(k, k);
+                if (!(d0 > 1e-14 * amax)) {
+                    err = "factorization failed: pivot block of supernode " + std::to_string(t) +
+                          " is not positive definite (a mesh component without constraints?)";
+                    return false;
+                }
+                const double d = std::sqrt(d0);
+                for (int r = k + 1; r < w; ++r) Pp(r, k) /= d;
+                Pp(k, k) = d;
+                for (int c = k + 1; c < w; ++c) {
+                    const double lck = Pp(c, k);
+                    if (lck == 0.0) continue;
+                    for (int r = c; r < w; ++r) Pp(r, c) -= Pp(r, k) * lck;
+                }
+            }
+            // inverse X of the Cholesky factor, right-looking: row k scaled, later rows updated
+            for (int r = 0; r < w; ++r)
+                for (int c = 0; c < w; ++c) Tp(r, c) = r == c ? 1.0 : 0.0;
+            for (int k = 0; k < w; ++k) {
+                const double lkk = Pp(k, k);
+                for (int c = 0; c <= k; ++c) Tp(k, c) = c == k ? 1.0 / lkk : Tp(k, c) / lkk;
+                for (int r = k + 1; r < w; ++r)
+                    for (int c = 0; c <= k; ++c) Tp(r, c) -= Pp(r, k) * Tp(k, c);
+            }
+            // D^{-1} = Linv^T Linv (symmetric)
+            for (int r = 0; r < w; ++r)
+                for (int c = 0; c <= r; ++c) {
+                    double acc = 0.0;
+                    for (int k = r; k < w; ++k) acc += Tp(k, r) * Tp(k, c);
+                    Pp(r, c) = acc;
+                    Pp(c, r) = acc;
+                }
+            if (no > 0) {
+                // G = F_os D^{-1} (panel rows [w, m))
+                for (int q = 0; q < w; ++q)
+                    for (int r = 0; r < no; ++r) {
+                        double acc = 0.0;
+                        for (int k = 0; k < w; ++k) acc += Sp(r, k) * Pp(k, q);
+                        Pp(w + r, q) = acc;
+                    }
+                // U -= G F_os^T (lower)
+                for (int b = 0; b < no; ++b)
+                    for (int a = b; a < no; ++a) {
+                        double acc = Up(a, b);
+                        for (int k = 0; k < w; ++k) {
+                            const double fb = Sp(b, k);
+                            if (fb != 0.0) acc -= Pp(w + a, k) * fb;
+                        }
+                        Up(a, b) = acc;
+                    }
+            }
+        }
+#undef MD_FCHECK
+    f.panel.swap(P);
     return true;
 }
 

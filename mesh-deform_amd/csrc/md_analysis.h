@@ -101,7 +101,14 @@ struct Factor {
     std::vector<int> rel;            // per off row of s: position in parent's rows [0, m_p)
     std::vector<int> child_ptr, child;
     std::vector<int64_t> poff;       // ns + 1: FP64 panel offsets (m x w, column-major)
-    std::vector<double> panel;       // FP64 panels
+    std::vector<double> panel;       // FP64 panels (host factorization)
+    // device numeric factorization (multifrontal, one height at a time): per height the
+    // supernodes of that height; per supernode the offsets of its workspace blocks: the
+    // off-diagonal front block (noff x w), the inverse of the pivot block's Cholesky factor
+    // (w x w) and the update matrix for the parent (noff x noff), all column-major FP64
+    std::vector<int> hptr, hlist;    // height h: hlist[hptr[h] .. hptr[h+1])
+    std::vector<int64_t> woff_s, woff_t, woff_u;  // ns + 1 each
+    std::vector<int> col_sn;         // supernode of each column
     // device layout
     std::vector<int> kc, kr;         // partial counts of u_s and x_s
     std::vector<int64_t> pc_off;     // ns + 1: column-major G_s (noff x w)
@@ -170,6 +177,12 @@ void device_layout(Factor& f, int sub_budget);
 bool numeric(const std::vector<int>& adj_ptr, const std::vector<int>& adj,
              const std::vector<double>& adj_val, const std::vector<double>& diag, Factor& f,
              std::string& err);
+
+// The device numeric factorization emulated on the host (same steps and index math, every
+// index checked); same inputs and result as numeric().
+bool numeric_device_emulation(const std::vector<int>& adj_ptr, const std::vector<int>& adj,
+                              const std::vector<double>& adj_val, const std::vector<double>& diag, Factor& f,
+                              std::string& err);
 
 // Host reference of the GPU solve on the same panels (used by host tests).
 void solve_host(const Factor& f, const double* b, double* x, int nrhs);

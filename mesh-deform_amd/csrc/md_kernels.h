@@ -73,6 +73,27 @@ struct MeshDev {
     double pcg_tol2;            // stop (freeze) when r.z <= tol^2 * (r.z)_0, per coordinate
 };
 
+// Device numeric factorization (multifrontal, FP64): per supernode record and the buffers.
+struct FacDev {
+    int c0, w, noff, roff;          // columns, off rows (offrows[roff ..], rel[roff ..])
+    int cb, ce, m, pad;             // children child[cb .. ce), m = w + noff
+    long long poff, soff, toff, uoff;  // FP64 panel (m x w), F_os copy (noff x w), L^{-1} (w x w),
+                                       // update matrix (noff x noff); all column-major
+    long long pc_off, pr_off;       // solve-precision panels of the sweeps
+};
+
+struct FacArgs {
+    const FacDev *sn;
+    const int *hlist;               // supernodes by height
+    const int *offrows, *rel, *child;
+    const int *slice_ptr, *sell_col;
+    const double *w64;              // cotan weights over the one-ring slots
+    int nf;
+    double *P, *S, *T, *U, *diag;
+    unsigned long long *amax;       // max |L_ii| as the bits of a non-negative double
+    int *flag;                      // 1: a pivot block is not positive definite; 2: structure
+};
+
 // A set of meshes processed by one launch. Block b belongs to mesh k where
 // prefix[k] <= b < prefix[k+1]; `meshes` and `prefix` are device arrays.
 struct LaunchSet {
@@ -105,6 +126,13 @@ void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_pcg_update(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
 void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
 void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+// Device numeric factorization: diagonal and its maximum, then per height (the `count`
+// supernodes from hlist[h0]) the front assembly and the pivot block's Cholesky and inverse (one
+// workgroup per supernode), D^{-1}, G and the update matrix (per_* workgroups per supernode;
+// per_g = 0 when no supernode of the height has off rows), then the solve-precision panels.
+void launch_fac_diag(hipStream_t s, const FacArgs &A);
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int per_dinv, int per_g, int per_u);
+void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64);
 // Plain copy of n16 16-byte elements (counter calibration, copy bandwidth).
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16);
 

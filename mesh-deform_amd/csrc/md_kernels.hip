@@ -1000,6 +1000,205 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
     pp[iv[t]] = v;
 }
 
+// ---- numeric factorization (arap.h:337, SimplicialLDLT::compute) ------------------------
+// Multifrontal block LDL^T in FP64, one height of the supernodal tree at a time, one
+// workgroup per supernode. Every supernode's front lives in its own workspace blocks; the
+// children's update matrices are added in child order, so the factor is deterministic.
+
+constexpr int kFacBlock = 256;
+
+// L_jj = sum of the weights of every neighbour of j, in slot order, and their maximum.
+__global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= A.nf) return;
+    const int sl = j / kSlice, l = j % kSlice;
+    double acc = 0.0;
+    for (int q = A.slice_ptr[sl]; q < A.slice_ptr[sl + 1]; ++q) {
+        const long long g = (long long)q * kSlice + l;
+        if (A.sell_col[g] != j) acc += A.w64[g];
+    }
+    A.diag[j] = acc;
+    atomicMax(A.amax, (unsigned long long)__double_as_longlong(fabs(acc)));
+}
+
+// Front of each supernode of one height: the pivot block and G rows (panel), F_os (copy),
+// the update matrix; original entries (L_jj and -w_ij for free i > j, slot order) first,
+// then the children's update matrices in child order.
+__global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
+    const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
+    double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
+    for (long long i = tid; i < (long long)m * w; i += kFacBlock) P[i] = 0.0;
+    for (long long i = tid; i < (long long)no * w; i += kFacBlock) F[i] = 0.0;
+    for (long long i = tid; i < (long long)no * no; i += kFacBlock) U[i] = 0.0;
+    __syncthreads();
+    const int *rows = A.offrows + S.roff;
+    for (int k = tid; k < w; k += kFacBlock) {
+        const int j = S.c0 + k, sl = j / kSlice, l = j % kSlice;
+        P[(long long)k * m + k] = A.diag[j];
+        for (int q = A.slice_ptr[sl]; q < A.slice_ptr[sl + 1]; ++q) {
+            const long long g = (long long)q * kSlice + l;
+            const int i = A.sell_col[g];
+            if (i <= j || i >= A.nf) continue;
+            const double v = -A.w64[g];
+            if (i < S.c0 + w) {
+                P[(long long)k * m + (i - S.c0)] += v;
+            } else {
+                int lo = 0, hi = no;  // first off row >= i
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rows[mid] < i) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < no && rows[lo] == i) F[(long long)k * no + lo] += v;
+                else *A.flag = 2;
+            }
+        }
+    }
+    __syncthreads();
+    for (int ci = S.cb; ci < S.ce; ++ci) {
+        const FacDev C = A.sn[A.child[ci]];
+        const int nc = C.noff;
+        const int *rel = A.rel + C.roff;
+        const double *Uc = A.U + C.uoff;
+        for (long long idx = tid; idx < (long long)nc * nc; idx += kFacBlock) {
+            const int b = (int)(idx / nc), a = (int)(idx % nc);
+            if (a < b) continue;
+            const double v = Uc[(long long)b * nc + a];
+            const int ra = rel[a], rb = rel[b];
+            if (ra >= m || rb > ra) { *A.flag = 2; continue; }
+            if (rb < w) {
+                if (ra < w) P[(long long)rb * m + ra] += v;
+                else F[(long long)rb * no + (ra - w)] += v;
+            } else {
+                U[(long long)(rb - w) * no + (ra - w)] += v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Pivot block of each front of one height (one workgroup per supernode): Cholesky in place
+// and its inverse X = L^{-1} (row-major), both right-looking. At step k the Cholesky scales
+// column k and updates the trailing triangle; the inverse scales row k and updates the later
+// rows. Every entry receives the same subtractions, in the same order, as in the left-looking
+// and column-by-column forms.
+constexpr int kCholBlock = 512;
+
+__global__ __launch_bounds__(kCholBlock) void k_fac_chol(FacArgs A, int h0) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
+    const int tid = threadIdx.x, w = S.w, m = S.m;
+    double *P = A.P + S.poff, *X = A.T + S.toff;
+    const double amax = __longlong_as_double((long long)*A.amax);
+    for (int k = 0; k < w; ++k) {
+        const double d0 = P[(long long)k * m + k];
+        if (tid == 0 && !(d0 > 1e-14 * amax)) *A.flag = 1;
+        const double d = sqrt(d0);
+        __syncthreads();  // every thread has read the pivot
+        for (int r = k + 1 + tid; r < w; r += kCholBlock) P[(long long)k * m + r] /= d;
+        if (tid == 0) P[(long long)k * m + k] = d;
+        __syncthreads();
+        // trailing update: L[r][c] -= L[r][k] L[c][k] for k < c <= r
+        const int nt = w - 1 - k;
+        for (long long idx = tid; idx < (long long)nt * nt; idx += kCholBlock) {
+            const int c = k + 1 + (int)(idx / nt), r = k + 1 + (int)(idx % nt);
+            if (r < c) continue;
+            const double lck = P[(long long)k * m + c];
+            if (lck != 0.0) P[(long long)c * m + r] -= P[(long long)k * m + r] * lck;
+        }
+        __syncthreads();
+    }
+    // X = I (lower part)
+    for (long long idx = tid; idx < (long long)w * w; idx += kCholBlock) {
+        const int r = (int)(idx / w), j = (int)(idx % w);
+        X[idx] = r == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < w; ++k) {
+        const double lkk = P[(long long)k * m + k];
+        // row k: X[k][j] = X[k][j] / L[k][k] for j <= k (j == k: 1 / L[k][k])
+        for (int j = tid; j <= k; j += kCholBlock)
+            X[(long long)k * w + j] = j == k ? 1.0 / lkk : X[(long long)k * w + j] / lkk;
+        __syncthreads();
+        // rows r > k: X[r][j] -= L[r][k] X[k][j]
+        const int nr = w - 1 - k, nj = k + 1;
+        for (long long idx = tid; idx < (long long)nr * nj; idx += kCholBlock) {
+            const int r = k + 1 + (int)(idx / nj), j = (int)(idx % nj);
+            X[(long long)r * w + j] -= P[(long long)k * m + r] * X[(long long)k * w + j];
+        }
+        __syncthreads();
+    }
+}
+
+// D^{-1} = X^T X into the pivot block of the panel (symmetric), `per` workgroups per supernode.
+__global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m;
+    double *P = A.P + S.poff;
+    const double *X = A.T + S.toff;
+    const long long n = (long long)w * w;
+    for (long long idx = (long long)(blockIdx.x % per) * kFacBlock + threadIdx.x; idx < n;
+         idx += (long long)per * kFacBlock) {
+        const int r = (int)(idx / w), c = (int)(idx % w);
+        if (c > r) continue;
+        double acc = 0.0;
+        for (int k = r; k < w; ++k) acc += X[(long long)k * w + r] * X[(long long)k * w + c];
+        P[(long long)c * m + r] = acc;
+        P[(long long)r * m + c] = acc;
+    }
+}
+
+// G = F_os D^{-1} into panel rows [w, m), `per` workgroups per supernode.
+__global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, no = S.noff, m = S.m;
+    double *P = A.P + S.poff;
+    const double *F = A.S + S.soff;
+    const long long n = (long long)no * w;
+    for (long long idx = (long long)(blockIdx.x % per) * kFacBlock + threadIdx.x; idx < n;
+         idx += (long long)per * kFacBlock) {
+        const int q = (int)(idx / no), r = (int)(idx % no);
+        double acc = 0.0;
+        for (int k = 0; k < w; ++k) acc += F[(long long)k * no + r] * P[(long long)q * m + k];
+        P[(long long)q * m + w + r] = acc;
+    }
+}
+
+// U -= G F_os^T (lower part), `per` workgroups per supernode.
+__global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, no = S.noff, m = S.m;
+    const double *P = A.P + S.poff, *F = A.S + S.soff;
+    double *U = A.U + S.uoff;
+    const long long n = (long long)no * no;
+    for (long long idx = (long long)(blockIdx.x % per) * kFacBlock + threadIdx.x; idx < n;
+         idx += (long long)per * kFacBlock) {
+        const int b = (int)(idx / no), a = (int)(idx % no);
+        if (a < b) continue;
+        double acc = U[idx];
+        for (int k = 0; k < w; ++k) {
+            const double fb = F[(long long)k * no + b];
+            if (fb != 0.0) acc -= P[(long long)k * m + w + a] * fb;
+        }
+        U[idx] = acc;
+    }
+}
+
+// FP64 panel -> the sweeps' layouts in the solve precision: column-major G (noff x w) and
+// row-major [D^{-1}; G] (m x w).
+template <typename T>
+__global__ __launch_bounds__(kFacBlock) void k_fac_convert(FacArgs A, T *pc, T *pr) {
+    const FacDev S = A.sn[blockIdx.x];
+    const int w = S.w, m = S.m, no = S.noff;
+    const double *P = A.P + S.poff;
+    for (long long idx = threadIdx.x; idx < (long long)m * w; idx += kFacBlock) {
+        const int q = (int)(idx / m), p = (int)(idx % m);
+        const T v = (T)P[idx];
+        pr[S.pr_off + (long long)p * w + q] = v;
+        if (p >= w) pc[S.pc_off + (long long)q * no + (p - w)] = v;
+    }
+}
+
 // Plain device copy of n 16-byte elements (grid-stride): the known-size reference kernel for
 // HBM counters and the achievable copy bandwidth.
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ src, f4 *__restrict__ dst, long long n) {
@@ -1008,6 +1207,25 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ sr
 }
 
 }  // namespace
+
+void launch_fac_diag(hipStream_t s, const FacArgs &A) {
+    if (A.nf > 0) hipLaunchKernelGGL(k_fac_diag, dim3((A.nf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A);
+}
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int per_dinv, int per_g, int per_u) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fac_assemble, dim3(count), dim3(kFacBlock), 0, s, A, h0);
+    hipLaunchKernelGGL(k_fac_chol, dim3(count), dim3(kCholBlock), 0, s, A, h0);
+    hipLaunchKernelGGL(k_fac_dinv, dim3(count * per_dinv), dim3(kFacBlock), 0, s, A, h0, per_dinv);
+    if (per_g > 0) {
+        hipLaunchKernelGGL(k_fac_g, dim3(count * per_g), dim3(kFacBlock), 0, s, A, h0, per_g);
+        hipLaunchKernelGGL(k_fac_u, dim3(count * per_u), dim3(kFacBlock), 0, s, A, h0, per_u);
+    }
+}
+void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64) {
+    if (ns <= 0) return;
+    if (fp64) hipLaunchKernelGGL(k_fac_convert<double>, dim3(ns), dim3(kFacBlock), 0, s, A, (double *)pc, (double *)pr);
+    else hipLaunchKernelGGL(k_fac_convert<float>, dim3(ns), dim3(kFacBlock), 0, s, A, (float *)pc, (float *)pr);
+}
 
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16) {
     if (n16 <= 0) return;

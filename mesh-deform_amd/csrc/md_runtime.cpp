@@ -236,6 +236,7 @@ extern "C" void md_config_default(md_config *cfg) {
     cfg->use_graph = 1;
     cfg->subtree_budget = kSubBudgetDefault;
     cfg->amalg_ratio = kAmalgRatioDefault;
+    cfg->factor_device = 1;
 }
 
 extern "C" const char *md_last_error(void) { return g_err.c_str(); }
@@ -445,6 +446,10 @@ int analyze_device(md_mesh *m) {
     size_t o_ch = A.reserve(sizeof(int) * std::max<size_t>(1, f.child.size()));
     size_t o_pc = A.reserve(psz * std::max<int64_t>(1, last(f.pc_off)));
     size_t o_pr = A.reserve(psz * std::max<int64_t>(1, last(f.pr_off)));
+    size_t o_fac = A.reserve(sizeof(FacDev) * std::max(1, ns));
+    size_t o_hl = A.reserve(sizeof(int) * std::max(1, ns));
+    size_t o_or = A.reserve(sizeof(int) * std::max<size_t>(1, f.offrows.size()));
+    size_t o_rel = A.reserve(sizeof(int) * std::max<size_t>(1, f.rel.size()));
     int eblocks = std::max(1, cdiv(t.nslices, kSlicesPerBlock));
     size_t o_en = A.reserve(sizeof(double) * eblocks);
     bool pcg = m->cfg.solver == MD_SOLVER_PCG;
@@ -483,6 +488,10 @@ int analyze_device(md_mesh *m) {
     d.r = A.at(o_r); d.z = A.at(o_z);
     d.child_ptr = (const int *)A.at(o_cp); d.child = (const int *)A.at(o_ch);
     d.pc = A.at(o_pc); d.pr = A.at(o_pr);
+    m->d_fac = (const FacDev *)A.at(o_fac);
+    m->d_hlist = (const int *)A.at(o_hl);
+    m->d_offrows = (const int *)A.at(o_or);
+    m->d_rel = (const int *)A.at(o_rel);
     d.energy_part = (double *)A.at(o_en);
     d.w32 = (float *)A.at(o_w32);
     d.diag = A.at(o_dg);
@@ -503,9 +512,21 @@ int analyze_device(md_mesh *m) {
     chk(up(d.tri, t.tri_internal.data(), sizeof(int) * 3 * (size_t)m->F));
     chk(up(d.slot_fe_ptr, t.slot_fe_ptr.data(), sizeof(int) * (slots + 1)));
     chk(up(d.slot_fe, t.slot_fe.data(), sizeof(int) * t.slot_fe.size()));
+    std::vector<FacDev> fh(ns);
+    for (int k = 0; k < ns; ++k) {
+        FacDev &x = fh[k];
+        x.c0 = f.c0[k]; x.w = f.w[k]; x.noff = f.noff[k]; x.roff = f.roff[k];
+        x.cb = f.child_ptr[k]; x.ce = f.child_ptr[k + 1]; x.m = f.w[k] + f.noff[k]; x.pad = 0;
+        x.poff = f.poff[k]; x.soff = f.woff_s[k]; x.toff = f.woff_t[k]; x.uoff = f.woff_u[k];
+        x.pc_off = f.pc_off[k]; x.pr_off = f.pr_off[k];
+    }
     if (ns > 0) {
         chk(up(d.child_ptr, f.child_ptr.data(), sizeof(int) * (ns + 1)));
         chk(up(d.child, f.child.data(), sizeof(int) * f.child.size()));
+        chk(up(m->d_fac, fh.data(), sizeof(FacDev) * ns));
+        chk(up(m->d_hlist, f.hlist.data(), sizeof(int) * ns));
+        chk(up(m->d_offrows, f.offrows.data(), sizeof(int) * f.offrows.size()));
+        chk(up(m->d_rel, f.rel.data(), sizeof(int) * f.rel.size()));
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
@@ -717,8 +738,78 @@ bool factor_host(md_mesh *m, std::string &err) {
     }
     m->fac.panel.clear();
     m->fac.panel.shrink_to_fit();
-    m->setup_ms_host += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    m->factor_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    m->setup_ms_host += m->factor_ms;
     return true;
+}
+
+// FP64 numeric factorization on the device from the device cotan weights, straight into
+// the sweeps' panels: diagonal, then per height of the supernodal tree the front assembly and
+// the dense partial factorization, then the conversion to the solve precision. The FP64
+// workspace lives for the call only.
+int factor_device(md_mesh *m) {
+    auto t0 = Clock::now();
+    const Factor &f = m->fac;
+    const int ns = f.ns, nf = m->topo.nf;
+    hipStream_t st = m->ctx->stream;
+    const size_t nP = (size_t)f.poff[ns], nS = (size_t)f.woff_s[ns], nT = (size_t)f.woff_t[ns],
+                 nU = (size_t)f.woff_u[ns];
+    const size_t bytes = sizeof(double) * (nP + nS + nT + nU + (size_t)nf) + 2 * sizeof(unsigned long long);
+    void *ws = nullptr;
+    if (hipMalloc(&ws, bytes) != hipSuccess) {
+        set_error("device factorization workspace of " + std::to_string(bytes) + " bytes: allocation failed");
+        return MD_ERR_NOMEM;
+    }
+    FacArgs A{};
+    A.sn = m->d_fac; A.hlist = m->d_hlist; A.offrows = m->d_offrows; A.rel = m->d_rel; A.child = m->dev.child;
+    A.slice_ptr = m->dev.slice_ptr; A.sell_col = m->dev.sell_col; A.w64 = m->dev.w64; A.nf = nf;
+    double *base = static_cast<double *>(ws);
+    A.P = base; A.S = A.P + nP; A.T = A.S + nS; A.U = A.T + nT; A.diag = A.U + nU;
+    A.amax = reinterpret_cast<unsigned long long *>(A.diag + nf);
+    A.flag = reinterpret_cast<int *>(A.amax + 1);
+    hipError_t e = hipMemsetAsync(A.amax, 0, 2 * sizeof(unsigned long long), st);
+    if (e == hipSuccess) {
+        launch_fac_diag(st, A);
+        const int nh = (int)f.hptr.size() - 1;
+        for (int h = 0; h < nh; ++h) {
+            // workgroups per supernode of the multi-workgroup stages: enough for the height's
+            // largest block, at most 64
+            int64_t mw2 = 0, mnw = 0, mno2 = 0;
+            for (int i = f.hptr[h]; i < f.hptr[h + 1]; ++i) {
+                const int64_t w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
+                mw2 = std::max(mw2, w * w);
+                mnw = std::max(mnw, no * w);
+                mno2 = std::max(mno2, no * no);
+            }
+            auto per = [](int64_t n) { return (int)std::min<int64_t>(64, std::max<int64_t>(1, (n + 255) / 256)); };
+            launch_fac_height(st, A, f.hptr[h], f.hptr[h + 1] - f.hptr[h], per(mw2), mnw > 0 ? per(mnw) : 0,
+                              per(mno2));
+        }
+        launch_fac_convert(st, A, ns, const_cast<void *>(m->dev.pc), const_cast<void *>(m->dev.pr), m->fp64());
+        e = hipGetLastError();
+    }
+    int flag = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, A.flag, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(ws);
+    if (e != hipSuccess) {
+        set_error(std::string("device factorization: ") + hipGetErrorString(e));
+        return MD_ERR_HIP;
+    }
+    m->factor_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    if (flag == 1) {
+        m->factored = false;
+        set_error("factorization failed: a pivot block is not positive definite (a mesh component "
+                  "without constraints?)");
+        return MD_ERR_FACTOR;
+    }
+    if (flag != 0) {
+        m->factored = false;
+        set_error("device factorization: inconsistent symbolic structure");
+        return MD_ERR_INVALID;
+    }
+    m->factored = true;
+    return MD_OK;
 }
 
 int factor_upload(md_mesh *m) {
@@ -775,14 +866,15 @@ int prep_device_a(md_mesh *m, Prep &p) {
     m->iterated = false;
     if (m->ncon == 0) { p.early = true; return MD_OK; }  // arap.h:113-114
     p.need_factor = m->topo.nf > 0 && (p.full || p.need_analysis || !m->factored);
-    if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT) return weights_download(m);
+    if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT && !m->cfg.factor_device) return weights_download(m);
     return MD_OK;
 }
 
 // Device work after the factorization.
 int prep_device_b(md_mesh *m, Prep &p) {
     if (p.need_factor) {
-        int rc = m->cfg.solver == MD_SOLVER_PCG ? factor_pcg(m) : factor_upload(m);
+        int rc = m->cfg.solver == MD_SOLVER_PCG ? factor_pcg(m) : m->cfg.factor_device ? factor_device(m)
+                                                                                       : factor_upload(m);
         if (rc) return rc;
     }
     m->dirty = false;
@@ -802,7 +894,7 @@ int setup(md_mesh *m, const double *xyz, bool *early) {
     int rc = prep_device_a(m, p);
     *early = p.early;
     if (rc || p.early) return rc;
-    if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT) {
+    if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT && !m->cfg.factor_device) {
         std::string err;
         if (!factor_host(m, err)) { set_error(err); return MD_ERR_FACTOR; }
     }
@@ -1046,7 +1138,7 @@ extern "C" int md_meshes_prepare(md_ctx *ctx, int count, md_mesh *const *meshes,
     std::vector<std::string> errs(nt);
     std::vector<char> bad(nt, 0);
     parallel_for(nt, nthreads, [&](int k) {
-        if (preps[k].need_factor && todo[k]->cfg.solver == MD_SOLVER_LDLT)
+        if (preps[k].need_factor && todo[k]->cfg.solver == MD_SOLVER_LDLT && !todo[k]->cfg.factor_device)
             bad[k] = !factor_host(todo[k], errs[k]);
     });
     for (int k = 0; k < nt; ++k)
@@ -1232,6 +1324,7 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->panel_entries = s.panel_entries;
     out->factor_nnz = s.strict_nnz_equiv;
     out->factor_nnz_exact = s.nnz_exact;
+    out->factor_ms = m->factor_ms;
     out->fwd_items = s.fwd_items;
     out->bwd_items = s.bwd_items;
     out->device_bytes = (int64_t)(m->arena.size() + m->larena.size());
@@ -1352,7 +1445,8 @@ extern "C" int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbp
 
 extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
                                     const double *xyz, int nrhs, const double *b, double *x,
-                                    int64_t *panel_entries, int *levels, int subtree_budget) {
+                                    int64_t *panel_entries, int *levels, int subtree_budget,
+                                    int device_factor_emulation) {
     if (n <= 0 || !rowptr || !col || !val || !xyz || nrhs < 0 || (nrhs > 0 && (!b || !x))) {
         set_error("md_host_factor_solve: invalid argument");
         return MD_ERR_INVALID;
@@ -1374,7 +1468,9 @@ extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, co
     Factor f;
     symbolic(n, ptr, adj, f, subtree_budget < 0 ? kSubBudgetDefault : std::min(subtree_budget, kSubBudgetMax));
     std::string err;
-    if (!numeric(ptr, adj, av, diag, f, err)) { set_error(err); return MD_ERR_FACTOR; }
+    const bool ok = device_factor_emulation ? numeric_device_emulation(ptr, adj, av, diag, f, err)
+                                            : numeric(ptr, adj, av, diag, f, err);
+    if (!ok) { set_error(err); return err.rfind("factor emulation", 0) == 0 ? MD_ERR_INVALID : MD_ERR_FACTOR; }
     std::vector<double> bp((size_t)n * nrhs), xp((size_t)n * nrhs);
     for (int r = 0; r < nrhs; ++r)
         for (int k = 0; k < n; ++k) bp[(size_t)r * n + k] = b[(size_t)r * n + perm[k]];

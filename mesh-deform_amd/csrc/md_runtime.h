@@ -135,6 +135,10 @@ struct md_mesh {
     md::Arena larena;              // sweep layout: partials, contribution lists, tiles, subtrees
     md::MeshDev dev{};
     uint64_t gen = 0;              // bumped whenever device pointers in dev change
+    // device numeric factorization: static structure (main arena)
+    const md::FacDev *d_fac = nullptr;
+    const int *d_hlist = nullptr, *d_offrows = nullptr, *d_rel = nullptr;
+    double factor_ms = 0.0;        // last numeric factorization (host or device), wall ms
     std::atomic<int> servers{0};   // servers this mesh is registered with (its layout is fixed)
     md::Plan plan;                 // single-mesh launch plan
     std::vector<int> adj_ptr, adj; // free-free pattern in solve order

@@ -32,7 +32,8 @@ _vp = ctypes.c_void_p
 class Config(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("solver", ctypes.c_int), ("refine_steps", ctypes.c_int),
                 ("rest_mode", ctypes.c_int), ("pcg_max_iters", ctypes.c_int), ("pcg_tol", ctypes.c_double),
-                ("use_graph", ctypes.c_int), ("subtree_budget", ctypes.c_int), ("amalg_ratio", ctypes.c_double)]
+                ("use_graph", ctypes.c_int), ("subtree_budget", ctypes.c_int), ("amalg_ratio", ctypes.c_double),
+                ("factor_device", ctypes.c_int)]
 
 
 class MeshStats(ctypes.Structure):
@@ -44,7 +45,8 @@ class MeshStats(ctypes.Structure):
                 ("device_bytes", ctypes.c_int64), ("setup_ms_host", ctypes.c_double),
                 ("bytes_local", ctypes.c_int64), ("bytes_residual", ctypes.c_int64),
                 ("bytes_forward", ctypes.c_int64), ("bytes_backward", ctypes.c_int64),
-                ("subtree_budget", ctypes.c_int), ("subtrees", ctypes.c_int), ("factor_nnz_exact", ctypes.c_int64)]
+                ("subtree_budget", ctypes.c_int), ("subtrees", ctypes.c_int), ("factor_nnz_exact", ctypes.c_int64),
+                ("factor_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -110,7 +112,8 @@ _SIGS = [
     ("md_server_destroy", ctypes.c_int, [_vp]),
     ("md_copy_probe", ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("md_host_factor_solve", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _f64p, _f64p,
-                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                            ctypes.c_int]),
 ]
 
 
@@ -145,7 +148,7 @@ def device_count():
 
 
 def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_graph=True, solver=SOLVER_LDLT,
-           pcg_iters=None, pcg_tol=None, subtree_budget=None, amalg_ratio=None):
+           pcg_iters=None, pcg_tol=None, subtree_budget=None, amalg_ratio=None, factor_device=None):
     c = Config()
     lib().md_config_default(ctypes.byref(c))
     c.precision = precision
@@ -161,6 +164,8 @@ def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_gr
         c.subtree_budget = int(subtree_budget)
     if amalg_ratio is not None:
         c.amalg_ratio = float(amalg_ratio)
+    if factor_device is not None:
+        c.factor_device = 1 if factor_device else 0
     return c
 
 
@@ -410,9 +415,10 @@ def copy_probe(ctx, nbytes, repeats=3):
     return g.value
 
 
-def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1):
+def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1, device_factor_emulation=False):
     """Host-only: the library's ordering + symbolic + FP64 block LDL^T, solved on the CPU by an
-    emulation of the device sweeps (subtree_budget as in Config; -1: the default)."""
+    emulation of the device sweeps (subtree_budget as in Config; -1: the default); with
+    device_factor_emulation the factorization is the emulation of the device kernels."""
     rp = np.ascontiguousarray(rp, np.int32)
     ci = np.ascontiguousarray(ci, np.int32)
     v = np.ascontiguousarray(v, np.float64)
@@ -423,5 +429,5 @@ def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1):
     lv = ctypes.c_int(0)
     check(lib().md_host_factor_solve(n, rp, ci, v, np.ascontiguousarray(xyz, np.float64).ravel(), b2.shape[0],
                                      np.ascontiguousarray(b2).ravel(), x.reshape(-1), ctypes.byref(pe),
-                                     ctypes.byref(lv), int(subtree_budget)))
+                                     ctypes.byref(lv), int(subtree_budget), 1 if device_factor_emulation else 0))
     return (x[0] if np.ndim(b) == 1 else x), pe.value, lv.value

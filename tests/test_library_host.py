@@ -73,6 +73,22 @@ def test_host_block_ldlt_matches_scipy(make, budget):
         assert lv_default < levels  # the default budget takes the bottom levels into subtrees
 
 
+@pytest.mark.parametrize("make", [meshgen.sphere_problem, lambda: meshgen.bar_problem(80, 8, 8),
+                                  lambda: meshgen.noisy_icosphere_problem(4)])
+def test_device_factorization_emulation_matches_scipy(make):
+    """The device numeric factorization, emulated on the host with every index checked (front
+    assembly, children's update matrices, Cholesky, inverse, G, update matrix), factors L: its
+    solves match scipy and the host multifrontal factorization."""
+    pr = make()
+    L, fx = _problem_system(pr)
+    b = np.random.default_rng(3).standard_normal((3, L.shape[0]))
+    x_dev, pe, lv = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, device_factor_emulation=True)
+    x_host, _, _ = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b)
+    ref = spla.spsolve(L.tocsc(), b.T).T
+    np.testing.assert_allclose(x_dev, ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
+    np.testing.assert_allclose(x_dev, x_host, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
+
 def test_host_block_ldlt_dense_small():
     """A dense SPD matrix is one supernode: the block LDL^T is D^{-1} itself."""
     rng = np.random.default_rng(2)
