@@ -859,32 +859,68 @@ bool numeric_device_emulation(const std::vector<int>& ptr, const std::vector<int
                     }
                 }
             }
-            // right-looking Cholesky of the pivot block (lower, in place)
-            for (int k = 0; k < w; ++k) {
-                const double d0 = Pp(k, k);
-                if (!(d0 > 1e-14 * amax)) {
-                    err = "factorization failed: pivot block of supernode " + std::to_string(t) +
-                          " is not positive definite (a mesh component without constraints?)";
-                    return false;
+            // Cholesky factor and its inverse X of the pivot block, blocked by 64 columns
+            // (right-looking); the device kernels follow the same operation order.
+            constexpr int kFb = 64;
+            const int nb = (w + kFb - 1) / kFb;
+            for (int s = 0; s < nb; ++s) {
+                const int kb = s * kFb, b = std::min(kFb, w - kb);
+                // diagonal block: unblocked Cholesky, then its inverse (row-major, upper part zero)
+                for (int k = 0; k < b; ++k) {
+                    const double d0 = Pp(kb + k, kb + k);
+                    if (!(d0 > 1e-14 * amax)) {
+                        err = "factorization failed: pivot block of supernode " + std::to_string(t) +
+                              " is not positive definite (a mesh component without constraints?)";
+                        return false;
+                    }
+                    const double d = std::sqrt(d0);
+                    for (int r = k + 1; r < b; ++r) Pp(kb + r, kb + k) /= d;
+                    Pp(kb + k, kb + k) = d;
+                    for (int c = k + 1; c < b; ++c)
+                        for (int r = c; r < b; ++r) Pp(kb + r, kb + c) -= Pp(kb + r, kb + k) * Pp(kb + c, kb + k);
                 }
-                const double d = std::sqrt(d0);
-                for (int r = k + 1; r < w; ++r) Pp(r, k) /= d;
-                Pp(k, k) = d;
-                for (int c = k + 1; c < w; ++c) {
-                    const double lck = Pp(c, k);
-                    if (lck == 0.0) continue;
-                    for (int r = c; r < w; ++r) Pp(r, c) -= Pp(r, k) * lck;
+                for (int r = 0; r < b; ++r)
+                    for (int c = 0; c < b; ++c) Tp(kb + r, kb + c) = r == c ? 1.0 : 0.0;
+                for (int k = 0; k < b; ++k) {
+                    const double lkk = Pp(kb + k, kb + k);
+                    for (int c = 0; c <= k; ++c) Tp(kb + k, kb + c) = c == k ? 1.0 / lkk : Tp(kb + k, kb + c) / lkk;
+                    for (int r = k + 1; r < b; ++r)
+                        for (int c = 0; c <= k; ++c) Tp(kb + r, kb + c) -= Pp(kb + r, kb + k) * Tp(kb + k, kb + c);
                 }
+                if (s == nb - 1) break;
+                // rows below: forward substitution against the diagonal block
+                for (int r = kb + kFb; r < w; ++r)
+                    for (int j = 0; j < kFb; ++j) {
+                        double v = Pp(r, kb + j);
+                        for (int q = 0; q < j; ++q) v -= Pp(r, kb + q) * Pp(kb + j, kb + q);
+                        Pp(r, kb + j) = v / Pp(kb + j, kb + j);
+                    }
+                // trailing lower part: A(r, c) -= sum_q L(r, kb + q) L(c, kb + q)
+                for (int c = kb + kFb; c < w; ++c)
+                    for (int r = c; r < w; ++r) {
+                        double acc = 0.0;
+                        for (int q = 0; q < kFb; ++q) acc += Pp(r, kb + q) * Pp(c, kb + q);
+                        Pp(r, c) -= acc;
+                    }
             }
-            // inverse X of the Cholesky factor, right-looking: row k scaled, later rows updated
-            for (int r = 0; r < w; ++r)
-                for (int c = 0; c < w; ++c) Tp(r, c) = r == c ? 1.0 : 0.0;
-            for (int k = 0; k < w; ++k) {
-                const double lkk = Pp(k, k);
-                for (int c = 0; c <= k; ++c) Tp(k, c) = c == k ? 1.0 / lkk : Tp(k, c) / lkk;
-                for (int r = k + 1; r < w; ++r)
-                    for (int c = 0; c <= k; ++c) Tp(r, c) -= Pp(r, k) * Tp(k, c);
-            }
+            // off-diagonal blocks of X, one block diagonal at a time:
+            // X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ
+            for (int d = 1; d < nb; ++d)
+                for (int I = d; I < nb; ++I) {
+                    const int J = I - d, r0 = I * kFb, c0 = J * kFb, rb = std::min(kFb, w - r0);
+                    std::vector<double> Y((size_t)kFb * kFb, 0.0);
+                    for (int K = J; K < I; ++K)
+                        for (int k = 0; k < kFb; ++k)
+                            for (int x = 0; x < rb; ++x)
+                                for (int y = 0; y < kFb; ++y)
+                                    Y[(size_t)x * kFb + y] += Pp(r0 + x, K * kFb + k) * Tp(K * kFb + k, c0 + y);
+                    for (int x = 0; x < rb; ++x)
+                        for (int y = 0; y < kFb; ++y) {
+                            double acc = 0.0;
+                            for (int k = 0; k < rb; ++k) acc += Tp(r0 + x, r0 + k) * Y[(size_t)k * kFb + y];
+                            Tp(r0 + x, c0 + y) = -acc;
+                        }
+                }
             // D^{-1} = Linv^T Linv (symmetric)
             for (int r = 0; r < w; ++r)
                 for (int c = 0; c <= r; ++c) {

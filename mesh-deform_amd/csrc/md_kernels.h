@@ -131,7 +131,10 @@ void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 // workgroup per supernode), D^{-1}, G and the update matrix (per_* workgroups per supernode;
 // per_g = 0 when no supernode of the height has off rows), then the solve-precision panels.
 void launch_fac_diag(hipStream_t s, const FacArgs &A);
-void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int per_dinv, int per_g, int per_u);
+// One height: assembly, blocked Cholesky and inverse of the pivot blocks (nb = column blocks
+// of 64 in the widest one), D^{-1}, G and the update matrices.
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_dinv, int per_g,
+                       int per_u);
 void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64);
 // Plain copy of n16 16-byte elements (counter calibration, copy bandwidth).
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16);

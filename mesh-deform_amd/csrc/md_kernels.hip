@@ -1083,51 +1083,176 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0) {
 // column k and updates the trailing triangle; the inverse scales row k and updates the later
 // rows. Every entry receives the same subtractions, in the same order, as in the left-looking
 // and column-by-column forms.
-constexpr int kCholBlock = 512;
+// The pivot block's Cholesky factor and its inverse, blocked by kFb columns (right-looking):
+// for each column block s, k_fac_potrf factors the diagonal block in LDS and inverts it (the
+// diagonal block of X = L^{-1}, row-major in the T workspace), k_fac_trsm solves the rows
+// below it, k_fac_syrk updates the trailing lower blocks; then k_fac_trtri forms the
+// off-diagonal blocks of X one block diagonal at a time:
+// X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ.
+constexpr int kFb = 64;
+constexpr int kFbPad = kFb + 1;
 
-__global__ __launch_bounds__(kCholBlock) void k_fac_chol(FacArgs A, int h0) {
+// Diagonal block s: unblocked right-looking Cholesky in LDS, then its inverse (right-looking,
+// row-major, the upper part zero).
+__global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int s) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
-    const int tid = threadIdx.x, w = S.w, m = S.m;
-    double *P = A.P + S.poff, *X = A.T + S.toff;
+    const int w = S.w, m = S.m, kb = s * kFb, tid = threadIdx.x;
+    if (kb >= w) return;
+    const int b = min(kFb, w - kb), lane = tid & (kFb - 1), wv = tid / kFb;
+    __shared__ double D[kFb][kFbPad];  // D[c][r] = L(kb + r, kb + c)
+    __shared__ double X[kFb][kFbPad];  // X[r][c] = Linv(kb + r, kb + c)
+    double *P = A.P + S.poff, *T = A.T + S.toff;
+    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
+        const int c = idx / kFb, r = idx % kFb;
+        D[c][r] = (r < b && c < b && r >= c) ? P[(long long)(kb + c) * m + kb + r] : 0.0;
+        X[c][r] = c == r ? 1.0 : 0.0;
+    }
     const double amax = __longlong_as_double((long long)*A.amax);
-    for (int k = 0; k < w; ++k) {
-        const double d0 = P[(long long)k * m + k];
+    __syncthreads();
+    for (int k = 0; k < b; ++k) {
+        const double d0 = D[k][k];
         if (tid == 0 && !(d0 > 1e-14 * amax)) *A.flag = 1;
         const double d = sqrt(d0);
         __syncthreads();  // every thread has read the pivot
-        for (int r = k + 1 + tid; r < w; r += kCholBlock) P[(long long)k * m + r] /= d;
-        if (tid == 0) P[(long long)k * m + k] = d;
+        if (tid < kFb && lane > k && lane < b) D[k][lane] /= d;
+        if (tid == 0) D[k][k] = d;
         __syncthreads();
-        // trailing update: L[r][c] -= L[r][k] L[c][k] for k < c <= r
-        const int nt = w - 1 - k;
-        for (long long idx = tid; idx < (long long)nt * nt; idx += kCholBlock) {
-            const int c = k + 1 + (int)(idx / nt), r = k + 1 + (int)(idx % nt);
-            if (r < c) continue;
-            const double lck = P[(long long)k * m + c];
-            if (lck != 0.0) P[(long long)c * m + r] -= P[(long long)k * m + r] * lck;
-        }
+        for (int c = k + 1 + wv; c < b; c += kFacBlock / kFb)
+            if (lane >= c && lane < b) D[c][lane] -= D[k][lane] * D[k][c];
         __syncthreads();
     }
-    // X = I (lower part)
-    for (long long idx = tid; idx < (long long)w * w; idx += kCholBlock) {
-        const int r = (int)(idx / w), j = (int)(idx % w);
-        X[idx] = r == j ? 1.0 : 0.0;
+    for (int k = 0; k < b; ++k) {
+        const double lkk = D[k][k];
+        if (tid < kFb && lane <= k) X[k][lane] = lane == k ? 1.0 / lkk : X[k][lane] / lkk;
+        __syncthreads();
+        for (int r = k + 1 + wv; r < b; r += kFacBlock / kFb)
+            if (lane <= k) X[r][lane] -= D[k][r] * X[k][lane];
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
+        const int c = idx / kFb, r = idx % kFb;
+        if (r < b && c < b) {
+            if (r >= c) P[(long long)(kb + c) * m + kb + r] = D[c][r];
+            T[(long long)(kb + c) * w + kb + r] = X[c][r];  // row kb + c of X
+        }
+    }
+}
+
+// Rows of row block s + 1 + (blockIdx.x % per) below diagonal block s:
+// L(r, kb + j) = (A(r, kb + j) - sum_{t<j} L(r, kb + t) L(kb + j, kb + t)) / L(kb + j, kb + j),
+// one row per lane.
+__global__ __launch_bounds__(kFb) void k_fac_trsm(FacArgs A, int h0, int s, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, kb = s * kFb, r0 = (s + 1 + (int)(blockIdx.x % per)) * kFb;
+    if (r0 >= w) return;  // also kb + kFb <= r0 < w: the diagonal block is full
+    const int lane = threadIdx.x, rb = min(kFb, w - r0);
+    __shared__ double D[kFb][kFbPad];  // D[t][j] = L(kb + j, kb + t)
+    __shared__ double R[kFb][kFbPad];  // R[t][i] = A(r0 + i, kb + t)
+    double *P = A.P + S.poff;
+    for (int c = 0; c < kFb; ++c) {
+        D[c][lane] = lane >= c ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
+        R[c][lane] = lane < rb ? P[(long long)(kb + c) * m + r0 + lane] : 0.0;
     }
     __syncthreads();
-    for (int k = 0; k < w; ++k) {
-        const double lkk = P[(long long)k * m + k];
-        // row k: X[k][j] = X[k][j] / L[k][k] for j <= k (j == k: 1 / L[k][k])
-        for (int j = tid; j <= k; j += kCholBlock)
-            X[(long long)k * w + j] = j == k ? 1.0 / lkk : X[(long long)k * w + j] / lkk;
+    for (int j = 0; j < kFb; ++j) {
+        double v = R[j][lane];
+        for (int t = 0; t < j; ++t) v -= R[t][lane] * D[t][j];
+        R[j][lane] = v / D[j][j];
+    }
+    __syncthreads();
+    if (lane < rb)
+        for (int c = 0; c < kFb; ++c) P[(long long)(kb + c) * m + r0 + lane] = R[c][lane];
+}
+
+// 64 x 64 product in LDS by 256 threads: acc[x][y] += sum_k As[k][ti + 16x] Bs[k][tj + 16y].
+__device__ __forceinline__ void tile_mma(const double (*As)[kFbPad], const double (*Bs)[kFbPad], int ti, int tj,
+                                         double acc[4][4]) {
+    for (int k = 0; k < kFb; ++k) {
+        double a[4], bb[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) a[x] = As[k][ti + 16 * x];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) bb[y] = Bs[k][tj + 16 * y];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * bb[y];
+    }
+}
+
+// Lower tile q = (I, J), s < J <= I, of the trailing block:
+// A(r, c) -= sum_{t<64} L(r, kb + t) L(c, kb + t).
+__global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
+    const int nt = (w + kFb - 1) / kFb - 1 - s;  // row blocks below diagonal block s
+    if (q >= nt * (nt + 1) / 2) return;
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= q) ++i;
+    const int r0 = (s + 1 + i) * kFb, c0 = (s + 1 + q - i * (i + 1) / 2) * kFb;
+    const int tid = threadIdx.x, ti = tid % 16, tj = tid / 16;
+    __shared__ double As[kFb][kFbPad];  // As[t][x] = L(r0 + x, kb + t)
+    __shared__ double Bs[kFb][kFbPad];  // Bs[t][y] = L(c0 + y, kb + t)
+    double *P = A.P + S.poff;
+    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
+        const int t = idx / kFb, x = idx % kFb;
+        const long long col = (long long)(kb + t) * m;
+        As[t][x] = r0 + x < w ? P[col + r0 + x] : 0.0;
+        Bs[t][x] = c0 + x < w ? P[col + c0 + x] : 0.0;
+    }
+    __syncthreads();
+    double acc[4][4] = {};
+    tile_mma(As, Bs, ti, tj, acc);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int r = r0 + ti + 16 * x, c = c0 + tj + 16 * y;
+            if (r < w && c < w && r >= c) P[(long long)c * m + r] -= acc[x][y];
+        }
+}
+
+// Off-diagonal blocks of X on block diagonal d: I = d + (blockIdx.x % per), J = I - d.
+__global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int d, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, I = d + (int)(blockIdx.x % per), J = I - d;
+    const int r0 = I * kFb, c0 = J * kFb;
+    if (r0 >= w) return;  // every block K in [J, I) is full
+    const int tid = threadIdx.x, ti = tid % 16, tj = tid / 16, rb = min(kFb, w - r0);
+    __shared__ double As[kFb][kFbPad];
+    __shared__ double Bs[kFb][kFbPad];
+    const double *P = A.P + S.poff;
+    double *T = A.T + S.toff;
+    double acc[4][4] = {};
+    // Y = sum_K L_IK X_KJ: As[k][x] = L(r0 + x, 64K + k), Bs[k][y] = X(64K + k, c0 + y)
+    for (int K = J; K < I; ++K) {
         __syncthreads();
-        // rows r > k: X[r][j] -= L[r][k] X[k][j]
-        const int nr = w - 1 - k, nj = k + 1;
-        for (long long idx = tid; idx < (long long)nr * nj; idx += kCholBlock) {
-            const int r = k + 1 + (int)(idx / nj), j = (int)(idx % nj);
-            X[(long long)r * w + j] -= P[(long long)k * m + r] * X[(long long)k * w + j];
+        for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
+            const int k = idx / kFb, x = idx % kFb;
+            As[k][x] = x < rb ? P[(long long)(K * kFb + k) * m + r0 + x] : 0.0;
+            Bs[k][x] = T[(long long)(K * kFb + k) * w + c0 + x];
         }
         __syncthreads();
+        tile_mma(As, Bs, ti, tj, acc);
     }
+    __syncthreads();
+    // X_IJ = -X_II Y: As[k][x] = X(r0 + x, r0 + k), Bs[k][y] = Y(k, y)
+    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
+        const int x = idx / kFb, k = idx % kFb;
+        As[k][x] = (x < rb && k < rb) ? T[(long long)(r0 + x) * w + r0 + k] : 0.0;
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) Bs[ti + 16 * x][tj + 16 * y] = acc[x][y];
+    __syncthreads();
+    double out[4][4] = {};
+    tile_mma(As, Bs, ti, tj, out);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            if (ti + 16 * x < rb) T[(long long)(r0 + ti + 16 * x) * w + c0 + tj + 16 * y] = -out[x][y];
 }
 
 // D^{-1} = X^T X into the pivot block of the panel (symmetric), `per` workgroups per supernode.
@@ -1211,10 +1336,21 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ sr
 void launch_fac_diag(hipStream_t s, const FacArgs &A) {
     if (A.nf > 0) hipLaunchKernelGGL(k_fac_diag, dim3((A.nf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A);
 }
-void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int per_dinv, int per_g, int per_u) {
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_dinv, int per_g,
+                       int per_u) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fac_assemble, dim3(count), dim3(kFacBlock), 0, s, A, h0);
-    hipLaunchKernelGGL(k_fac_chol, dim3(count), dim3(kCholBlock), 0, s, A, h0);
+    for (int b = 0; b < nb; ++b) {
+        hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
+        const int below = nb - 1 - b;
+        if (below > 0) {
+            hipLaunchKernelGGL(k_fac_trsm, dim3(count * below), dim3(kFb), 0, s, A, h0, b, below);
+            const int tiles = below * (below + 1) / 2;
+            hipLaunchKernelGGL(k_fac_syrk, dim3(count * tiles), dim3(kFacBlock), 0, s, A, h0, b, tiles);
+        }
+    }
+    for (int d = 1; d < nb; ++d)
+        hipLaunchKernelGGL(k_fac_trtri, dim3(count * (nb - d)), dim3(kFacBlock), 0, s, A, h0, d, nb - d);
     hipLaunchKernelGGL(k_fac_dinv, dim3(count * per_dinv), dim3(kFacBlock), 0, s, A, h0, per_dinv);
     if (per_g > 0) {
         hipLaunchKernelGGL(k_fac_g, dim3(count * per_g), dim3(kFacBlock), 0, s, A, h0, per_g);

@@ -774,16 +774,17 @@ int factor_device(md_mesh *m) {
         for (int h = 0; h < nh; ++h) {
             // workgroups per supernode of the multi-workgroup stages: enough for the height's
             // largest block, at most 64
-            int64_t mw2 = 0, mnw = 0, mno2 = 0;
+            int64_t mw2 = 0, mnw = 0, mno2 = 0, mw = 0;
             for (int i = f.hptr[h]; i < f.hptr[h + 1]; ++i) {
                 const int64_t w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
+                mw = std::max(mw, w);
                 mw2 = std::max(mw2, w * w);
                 mnw = std::max(mnw, no * w);
                 mno2 = std::max(mno2, no * no);
             }
             auto per = [](int64_t n) { return (int)std::min<int64_t>(64, std::max<int64_t>(1, (n + 255) / 256)); };
-            launch_fac_height(st, A, f.hptr[h], f.hptr[h + 1] - f.hptr[h], per(mw2), mnw > 0 ? per(mnw) : 0,
-                              per(mno2));
+            launch_fac_height(st, A, f.hptr[h], f.hptr[h + 1] - f.hptr[h], (int)((mw + 63) / 64), per(mw2),
+                              mnw > 0 ? per(mnw) : 0, per(mno2));
         }
         launch_fac_convert(st, A, ns, const_cast<void *>(m->dev.pc), const_cast<void *>(m->dev.pr), m->fp64());
         e = hipGetLastError();

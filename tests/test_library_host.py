@@ -89,6 +89,23 @@ def test_device_factorization_emulation_matches_scipy(make):
     np.testing.assert_allclose(x_dev, x_host, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
 
 
+def test_device_factorization_emulation_dense_blocks():
+    """A dense SPD matrix of 150 unknowns is one supernode of three column blocks (64, 64, 22):
+    the emulated blocked Cholesky, its triangular inverse (diagonal, off-diagonal and partial
+    blocks) and D^{-1} give the dense solve."""
+    rng = np.random.default_rng(5)
+    n = 150
+    A = rng.standard_normal((n, n))
+    A = A @ A.T + n * np.eye(n)
+    S = sp.csr_matrix(A)
+    b = rng.standard_normal((2, n))
+    xyz = rng.standard_normal((n, 3))
+    x, pe, _ = mdarap.host_factor_solve(S.indptr, S.indices, S.data, xyz, b, subtree_budget=0,
+                                        device_factor_emulation=True)
+    assert pe == n * n
+    np.testing.assert_allclose(x, np.linalg.solve(A, b.T).T, rtol=1e-11, atol=1e-13)
+
+
 def test_host_block_ldlt_dense_small():
     """A dense SPD matrix is one supernode: the block LDL^T is D^{-1} itself."""
     rng = np.random.default_rng(2)
