@@ -355,6 +355,9 @@ template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
 // one load; more are loaded 8 at a time (clamped addresses, masked values) so their
 // latencies overlap.
 constexpr int kPartialBatch = 8;
+// Level launches of at most this many tiles load their panel entries ahead of the vector
+// (latency-bound: few waves); larger ones keep the register count low for occupancy.
+constexpr int kPreloadMaxTiles = 2048;
 
 template <typename V>
 __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e) {
@@ -375,12 +378,11 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
     return a;
 }
 
-// Children's contributions to position pos of supernode S's front, summed in child order:
-// the first from the position's record, the rest from its overflow list.
+// Children's contributions to a position of a front, summed in child order: the first from
+// the position's record r (M.cpos), the rest from its overflow list.
 template <typename V>
-__device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S, int pos) {
+__device__ __forceinline__ V pull_record(const MeshDev &M, const int4 r) {
     const V *U = reinterpret_cast<const V *>(M.upart);
-    const int4 r = M.cpos[S.cp_off + pos];
     V a;
     a.x = 0; a.y = 0; a.z = 0; a.w = 0;
     if (r.z > 0) a = sum_partials<V>(U + r.x, r.z, r.y, 0);
@@ -392,6 +394,30 @@ __device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S
             a.x += u.x; a.y += u.y; a.z += u.z;
         }
     return a;
+}
+
+template <typename V>
+__device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S, int pos) {
+    return pull_record<V>(M, M.cpos[S.cp_off + pos]);
+}
+
+// g[i] = P[min(i, n - 1) * stride] for i < N (n >= 1): a column of a panel tile loaded into
+// registers ahead of the vector it multiplies; entries past n are valid panel values that meet
+// zero vector entries.
+template <int N, typename T>
+__device__ __forceinline__ void panel_load(const T *P, long long stride, int n, T *g) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) g[i] = P[(long long)min(i, n - 1) * stride];
+}
+
+// acc += sum_{i < N} g[i] * v[i] (v[i] = 0 for i >= n).
+template <int N, typename T, typename V>
+__device__ __forceinline__ void panel_dot(const T *g, const V *v, T &ax, T &ay, T &az) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const V x = v[i];
+        ax += g[i] * x.x; ay += g[i] * x.y; az += g[i] * x.z;
+    }
 }
 
 // acc += sum_{i < n} P[i * stride] * v[i] for n <= N (N = 16, 32 or 64): all N loads are
@@ -423,19 +449,56 @@ __device__ __forceinline__ void tile_dot_any(const T *P, long long stride, const
 //   partial q0/64 of u_s on rows [r0, r0+64) = (q0 == 0 ? v_o : 0) - G_s[rows, cols] v_d.
 // Children's updates arrive as kc partial vectors; each lane pulls the contributions listed
 // for its position (host-built lists in child order), so no two lanes write the same place.
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
+// fwd_tile loads the tile's G entries (N = 16, 32 or 64 columns) while the contributions arrive:
+// a shorter chain of dependent loads for twice the registers (launches of few tiles);
+// fwd_tile_lean loads them after the contributions.
+template <int N, typename T>
+__device__ __forceinline__ void fwd_tile(const MeshDev &M, const SnDev &S, int r0, int q0, int qn, int rn) {
     using V = typename V4T<T>::type;
     __shared__ V vd[kTile];
     __shared__ V vo[kTile];
-    const int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
-    const TileDev tl = M.ftile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
-    const SnDev &S = tl.s;
-    const int r0 = tl.a, q0 = tl.b, lane = threadIdx.x;
-    const int w = S.w, no = S.noff;
-    const int qn = min(kTile, w - q0), rn = max(0, min(kTile, no - r0));
-    const int kcb = q0 / kTile;
+    const int lane = threadIdx.x, w = S.w, no = S.noff, kcb = q0 / kTile;
+    const V *R = reinterpret_cast<const V *>(M.r);
+    V b, o;
+    b.x = 0; b.y = 0; b.z = 0; b.w = 0;
+    o = b;
+    int4 rd = make_int4(0, 0, 0, -1), ro = make_int4(0, 0, 0, -1);
+    if (lane < qn) {
+        b = R[S.c0 + q0 + lane];
+        rd = M.cpos[S.cp_off + q0 + lane];
+    }
+    if (kcb == 0 && lane < rn) ro = M.cpos[S.cp_off + w + r0 + lane];
+    // column-major G_s: entry (row r, column q) at q * no + r
+    T g[N];
+    if (rn > 0) panel_load<N, T>(tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1), no, qn, g);
+    if (lane < qn) {
+        const V c = pull_record<V>(M, rd);
+        b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
+    }
+    vd[lane] = b;
+    if (kcb == 0 && lane < rn) o = pull_record<V>(M, ro);
+    vo[lane] = o;
+    __syncthreads();
+    if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
+    if (rn == 0) return;
+    T ax = 0, ay = 0, az = 0;
+    panel_dot<N, T, V>(g, vd, ax, ay, az);
+    if (lane < rn) {
+        V out;
+        out.x = (kcb == 0 ? o.x : T(0)) - ax;
+        out.y = (kcb == 0 ? o.y : T(0)) - ay;
+        out.z = (kcb == 0 ? o.z : T(0)) - az;
+        out.w = 0;
+        reinterpret_cast<V *>(M.upart)[S.uoff + (long long)kcb * no + r0 + lane] = out;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void fwd_tile_lean(const MeshDev &M, const SnDev &S, int r0, int q0, int qn, int rn) {
+    using V = typename V4T<T>::type;
+    __shared__ V vd[kTile];
+    __shared__ V vo[kTile];
+    const int lane = threadIdx.x, w = S.w, no = S.noff, kcb = q0 / kTile;
     const V *R = reinterpret_cast<const V *>(M.r);
     {
         V b;
@@ -454,7 +517,6 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     __syncthreads();
     if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
     if (rn == 0) return;
-    // column-major G_s: entry (row r, column q) at q * no + r
     const T *G = tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1);
     T ax = 0, ay = 0, az = 0;
     tile_dot_any<T, V>(G, no, vd, qn, ax, ay, az);
@@ -468,21 +530,62 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     }
 }
 
+template <typename T, bool Pre>
+__global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
+    const int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    const TileDev tl = M.ftile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
+    const SnDev &S = tl.s;
+    const int r0 = tl.a, q0 = tl.b;
+    const int qn = min(kTile, S.w - q0), rn = max(0, min(kTile, S.noff - r0));
+    if (!Pre) fwd_tile_lean<T>(M, S, r0, q0, qn, rn);
+    else if (qn <= 16) fwd_tile<16, T>(M, S, r0, q0, qn, rn);
+    else if (qn <= 32) fwd_tile<32, T>(M, S, r0, q0, qn, rn);
+    else fwd_tile<64, T>(M, S, r0, q0, qn, rn);
+}
+
 // Backward tile (s, q0, p0), one wavefront: partial p0/64 of x_s on columns [q0, q0+64):
 //   sum over rows p in [p0, p0+64) of [Dinv_s; G_s][p, q] * t[p],  t = [z_s; -x_off],
 // read from the row-major copy (lanes on consecutive columns). x at off rows is the sum of
-// the owner's partials (xloc gives where they are).
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
+// the owner's partials (xloc gives where they are). bwd_tile loads the tile's panel rows (N =
+// 16, 32 or 64) while t is gathered; bwd_tile_lean loads them afterwards.
+template <int N, typename T>
+__device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q0, int p0, int qn, int pn) {
     using V = typename V4T<T>::type;
     __shared__ V tv[kTile];
-    const int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
-    const TileDev tl = M.btile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
-    const SnDev &S = tl.s;
-    const int q0 = tl.a, p0 = tl.b, lane = threadIdx.x;
-    const int w = S.w, m = S.w + S.noff;
-    const int qn = min(kTile, w - q0), pn = min(kTile, m - p0);
+    const int lane = threadIdx.x, w = S.w;
+    const V *X = reinterpret_cast<const V *>(M.xpart);
+    const int p = p0 + lane;
+    V t;
+    t.x = 0; t.y = 0; t.z = 0; t.w = 0;
+    int4 xl = make_int4(0, 0, 0, 0);
+    if (lane < pn) {
+        if (p < w) t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
+        else xl = M.xloc[S.roff + p - w];
+    }
+    T g[N];
+    panel_load<N, T>(tptr<T>(M.pr) + S.pr_off + (long long)p0 * w + q0 + min(lane, qn - 1), w, pn, g);
+    if (lane < pn && p >= w) {
+        const V a = sum_partials<V>(X + xl.x, xl.z, xl.y, 0);
+        t.x = -a.x; t.y = -a.y; t.z = -a.z;
+    }
+    t.w = 0;
+    tv[lane] = t;
+    __syncthreads();
+    T ax = 0, ay = 0, az = 0;
+    panel_dot<N, T, V>(g, tv, ax, ay, az);
+    if (lane < qn) {
+        V out;
+        out.x = ax; out.y = ay; out.z = az; out.w = 0;
+        reinterpret_cast<V *>(M.xpart)[S.xoff + (long long)(p0 / kTile) * w + q0 + lane] = out;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bwd_tile_lean(const MeshDev &M, const SnDev &S, int q0, int p0, int qn, int pn) {
+    using V = typename V4T<T>::type;
+    __shared__ V tv[kTile];
+    const int lane = threadIdx.x, w = S.w;
     const V *X = reinterpret_cast<const V *>(M.xpart);
     {
         V t;
@@ -509,6 +612,20 @@ __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
         out.x = ax; out.y = ay; out.z = az; out.w = 0;
         reinterpret_cast<V *>(M.xpart)[S.xoff + (long long)(p0 / kTile) * w + q0 + lane] = out;
     }
+}
+
+template <typename T, bool Pre>
+__global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
+    const int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    const TileDev tl = M.btile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
+    const SnDev &S = tl.s;
+    const int q0 = tl.a, p0 = tl.b;
+    const int qn = min(kTile, S.w - q0), pn = min(kTile, S.w + S.noff - p0);
+    if (!Pre) bwd_tile_lean<T>(M, S, q0, p0, qn, pn);
+    else if (pn <= 16) bwd_tile<16, T>(M, S, q0, p0, qn, pn);
+    else if (pn <= 32) bwd_tile<32, T>(M, S, q0, p0, qn, pn);
+    else bwd_tile<64, T>(M, S, q0, p0, qn, pn);
 }
 
 // acc += sum_{i < n} P[i * stride] * v[i] (n <= 64; n <= 0 adds nothing but still reads P[0]
@@ -1418,13 +1535,23 @@ void launch_residual(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
 }
 void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
     if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_fwd<double>, dim3(blocks), dim3(kTile), 0, s, L, level);
-    else hipLaunchKernelGGL(k_fwd<float>, dim3(blocks), dim3(kTile), 0, s, L, level);
+    if (blocks <= kPreloadMaxTiles) {
+        if (fp64) hipLaunchKernelGGL((k_fwd<double, true>), dim3(blocks), dim3(kTile), 0, s, L, level);
+        else hipLaunchKernelGGL((k_fwd<float, true>), dim3(blocks), dim3(kTile), 0, s, L, level);
+    } else {
+        if (fp64) hipLaunchKernelGGL((k_fwd<double, false>), dim3(blocks), dim3(kTile), 0, s, L, level);
+        else hipLaunchKernelGGL((k_fwd<float, false>), dim3(blocks), dim3(kTile), 0, s, L, level);
+    }
 }
 void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
     if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_bwd<double>, dim3(blocks), dim3(kTile), 0, s, L, level);
-    else hipLaunchKernelGGL(k_bwd<float>, dim3(blocks), dim3(kTile), 0, s, L, level);
+    if (blocks <= kPreloadMaxTiles) {
+        if (fp64) hipLaunchKernelGGL((k_bwd<double, true>), dim3(blocks), dim3(kTile), 0, s, L, level);
+        else hipLaunchKernelGGL((k_bwd<float, true>), dim3(blocks), dim3(kTile), 0, s, L, level);
+    } else {
+        if (fp64) hipLaunchKernelGGL((k_bwd<double, false>), dim3(blocks), dim3(kTile), 0, s, L, level);
+        else hipLaunchKernelGGL((k_bwd<float, false>), dim3(blocks), dim3(kTile), 0, s, L, level);
+    }
 }
 void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64) {
     if (blocks <= 0) return;
