@@ -529,7 +529,7 @@ void device_layout(Factor& f, int sub_budget) {
         f.xoff[t + 1] = f.xoff[t] + (int64_t)f.kr[t] * f.w[t];
     }
     // contributions of the children's update entries to each position of a front, in child
-    // order: the forward gather pulls them per position (no write conflicts). The first one is
+    // order: the forward gather pulls them per position (no write conflicts). The first two are
     // inline in the position's record, the rest in a list ended by a zero count.
     f.cp_off.assign(nn, 0);
     f.cpos.clear();
@@ -544,19 +544,19 @@ void device_layout(Factor& f, int sub_budget) {
                 int c = f.child[k];
                 for (int e = 0; e < f.noff[c]; ++e) lists[f.rel[f.roff[c] + e]].emplace_back(c, e);
             }
-            f.cp_off[t] = (int)(f.cpos.size() / 4);
+            f.cp_off[t] = (int)(f.cpos.size() / 8);
             for (int p = 0; p < m; ++p) {
                 const auto& L = lists[p];
-                if (L.empty()) {
-                    int rec[4] = {-1, 0, 0, -1};
-                    f.cpos.insert(f.cpos.end(), rec, rec + 4);
-                    continue;
+                int rec[8] = {-1, 0, 0, -1, -1, 0, 0, 0};
+                for (size_t i = 0; i < std::min<size_t>(2, L.size()); ++i) {
+                    const int c = L[i].first;
+                    rec[4 * i] = (int)(f.uoff[c] + L[i].second);
+                    rec[4 * i + 1] = f.noff[c];
+                    rec[4 * i + 2] = f.kc[c];
                 }
-                int rec[4] = {(int)(f.uoff[L[0].first] + L[0].second), f.noff[L[0].first], f.kc[L[0].first],
-                              L.size() > 1 ? (int)(f.cmore.size() / 4) : -1};
-                f.cpos.insert(f.cpos.end(), rec, rec + 4);
-                if (L.size() > 1) {
-                    for (size_t i = 1; i < L.size(); ++i) {
+                if (L.size() > 2) {
+                    rec[3] = (int)(f.cmore.size() / 4);
+                    for (size_t i = 2; i < L.size(); ++i) {
                         int c = L[i].first, e = L[i].second;
                         int ent[4] = {(int)(f.uoff[c] + e), f.noff[c], f.kc[c], 0};
                         f.cmore.insert(f.cmore.end(), ent, ent + 4);
@@ -564,6 +564,7 @@ void device_layout(Factor& f, int sub_budget) {
                     int end[4] = {0, 0, 0, 0};
                     f.cmore.insert(f.cmore.end(), end, end + 4);
                 }
+                f.cpos.insert(f.cpos.end(), rec, rec + 8);
             }
         }
     }
@@ -1159,8 +1160,8 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
             for (int i = 0; i < qn; ++i)
                 for (int c = 0; c < 3; ++c) vd[c][i] = b[(size_t)c * n + c0 + q0 + i];
             auto pull = [&](int pos, double* acc3) -> bool {
-                size_t r = 4 * ((size_t)f.cp_off[s] + pos);
-                if (r + 3 >= f.cpos.size()) return false;
+                size_t r = 8 * ((size_t)f.cp_off[s] + pos);
+                if (r + 7 >= f.cpos.size()) return false;
                 const int* rec = f.cpos.data() + r;
                 auto add = [&](const int* ce) -> bool {
                     for (int c = 0; c < 3; ++c) {
@@ -1171,8 +1172,9 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                     return true;
                 };
                 if (rec[2] > 0 && !add(rec)) return false;
+                if (rec[6] > 0 && (rec[2] <= 0 || !add(rec + 4))) return false;  // second follows a first
                 if (rec[3] >= 0) {
-                    if (rec[2] <= 0) return false;  // overflow entries follow a first one
+                    if (rec[6] <= 0) return false;  // overflow entries follow the inline two
                     for (size_t q = (size_t)rec[3];; ++q) {
                         if (4 * q + 3 >= f.cmore.size()) return false;
                         const int* ce = f.cmore.data() + 4 * q;

@@ -116,8 +116,9 @@ struct Factor {
     std::vector<int64_t> uoff;       // ns + 1: kc * noff partial update entries
     std::vector<int64_t> xoff;       // ns + 1: kr * w partial solution entries
     std::vector<int> cp_off;         // per supernode: its first position record
-    std::vector<int> cpos;           // 4 per front position: first contribution (child uoff + e,
-                                     //   child noff, child kc; count 0: none), more: cmore index or -1
+    std::vector<int> cpos;           // 8 per front position: first contribution (child uoff + e,
+                                     //   child noff, child kc; count 0: none, then the cmore index
+                                     //   of the third onward or -1), second contribution (same, 0)
     std::vector<int> cmore;          // 4 per further contribution (same fields, 0), lists end with count 0
     std::vector<int> rowinfo;        // 4 per row: first partial, stride, count, 0
     std::vector<int> xloc;           // 4 per off row entry: rowinfo of that row

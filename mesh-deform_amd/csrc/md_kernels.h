@@ -49,8 +49,9 @@ struct MeshDev {
     void *r, *z;                // residual and forward result (V4 of the solve type), nf each
     void *upart, *xpart;        // partial update / solution vectors (V4)
     const int *child_ptr, *child;
-    const int4 *cpos;           // per front position: first contribution (child partial base,
-                                //   stride, count; count 0: none), index of more in cmore or -1
+    const int4 *cpos;           // per front position, two records: the first contribution (child
+                                //   partial base, stride, count; count 0: none; index of the third
+                                //   onward in cmore or -1) and the second (same fields, 0)
     const int4 *cmore;          // further contributions, lists ended by a zero count
     const int4 *rowinfo;        // per row: (first partial, stride, count, 0)
     const int4 *xloc;           // per off row entry: rowinfo of that row
@@ -102,6 +103,7 @@ struct LaunchSet {
     int nactive;
     const int *bmesh;  // optional: mesh of every block (static batches), else a binary search
     const int *tbase;  // per mesh: first tile of the launch's level (level sweeps), else 0
+    const TileDev *tiles;  // one-mesh level launches: the level's first tile (else null)
 };
 
 // Launchers (grid = total blocks of the launch, from the host-side prefix table).

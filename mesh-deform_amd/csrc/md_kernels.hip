@@ -378,14 +378,17 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
     return a;
 }
 
-// Children's contributions to a position of a front, summed in child order: the first from
-// the position's record r (M.cpos), the rest from its overflow list.
+// Children's contributions to a position of a front, summed in child order: the first two from
+// the position's records r, r2 (M.cpos), the rest from its overflow list.
 template <typename V>
-__device__ __forceinline__ V pull_record(const MeshDev &M, const int4 r) {
+__device__ __forceinline__ V pull_record(const MeshDev &M, const int4 r, const int4 r2) {
     const V *U = reinterpret_cast<const V *>(M.upart);
-    V a;
+    V a, b;
     a.x = 0; a.y = 0; a.z = 0; a.w = 0;
+    b = a;
     if (r.z > 0) a = sum_partials<V>(U + r.x, r.z, r.y, 0);
+    if (r2.z > 0) b = sum_partials<V>(U + r2.x, r2.z, r2.y, 0);
+    a.x += b.x; a.y += b.y; a.z += b.z;
     if (r.w >= 0)
         for (int q = r.w;; ++q) {
             const int4 ce = M.cmore[q];
@@ -398,7 +401,8 @@ __device__ __forceinline__ V pull_record(const MeshDev &M, const int4 r) {
 
 template <typename V>
 __device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S, int pos) {
-    return pull_record<V>(M, M.cpos[S.cp_off + pos]);
+    const int4 *rec = M.cpos + 2 * ((long long)S.cp_off + pos);
+    return pull_record<V>(M, rec[0], rec[1]);
 }
 
 // g[i] = P[min(i, n - 1) * stride] for i < N (n >= 1): a column of a panel tile loaded into
@@ -462,21 +466,26 @@ __device__ __forceinline__ void fwd_tile(const MeshDev &M, const SnDev &S, int r
     V b, o;
     b.x = 0; b.y = 0; b.z = 0; b.w = 0;
     o = b;
-    int4 rd = make_int4(0, 0, 0, -1), ro = make_int4(0, 0, 0, -1);
+    int4 rd = make_int4(0, 0, 0, -1), ro = rd, rd2 = make_int4(0, 0, 0, 0), ro2 = rd2;
+    const int4 *cp = M.cpos + 2 * (long long)S.cp_off;
     if (lane < qn) {
         b = R[S.c0 + q0 + lane];
-        rd = M.cpos[S.cp_off + q0 + lane];
+        rd = cp[2 * (q0 + lane)];
+        rd2 = cp[2 * (q0 + lane) + 1];
     }
-    if (kcb == 0 && lane < rn) ro = M.cpos[S.cp_off + w + r0 + lane];
+    if (kcb == 0 && lane < rn) {
+        ro = cp[2 * (w + r0 + lane)];
+        ro2 = cp[2 * (w + r0 + lane) + 1];
+    }
     // column-major G_s: entry (row r, column q) at q * no + r
     T g[N];
     if (rn > 0) panel_load<N, T>(tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1), no, qn, g);
     if (lane < qn) {
-        const V c = pull_record<V>(M, rd);
+        const V c = pull_record<V>(M, rd, rd2);
         b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
     }
     vd[lane] = b;
-    if (kcb == 0 && lane < rn) o = pull_record<V>(M, ro);
+    if (kcb == 0 && lane < rn) o = pull_record<V>(M, ro, ro2);
     vo[lane] = o;
     __syncthreads();
     if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
@@ -534,7 +543,7 @@ template <typename T, bool Pre>
 __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    const TileDev tl = M.ftile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
+    const TileDev tl = L.tiles ? L.tiles[blockIdx.x] : M.ftile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
     const SnDev &S = tl.s;
     const int r0 = tl.a, q0 = tl.b;
     const int qn = min(kTile, S.w - q0), rn = max(0, min(kTile, S.noff - r0));
@@ -618,7 +627,7 @@ template <typename T, bool Pre>
 __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    const TileDev tl = M.btile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
+    const TileDev tl = L.tiles ? L.tiles[blockIdx.x] : M.btile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
     const SnDev &S = tl.s;
     const int q0 = tl.a, p0 = tl.b;
     const int qn = min(kTile, S.w - q0), pn = min(kTile, S.w + S.noff - p0);

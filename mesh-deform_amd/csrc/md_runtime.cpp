@@ -187,10 +187,10 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
         }
         launch_sub_fwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_fwd, plan.fp64);
         for (int l = 0; l < plan.nlevels; ++l)
-            launch_fwd(stream, plan.set(plan.off_fwd[l]), plan.blk_fwd[l], l, plan.fp64);
+            launch_fwd(stream, plan.level_set(plan.off_fwd[l], true), plan.blk_fwd[l], l, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[3], stream);
         for (int l = plan.nlevels - 1; l >= 0; --l)
-            launch_bwd(stream, plan.set(plan.off_bwd[l]), plan.blk_bwd[l], l, plan.fp64);
+            launch_bwd(stream, plan.level_set(plan.off_bwd[l], false), plan.blk_bwd[l], l, plan.fp64);
         launch_sub_bwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_bwd, plan.fp64);
         launch_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[4], stream);
@@ -1361,8 +1361,10 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     // off-row sources (backward)
     const int64_t sub_fwd_idx = 4 * (int64_t)(f.snode.size() + f.scp.size() + f.sfc.size() + f.sub_h.size());  // scp: 2 ints per position
     const int64_t sub_bwd_idx = 4 * (int64_t)(f.snode.size() + f.sbo.size() + f.sub_h.size());
-    // forward: G panel, r in, z out, update partials out and read back by the parent, rel
-    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + offr * 4 + sub_fwd_idx;
+    // forward: G panel, r in, z out, update partials out and read back by the parent, the
+    // level fronts' contribution records
+    const int64_t lvl_fwd_idx = 4 * (int64_t)(f.cpos.size() + f.cmore.size());
+    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + lvl_fwd_idx + sub_fwd_idx;
     // backward + update: Dinv and G panels, z in, x partials of ancestors gathered (with
     // their locators), x partials out, final sum and the p' update
     out->bytes_backward = (dw + gw) * es + (int64_t)t.nf * vs + xloc_reads * vs + offr * 16 + 2 * xp * vs +

@@ -72,7 +72,15 @@ struct Plan {
     LaunchSet set(int off) const {
         size_t t = (size_t)off / (size_t)(nactive + 1);
         const int *bm = (d_bmesh && t < bmesh_off.size() && bmesh_off[t] != (size_t)-1) ? d_bmesh + bmesh_off[t] : nullptr;
-        return LaunchSet{d_meshes, d_prefix + off, nactive, bm, d_tbase + off};
+        return LaunchSet{d_meshes, d_prefix + off, nactive, bm, d_tbase + off, nullptr};
+    }
+    // Level launch: a one-mesh plan also passes the level's first tile, so that blocks load
+    // their tile without first loading the mesh record.
+    LaunchSet level_set(int off, bool fwd) const {
+        LaunchSet L = set(off);
+        if (nactive == 1 && !h_meshes.empty())
+            L.tiles = (fwd ? h_meshes[0].ftile : h_meshes[0].btile) + h_tbase[(size_t)off];
+        return L;
     }
 };
 
