@@ -581,10 +581,12 @@ void device_layout(Factor& f, int sub_budget) {
         for (int k = 0; k < 4; ++k) f.xloc[4 * e + k] = f.rowinfo[4 * (size_t)f.offrows[e] + k];
     f.ft_ptr.assign(f.nlevels + 1, 0);
     f.bt_ptr.assign(f.nlevels + 1, 0);
+    // a level supernode without off rows (a root) has no forward tiles: its backward tiles
+    // gather its right-hand side and children's contributions themselves
     for (int t = 0; t < nn; ++t) {
         if (f.level[t] < 0) continue;
         int m = f.w[t] + f.noff[t];
-        f.ft_ptr[f.level[t] + 1] += std::max(1, cdiv(f.noff[t], kTile)) * f.kc[t];
+        if (f.noff[t] > 0) f.ft_ptr[f.level[t] + 1] += cdiv(f.noff[t], kTile) * f.kc[t];
         f.bt_ptr[f.level[t] + 1] += f.kc[t] * cdiv(m, kTile);
     }
     for (int l = 0; l < f.nlevels; ++l) {
@@ -597,7 +599,7 @@ void device_layout(Factor& f, int sub_budget) {
     for (int t = 0; t < nn; ++t) {
         int l = f.level[t], m = f.w[t] + f.noff[t];
         if (l < 0) continue;
-        int rbs = std::max(1, cdiv(f.noff[t], kTile));
+        int rbs = cdiv(f.noff[t], kTile);
         for (int rb = 0; rb < rbs; ++rb)
             for (int cb = 0; cb < f.kc[t]; ++cb) {
                 int* tt = &f.ft[4 * (size_t)ff[l]++];
@@ -1149,6 +1151,32 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
             }
         }
     }
+    // children's contributions to position pos of level supernode s, in the kernels' order
+    auto pull_at = [&](int s, int pos, double* acc3) -> bool {
+        size_t r = 8 * ((size_t)f.cp_off[s] + pos);
+        if (r + 7 >= f.cpos.size()) return false;
+        const int* rec = f.cpos.data() + r;
+        auto add = [&](const int* ce) -> bool {
+            for (int c = 0; c < 3; ++c) {
+                double v;
+                if (!sum_part(up, ce[0], ce[2], ce[1], 0, c, v)) return false;
+                acc3[c] += v;
+            }
+            return true;
+        };
+        if (rec[2] > 0 && !add(rec)) return false;
+        if (rec[6] > 0 && (rec[2] <= 0 || !add(rec + 4))) return false;  // second follows a first
+        if (rec[3] >= 0) {
+            if (rec[6] <= 0) return false;  // overflow entries follow the inline two
+            for (size_t q = (size_t)rec[3];; ++q) {
+                if (4 * q + 3 >= f.cmore.size()) return false;
+                const int* ce = f.cmore.data() + 4 * q;
+                if (ce[2] == 0) break;
+                if (!add(ce)) return false;
+            }
+        }
+        return true;
+    };
     for (int l = 0; l < f.nlevels; ++l) {
         for (int t = f.ft_ptr[l]; t < f.ft_ptr[l + 1]; ++t) {
             int s = f.ft[4 * (size_t)t], r0 = f.ft[4 * (size_t)t + 1], q0 = f.ft[4 * (size_t)t + 2];
@@ -1159,31 +1187,7 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
             double vd[3][kTile] = {}, vo[3][kTile] = {};
             for (int i = 0; i < qn; ++i)
                 for (int c = 0; c < 3; ++c) vd[c][i] = b[(size_t)c * n + c0 + q0 + i];
-            auto pull = [&](int pos, double* acc3) -> bool {
-                size_t r = 8 * ((size_t)f.cp_off[s] + pos);
-                if (r + 7 >= f.cpos.size()) return false;
-                const int* rec = f.cpos.data() + r;
-                auto add = [&](const int* ce) -> bool {
-                    for (int c = 0; c < 3; ++c) {
-                        double v;
-                        if (!sum_part(up, ce[0], ce[2], ce[1], 0, c, v)) return false;
-                        acc3[c] += v;
-                    }
-                    return true;
-                };
-                if (rec[2] > 0 && !add(rec)) return false;
-                if (rec[6] > 0 && (rec[2] <= 0 || !add(rec + 4))) return false;  // second follows a first
-                if (rec[3] >= 0) {
-                    if (rec[6] <= 0) return false;  // overflow entries follow the inline two
-                    for (size_t q = (size_t)rec[3];; ++q) {
-                        if (4 * q + 3 >= f.cmore.size()) return false;
-                        const int* ce = f.cmore.data() + 4 * q;
-                        if (ce[2] == 0) break;
-                        if (!add(ce)) return false;
-                    }
-                }
-                return true;
-            };
+            auto pull = [&](int pos, double* acc3) -> bool { return pull_at(s, pos, acc3); };
             for (int i = 0; i < qn; ++i) {
                 double a[3] = {0, 0, 0};
                 MD_CHECK(pull(q0 + i, a), "column contributions");
@@ -1222,7 +1226,9 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
             for (int lane = 0; lane < pn; ++lane) {
                 int p = p0 + lane;
                 for (int c = 0; c < 3; ++c) {
-                    if (p < w) {
+                    if (p < w && f.noff[s] == 0) {
+                        tv[c][lane] = b[(size_t)c * n + f.c0[s] + p];  // a root: no forward tile
+                    } else if (p < w) {
                         tv[c][lane] = z[3 * (size_t)(f.c0[s] + p) + c];
                     } else {
                         size_t e = (size_t)f.roff[s] + p - w;
@@ -1232,6 +1238,11 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                         MD_CHECK(sum_part(xp, xl[0], xl[2], xl[1], 0, c, v), "solution partial");
                         tv[c][lane] = -v;
                     }
+                }
+                if (p < w && f.noff[s] == 0) {
+                    double a[3] = {0, 0, 0};
+                    MD_CHECK(pull_at(s, p, a), "root contributions");
+                    for (int c = 0; c < 3; ++c) tv[c][lane] += a[c];
                 }
             }
             for (int lane = 0; lane < qn; ++lane) {

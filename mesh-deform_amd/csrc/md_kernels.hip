@@ -565,18 +565,33 @@ __device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q
     const int lane = threadIdx.x, w = S.w;
     const V *X = reinterpret_cast<const V *>(M.xpart);
     const int p = p0 + lane;
+    // a root (no off rows) has no forward tile: its right-hand side and children's
+    // contributions are gathered here (z = r + contributions, as the forward tile forms it)
+    const bool root = S.noff == 0;
     V t;
     t.x = 0; t.y = 0; t.z = 0; t.w = 0;
-    int4 xl = make_int4(0, 0, 0, 0);
+    int4 xl = make_int4(0, 0, 0, 0), c1 = make_int4(0, 0, 0, -1), c2 = make_int4(0, 0, 0, 0);
     if (lane < pn) {
-        if (p < w) t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
-        else xl = M.xloc[S.roff + p - w];
+        if (p >= w) {
+            xl = M.xloc[S.roff + p - w];
+        } else if (root) {
+            t = reinterpret_cast<const V *>(M.r)[S.c0 + p];
+            const int4 *cp = M.cpos + 2 * ((long long)S.cp_off + p);
+            c1 = cp[0];
+            c2 = cp[1];
+        } else {
+            t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
+        }
     }
     T g[N];
     panel_load<N, T>(tptr<T>(M.pr) + S.pr_off + (long long)p0 * w + q0 + min(lane, qn - 1), w, pn, g);
     if (lane < pn && p >= w) {
         const V a = sum_partials<V>(X + xl.x, xl.z, xl.y, 0);
         t.x = -a.x; t.y = -a.y; t.z = -a.z;
+    }
+    if (lane < pn && p < w && root) {
+        const V c = pull_record<V>(M, c1, c2);
+        t.x += c.x; t.y += c.y; t.z += c.z;
     }
     t.w = 0;
     tv[lane] = t;
@@ -601,7 +616,11 @@ __device__ __forceinline__ void bwd_tile_lean(const MeshDev &M, const SnDev &S, 
         t.x = 0; t.y = 0; t.z = 0;
         int p = p0 + lane;
         if (lane < pn) {
-            if (p < w) {
+            if (p < w && S.noff == 0) {  // a root: no forward tile (as in bwd_tile)
+                t = reinterpret_cast<const V *>(M.r)[S.c0 + p];
+                const V c = pull_contributions<V>(M, S, p);
+                t.x += c.x; t.y += c.y; t.z += c.z;
+            } else if (p < w) {
                 t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
             } else {
                 int4 xl = M.xloc[S.roff + p - w];
