@@ -64,6 +64,9 @@ struct MeshDev {
     const int2 *scp;            // forward positions: first two child update entries (LDS), overflow
     const int *sfc;             // -1-terminated overflow lists of child update entries
     const int2 *sbo;            // backward off-row sources
+    int nsub;                   // bottom subtrees (their rows' x is added to p' by k_sub_bwd)
+    int nlrows;                 // rows of level-swept supernodes, increasing
+    const int *lrows;           //   (x summed from partials and added to p' by k_sub_bwd too)
     double *energy_part;        // one partial sum per 256 vertices
     // Jacobi-PCG alternative to the triangular sweeps (solve type vectors, nf entries)
     float *w32;                 // slots: cotan weights in FP32
@@ -113,7 +116,6 @@ void launch_local(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_residual(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
-void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 // Bottom-subtree sweeps: one 64-lane block per subtree; LDS for `nodes` supernode records
 // and `lds_entries` vector entries.
 void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64);

@@ -778,10 +778,25 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
     V *F = reinterpret_cast<V *>(sub_smem + sizeof(SubNode) * node_cap);
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    const int4 sd = M.sub[blockIdx.x - L.prefix[k]];
     const int lane = threadIdx.x;
+    const V *X = reinterpret_cast<const V *>(M.xpart);
+    const int bk = blockIdx.x - L.prefix[k];
+    if (bk >= M.nsub) {
+        // blocks past the subtrees: p' += x for 64 rows of level supernodes (x summed from
+        // the partials of the level sweeps)
+        const int j = (bk - M.nsub) * kTile + lane;
+        if (j < M.nlrows) {
+            const int i = M.lrows[j];
+            const int4 ri = M.rowinfo[i];
+            const V a = sum_partials<V>(X + ri.x, ri.z, ri.y, 0);
+            d4 pv = M.pp[i];
+            pv.x += (double)a.x; pv.y += (double)a.y; pv.z += (double)a.z;
+            M.pp[i] = pv;
+        }
+        return;
+    }
+    const int4 sd = M.sub[bk];
     const V *Zs = reinterpret_cast<const V *>(M.z);
-    V *X = reinterpret_cast<V *>(M.xpart);
     const T *P = tptr<T>(M.pr);
     sub_records(M, sd, N);
     for (int h = sd.w - 1; h >= 0; --h) {
@@ -822,7 +837,11 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
                 V o;
                 o.x = ax; o.y = ay; o.z = az; o.w = 0;
                 F[S.xs + q] = o;
-                X[S.xo + q] = o;
+                // x of a subtree row is final here and read by nothing outside the subtree:
+                // p' += x directly
+                d4 pv = M.pp[S.c0 + q];
+                pv.x += (double)o.x; pv.y += (double)o.y; pv.z += (double)o.z;
+                M.pp[S.c0 + q] = pv;
             }
         }
         __syncthreads();
@@ -830,20 +849,6 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
 }
 
 // p'_i += x_i for every free vertex, x_i the sum of its supernode's partials.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_apply(LaunchSet L) {
-    using V = typename V4T<T>::type;
-    int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
-    int i = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
-    if (i >= M.nf) return;
-    int4 ri = M.rowinfo[i];
-    V a = sum_partials<V>(reinterpret_cast<const V *>(M.xpart) + ri.x, ri.z, ri.y, 0);
-    d4 pv = M.pp[i];
-    pv.x += (double)a.x; pv.y += (double)a.y; pv.z += (double)a.z;
-    M.pp[i] = pv;
-}
-
 // ---- Jacobi-preconditioned CG on the correction system (alternative to the sweeps) -------
 //
 // Solves A d = r for the three coordinates at once (three independent CG recurrences, one
@@ -1592,11 +1597,6 @@ void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_e
     const size_t rec = sizeof(SubNode) * (size_t)nodes;
     if (fp64) hipLaunchKernelGGL(k_sub_bwd<double>, dim3(blocks), dim3(kTile), rec + sizeof(d4) * lds_entries, s, L, nodes);
     else hipLaunchKernelGGL(k_sub_bwd<float>, dim3(blocks), dim3(kTile), rec + sizeof(f4) * lds_entries, s, L, nodes);
-}
-void launch_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
-    if (blocks <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_apply<double>, dim3(blocks), dim3(kBlock), 0, s, L);
-    else hipLaunchKernelGGL(k_apply<float>, dim3(blocks), dim3(kBlock), 0, s, L);
 }
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
     if (blocks <= 0) return;

@@ -71,7 +71,7 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     plan.nlevels = 0;
     for (const Factor *f : factors) plan.nlevels = std::max(plan.nlevels, f->nlevels);
     int nl = plan.nlevels;
-    int ntables = 6 + 2 * nl;
+    int ntables = 7 + 2 * nl;
     plan.h_prefix.assign((size_t)ntables * (na + 1), 0);
     plan.h_tbase.assign((size_t)ntables * (na + 1), 0);
     plan.off_fwd.assign(nl, 0);
@@ -93,6 +93,8 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     table(3, plan.off_ident, plan.blk_ident, [&](int k) { return cdiv(meshes[k]->n, kBlock); });
     table(4, plan.off_apply, plan.blk_apply, [&](int k) { return cdiv(meshes[k]->nf, kBlock); });
     table(5, plan.off_sub, plan.blk_sub, [&](int k) { return factors[k]->nsub; });
+    table(6, plan.off_subb, plan.blk_subb,
+          [&](int k) { return factors[k]->nsub + cdiv(meshes[k]->nlrows, kTile); });
     plan.sub_lds_fwd = 0;
     plan.sub_lds_bwd = 0;
     plan.sub_nodes = 0;
@@ -102,11 +104,11 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
         plan.sub_nodes = std::max(plan.sub_nodes, f->sub_nodes);
     }
     for (int l = 0; l < nl; ++l) {
-        table(6 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
+        table(7 + 2 * l, plan.off_fwd[l], plan.blk_fwd[l], [&](int k) {
             const Factor *f = factors[k];
             return l < f->nlevels ? f->ft_ptr[l + 1] - f->ft_ptr[l] : 0;
         });
-        table(7 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
+        table(8 + 2 * l, plan.off_bwd[l], plan.blk_bwd[l], [&](int k) {
             const Factor *f = factors[k];
             return l < f->nlevels ? f->bt_ptr[l + 1] - f->bt_ptr[l] : 0;
         });
@@ -191,8 +193,8 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
         if (ev && r == 0) hipEventRecord(ev[3], stream);
         for (int l = plan.nlevels - 1; l >= 0; --l)
             launch_bwd(stream, plan.level_set(plan.off_bwd[l], false), plan.blk_bwd[l], l, plan.fp64);
-        launch_sub_bwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_bwd, plan.fp64);
-        launch_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
+        // subtrees, then p' += x (subtree rows inside their blocks, level rows in extra blocks)
+        launch_sub_bwd(stream, plan.set(plan.off_subb), plan.blk_subb, plan.sub_nodes, plan.sub_lds_bwd, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[4], stream);
     }
 }
@@ -566,6 +568,11 @@ int upload_layout(md_mesh *m) {
     size_t o_scp = A.reserve(sizeof(int) * std::max<size_t>(2, f.scp.size()));
     size_t o_sfc = A.reserve(sizeof(int) * std::max<size_t>(1, f.sfc.size()));
     size_t o_sbo = A.reserve(sizeof(int) * std::max<size_t>(2, f.sbo.size()));
+    std::vector<int> lrows;  // rows of level-swept supernodes
+    for (int k = 0; k < ns; ++k)
+        if (f.level[k] >= 0)
+            for (int q = 0; q < f.w[k]; ++q) lrows.push_back(f.c0[k] + q);
+    size_t o_lr = A.reserve(sizeof(int) * std::max<size_t>(1, lrows.size()));
     if (A.allocate() != hipSuccess) {
         set_error("device allocation of " + std::to_string(A.size()) + " bytes (sweep layout) failed");
         A.reset();
@@ -579,6 +586,9 @@ int upload_layout(md_mesh *m) {
     d.sub = (const int4 *)A.at(o_sub); d.sub_h = (const int *)A.at(o_subh);
     d.snode = (const int4 *)A.at(o_snode); d.scp = (const int2 *)A.at(o_scp);
     d.sfc = (const int *)A.at(o_sfc); d.sbo = (const int2 *)A.at(o_sbo);
+    d.nsub = f.nsub;
+    d.nlrows = (int)lrows.size();
+    d.lrows = (const int *)A.at(o_lr);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -611,6 +621,7 @@ int upload_layout(md_mesh *m) {
         chk(up(d.scp, f.scp.data(), sizeof(int) * f.scp.size()));
         chk(up(d.sfc, f.sfc.data(), sizeof(int) * f.sfc.size()));
         chk(up(d.sbo, f.sbo.data(), sizeof(int) * f.sbo.size()));
+        chk(up(d.lrows, lrows.data(), sizeof(int) * lrows.size()));
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
@@ -1342,13 +1353,16 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->bytes_local = slots * (4 + 3 * es) + (int64_t)t.n * (ps + vs);
     // residual: ids, edges and FP64 weights per slot, p' and rotation per vertex, residual out
     out->bytes_residual = free_slots * (4 + 3 * es + 8) + (int64_t)t.n * (ps + vs) + (int64_t)t.nf * vs;
-    int64_t gw = 0, dw = 0, upd = 0, xp = 0, offr = 0, xloc_reads = 0;
+    int64_t gw = 0, dw = 0, upd = 0, xp = 0, offr = 0, xloc_reads = 0, lrows = 0;
     for (int k = 0; k < f.ns; ++k) {
         gw += (int64_t)f.noff[k] * f.w[k];
         dw += (int64_t)f.w[k] * f.w[k];
         upd += (int64_t)f.kc[k] * f.noff[k];
-        xp += (int64_t)f.kr[k] * f.w[k];
         offr += f.noff[k];
+        if (f.level[k] >= 0) {  // subtree rows' x stays in LDS and goes straight into p'
+            xp += (int64_t)f.kr[k] * f.w[k];
+            lrows += f.w[k];
+        }
     }
     // solution partials gathered from HBM: off rows of level-swept supernodes, and off rows
     // of subtree supernodes that lie outside their subtree (the others are read from LDS)
@@ -1366,9 +1380,10 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     const int64_t lvl_fwd_idx = 4 * (int64_t)(f.cpos.size() + f.cmore.size());
     out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + lvl_fwd_idx + sub_fwd_idx;
     // backward + update: Dinv and G panels, z in, x partials of ancestors gathered (with
-    // their locators), x partials out, final sum and the p' update
+    // their locators), level rows' x partials out and summed (row list, row locators), the
+    // p' update
     out->bytes_backward = (dw + gw) * es + (int64_t)t.nf * vs + xloc_reads * vs + offr * 16 + 2 * xp * vs +
-                          (int64_t)t.nf * (16 + 2 * ps) + sub_bwd_idx;
+                          lrows * (4 + 16) + (int64_t)t.nf * 2 * ps + sub_bwd_idx;
     return MD_OK;
 }
 

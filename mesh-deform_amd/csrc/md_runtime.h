@@ -48,6 +48,7 @@ struct Plan {
     int nactive = 0;
     int nlevels = 0;
     int blk_face = 0, blk_slice = 0, blk_resid = 0, blk_ident = 0, blk_apply = 0, blk_sub = 0;
+    int blk_subb = 0, off_subb = 0;    // k_sub_bwd: subtrees, then the p' update of level rows
     std::vector<int> blk_fwd, blk_bwd;
     int off_face = 0, off_slice = 0, off_resid = 0, off_ident = 0, off_apply = 0, off_sub = 0;
     int sub_lds_fwd = 0, sub_lds_bwd = 0;  // LDS vector entries of the largest bottom subtree
