@@ -615,6 +615,8 @@ void device_layout(Factor& f, int sub_budget) {
     // records by height, contribution entries, backward off-row sources
     f.nsub = 0; f.sub_lds_fwd = 0; f.sub_lds_bwd = 0; f.sub_nodes = 0;
     f.sub.clear(); f.sub_h.clear(); f.snode.clear(); f.scp.clear(); f.sfc.clear(); f.sbo.clear();
+    f.srow.clear();
+    f.sfi.assign(n, -1);
     std::vector<int> fb(nn, 0), fbb(nn, 0);
     for (int r = 0; r < nn; ++r) {
         if (sub_root[r] != r) continue;
@@ -626,6 +628,10 @@ void device_layout(Factor& f, int sub_budget) {
         int fronts = 0, height_fronts = 0;
         for (int t = first; t <= r; ++t) { fb[t] = fronts; fronts += f.w[t] + f.noff[t]; }
         const int ncols = col1 - col0;
+        f.srow.push_back(col0);
+        f.srow.push_back(ncols);
+        for (int t = first; t <= r; ++t)
+            for (int q = 0; q < f.w[t]; ++q) f.sfi[f.c0[t] + q] = fb[t] + q;
         for (int h = 0; h < nh; ++h) {
             int off = ncols;
             for (int t : at[h]) { fbb[t] = off; off += f.w[t] + f.noff[t]; }

@@ -136,6 +136,8 @@ struct Factor {
     std::vector<int> scp;            // 2 per forward position: first two child entries (see above)
     std::vector<int> sfc;            // -1-terminated overflow lists of child entries
     std::vector<int> sbo;            // 2 per off-row position of a subtree supernode (backward source)
+    std::vector<int> srow;           // 2 per subtree: first row, rows
+    std::vector<int> sfi;            // per row (n): forward front slot of a subtree row, else -1
     int64_t nnz_exact = 0;           // strictly-lower nonzeros of the exact factor (no amalgamation zeros)
     int64_t factor_nnz() const { return poff.empty() ? 0 : poff.back(); }
 };

@@ -65,8 +65,11 @@ struct MeshDev {
     const int *sfc;             // -1-terminated overflow lists of child update entries
     const int2 *sbo;            // backward off-row sources
     int nsub;                   // bottom subtrees (their rows' x is added to p' by k_sub_bwd)
+    const int2 *srow;           // per subtree: first row, rows (their residual is formed in LDS)
+    const int *sfi;             // per free row of a subtree: its forward front slot in LDS
     int nlrows;                 // rows of level-swept supernodes, increasing
-    const int *lrows;           //   (x summed from partials and added to p' by k_sub_bwd too)
+    const int *lrows;           //   (their residual: extra blocks of k_sub_fwd; x summed from the
+                                //   partials and added to p': extra blocks of k_sub_bwd)
     double *energy_part;        // one partial sum per 256 vertices
     // Jacobi-PCG alternative to the triangular sweeps (solve type vectors, nf entries)
     float *w32;                 // slots: cotan weights in FP32
@@ -118,7 +121,9 @@ void launch_fwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64);
 // Bottom-subtree sweeps: one 64-lane block per subtree; LDS for `nodes` supernode records
 // and `lds_entries` vector entries.
-void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64);
+// fused: the launch also forms the residual (one-mesh plans; blocks past the subtrees take
+// the level rows), else k_residual ran before it.
+void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fused, bool fp64);
 void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);

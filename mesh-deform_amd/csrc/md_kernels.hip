@@ -282,16 +282,11 @@ __global__ __launch_bounds__(kBlock) void k_identity_quat(LaunchSet L) {
 // b_i - (L p')_i for the reference's b = bFixed + sum_j w_ij/2 (R_i + R_j)(p_i - p_j)
 // (arap.h:395-414) and L of arap.h:325-332, so the solve returns the correction to p'.
 // The rotation term is in T, the weight term in FP64.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
+// r_i of one free vertex, slots kU at a time (ids and edges, then the neighbour gathers).
+template <typename T, int kU>
+__device__ __forceinline__ typename V4T<T>::type residual_of(const MeshDev &M, int iv) {
     using V = typename V4T<T>::type;
-    int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
-    int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
-    int lane = threadIdx.x & 63;
-    if (slice >= M.nslices_free) return;
-    int iv = slice * kSlice + lane;
-    if (iv >= M.nf) return;
+    const int slice = iv / kSlice, lane = iv % kSlice;
     const T *A = aplanes<T>(M);
     const V *Q = reinterpret_cast<const V *>(M.quat);
     int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
@@ -300,7 +295,6 @@ __global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
     T sax = 0, say = 0, saz = 0;   // sum_j a_ij
     T rx = 0, ry = 0, rz = 0;      // sum_j R_j a_ij
     double lx = 0, ly = 0, lz = 0; // sum_j w_ij (p'_i - p'_j)
-    constexpr int kU = 8;
     for (int q0 = 0; q0 < wd; q0 += kU) {
         int jj[kU];
         T ax_[kU], ay_[kU], az_[kU];
@@ -342,7 +336,20 @@ __global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
     out.y = (T)((double)by - ly);
     out.z = (T)((double)bz - lz);
     out.w = 0;
-    reinterpret_cast<V *>(M.r)[iv] = out;
+    return out;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
+    using V = typename V4T<T>::type;
+    int k = find_mesh(L, blockIdx.x);
+    const MeshDev &M = L.meshes[k];
+    int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (slice >= M.nslices_free) return;
+    int iv = slice * kSlice + lane;
+    if (iv >= M.nf) return;
+    reinterpret_cast<V *>(M.r)[iv] = residual_of<T, 8>(M, iv);
 }
 
 // ---- block LDL^T sweeps (arap.h:416-429) --------------------------------------------------
@@ -699,7 +706,10 @@ __device__ __forceinline__ void sub_records(const MeshDev &M, const int4 &sd, Su
 //   positions: front[p] = (p < w ? r[c0 + p] : 0) + children's update entries; z = front[p < w],
 //   off rows:  front[w + i] -= G_s[i, 0..w) front[0 .. w)  (u_s in place; the root also stores
 //              u_s for its parent outside the subtree).
-template <typename T>
+// With Fused (one-mesh launches), the residual is formed here: subtree rows straight into
+// their front positions, level rows by blocks past the subtrees (no k_residual launch; more
+// registers, so batches keep the separate residual).
+template <typename T, bool Fused>
 __global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L, int node_cap) {
     using V = typename V4T<T>::type;
     extern __shared__ __align__(32) unsigned char sub_smem[];
@@ -707,13 +717,29 @@ __global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L, int node_cap) {
     V *F = reinterpret_cast<V *>(sub_smem + sizeof(SubNode) * node_cap);
     const int k = find_mesh(L, blockIdx.x);
     const MeshDev &M = L.meshes[k];
-    const int4 sd = M.sub[blockIdx.x - L.prefix[k]];
     const int lane = threadIdx.x;
+    const int bk = blockIdx.x - L.prefix[k];
+    if (Fused && bk >= M.nsub) {
+        // blocks past the subtrees: the residual of 64 rows of level supernodes
+        const int j = (bk - M.nsub) * kTile + lane;
+        if (j < M.nlrows) {
+            const int i = M.lrows[j];
+            reinterpret_cast<V *>(M.r)[i] = residual_of<T, 8>(M, i);
+        }
+        return;
+    }
+    const int4 sd = M.sub[bk];
     const V *R = reinterpret_cast<const V *>(M.r);
     V *Z = reinterpret_cast<V *>(M.z);
     V *U = reinterpret_cast<V *>(M.upart);
     const T *G = tptr<T>(M.pc);
-    sub_records(M, sd, N);
+    if (Fused) {
+        // the residual of the subtree's rows, straight into their front positions (it is
+        // read by nothing else)
+        const int2 rr = M.srow[bk];
+        for (int t = lane; t < rr.y; t += kTile) F[M.sfi[rr.x + t]] = residual_of<T, 8>(M, rr.x + t);
+    }
+    sub_records(M, sd, N);  // ends with a barrier
     for (int h = 0; h < sd.w; ++h) {
         const int a = M.sub_h[sd.z + h] - sd.x, b = M.sub_h[sd.z + h + 1] - sd.x;
         const int npos = N[b - 1].pa + N[b - 1].w + N[b - 1].noff;
@@ -722,7 +748,7 @@ __global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L, int node_cap) {
             const int p = t - S.pa;
             V v;
             v.x = 0; v.y = 0; v.z = 0; v.w = 0;
-            if (p < S.w) v = R[S.c0 + p];
+            if (p < S.w) v = Fused ? F[S.fb + p] : R[S.c0 + p];  // the residual
             // children's update entries: two inline, the rest from a -1-terminated list
             const int2 e = M.scp[S.cp + p];
             if (e.x >= 0) {
@@ -1586,11 +1612,17 @@ void launch_bwd(hipStream_t s, LaunchSet L, int blocks, int level, bool fp64) {
         else hipLaunchKernelGGL((k_bwd<float, false>), dim3(blocks), dim3(kTile), 0, s, L, level);
     }
 }
-void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64) {
+void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fused, bool fp64) {
     if (blocks <= 0) return;
     const size_t rec = sizeof(SubNode) * (size_t)nodes;
-    if (fp64) hipLaunchKernelGGL(k_sub_fwd<double>, dim3(blocks), dim3(kTile), rec + sizeof(d4) * lds_entries, s, L, nodes);
-    else hipLaunchKernelGGL(k_sub_fwd<float>, dim3(blocks), dim3(kTile), rec + sizeof(f4) * lds_entries, s, L, nodes);
+    const size_t lds = rec + (fp64 ? sizeof(d4) : sizeof(f4)) * lds_entries;
+    if (fused) {
+        if (fp64) hipLaunchKernelGGL((k_sub_fwd<double, true>), dim3(blocks), dim3(kTile), lds, s, L, nodes);
+        else hipLaunchKernelGGL((k_sub_fwd<float, true>), dim3(blocks), dim3(kTile), lds, s, L, nodes);
+    } else {
+        if (fp64) hipLaunchKernelGGL((k_sub_fwd<double, false>), dim3(blocks), dim3(kTile), lds, s, L, nodes);
+        else hipLaunchKernelGGL((k_sub_fwd<float, false>), dim3(blocks), dim3(kTile), lds, s, L, nodes);
+    }
 }
 void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64) {
     if (blocks <= 0) return;

@@ -167,14 +167,19 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     return sync ? hipStreamSynchronize(stream) : hipSuccess;
 }
 
+// One-mesh plans with at most this many bottom subtrees form the residual inside the subtree
+// forward launch: its registers allow 3 waves per SIMD, and 3 x 1024 SIMDs of a 256-CU device
+// then hold every subtree at once (more subtrees would queue behind the longer waves).
+constexpr int kFusedResidualMaxSubtrees = 3072;
+
 void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
     if (ev) hipEventRecord(ev[0], stream);
     launch_local(stream, plan.set(plan.off_slice), plan.blk_slice, plan.fp64);
     if (ev) hipEventRecord(ev[1], stream);
     for (int r = 0; r < plan.refine_steps; ++r) {
-        launch_residual(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
-        if (ev && r == 0) hipEventRecord(ev[2], stream);
         if (plan.pcg) {
+            launch_residual(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
+            if (ev && r == 0) hipEventRecord(ev[2], stream);
             // Jacobi-PCG on the correction: init, K x (A p, update, direction), apply
             launch_pcg_init(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
             for (int k = 0; k < plan.pcg_iters; ++k) {
@@ -187,7 +192,18 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
             if (ev && r == 0) hipEventRecord(ev[4], stream);
             continue;
         }
-        launch_sub_fwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_fwd, plan.fp64);
+        // one-mesh plans form the residual inside the subtree forward launch (subtree rows in
+        // LDS, level rows by extra blocks): its phase is then empty and the forward phase
+        // holds it; batches and server mixes keep k_residual (occupancy of the subtree sweep)
+        const bool fused = plan.nactive == 1 && plan.blk_sub <= kFusedResidualMaxSubtrees;
+        if (!fused) launch_residual(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
+        if (ev && r == 0) hipEventRecord(ev[2], stream);
+        if (fused)
+            launch_sub_fwd(stream, plan.set(plan.off_subb), plan.blk_subb, plan.sub_nodes, plan.sub_lds_fwd, true,
+                           plan.fp64);
+        else
+            launch_sub_fwd(stream, plan.set(plan.off_sub), plan.blk_sub, plan.sub_nodes, plan.sub_lds_fwd, false,
+                           plan.fp64);
         for (int l = 0; l < plan.nlevels; ++l)
             launch_fwd(stream, plan.level_set(plan.off_fwd[l], true), plan.blk_fwd[l], l, plan.fp64);
         if (ev && r == 0) hipEventRecord(ev[3], stream);
@@ -573,6 +589,8 @@ int upload_layout(md_mesh *m) {
         if (f.level[k] >= 0)
             for (int q = 0; q < f.w[k]; ++q) lrows.push_back(f.c0[k] + q);
     size_t o_lr = A.reserve(sizeof(int) * std::max<size_t>(1, lrows.size()));
+    size_t o_srow = A.reserve(sizeof(int) * std::max<size_t>(2, f.srow.size()));
+    size_t o_sfi = A.reserve(sizeof(int) * std::max<size_t>(1, f.sfi.size()));
     if (A.allocate() != hipSuccess) {
         set_error("device allocation of " + std::to_string(A.size()) + " bytes (sweep layout) failed");
         A.reset();
@@ -589,6 +607,8 @@ int upload_layout(md_mesh *m) {
     d.nsub = f.nsub;
     d.nlrows = (int)lrows.size();
     d.lrows = (const int *)A.at(o_lr);
+    d.srow = (const int2 *)A.at(o_srow);
+    d.sfi = (const int *)A.at(o_sfi);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -622,6 +642,8 @@ int upload_layout(md_mesh *m) {
         chk(up(d.sfc, f.sfc.data(), sizeof(int) * f.sfc.size()));
         chk(up(d.sbo, f.sbo.data(), sizeof(int) * f.sbo.size()));
         chk(up(d.lrows, lrows.data(), sizeof(int) * lrows.size()));
+        chk(up(d.srow, f.srow.data(), sizeof(int) * f.srow.size()));
+        chk(up(d.sfi, f.sfi.data(), sizeof(int) * f.sfi.size()));
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
