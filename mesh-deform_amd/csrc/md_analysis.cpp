@@ -1086,7 +1086,17 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
         MD_CHECK(f.sub[4 * (size_t)i + 1] - f.sub[4 * (size_t)i] <= f.sub_nodes, "subtree records");
         lds.assign(3 * (size_t)entries, 0.0);
         stamp.assign(entries, 0);
-        int phase = 0;
+        int phase = 1;
+        // the one-mesh form first writes the residual of the subtree's rows into their front
+        // positions (sfi); the gathers below read it from there
+        MD_CHECK(2 * (size_t)i + 1 < f.srow.size(), "subtree row range");
+        const int col0 = f.srow[2 * (size_t)i], ncols = f.srow[2 * (size_t)i + 1];
+        MD_CHECK(col0 >= 0 && ncols > 0 && (size_t)col0 + ncols <= f.sfi.size(), "subtree rows");
+        for (int t = 0; t < ncols; ++t) {
+            const int e = f.sfi[(size_t)col0 + t];
+            MD_CHECK(lds_claim(e, entries, phase), "subtree residual write");
+            for (int c = 0; c < 3; ++c) lds[3 * (size_t)e + c] = b[(size_t)c * n + col0 + t];
+        }
         for (int h = 0; h < nh; ++h) {
             int ra, rb;
             MD_CHECK(sub_heights(i, h, ra, rb), "subtree height range");
@@ -1115,8 +1125,9 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                 double v[3] = {0, 0, 0};
                 for (int c = 0; c < 3; ++c) {
                     if (p < w) {
-                        MD_CHECK(R[0] + p < n, "subtree rhs index");
-                        v[c] = b[(size_t)c * n + R[0] + p];
+                        MD_CHECK(R[0] + p < n && f.sfi[R[0] + p] == R[3] + p, "subtree residual slot");
+                        MD_CHECK(lds_read(R[3] + p, entries, phase, c, v[c]), "subtree residual read");
+                        MD_CHECK(v[c] == b[(size_t)c * n + R[0] + p], "subtree residual value");
                     }
                     for (int e : src) {
                         double u;
@@ -1269,7 +1280,13 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
         MD_CHECK(entries > 0 && entries <= f.sub_lds_bwd, "subtree extent (backward)");
         lds.assign(3 * (size_t)entries, 0.0);
         stamp.assign(entries, 0);
-        int phase = 0;
+        // the column region starts with z of the subtree's rows; each row's x replaces it
+        int phase = 1;
+        const int col0 = f.srow[2 * (size_t)i], ncols = f.srow[2 * (size_t)i + 1];
+        for (int t = 0; t < ncols; ++t) {
+            MD_CHECK(lds_claim(t, entries, phase), "subtree z write");
+            for (int c = 0; c < 3; ++c) lds[3 * (size_t)t + c] = z[3 * (size_t)(col0 + t) + c];
+        }
         for (int h = nh - 1; h >= 0; --h) {
             int ra, rb;
             MD_CHECK(sub_heights(i, h, ra, rb), "subtree height range");
@@ -1281,7 +1298,8 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                 const int p = t - R[11], w = R[1];
                 double v[3];
                 if (p < w) {
-                    for (int c = 0; c < 3; ++c) v[c] = z[3 * (size_t)(R[0] + p) + c];
+                    MD_CHECK(R[10] + p < ncols && col0 + R[10] + p == R[0] + p, "subtree column slot");
+                    for (int c = 0; c < 3; ++c) MD_CHECK(lds_read(R[10] + p, entries, phase, c, v[c]), "subtree z read");
                 } else {
                     const size_t e = (size_t)R[7] + p - w;
                     MD_CHECK(R[7] >= 0 && 2 * e + 1 < f.sbo.size(), "subtree off-row source");

@@ -822,9 +822,15 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
         return;
     }
     const int4 sd = M.sub[bk];
-    const V *Zs = reinterpret_cast<const V *>(M.z);
     const T *P = tptr<T>(M.pr);
-    sub_records(M, sd, N);
+    // the column region F[0 .. ncols) starts with z of the subtree's rows (read at the row's
+    // height, then replaced by its x)
+    const int2 rr = M.srow[bk];
+    {
+        const V *Zs = reinterpret_cast<const V *>(M.z);
+        for (int t = lane; t < rr.y; t += kTile) F[t] = Zs[rr.x + t];
+    }
+    sub_records(M, sd, N);  // ends with a barrier
     for (int h = sd.w - 1; h >= 0; --h) {
         const int a = M.sub_h[sd.z + h] - sd.x, b = M.sub_h[sd.z + h + 1] - sd.x;
         const int npos = N[b - 1].pa + N[b - 1].w + N[b - 1].noff;
@@ -833,7 +839,7 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
             const int p = t - S.pa;
             V v;
             if (p < S.w) {
-                v = Zs[S.c0 + p];
+                v = F[S.xs + p];
             } else {
                 const int2 d = M.sbo[S.ob + p - S.w];
                 // x of the row: from LDS, or the owner's partials loaded 8 at a time
@@ -863,14 +869,16 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
                 V o;
                 o.x = ax; o.y = ay; o.z = az; o.w = 0;
                 F[S.xs + q] = o;
-                // x of a subtree row is final here and read by nothing outside the subtree:
-                // p' += x directly
-                d4 pv = M.pp[S.c0 + q];
-                pv.x += (double)o.x; pv.y += (double)o.y; pv.z += (double)o.z;
-                M.pp[S.c0 + q] = pv;
             }
         }
         __syncthreads();
+    }
+    // x of the subtree's rows is final and read by nothing outside the subtree: p' += x
+    for (int t = lane; t < rr.y; t += kTile) {
+        const V o = F[t];
+        d4 pv = M.pp[rr.x + t];
+        pv.x += (double)o.x; pv.y += (double)o.y; pv.z += (double)o.z;
+        M.pp[rr.x + t] = pv;
     }
 }
 
