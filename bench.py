@@ -424,9 +424,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None if traffic is None else int(traffic),
-                     "kernel": "one ARAP iteration = one hipGraph launch (k_local, k_residual, k_sub_fwd, "
-                               "k_fwd per level, k_bwd per level, k_sub_bwd, k_apply), HIP events on the "
-                               "library stream",
+                     "kernel": "one ARAP iteration = one hipGraph launch (k_local, k_sub_fwd with the "
+                               "residual, k_fwd per level, k_bwd per level, k_sub_bwd with the p' update), "
+                               "HIP events on the library stream",
                      "algorithmic_bytes": cb["B_iter"], "ms_per_launch": round(ev_step_ms, 5),
                      "canonical": cb,
                      "counted_bytes": {"iteration": int(iter_bytes), "sweeps": int(solve_bytes),
