@@ -516,7 +516,7 @@ void device_layout(Factor& f, int sub_budget) {
     }
     // device layout: tiles, partial vectors, bucket pointers, row locators
     f.pc_off.assign(nn + 1, 0); f.pr_off.assign(nn + 1, 0);
-    f.uoff.assign(nn + 1, 0); f.xoff.assign(nn + 1, 0);
+    f.xoff.assign(nn + 1, 0);
     f.kc.resize(nn); f.kr.resize(nn);
     for (int t = 0; t < nn; ++t) {
         int m = f.w[t] + f.noff[t];
@@ -525,47 +525,32 @@ void device_layout(Factor& f, int sub_budget) {
         f.kr[t] = in_sub ? 1 : cdiv(m, kTile);
         f.pc_off[t + 1] = f.pc_off[t] + (int64_t)f.noff[t] * f.w[t];
         f.pr_off[t + 1] = f.pr_off[t] + (int64_t)m * f.w[t];
-        f.uoff[t + 1] = f.uoff[t] + (int64_t)f.kc[t] * f.noff[t];
         f.xoff[t + 1] = f.xoff[t] + (int64_t)f.kr[t] * f.w[t];
     }
-    // contributions of the children's update entries to each position of a front, in child
-    // order: the forward gather pulls them per position (no write conflicts). The first two are
-    // inline in the position's record, the rest in a list ended by a zero count.
-    f.cp_off.assign(nn, 0);
-    f.cpos.clear();
-    f.cmore.clear();
+    // update slots: per level front position as many slots as its largest count of child
+    // partials; each child entry's partials go to consecutive slots in child order
+    f.ubase.assign(nn + 1, 0);
+    f.uk.assign(nn, 0);
+    f.udst.assign(f.offrows.size(), -1);
     {
-        std::vector<std::vector<std::pair<int, int>>> lists;  // per position: (child, entry)
+        std::vector<int> cnt;
         for (int t = 0; t < nn; ++t) {
-            if (sub_root[t] >= 0) { f.cp_off[t] = 0; continue; }  // swept by its subtree
-            int m = f.w[t] + f.noff[t];
-            lists.assign(m, {});
+            f.ubase[t + 1] = f.ubase[t];
+            if (sub_root[t] >= 0) continue;  // swept by its subtree
+            const int m = f.w[t] + f.noff[t];
+            cnt.assign(m, 0);
             for (int k = f.child_ptr[t]; k < f.child_ptr[t + 1]; ++k) {
-                int c = f.child[k];
-                for (int e = 0; e < f.noff[c]; ++e) lists[f.rel[f.roff[c] + e]].emplace_back(c, e);
-            }
-            f.cp_off[t] = (int)(f.cpos.size() / 8);
-            for (int p = 0; p < m; ++p) {
-                const auto& L = lists[p];
-                int rec[8] = {-1, 0, 0, -1, -1, 0, 0, 0};
-                for (size_t i = 0; i < std::min<size_t>(2, L.size()); ++i) {
-                    const int c = L[i].first;
-                    rec[4 * i] = (int)(f.uoff[c] + L[i].second);
-                    rec[4 * i + 1] = f.noff[c];
-                    rec[4 * i + 2] = f.kc[c];
+                const int c = f.child[k];
+                for (int e = 0; e < f.noff[c]; ++e) {
+                    const int p = f.rel[f.roff[c] + e];
+                    f.udst[f.roff[c] + e] = f.ubase[t] + (int64_t)cnt[p] * m + p;
+                    cnt[p] += f.kc[c];
                 }
-                if (L.size() > 2) {
-                    rec[3] = (int)(f.cmore.size() / 4);
-                    for (size_t i = 2; i < L.size(); ++i) {
-                        int c = L[i].first, e = L[i].second;
-                        int ent[4] = {(int)(f.uoff[c] + e), f.noff[c], f.kc[c], 0};
-                        f.cmore.insert(f.cmore.end(), ent, ent + 4);
-                    }
-                    int end[4] = {0, 0, 0, 0};
-                    f.cmore.insert(f.cmore.end(), end, end + 4);
-                }
-                f.cpos.insert(f.cpos.end(), rec, rec + 8);
             }
+            int K = 0;
+            for (int p = 0; p < m; ++p) K = std::max(K, cnt[p]);
+            f.uk[t] = K;
+            f.ubase[t + 1] = f.ubase[t] + (int64_t)K * m;
         }
     }
     f.rowinfo.assign(4 * (size_t)n, 0);
@@ -654,7 +639,7 @@ void device_layout(Factor& f, int sub_budget) {
             for (int t : at[h]) {
                 const int w = f.w[t], no = f.noff[t], m = w + no;
                 int rec[16] = {f.c0[t], w, no, fb[t], (int)f.pc_off[t], (int)f.pr_off[t], (int)(f.scp.size() / 2),
-                               (int)(f.sbo.size() / 2), t == r ? (int)f.uoff[t] : -1, (int)f.xoff[t],
+                               (int)(f.sbo.size() / 2), t == r ? f.roff[t] : -1, (int)f.xoff[t],
                                f.c0[t] - col0, pa, pb, pq, fbb[t], 0};
                 f.snode.insert(f.snode.end(), rec, rec + 16);
                 pa += m; pb += no; pq += w;
@@ -1004,7 +989,7 @@ void solve_host(const Factor& f, const double* b, double* x, int nrhs) {
 // buffer it reads or writes. Returns false on an out-of-range access.
 bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& err) {
     const int n = f.n;
-    const int64_t nu = f.uoff.empty() ? 0 : f.uoff.back(), nx = f.xoff.empty() ? 0 : f.xoff.back();
+    const int64_t nu = f.ubase.empty() ? 0 : f.ubase.back(), nx = f.xoff.empty() ? 0 : f.xoff.back();
     std::vector<double> z(3 * (size_t)n, 0.0), up(3 * (size_t)nu, 0.0), xp(3 * (size_t)nx, 0.0);
     std::vector<double> pc(f.pc_off.back()), pr(f.pr_off.back());
     for (int s = 0; s < f.ns; ++s) {
@@ -1159,38 +1144,30 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                 const int dst = R[3] + w + i_;
                 for (int c = 0; c < 3; ++c) MD_CHECK(lds_read(dst, entries, phase, c, o[c]), "subtree off-row read");
                 MD_CHECK(lds_claim(dst, entries, phase), "subtree update write");
-                MD_CHECK(R[8] < 0 || R[8] + i_ < nu, "subtree update output");
+                int64_t udst = -1;
+                if (R[8] >= 0) {  // the root: its update entry goes to the parent's slot
+                    MD_CHECK((size_t)R[8] + i_ < f.udst.size(), "subtree update entry");
+                    udst = f.udst[R[8] + i_];
+                    MD_CHECK(udst >= 0 && udst < nu, "subtree update output");
+                }
                 for (int c = 0; c < 3; ++c) {
                     double u = o[c] - acc[c];
                     lds[3 * (size_t)dst + c] = u;
-                    if (R[8] >= 0) up[3 * (size_t)(R[8] + i_) + c] = u;
+                    if (udst >= 0) up[3 * (size_t)udst + c] = u;
                 }
             }
         }
     }
-    // children's contributions to position pos of level supernode s, in the kernels' order
+    // children's contributions to position pos of level supernode s: the sum of its uk slots
+    // (unused ones zero), in slot order as the kernels sum them
     auto pull_at = [&](int s, int pos, double* acc3) -> bool {
-        size_t r = 8 * ((size_t)f.cp_off[s] + pos);
-        if (r + 7 >= f.cpos.size()) return false;
-        const int* rec = f.cpos.data() + r;
-        auto add = [&](const int* ce) -> bool {
-            for (int c = 0; c < 3; ++c) {
-                double v;
-                if (!sum_part(up, ce[0], ce[2], ce[1], 0, c, v)) return false;
-                acc3[c] += v;
-            }
-            return true;
-        };
-        if (rec[2] > 0 && !add(rec)) return false;
-        if (rec[6] > 0 && (rec[2] <= 0 || !add(rec + 4))) return false;  // second follows a first
-        if (rec[3] >= 0) {
-            if (rec[6] <= 0) return false;  // overflow entries follow the inline two
-            for (size_t q = (size_t)rec[3];; ++q) {
-                if (4 * q + 3 >= f.cmore.size()) return false;
-                const int* ce = f.cmore.data() + 4 * q;
-                if (ce[2] == 0) break;
-                if (!add(ce)) return false;
-            }
+        const int m = f.w[s] + f.noff[s];
+        if (pos < 0 || pos >= m || f.uk[s] < 0) return false;
+        if (f.ubase[s] + (int64_t)f.uk[s] * m > f.ubase[s + 1]) return false;
+        for (int c = 0; c < 3; ++c) {
+            double v;
+            if (!sum_part(up, f.ubase[s], f.uk[s], m, pos, c, v)) return false;
+            acc3[c] += v;
         }
         return true;
     };
@@ -1226,8 +1203,10 @@ bool solve_tiles_host(const Factor& f, const double* b, double* x, std::string& 
                     MD_CHECK(idx < f.pc_off[s + 1], "forward panel");
                     for (int c = 0; c < 3; ++c) acc[c] += pc[idx] * vd[c][i];
                 }
-                int64_t o = f.uoff[s] + (int64_t)kcb * no + r0 + lane;
-                MD_CHECK(o < f.uoff[s + 1], "update output");
+                const int par = f.parent[s];
+                MD_CHECK(par >= 0 && (size_t)f.roff[s] + r0 + lane < f.udst.size(), "update destination");
+                const int64_t o = f.udst[f.roff[s] + r0 + lane] + (int64_t)kcb * (f.w[par] + f.noff[par]);
+                MD_CHECK(f.udst[f.roff[s] + r0 + lane] >= f.ubase[par] && o < f.ubase[par + 1], "update output");
                 for (int c = 0; c < 3; ++c) up[3 * o + c] = (kcb == 0 ? vo[c][lane] : 0.0) - acc[c];
             }
         }

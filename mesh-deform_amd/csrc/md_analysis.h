@@ -113,13 +113,15 @@ struct Factor {
     std::vector<int> kc, kr;         // partial counts of u_s and x_s
     std::vector<int64_t> pc_off;     // ns + 1: column-major G_s (noff x w)
     std::vector<int64_t> pr_off;     // ns + 1: row-major [Dinv_s; G_s] (m x w)
-    std::vector<int64_t> uoff;       // ns + 1: kc * noff partial update entries
+    // update slots of the forward sweep: a level supernode's front has uk slots per position
+    // (slot k of position p at ubase + k * m + p, k-major); the children's partial updates for
+    // p fill consecutive slots in child order (kc each), the others stay zero, so a front sums
+    // its slots at addresses known from its tile alone
+    std::vector<int64_t> ubase;      // ns + 1: first slot of each supernode's front
+    std::vector<int> uk;             // per supernode: slots per front position
+    std::vector<int64_t> udst;       // per off-row entry of a level supernode or subtree root:
+                                     //   its slot in the parent's front (partial cb: + cb * m_parent)
     std::vector<int64_t> xoff;       // ns + 1: kr * w partial solution entries
-    std::vector<int> cp_off;         // per supernode: its first position record
-    std::vector<int> cpos;           // 8 per front position: first contribution (child uoff + e,
-                                     //   child noff, child kc; count 0: none, then the cmore index
-                                     //   of the third onward or -1), second contribution (same, 0)
-    std::vector<int> cmore;          // 4 per further contribution (same fields, 0), lists end with count 0
     std::vector<int> rowinfo;        // 4 per row: first partial, stride, count, 0
     std::vector<int> xloc;           // 4 per off row entry: rowinfo of that row
     // tiles grouped by height level: level l occupies [ft_ptr[l], ft_ptr[l+1]) (4 ints each)

@@ -16,9 +16,10 @@ template <> struct V4T<double> { using type = d4; };
 // Per-supernode record of the block LDL^T sweeps (see Factor in md_analysis.h).
 struct SnDev {
     int c0, w, noff, roff;     // columns [c0, c0+w), off rows offrows[roff .. roff+noff)
-    int kc, kr, cp_off, pad;   // partial counts of u_s / x_s, contribution pointers
+    int kc, kr, uk, dstride;   // partial counts of u_s / x_s, update slots per front position,
+                               //   the parent's front size (stride of u_s's partials in its slots)
     long long pc_off, pr_off;  // column-major G_s, row-major [Dinv_s; G_s]
-    long long uoff, xoff;      // partial update / partial solution entries
+    long long ubase, xoff;     // first update slot of the front / partial solution entries
 };
 
 // One level tile of a sweep: the supernode's record and the tile origin, so a tile is one load.
@@ -47,12 +48,10 @@ struct MeshDev {
     d4 *pp;                     // current positions p'
     void *quat;                 // rotations as quaternions (x, y, z, w=scalar): f4 or d4
     void *r, *z;                // residual and forward result (V4 of the solve type), nf each
-    void *upart, *xpart;        // partial update / solution vectors (V4)
+    void *upart, *xpart;        // update slots of the level fronts / partial solutions (V4)
     const int *child_ptr, *child;
-    const int4 *cpos;           // per front position, two records: the first contribution (child
-                                //   partial base, stride, count; count 0: none; index of the third
-                                //   onward in cmore or -1) and the second (same fields, 0)
-    const int4 *cmore;          // further contributions, lists ended by a zero count
+    const long long *udst;      // per off-row entry of a level supernode or subtree root: its
+                                //   update slot in the parent's front (partial cb: + cb * dstride)
     const int4 *rowinfo;        // per row: (first partial, stride, count, 0)
     const int4 *xloc;           // per off row entry: rowinfo of that row
     const void *pc, *pr;        // panels (float or double), forward and backward layouts

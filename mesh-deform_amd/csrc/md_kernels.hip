@@ -385,31 +385,12 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
     return a;
 }
 
-// Children's contributions to a position of a front, summed in child order: the first two from
-// the position's records r, r2 (M.cpos), the rest from its overflow list.
+// Children's partial updates at position pos of a level front: the sum of its uk slots (slot k
+// at k * m + pos; the children's partials fill them in child order, unused ones are zero), so
+// their addresses follow from the tile alone.
 template <typename V>
-__device__ __forceinline__ V pull_record(const MeshDev &M, const int4 r, const int4 r2) {
-    const V *U = reinterpret_cast<const V *>(M.upart);
-    V a, b;
-    a.x = 0; a.y = 0; a.z = 0; a.w = 0;
-    b = a;
-    if (r.z > 0) a = sum_partials<V>(U + r.x, r.z, r.y, 0);
-    if (r2.z > 0) b = sum_partials<V>(U + r2.x, r2.z, r2.y, 0);
-    a.x += b.x; a.y += b.y; a.z += b.z;
-    if (r.w >= 0)
-        for (int q = r.w;; ++q) {
-            const int4 ce = M.cmore[q];
-            if (ce.z == 0) break;
-            const V u = sum_partials<V>(U + ce.x, ce.z, ce.y, 0);
-            a.x += u.x; a.y += u.y; a.z += u.z;
-        }
-    return a;
-}
-
-template <typename V>
-__device__ __forceinline__ V pull_contributions(const MeshDev &M, const SnDev &S, int pos) {
-    const int4 *rec = M.cpos + 2 * ((long long)S.cp_off + pos);
-    return pull_record<V>(M, rec[0], rec[1]);
+__device__ __forceinline__ V pull_slots(const MeshDev &M, const SnDev &S, int pos) {
+    return sum_partials<V>(reinterpret_cast<const V *>(M.upart) + S.ubase, S.uk, S.w + S.noff, pos);
 }
 
 // g[i] = P[min(i, n - 1) * stride] for i < N (n >= 1): a column of a panel tile loaded into
@@ -473,26 +454,17 @@ __device__ __forceinline__ void fwd_tile(const MeshDev &M, const SnDev &S, int r
     V b, o;
     b.x = 0; b.y = 0; b.z = 0; b.w = 0;
     o = b;
-    int4 rd = make_int4(0, 0, 0, -1), ro = rd, rd2 = make_int4(0, 0, 0, 0), ro2 = rd2;
-    const int4 *cp = M.cpos + 2 * (long long)S.cp_off;
-    if (lane < qn) {
-        b = R[S.c0 + q0 + lane];
-        rd = cp[2 * (q0 + lane)];
-        rd2 = cp[2 * (q0 + lane) + 1];
-    }
-    if (kcb == 0 && lane < rn) {
-        ro = cp[2 * (w + r0 + lane)];
-        ro2 = cp[2 * (w + r0 + lane) + 1];
-    }
+    if (lane < qn) b = R[S.c0 + q0 + lane];
     // column-major G_s: entry (row r, column q) at q * no + r
     T g[N];
     if (rn > 0) panel_load<N, T>(tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1), no, qn, g);
+    const long long dst = lane < rn ? M.udst[S.roff + r0 + lane] : 0;  // this row's slot in the parent
     if (lane < qn) {
-        const V c = pull_record<V>(M, rd, rd2);
+        const V c = pull_slots<V>(M, S, q0 + lane);
         b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
     }
     vd[lane] = b;
-    if (kcb == 0 && lane < rn) o = pull_record<V>(M, ro, ro2);
+    if (kcb == 0 && lane < rn) o = pull_slots<V>(M, S, w + r0 + lane);
     vo[lane] = o;
     __syncthreads();
     if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
@@ -505,7 +477,7 @@ __device__ __forceinline__ void fwd_tile(const MeshDev &M, const SnDev &S, int r
         out.y = (kcb == 0 ? o.y : T(0)) - ay;
         out.z = (kcb == 0 ? o.z : T(0)) - az;
         out.w = 0;
-        reinterpret_cast<V *>(M.upart)[S.uoff + (long long)kcb * no + r0 + lane] = out;
+        reinterpret_cast<V *>(M.upart)[dst + (long long)kcb * S.dstride] = out;
     }
 }
 
@@ -521,13 +493,13 @@ __device__ __forceinline__ void fwd_tile_lean(const MeshDev &M, const SnDev &S, 
         b.x = 0; b.y = 0; b.z = 0; b.w = 0;
         if (lane < qn) {
             b = R[S.c0 + q0 + lane];
-            V c = pull_contributions<V>(M, S, q0 + lane);
+            V c = pull_slots<V>(M, S, q0 + lane);
             b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
         }
         vd[lane] = b;
         V o;
         o.x = 0; o.y = 0; o.z = 0; o.w = 0;
-        if (kcb == 0 && lane < rn) o = pull_contributions<V>(M, S, w + r0 + lane);
+        if (kcb == 0 && lane < rn) o = pull_slots<V>(M, S, w + r0 + lane);
         vo[lane] = o;
     }
     __syncthreads();
@@ -542,7 +514,7 @@ __device__ __forceinline__ void fwd_tile_lean(const MeshDev &M, const SnDev &S, 
         out.y = (kcb == 0 ? o.y : T(0)) - ay;
         out.z = (kcb == 0 ? o.z : T(0)) - az;
         out.w = 0;
-        reinterpret_cast<V *>(M.upart)[S.uoff + (long long)kcb * no + r0 + lane] = out;
+        reinterpret_cast<V *>(M.upart)[M.udst[S.roff + r0 + lane] + (long long)kcb * S.dstride] = out;
     }
 }
 
@@ -577,18 +549,11 @@ __device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q
     const bool root = S.noff == 0;
     V t;
     t.x = 0; t.y = 0; t.z = 0; t.w = 0;
-    int4 xl = make_int4(0, 0, 0, 0), c1 = make_int4(0, 0, 0, -1), c2 = make_int4(0, 0, 0, 0);
+    int4 xl = make_int4(0, 0, 0, 0);
     if (lane < pn) {
-        if (p >= w) {
-            xl = M.xloc[S.roff + p - w];
-        } else if (root) {
-            t = reinterpret_cast<const V *>(M.r)[S.c0 + p];
-            const int4 *cp = M.cpos + 2 * ((long long)S.cp_off + p);
-            c1 = cp[0];
-            c2 = cp[1];
-        } else {
-            t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
-        }
+        if (p >= w) xl = M.xloc[S.roff + p - w];
+        else if (root) t = reinterpret_cast<const V *>(M.r)[S.c0 + p];
+        else t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
     }
     T g[N];
     panel_load<N, T>(tptr<T>(M.pr) + S.pr_off + (long long)p0 * w + q0 + min(lane, qn - 1), w, pn, g);
@@ -597,7 +562,7 @@ __device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q
         t.x = -a.x; t.y = -a.y; t.z = -a.z;
     }
     if (lane < pn && p < w && root) {
-        const V c = pull_record<V>(M, c1, c2);
+        const V c = pull_slots<V>(M, S, p);
         t.x += c.x; t.y += c.y; t.z += c.z;
     }
     t.w = 0;
@@ -625,7 +590,7 @@ __device__ __forceinline__ void bwd_tile_lean(const MeshDev &M, const SnDev &S, 
         if (lane < pn) {
             if (p < w && S.noff == 0) {  // a root: no forward tile (as in bwd_tile)
                 t = reinterpret_cast<const V *>(M.r)[S.c0 + p];
-                const V c = pull_contributions<V>(M, S, p);
+                const V c = pull_slots<V>(M, S, p);
                 t.x += c.x; t.y += c.y; t.z += c.z;
             } else if (p < w) {
                 t = reinterpret_cast<const V *>(M.z)[S.c0 + p];
@@ -677,7 +642,8 @@ __device__ __forceinline__ void wave_dot(const T *P, long long stride, const V *
 struct SubNode {
     int c0, w, noff, fb;     // columns, off rows, forward front base in LDS
     int pc, pr, cp, ob;      // G offset, [Dinv; G] offset, contribution entries, off-row sources
-    int uo, xo, xs, pa;      // u output (root) or -1, x output, LDS column base of x, position prefix
+    int uo, xo, xs, pa;      // root: first update-slot entry (udst) or -1; x output, LDS column base
+                             //   of x, position prefix
     int pb, pq, fbb, pad;    // off-row prefix, column prefix, backward front base in LDS
 };
 
@@ -784,7 +750,7 @@ __global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L, int node_cap) {
                 V o = F[S.fb + S.w + i];
                 o.x -= ax; o.y -= ay; o.z -= az;
                 F[S.fb + S.w + i] = o;
-                if (S.uo >= 0) U[S.uo + i] = o;
+                if (S.uo >= 0) U[M.udst[S.uo + i]] = o;  // the root: into the parent's slot
             }
         }
         __syncthreads();

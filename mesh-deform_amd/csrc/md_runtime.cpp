@@ -569,11 +569,10 @@ int upload_layout(md_mesh *m) {
     auto last = [](const std::vector<int64_t> &v) { return v.empty() ? (int64_t)0 : v.back(); };
     Arena &A = m->larena;
     A.reset();
-    size_t o_up = A.reserve(vsz * std::max<int64_t>(1, last(f.uoff)));
+    size_t o_up = A.reserve(vsz * std::max<int64_t>(1, last(f.ubase)));
     size_t o_xp = A.reserve(vsz * std::max<int64_t>(1, last(f.xoff)));
     const size_t nft = f.ft.size() / 4, nbt = f.bt.size() / 4;
-    size_t o_cpos = A.reserve(sizeof(int) * std::max<size_t>(4, f.cpos.size()));
-    size_t o_cmore = A.reserve(sizeof(int) * std::max<size_t>(4, f.cmore.size()));
+    size_t o_udst = A.reserve(sizeof(int64_t) * std::max<size_t>(1, f.udst.size()));
     size_t o_ri = A.reserve(sizeof(int) * std::max<size_t>(4, f.rowinfo.size()));
     size_t o_xl = A.reserve(sizeof(int) * std::max<size_t>(4, f.xloc.size()));
     size_t o_ftile = A.reserve(sizeof(TileDev) * std::max<size_t>(1, nft));
@@ -598,7 +597,7 @@ int upload_layout(md_mesh *m) {
     }
     MeshDev &d = m->dev;
     d.upart = A.at(o_up); d.xpart = A.at(o_xp);
-    d.cpos = (const int4 *)A.at(o_cpos); d.cmore = (const int4 *)A.at(o_cmore);
+    d.udst = (const long long *)A.at(o_udst);
     d.rowinfo = (const int4 *)A.at(o_ri); d.xloc = (const int4 *)A.at(o_xl);
     d.ftile = (const TileDev *)A.at(o_ftile); d.btile = (const TileDev *)A.at(o_btile);
     d.sub = (const int4 *)A.at(o_sub); d.sub_h = (const int *)A.at(o_subh);
@@ -617,8 +616,9 @@ int upload_layout(md_mesh *m) {
     auto sn_of = [&](int k) {
         SnDev x;
         x.c0 = f.c0[k]; x.w = f.w[k]; x.noff = f.noff[k]; x.roff = f.roff[k];
-        x.kc = f.kc[k]; x.kr = f.kr[k]; x.cp_off = f.cp_off[k]; x.pad = 0;
-        x.pc_off = f.pc_off[k]; x.pr_off = f.pr_off[k]; x.uoff = f.uoff[k]; x.xoff = f.xoff[k];
+        x.kc = f.kc[k]; x.kr = f.kr[k]; x.uk = f.uk[k];
+        x.dstride = f.parent[k] >= 0 ? f.w[f.parent[k]] + f.noff[f.parent[k]] : 0;
+        x.pc_off = f.pc_off[k]; x.pr_off = f.pr_off[k]; x.ubase = f.ubase[k]; x.xoff = f.xoff[k];
         return x;
     };
     std::vector<TileDev> fth(nft), bth(nbt);
@@ -629,8 +629,9 @@ int upload_layout(md_mesh *m) {
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
     if (ns > 0) {
-        chk(up(d.cpos, f.cpos.data(), sizeof(int) * f.cpos.size()));
-        chk(up(d.cmore, f.cmore.data(), sizeof(int) * f.cmore.size()));
+        chk(up(d.udst, f.udst.data(), sizeof(int64_t) * f.udst.size()));
+        // slots no child writes must read as zero, every iteration
+        chk(hipMemsetAsync(d.upart, 0, vsz * std::max<int64_t>(1, last(f.ubase)), st));
         chk(up(d.rowinfo, f.rowinfo.data(), sizeof(int) * f.rowinfo.size()));
         chk(up(d.xloc, f.xloc.data(), sizeof(int) * f.xloc.size()));
         chk(up(d.ftile, fth.data(), sizeof(TileDev) * nft));
@@ -1397,10 +1398,12 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     // off-row sources (backward)
     const int64_t sub_fwd_idx = 4 * (int64_t)(f.snode.size() + f.scp.size() + f.sfc.size() + f.sub_h.size());  // scp: 2 ints per position
     const int64_t sub_bwd_idx = 4 * (int64_t)(f.snode.size() + f.sbo.size() + f.sub_h.size());
-    // forward: G panel, r in, z out, update partials out and read back by the parent, the
-    // level fronts' contribution records
-    const int64_t lvl_fwd_idx = 4 * (int64_t)(f.cpos.size() + f.cmore.size());
-    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + 2 * upd * vs + lvl_fwd_idx + sub_fwd_idx;
+    // forward: G panel, r in, z out, update partials out (with their slot addresses) and every
+    // update slot of the level fronts read back (unused slots included)
+    const int64_t uslots = f.ubase.empty() ? 0 : f.ubase.back();
+    int64_t udst_reads = 0;
+    for (size_t e = 0; e < f.udst.size(); ++e) udst_reads += f.udst[e] >= 0;
+    out->bytes_forward = gw * es + 2 * (int64_t)t.nf * vs + upd * vs + uslots * vs + 8 * udst_reads + sub_fwd_idx;
     // backward + update: Dinv and G panels, z in, x partials of ancestors gathered (with
     // their locators), level rows' x partials out and summed (row list, row locators), the
     // p' update
