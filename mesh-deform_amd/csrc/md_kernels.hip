@@ -362,22 +362,25 @@ template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
 // one load; more are loaded 8 at a time (clamped addresses, masked values) so their
 // latencies overlap.
 constexpr int kPartialBatch = 8;
+// The same in the panel-preloading backward tiles (few waves: registers to spare), so the 9-16
+// partials of a wide owner's x arrive in one round.
+constexpr int kWidePartialBatch = 16;
 // Level launches of at most this many tiles load their panel entries ahead of the vector
 // (latency-bound: few waves); larger ones keep the register count low for occupancy.
 constexpr int kPreloadMaxTiles = 2048;
 
-template <typename V>
+template <typename V, int B = kPartialBatch>
 __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e) {
     using T = decltype(V{}.x);
     if (cnt == 1) return U[e];
     V a;
     a.x = 0; a.y = 0; a.z = 0; a.w = 0;
-    for (int k0 = 0; k0 < cnt; k0 += kPartialBatch) {
-        V b[kPartialBatch];
+    for (int k0 = 0; k0 < cnt; k0 += B) {
+        V b[B];
 #pragma unroll
-        for (int u = 0; u < kPartialBatch; ++u) b[u] = U[(long long)min(k0 + u, cnt - 1) * stride + e];
+        for (int u = 0; u < B; ++u) b[u] = U[(long long)min(k0 + u, cnt - 1) * stride + e];
 #pragma unroll
-        for (int u = 0; u < kPartialBatch; ++u) {
+        for (int u = 0; u < B; ++u) {
             T m = (k0 + u < cnt) ? T(1) : T(0);
             a.x += m * b[u].x; a.y += m * b[u].y; a.z += m * b[u].z;
         }
@@ -388,9 +391,9 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
 // Children's partial updates at position pos of a level front: the sum of its uk slots (slot k
 // at k * m + pos; the children's partials fill them in child order, unused ones are zero), so
 // their addresses follow from the tile alone.
-template <typename V>
+template <typename V, int B = kPartialBatch>
 __device__ __forceinline__ V pull_slots(const MeshDev &M, const SnDev &S, int pos) {
-    return sum_partials<V>(reinterpret_cast<const V *>(M.upart) + S.ubase, S.uk, S.w + S.noff, pos);
+    return sum_partials<V, B>(reinterpret_cast<const V *>(M.upart) + S.ubase, S.uk, S.w + S.noff, pos);
 }
 
 // g[i] = P[min(i, n - 1) * stride] for i < N (n >= 1): a column of a panel tile loaded into
@@ -558,11 +561,11 @@ __device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q
     T g[N];
     panel_load<N, T>(tptr<T>(M.pr) + S.pr_off + (long long)p0 * w + q0 + min(lane, qn - 1), w, pn, g);
     if (lane < pn && p >= w) {
-        const V a = sum_partials<V>(X + xl.x, xl.z, xl.y, 0);
+        const V a = sum_partials<V, kWidePartialBatch>(X + xl.x, xl.z, xl.y, 0);
         t.x = -a.x; t.y = -a.y; t.z = -a.z;
     }
     if (lane < pn && p < w && root) {
-        const V c = pull_slots<V>(M, S, p);
+        const V c = pull_slots<V, kWidePartialBatch>(M, S, p);
         t.x += c.x; t.y += c.y; t.z += c.z;
     }
     t.w = 0;
