@@ -226,8 +226,13 @@ def batched_leg(ctx, count, iters, seed, tune=True, baseline=True):
     ms = b.time_iterations(0, iters)
     b.close()
     total_iters = count * iters
+    # canonical bytes of one batched iteration (every mesh once) over its event time
+    batch_bytes = sum(canonical_bytes(pr, m.stats())["B_iter"] for pr, m in zip(probs, meshes))
+    achieved = batch_bytes / (ms / iters * 1e-3) / 1e9
     return dict(meshes=count, total_vertices=int(sum(p.n for p in probs)), iterations=iters,
                 batched_iters_per_s=total_iters / (ms / 1e3),
+                batched_ms_per_iteration=round(ms / iters, 4), canonical_bytes_per_batched_iteration=int(batch_bytes),
+                achieved_GBs=round(achieved, 1), hbm_frac=round(achieved / PEAK_HBM_GBS, 4),
                 plain_iters_per_s=None if plain_s is None else total_iters / plain_s,
                 speedup=None if plain_s is None else (total_iters / (ms / 1e3)) / (total_iters / plain_s),
                 batch_subtree_budget=budget,
