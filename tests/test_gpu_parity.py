@@ -72,6 +72,81 @@ def test_cotan_matches_oracle(ctx, golden_dir, name):
     np.testing.assert_allclose(v, g["w"], rtol=1e-12, atol=0)
 
 
+def _degenerate_grid():
+    """A 7 x 7 grid patch with degenerate faces: two coincident vertices (zero-length edge,
+    zero-area faces), a needle (a vertex moved almost onto an edge), a NaN-prone flat triangle,
+    and obtuse corners; anchors on the first row, handles on the last row lifted 0.3."""
+    k = 7
+    g = np.arange(k, dtype=np.float64)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    xyz = np.stack([X.ravel(), Y.ravel(), 0.01 * np.sin(X.ravel() * 1.3 + Y.ravel())], axis=1) / (k - 1)
+    tri = []
+    for j in range(k - 1):
+        for i in range(k - 1):
+            a, b, c, d = j * k + i, j * k + i + 1, (j + 1) * k + i, (j + 1) * k + i + 1
+            tri += [(a, b, d), (a, d, c)]
+    tri = np.array(tri, np.int32)
+    xyz[17] = xyz[18]                                  # coincident vertices
+    xyz[31] = 0.5 * (xyz[30] + xyz[32]) + [0, 1e-9, 0]  # needle: almost on the edge 30-32
+    xyz[38, 1] += 0.4 / (k - 1)                         # obtuse corners around vertex 38
+    anchors = np.arange(k)
+    handles = np.arange(k * (k - 1), k * k)
+    idx = np.concatenate([anchors, handles]).astype(np.int32)
+    tgt = xyz[idx].copy()
+    tgt[len(anchors):, 2] += 0.3
+    return xyz, tri, idx, tgt
+
+
+def test_cotan_and_deform_on_degenerate_faces(ctx):
+    """Degenerate geometry (arap.h:199-218 clamps: max(1e-8, .) lengths and Heron area,
+    max(1e-10, .) cotangents): the device weights equal the oracle's, and the FP64 deformation
+    follows the oracle (loosely: the clamped weights make the system ill-conditioned)."""
+    say("case: degenerate faces")
+    xyz, tri, idx, tgt = _degenerate_grid()
+    rp0, ci0, w0 = oracle.cotan_csr(xyz, tri)
+    assert np.all(w0 > 0) and np.any(w0 < 1e-6)  # the clamps are exercised
+    m, ok = run_gpu(ctx, xyz, tri, idx, tgt, 5)
+    assert ok
+    rp, ci, v = m.cotan()
+    assert np.array_equal(rp, rp0) and np.array_equal(ci, ci0)
+    np.testing.assert_allclose(v, w0, rtol=1e-12, atol=0)
+    a = oracle.Arap(xyz, tri)
+    a.set_constraints(idx, tgt)
+    assert a.deform(5)
+    assert np.all(np.isfinite(m.mesh))
+    assert meshgen.relative_error(m.mesh, a.mesh) < 1e-6
+
+
+@pytest.mark.parametrize("budget", [0, 256])
+def test_disconnected_components_match_oracle(ctx, budget):
+    """Three disconnected components in one mesh (a UV sphere, a small bar, a small icosphere),
+    each with its own anchors and handles: the factor is a forest with several roots (roots
+    without off rows, whose backward tiles gather their right-hand side); FP64 at 1e-9 and
+    FP32 + refinement at 1e-5 against the oracle."""
+    say("case: disconnected components, budget", budget)
+    parts = [meshgen.sphere_problem(12, 24), meshgen.bar_problem(40, 4, 4), meshgen.noisy_icosphere_problem(3, 0.01, 7)]
+    xyz, tri, idx, tgt, base = [], [], [], [], 0
+    for k, pr in enumerate(parts):
+        shift = np.array([3.0 * k, 0.0, 0.0])
+        xyz.append(pr.xyz + shift)
+        tri.append(pr.tri + base)
+        ci, ct = pr.constraints()
+        idx.append(ci + base)
+        tgt.append(ct + shift)
+        base += pr.n
+    xyz, tri = np.concatenate(xyz), np.concatenate(tri).astype(np.int32)
+    idx, tgt = np.concatenate(idx).astype(np.int32), np.concatenate(tgt)
+    a = oracle.Arap(xyz, tri)
+    a.set_constraints(idx, tgt)
+    assert a.deform(8)
+    for prec, tol in ((mdarap.PREC_FP64, POS_TOL_FP64), (mdarap.PREC_FP32_REFINE, POS_TOL_FP32)):
+        m, ok = run_gpu(ctx, xyz, tri, idx, tgt, 8, precision=prec, subtree_budget=budget)
+        assert ok
+        err = meshgen.relative_error(m.mesh, a.mesh)
+        say("  precision", prec, "levels", m.stats()["levels"], "rel err", err)
+        assert err < tol
+
+
 # ---- local step (arap.h:354-384) ----------------------------------------------------------
 
 @pytest.mark.parametrize("name", ["sphere_obj", "bar_obj", "plane_obj", "uv_sphere"])
