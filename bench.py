@@ -89,6 +89,7 @@ def plain_leg(ctx, pr, steps, warmup, tune=True, budget=None, fixed_ratio=None):
     st = m.stats()
     st["subtree_budget"] = budget
     st["amalg_ratio"] = ratio if ratio is not None else mdarap.config().amalg_ratio
+    st["level_pairs"] = mdarap.config().level_pairs
     st["autotune_s"] = tune_s
     return m, st, setup_s, wall, ev_total_ms, phase_ms
 
@@ -287,7 +288,7 @@ def traffic_probe(ctx, iters, budget=None, ratio=None):
     m.close()
     print(json.dumps({"probe": "traffic", "copy_bytes": nbytes, "copies": 4, "copy_GBs": round(gbps, 1),
                       "iterations": iters, "warmup_iterations": 2, "subtree_budget": used,
-                      "amalg_ratio": ratio}), flush=True)
+                      "amalg_ratio": ratio, "level_pairs": mdarap.config().level_pairs}), flush=True)
 
 
 def traffic_summary(fetch_csv, write_csv, iters):
@@ -324,10 +325,10 @@ def traffic_summary(fetch_csv, write_csv, iters):
     return out
 
 
-def read_traffic(path, budget, ratio):
+def read_traffic(path, budget, ratio, pairs):
     """traffic_bytes_per_iter from a committed traffic summary (lines 'traffic_bytes_per_iter: N'
-    and 'traffic_config: subtree_budget=B amalg_ratio=R'), when it was measured on the
-    factor and sweep layout the plain leg runs."""
+    and 'traffic_config: subtree_budget=B amalg_ratio=R level_pairs=P'), when it was measured
+    on the factor and sweep layout the plain leg runs."""
     val, cfg = None, None
     try:
         with open(path) as f:
@@ -337,10 +338,10 @@ def read_traffic(path, budget, ratio):
                     val = float(line.split(":", 1)[1])
                 elif line.startswith("traffic_config:"):
                     kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
-                    cfg = (int(kv["subtree_budget"]), float(kv["amalg_ratio"]))
+                    cfg = (int(kv["subtree_budget"]), float(kv["amalg_ratio"]), int(kv.get("level_pairs", 0)))
     except (OSError, ValueError, KeyError):
         return None
-    if cfg is None or cfg[0] != budget or abs(cfg[1] - ratio) > 1e-9:
+    if cfg is None or cfg[0] != budget or abs(cfg[1] - ratio) > 1e-9 or cfg[2] != int(pairs):
         return None
     return val
 
@@ -409,7 +410,7 @@ def main():
     iter_bytes = st["bytes_local"] + st["bytes_residual"] + solve_bytes
     cb = canonical_bytes(pr, st)
     achieved = cb["B_iter"] / (ev_step_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.traffic_summary, st["subtree_budget"], st["amalg_ratio"])
+    traffic = read_traffic(args.traffic_summary, st["subtree_budget"], st["amalg_ratio"], st["level_pairs"])
     log("plain: %.1f iters/s (%.4f ms/iter wall, %.4f ms event); phases ms %s" %
         (value, ms_step, ev_step_ms, np.round(ph_step, 4).tolist()))
     out = {
@@ -425,7 +426,7 @@ def main():
                    "levels": st["levels"], "panel_entries": st["panel_entries"],
                    "factor_nnz": st["factor_nnz"], "setup_s": round(setup_s, 3),
                    "subtree_budget": st["subtree_budget"], "amalg_ratio": st["amalg_ratio"],
-                   "autotune_s": round(st["autotune_s"], 3)},
+                   "level_pairs": st["level_pairs"], "autotune_s": round(st["autotune_s"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None if traffic is None else int(traffic),

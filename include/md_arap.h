@@ -71,6 +71,9 @@ typedef struct {
     int factor_device; /* MD_SOLVER_LDLT: 1 (default) runs the FP64 numeric factorization on the
                           device (multifrontal, one height of the supernodal tree per launch
                           pair); 0 runs it on the host */
+    int level_pairs;   /* MD_SOLVER_LDLT: 1 orders the free vertices so that every other sweep level
+                          merges into its parents (about half the level launches, denser panels,
+                          the same fill); 0 keeps the plain nested-dissection order; default 1 */
 } md_config;
 
 typedef struct {
@@ -224,13 +227,14 @@ int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbps);
 /* Factors the SPD matrix given as full CSR (n rows, ascending columns), with coordinates
  * xyz (3 per row) for the ordering, by the same host analysis and FP64 block-LDL^T the
  * library uploads, then solves nrhs right-hand sides (b[r*n + i]) on the host by an
- * emulation of the device sweeps (subtree_budget as in md_config; negative: the default).
+ * emulation of the device sweeps (subtree_budget and level_pairs as in md_config; negative
+ * subtree_budget: the default).
  * device_factor_emulation != 0 factors by a host emulation of the device factorization (the
  * same steps and index math as its kernels, every index checked) instead. */
 int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
                          const double *xyz, int nrhs, const double *b, double *x,
                          int64_t *panel_entries, int *levels, int subtree_budget,
-                         int device_factor_emulation);
+                         int device_factor_emulation, int level_pairs);
 
 #ifdef __cplusplus
 }

@@ -171,7 +171,8 @@ std::vector<int> order_graph(int m, const int* rowptr, const int* col, const dou
 }
 
 std::vector<int> order_free_vertices(int n, int F, const int* tri, const double* xyz,
-                                     const std::vector<char>& is_con) {
+                                     const std::vector<char>& is_con, std::vector<int>* pair_group,
+                                     double amalg_ratio) {
     std::vector<int> rank_of, vert_of, ptr, adj;
     free_graph(n, F, tri, is_con, rank_of, vert_of, ptr, adj);
     int m = (int)vert_of.size();
@@ -179,6 +180,7 @@ std::vector<int> order_free_vertices(int n, int F, const int* tri, const double*
     for (int k = 0; k < m; ++k)
         for (int a = 0; a < 3; ++a) fx[3 * k + a] = xyz[3 * (size_t)vert_of[k] + a];
     std::vector<int> perm = nd_postorder(m, ptr, adj, fx);
+    if (pair_group) perm = pair_levels(perm, ptr, adj, kPairBudget, amalg_ratio, *pair_group);
     for (int k = 0; k < m; ++k) perm[k] = vert_of[perm[k]];
     return perm;
 }
@@ -296,7 +298,7 @@ void build_topology(int n, int F, const int* tri, const std::vector<int>& perm,
 constexpr int kAmalgSmall = 16;
 
 void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, Factor& f, int sub_budget,
-              double amalg_ratio) {
+              double amalg_ratio, const std::vector<int>* group) {
     f = Factor();
     f.n = n;
     std::vector<int> ident(n);
@@ -372,10 +374,12 @@ void symbolic(int n, const std::vector<int>& ptr, const std::vector<int>& adj, F
             int64_t z = zeros[c] + zeros[p] + (after - before);
             double frac = (double)z / (double)after;
             int64_t wm = wc + wp;
-            if (wm > kMaxSnWidth) break;
+            // a level pair (pair_levels placed its columns together) always merges
+            const bool grouped = group && (*group)[lo[c]] >= 0 && (*group)[lo[c]] == (*group)[lo[p]];
+            if (wm > (grouped ? kMaxGroupWidth : kMaxSnWidth)) break;
             // relaxed amalgamation: tiny supernodes always merge, wider ones only while the
             // share of explicit zeros stays small
-            bool ok = wm <= kAmalgSmall || (wm <= 16 && frac < 0.5) ||
+            bool ok = grouped || wm <= kAmalgSmall || (wm <= 16 && frac < 0.5) ||
                       (wm <= 48 && frac < amalg_ratio) || frac < 0.5 * amalg_ratio;
             if (!ok) break;
             zeros[p] = z;
@@ -675,6 +679,85 @@ void device_layout(Factor& f, int sub_budget) {
             }
         }
     }
+}
+
+// ---- level pairs ------------------------------------------------------------------------
+
+std::vector<int> pair_levels(const std::vector<int>& order, const std::vector<int>& ptr, const std::vector<int>& adj,
+                             int sub_budget, double amalg_ratio, std::vector<int>& group) {
+    const int m = (int)order.size();
+    group.assign(m, -1);
+    if (m == 0) return order;
+    // the pattern in this order, and its factor as it would be laid out
+    std::vector<int> iorder(m);
+    for (int k = 0; k < m; ++k) iorder[order[k]] = k;
+    std::vector<int> p2(m + 1, 0), a2;
+    a2.reserve(adj.size());
+    for (int k = 0; k < m; ++k) {
+        const int i = order[k];
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) a2.push_back(iorder[adj[e]]);
+        p2[k + 1] = (int)a2.size();
+    }
+    Factor f;
+    symbolic(m, p2, a2, f, sub_budget, amalg_ratio);
+    const int ns = f.ns, top = f.nlevels - 1;
+    if (top < 2) return order;
+    // members: level supernodes whose level is an odd distance below the top level and whose
+    // parent sits one level up; they join their parent's group
+    std::vector<int> head(ns, -1);
+    for (int t = ns - 1; t >= 0; --t) {
+        if (f.level[t] < 0) continue;
+        const int p = f.parent[t];
+        const bool member = p >= 0 && (top - f.level[t]) % 2 == 0 && f.level[p] == f.level[t] + 1 && head[p] == p;
+        head[t] = member ? p : t;
+    }
+    // children lists in id order
+    std::vector<std::vector<int>> kids(ns);
+    std::vector<int> roots;
+    for (int t = 0; t < ns; ++t) {
+        if (f.parent[t] >= 0) kids[f.parent[t]].push_back(t);
+        else roots.push_back(t);
+    }
+    // new order of positions: a head's members' subtrees first, then the members' columns,
+    // then the head's; every column still follows its descendants (same fill)
+    std::vector<int> pos;
+    pos.reserve(m);
+    std::vector<int> gpos;
+    gpos.reserve(m);
+    auto cols = [&](int t, int g) {
+        for (int j = f.c0[t]; j < f.c0[t] + f.w[t]; ++j) { pos.push_back(j); gpos.push_back(g); }
+    };
+    struct Frame { int t, stage; };
+    std::vector<Frame> st;
+    for (int r : roots) {
+        st.push_back({r, 0});
+        while (!st.empty()) {
+            const int t = st.back().t;
+            const int stage = st.back().stage;
+            const std::vector<int>& ch = kids[t];
+            if (stage < (int)ch.size()) {
+                st.back().stage = stage + 1;
+                const int c = ch[stage];
+                if (head[c] == t && c != t) {  // a member: its children's subtrees now
+                    for (int q = (int)kids[c].size() - 1; q >= 0; --q) st.push_back({kids[c][q], 0});
+                } else {
+                    st.push_back({c, 0});
+                }
+                continue;
+            }
+            const bool is_head = head[t] == t;
+            bool grouped = false;
+            for (int c : ch)
+                if (head[c] == t && c != t) { cols(c, t); grouped = true; }
+            cols(t, is_head && grouped ? t : -1);
+            st.pop_back();
+        }
+    }
+    std::vector<int> out(m);
+    for (int k = 0; k < m; ++k) out[k] = order[pos[k]];
+    // members' columns carry the group too
+    for (int k = 0; k < m; ++k) group[k] = gpos[k];
+    return out;
 }
 
 // ---- numeric ----------------------------------------------------------------------------

@@ -15,6 +15,12 @@ namespace md {
 
 // Maximum columns of one supernode (wider separators are split into chains).
 constexpr int kMaxSnWidth = 1024;
+// Maximum columns of a supernode merged from a level pair (a supernode with its children).
+constexpr int kMaxGroupWidth = 4096;
+// Subtree budget whose sweep levels the level pairing follows, whatever budget the layout then
+// uses: the factor depends on the amalgamation and the pairing only, so a re-layout with any
+// budget keeps it (and the small budget pairs the low levels as well).
+constexpr int kPairBudget = 128;
 // Edge of the square panel tile one wavefront streams in the triangular sweeps.
 constexpr int kTile = 64;
 // Vertices per one-ring slice (one wavefront).
@@ -158,8 +164,11 @@ void build_topology(int n, int F, const int* tri, const std::vector<int>& perm,
 
 // Fill-reducing order of the free vertices: geometric nested dissection on the free-free
 // graph, followed by an elimination-tree postorder. Returns original vertex ids.
+// With pair_group, the order also pairs sweep levels (pair_levels with kPairBudget and the
+// given amalgamation) and pair_group receives the group of every position.
 std::vector<int> order_free_vertices(int n, int F, const int* tri, const double* xyz,
-                                     const std::vector<char>& is_con);
+                                     const std::vector<char>& is_con, std::vector<int>* pair_group = nullptr,
+                                     double amalg_ratio = 0.0);
 
 // The same ordering for a general symmetric pattern (full CSR, m rows) with coordinates.
 std::vector<int> order_graph(int m, const int* rowptr, const int* col, const double* xyz);
@@ -168,8 +177,21 @@ std::vector<int> order_graph(int m, const int* rowptr, const int* col, const dou
 // adj_ptr/adj (CSR over solve indices, no diagonal needed) describe A's pattern.
 // sub_budget: LDS vector entries of one bottom subtree (0: no subtree sweeps); amalg_ratio:
 // relaxed-amalgamation limit (more merges: fewer, wider supernodes and levels, more zeros).
+// group (per column, -1: none): columns of one group that end up adjacent in the supernodal
+// tree always merge (up to kMaxGroupWidth columns).
 void symbolic(int n, const std::vector<int>& adj_ptr, const std::vector<int>& adj, Factor& f,
-              int sub_budget = kSubBudgetDefault, double amalg_ratio = kAmalgRatioDefault);
+              int sub_budget = kSubBudgetDefault, double amalg_ratio = kAmalgRatioDefault,
+              const std::vector<int>* group = nullptr);
+
+// Level pairs: in the factor of this order (pattern ptr/adj over the original indices, laid
+// out with the given subtree budget and amalgamation), each sweep level at an even distance
+// below the top level joins the supernodes one level up that are its parents: the returned
+// order places a group's member supernodes right before their parent (after the members'
+// own subtrees, so the fill is unchanged) and group receives each position's group (-1:
+// none), so that symbolic() merges every group into one supernode: about half the levels,
+// for denser panels.
+std::vector<int> pair_levels(const std::vector<int>& order, const std::vector<int>& ptr, const std::vector<int>& adj,
+                             int sub_budget, double amalg_ratio, std::vector<int>& group);
 
 // Device layout of a symbolic factor (partials, contribution lists, tiles, bottom subtrees)
 // for the given subtree budget; symbolic() calls it, and it may be called again to re-lay

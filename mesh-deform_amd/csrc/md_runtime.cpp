@@ -255,6 +255,7 @@ extern "C" void md_config_default(md_config *cfg) {
     cfg->subtree_budget = kSubBudgetDefault;
     cfg->amalg_ratio = kAmalgRatioDefault;
     cfg->factor_device = 1;
+    cfg->level_pairs = 1;
 }
 
 extern "C" const char *md_last_error(void) { return g_err.c_str(); }
@@ -404,7 +405,10 @@ using Clock = std::chrono::steady_clock;
 // Host part of the analysis: ordering, topology, free-free pattern, symbolic factor.
 void analyze_host(md_mesh *m) {
     int n = m->n;
-    std::vector<int> perm = order_free_vertices(n, m->F, m->tri.data(), m->rest.data(), m->is_con);
+    const bool pairs = m->cfg.level_pairs != 0 && m->cfg.solver == MD_SOLVER_LDLT;
+    std::vector<int> group;
+    std::vector<int> perm = order_free_vertices(n, m->F, m->tri.data(), m->rest.data(), m->is_con,
+                                                pairs ? &group : nullptr, m->cfg.amalg_ratio);
     m->topo = Topology();
     build_topology(n, m->F, m->tri.data(), perm, m->is_con, m->topo);
     const Topology &t = m->topo;
@@ -425,7 +429,7 @@ void analyze_host(md_mesh *m) {
     }
     m->fac = Factor();
     if (nf > 0 && nf < n && m->cfg.solver == MD_SOLVER_LDLT)
-        symbolic(nf, m->adj_ptr, m->adj, m->fac, m->cfg.subtree_budget, m->cfg.amalg_ratio);
+        symbolic(nf, m->adj_ptr, m->adj, m->fac, m->cfg.subtree_budget, m->cfg.amalg_ratio, pairs ? &group : nullptr);
     else m->fac.n = nf;
 }
 
@@ -1489,12 +1493,23 @@ extern "C" int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbp
 extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
                                     const double *xyz, int nrhs, const double *b, double *x,
                                     int64_t *panel_entries, int *levels, int subtree_budget,
-                                    int device_factor_emulation) {
+                                    int device_factor_emulation, int level_pairs) {
     if (n <= 0 || !rowptr || !col || !val || !xyz || nrhs < 0 || (nrhs > 0 && (!b || !x))) {
         set_error("md_host_factor_solve: invalid argument");
         return MD_ERR_INVALID;
     }
+    const int budget = subtree_budget < 0 ? kSubBudgetDefault : std::min(subtree_budget, kSubBudgetMax);
     std::vector<int> perm = order_graph(n, rowptr, col, xyz);
+    std::vector<int> group;
+    if (level_pairs) {
+        std::vector<int> ptr0(n + 1, 0), adj0;
+        for (int i = 0; i < n; ++i) {
+            for (int e = rowptr[i]; e < rowptr[i + 1]; ++e)
+                if (col[e] != i) adj0.push_back(col[e]);
+            ptr0[i + 1] = (int)adj0.size();
+        }
+        perm = pair_levels(perm, ptr0, adj0, kPairBudget, kAmalgRatioDefault, group);
+    }
     std::vector<int> iperm(n);
     for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
     std::vector<int> ptr(n + 1, 0), adj;
@@ -1509,7 +1524,7 @@ extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, co
         ptr[k + 1] = (int)adj.size();
     }
     Factor f;
-    symbolic(n, ptr, adj, f, subtree_budget < 0 ? kSubBudgetDefault : std::min(subtree_budget, kSubBudgetMax));
+    symbolic(n, ptr, adj, f, budget, kAmalgRatioDefault, level_pairs ? &group : nullptr);
     std::string err;
     const bool ok = device_factor_emulation ? numeric_device_emulation(ptr, adj, av, diag, f, err)
                                             : numeric(ptr, adj, av, diag, f, err);

@@ -20,9 +20,9 @@ SOLVER_LDLT, SOLVER_PCG = 0, 1
 REST_REFERENCE, REST_FIXED = 0, 1
 
 # candidate bottom-subtree budgets of the sweep-layout autotuner (LDS vector entries; 0: none)
-AUTOTUNE_BUDGETS = (0, 256, 512, 1024)
+AUTOTUNE_BUDGETS = (0, 128, 256, 512, 1024)
 # candidate relaxed-amalgamation ratios of the factor-layout autotuner
-AUTOTUNE_RATIOS = (0.3, 0.5, 0.7, 1.0)
+AUTOTUNE_RATIOS = (0.3, 0.5, 0.7, 0.9)
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -33,7 +33,7 @@ class Config(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("solver", ctypes.c_int), ("refine_steps", ctypes.c_int),
                 ("rest_mode", ctypes.c_int), ("pcg_max_iters", ctypes.c_int), ("pcg_tol", ctypes.c_double),
                 ("use_graph", ctypes.c_int), ("subtree_budget", ctypes.c_int), ("amalg_ratio", ctypes.c_double),
-                ("factor_device", ctypes.c_int)]
+                ("factor_device", ctypes.c_int), ("level_pairs", ctypes.c_int)]
 
 
 class MeshStats(ctypes.Structure):
@@ -113,7 +113,7 @@ _SIGS = [
     ("md_copy_probe", ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("md_host_factor_solve", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _f64p, _f64p,
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
-                                            ctypes.c_int]),
+                                            ctypes.c_int, ctypes.c_int]),
 ]
 
 
@@ -148,7 +148,8 @@ def device_count():
 
 
 def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_graph=True, solver=SOLVER_LDLT,
-           pcg_iters=None, pcg_tol=None, subtree_budget=None, amalg_ratio=None, factor_device=None):
+           pcg_iters=None, pcg_tol=None, subtree_budget=None, amalg_ratio=None, factor_device=None,
+           level_pairs=None):
     c = Config()
     lib().md_config_default(ctypes.byref(c))
     c.precision = precision
@@ -166,6 +167,8 @@ def config(precision=PREC_FP64, rest_mode=REST_REFERENCE, refine_steps=1, use_gr
         c.amalg_ratio = float(amalg_ratio)
     if factor_device is not None:
         c.factor_device = 1 if factor_device else 0
+    if level_pairs is not None:
+        c.level_pairs = 1 if level_pairs else 0
     return c
 
 
@@ -415,7 +418,7 @@ def copy_probe(ctx, nbytes, repeats=3):
     return g.value
 
 
-def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1, device_factor_emulation=False):
+def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1, device_factor_emulation=False, level_pairs=False):
     """Host-only: the library's ordering + symbolic + FP64 block LDL^T, solved on the CPU by an
     emulation of the device sweeps (subtree_budget as in Config; -1: the default); with
     device_factor_emulation the factorization is the emulation of the device kernels."""
@@ -429,5 +432,6 @@ def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1, device_factor_emulat
     lv = ctypes.c_int(0)
     check(lib().md_host_factor_solve(n, rp, ci, v, np.ascontiguousarray(xyz, np.float64).ravel(), b2.shape[0],
                                      np.ascontiguousarray(b2).ravel(), x.reshape(-1), ctypes.byref(pe),
-                                     ctypes.byref(lv), int(subtree_budget), 1 if device_factor_emulation else 0))
+                                     ctypes.byref(lv), int(subtree_budget), 1 if device_factor_emulation else 0,
+                                     1 if level_pairs else 0))
     return (x[0] if np.ndim(b) == 1 else x), pe.value, lv.value

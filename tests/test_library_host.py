@@ -56,21 +56,26 @@ def _problem_system(pr):
 @pytest.mark.parametrize("make", [meshgen.sphere_problem, lambda: meshgen.bar_problem(80, 8, 8),
                                   lambda: meshgen.noisy_icosphere_problem(4)])
 @pytest.mark.parametrize("budget", [0, 64, 512, 1024])
-def test_host_block_ldlt_matches_scipy(make, budget):
+@pytest.mark.parametrize("pairs", [False, True])
+def test_host_block_ldlt_matches_scipy(make, budget, pairs):
     """The device sweeps (level tiles, and bottom subtrees under each LDS budget), emulated
     on the host with every index bounds-checked and LDS phases race-checked, solve L x = b."""
     pr = make()
     L, fx = _problem_system(pr)
     b = np.random.default_rng(1).standard_normal((3, L.shape[0]))
-    x, panel_entries, levels = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, subtree_budget=budget)
+    x, panel_entries, levels = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, subtree_budget=budget,
+                                                        level_pairs=pairs)
     ref = spla.spsolve(L.tocsc(), b.T).T
     np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
     assert np.abs(L @ x.T - b.T).max() < 1e-10 * np.abs(b).max()
     assert panel_entries >= L.nnz // 2
     assert levels >= 1 if budget == 0 else levels >= 0
-    if budget == 0:
+    if budget == 0 and not pairs:
         _, _, lv_default = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b)
         assert lv_default < levels  # the default budget takes the bottom levels into subtrees
+    if pairs:
+        _, _, lv_plain = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, subtree_budget=budget)
+        assert levels <= lv_plain  # paired levels never add launches
 
 
 @pytest.mark.parametrize("make", [meshgen.sphere_problem, lambda: meshgen.bar_problem(80, 8, 8),
@@ -83,6 +88,10 @@ def test_device_factorization_emulation_matches_scipy(make):
     L, fx = _problem_system(pr)
     b = np.random.default_rng(3).standard_normal((3, L.shape[0]))
     x_dev, pe, lv = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, device_factor_emulation=True)
+    x_pair, _, _ = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b, device_factor_emulation=True,
+                                            level_pairs=True)
+    np.testing.assert_allclose(x_pair, spla.spsolve(L.tocsc(), b.T).T, rtol=1e-9,
+                               atol=1e-11 * np.abs(x_dev).max())  # wide merged fronts
     x_host, _, _ = mdarap.host_factor_solve(L.indptr, L.indices, L.data, fx, b)
     ref = spla.spsolve(L.tocsc(), b.T).T
     np.testing.assert_allclose(x_dev, ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
