@@ -326,24 +326,24 @@ def traffic_summary(fetch_csv, write_csv, iters):
 
 
 def read_traffic(path, budget, ratio, pairs):
-    """traffic_bytes_per_iter from a committed traffic summary (lines 'traffic_bytes_per_iter: N'
-    and 'traffic_config: subtree_budget=B amalg_ratio=R level_pairs=P'), when it was measured
-    on the factor and sweep layout the plain leg runs."""
-    val, cfg = None, None
+    """traffic_bytes_per_iter from a committed traffic summary, when it was measured on the
+    factor and sweep layout the plain leg runs. The summary holds one or more pairs of lines
+    'traffic_bytes_per_iter: N' then 'traffic_config: subtree_budget=B amalg_ratio=R level_pairs=P'."""
+    found, val = {}, None
     try:
         with open(path) as f:
             for line in f:
                 line = line.strip()
                 if line.startswith("traffic_bytes_per_iter:"):
                     val = float(line.split(":", 1)[1])
-                elif line.startswith("traffic_config:"):
+                elif line.startswith("traffic_config:") and val is not None:
                     kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
-                    cfg = (int(kv["subtree_budget"]), float(kv["amalg_ratio"]), int(kv.get("level_pairs", 0)))
+                    cfg = (int(kv["subtree_budget"]), round(float(kv["amalg_ratio"]), 6), int(kv.get("level_pairs", 0)))
+                    found[cfg] = val
+                    val = None
     except (OSError, ValueError, KeyError):
         return None
-    if cfg is None or cfg[0] != budget or abs(cfg[1] - ratio) > 1e-9 or cfg[2] != int(pairs):
-        return None
-    return val
+    return found.get((int(budget), round(float(ratio), 6), int(pairs)))
 
 
 def main():
