@@ -67,7 +67,8 @@ struct Plan {
     int refine_steps = 1;
     bool pcg = false;
     int pcg_iters = 0;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;        // one iteration
+    hipGraphExec_t graph_multi = nullptr;  // kGraphIterations iterations
     ~Plan();
     void drop_graph();
     LaunchSet set(int off) const {

@@ -343,6 +343,46 @@ def test_batch_equals_individual(ctx):
         np.testing.assert_array_equal(m.download(), ref)
 
 
+def test_graph_replays_equal_direct_launches(ctx):
+    """Long runs replay a graph of 8 iterations, the rest a graph of one: any split of the
+    iterations gives bitwise what direct launches give, for one mesh and for a batch."""
+    say("case: graph replays")
+    probs = [meshgen.noisy_icosphere_problem(4)] + meshgen.mixed_batch(3, 500, 4000, seed=11)
+    ref = []
+    for pr in probs:
+        idx, tgt = pr.constraints()
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, use_graph=False))
+        m.set_constraints(idx, tgt)
+        m.prepare(pr.xyz)
+        m.iterate(22)
+        ref.append(m.download())
+        m.close()
+    pr = probs[0]
+    idx, tgt = pr.constraints()
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+    m.set_constraints(idx, tgt)
+    m.prepare(pr.xyz)
+    for k in (3, 19):  # one-iteration graph, then two 8-iteration replays and three single ones
+        m.iterate(k)
+    np.testing.assert_array_equal(m.download(), ref[0])
+    m.close()
+    bm = []
+    for pr in probs:
+        idx, tgt = pr.constraints()
+        mm = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
+        mm.set_constraints(idx, tgt)
+        mm.prepare(pr.xyz)
+        bm.append(mm)
+    b = mdarap.Batch(ctx, bm)
+    b.iterate(16)
+    b.iterate(6)
+    for mm, r in zip(bm, ref):
+        np.testing.assert_array_equal(mm.download(), r)
+    b.close()
+    for mm in bm:
+        mm.close()
+
+
 def _prepared(ctx, probs, budget=None):
     out = []
     for pr in probs:
