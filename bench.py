@@ -11,7 +11,9 @@ Legs:
           K timed iterations replayed from hipGraphs; iterations/s.
   check   position error of that path (and of the FP64 path) against the FP64 CPU oracle
           after the same number of iterations from the same start (100, as many as fit the
-          CPU time cap), and the ARAP energy of each.
+          CPU time cap), and the ARAP energy of each; the same for the Jacobi-PCG alternative
+          of the global step (FP32, 1000 CG iterations per solve), with its own time per
+          iteration.
   cpu     the oracle (single-threaded C restatement of inc/deform/arap.h) on the same mesh:
           setup plus a bounded number of iterations, timed on this host.
   served  64 resident meshes (log-uniform 5k-164k vertices), Poisson drag requests at
@@ -44,6 +46,7 @@ import trajectory as tj  # noqa: E402
 
 METRIC = "ARAP iters/s @164k verts + p99 drag latency, served vs plain, rel err≤1e-5"
 PEAK_HBM_GBS = 8000.0
+PCG_ITERS = 1000  # CG iterations per solve of the Jacobi-PCG alternative (1e-5 on the 164k icosphere)
 
 
 def log(*a):
@@ -122,7 +125,19 @@ def check_leg(ctx, pr, iters, cpu_seconds_cap, budget, ratio):
         errs[name] = meshgen.relative_error(m.mesh, a.mesh)
         energies[name] = m.energy()
         m.close()
-    return dict(setup_s=setup_s, iters=n_it, iter_s=cpu_iter_s, errors=errs, energy=energies)
+    # the Jacobi-PCG alternative of the global step (FP32, warm-started from the previous
+    # iterate, FP64 residual): CG iterations per solve for the 1e-5 bound on this mesh
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE, solver=mdarap.SOLVER_PCG,
+                                                       pcg_iters=PCG_ITERS, pcg_tol=1e-7))
+    m.set_constraints(idx, tgt)
+    assert m.deform(n_it)
+    errs["fp32_refine_pcg"] = meshgen.relative_error(m.mesh, a.mesh)
+    energies["fp32_refine_pcg"] = m.energy()
+    pcg_ms, _ = m.time_iterations(1, 10)  # HIP events over 10 further iterations
+    m.close()
+    pcg = {"cg_iterations_per_solve": PCG_ITERS, "ms_per_iteration": round(pcg_ms / 10, 3),
+           "iters_per_s": round(1e4 / pcg_ms, 2)}
+    return dict(setup_s=setup_s, iters=n_it, iter_s=cpu_iter_s, errors=errs, energy=energies, pcg=pcg)
 
 
 def served_problems(count, lo, hi, seed):
@@ -449,6 +464,7 @@ def main():
         out["rel_err_vs_fp64_oracle"] = {k: float("%.3e" % v) for k, v in ck["errors"].items()}
         out["rel_err_iters"] = ck["iters"]
         out["arap_energy"] = {k: float("%.10e" % v) for k, v in ck["energy"].items()}
+        out["pcg_alternative"] = ck["pcg"]
         out["cpu_baseline"] = {"value": round(1.0 / ck["iter_s"], 3), "unit": "iters/s", "cores": 1,
                                "kind": "port",
                                "sample": "oracle (FP64 C restatement of arap.h, up-looking sparse LDL^T, "

@@ -238,12 +238,15 @@ hipError_t run_iterations(Plan &plan, hipStream_t stream, int iters, bool use_gr
         for (int i = 0; i < iters; ++i) enqueue_iteration(plan, stream, nullptr);
         return hipGetLastError();
     }
-    if (!plan.graph || !plan.graph_multi) {  // both at the first replay: a warm-up call captures them
+    // a PCG iteration is already hundreds of launches: one-iteration graphs only
+    const bool multi = !plan.pcg;
+    if (!plan.graph || (multi && !plan.graph_multi)) {  // both at the first replay: a warm-up call captures them
         plan.drop_graph();
         MD_HIP_E(capture_iterations(plan, stream, 1, &plan.graph));
-        MD_HIP_E(capture_iterations(plan, stream, kGraphIterations, &plan.graph_multi));
+        if (multi) MD_HIP_E(capture_iterations(plan, stream, kGraphIterations, &plan.graph_multi));
     }
-    for (; iters >= kGraphIterations; iters -= kGraphIterations) MD_HIP_E(hipGraphLaunch(plan.graph_multi, stream));
+    if (multi)
+        for (; iters >= kGraphIterations; iters -= kGraphIterations) MD_HIP_E(hipGraphLaunch(plan.graph_multi, stream));
     for (; iters > 0; --iters) MD_HIP_E(hipGraphLaunch(plan.graph, stream));
     return hipGetLastError();
 }
