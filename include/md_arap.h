@@ -59,7 +59,9 @@ typedef struct {
     int rest_mode;     /* md_rest_mode, default MD_REST_REFERENCE */
     int pcg_max_iters; /* MD_SOLVER_PCG: CG iterations per solve, default 200 */
     double pcg_tol;    /* MD_SOLVER_PCG: relative residual target, default 1e-7 */
-    int use_graph;     /* replay iterations from a captured hipGraph, default 1 */
+    int use_graph;     /* replay iterations from captured hipGraphs (one of 8 iterations, one of a
+                          single iteration for the rest; PCG: single only), captured at the
+                          first md_mesh_iterate / md_batch_iterate after a (re)layout; default 1 */
     int subtree_budget;/* MD_SOLVER_LDLT: LDS vector entries (at most 1024) of one bottom
                           subtree of the factor swept by a single wavefront (fronts + columns of
                           its supernodes); 0 sweeps every supernode by level launches;
