@@ -14,6 +14,16 @@ template <> struct V4T<float> { using type = f4; };
 template <> struct V4T<double> { using type = d4; };
 
 // Per-supernode record of the block LDL^T sweeps (see Factor in md_analysis.h).
+// Pointers of the device records below are in the global address space in the device pass of
+// the kernels' file (md_kernels.hip defines MD_KERNELS_DEVICE), so the kernels' loads and stores
+// are global (not flat) instructions: they wait on the vector memory counter alone, in issue
+// order, instead of on the LDS counter as well. Host code sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(MD_KERNELS_DEVICE)
+#define MD_G __attribute__((address_space(1)))
+#else
+#define MD_G
+#endif
+
 struct SnDev {
     int c0, w, noff, roff;     // columns [c0, c0+w), off rows offrows[roff .. roff+noff)
     int kc, kr, uk, dstride;   // partial counts of u_s / x_s, update slots per front position,
@@ -35,46 +45,46 @@ struct MeshDev {
     int nslices_free;           // slices covering internal ids [0, nf)
     int F;
     long long slots;            // 64 * slice_ptr[nslices]
-    const int *slice_ptr;       // nslices + 1 (64-wide slot columns)
-    const int *sell_col;        // slots: neighbour internal id (self = padding)
-    float *a32;                 // 3 planes of slots: w_ij (p_i - p_j), FP32
-    double *a64;                // 3 planes of slots, FP64
-    double *w64;                // slots: cotan weight w_ij
-    const int *tri;             // 3F internal ids
-    double *fe_val;             // 3F half-cotangents (face edge 3f+k)
-    const int *slot_fe_ptr;     // slots + 1
-    const int *slot_fe;         // face-edge ids feeding each slot, face order
-    d4 *p;                      // rest positions (internal order)
-    d4 *pp;                     // current positions p'
-    void *quat;                 // rotations as quaternions (x, y, z, w=scalar): f4 or d4
-    void *r, *z;                // residual and forward result (V4 of the solve type), nf each
-    void *upart, *xpart;        // update slots of the level fronts / partial solutions (V4)
-    const int *child_ptr, *child;
-    const long long *udst;      // per off-row entry of a level supernode or subtree root: its
+    const MD_G int *slice_ptr;  // nslices + 1 (64-wide slot columns)
+    const MD_G int *sell_col;   // slots: neighbour internal id (self = padding)
+    MD_G float *a32;            // 3 planes of slots: w_ij (p_i - p_j), FP32
+    MD_G double *a64;           // 3 planes of slots, FP64
+    MD_G double *w64;           // slots: cotan weight w_ij
+    const MD_G int *tri;        // 3F internal ids
+    MD_G double *fe_val;        // 3F half-cotangents (face edge 3f+k)
+    const MD_G int *slot_fe_ptr; // slots + 1
+    const MD_G int *slot_fe;    // face-edge ids feeding each slot, face order
+    MD_G d4 *p;                 // rest positions (internal order)
+    MD_G d4 *pp;                // current positions p'
+    MD_G void *quat;            // rotations as quaternions (x, y, z, w=scalar): f4 or d4
+    MD_G void *r, *z;           // residual and forward result (V4 of the solve type), nf each
+    MD_G void *upart, *xpart;   // update slots of the level fronts / partial solutions (V4)
+    const MD_G int *child_ptr, *child;
+    const MD_G long long *udst; // per off-row entry of a level supernode or subtree root: its
                                 //   update slot in the parent's front (partial cb: + cb * dstride)
-    const int4 *rowinfo;        // per row: (first partial, stride, count, 0)
-    const int4 *xloc;           // per off row entry: rowinfo of that row
-    const void *pc, *pr;        // panels (float or double), forward and backward layouts
-    const TileDev *ftile, *btile;  // level tiles, level by level (bases in the launch tables)
+    const MD_G int4 *rowinfo;   // per row: (first partial, stride, count, 0)
+    const MD_G int4 *xloc;      // per off row entry: rowinfo of that row
+    const MD_G void *pc, *pr;   // panels (float or double), forward and backward layouts
+    const MD_G TileDev *ftile, *btile; // level tiles, level by level (bases in the launch tables)
     // bottom subtrees, one wavefront each (layouts in md_analysis.h, Factor)
-    const int4 *sub;            // (first record, end record, height table offset, heights)
-    const int *sub_h;           // first record of each (subtree, height), + a terminal per subtree
-    const int4 *snode;          // supernode records, four int4 each (SubNode)
-    const int2 *scp;            // forward positions: first two child update entries (LDS), overflow
-    const int *sfc;             // -1-terminated overflow lists of child update entries
-    const int2 *sbo;            // backward off-row sources
+    const MD_G int4 *sub;       // (first record, end record, height table offset, heights)
+    const MD_G int *sub_h;      // first record of each (subtree, height), + a terminal per subtree
+    const MD_G int4 *snode;     // supernode records, four int4 each (SubNode)
+    const MD_G int2 *scp;       // forward positions: first two child update entries (LDS), overflow
+    const MD_G int *sfc;        // -1-terminated overflow lists of child update entries
+    const MD_G int2 *sbo;       // backward off-row sources
     int nsub;                   // bottom subtrees (their rows' x is added to p' by k_sub_bwd)
-    const int2 *srow;           // per subtree: first row, rows (their residual is formed in LDS)
-    const int *sfi;             // per free row of a subtree: its forward front slot in LDS
+    const MD_G int2 *srow;      // per subtree: first row, rows (their residual is formed in LDS)
+    const MD_G int *sfi;        // per free row of a subtree: its forward front slot in LDS
     int nlrows;                 // rows of level-swept supernodes, increasing
-    const int *lrows;           //   (their residual: extra blocks of k_sub_fwd; x summed from the
+    const MD_G int *lrows;      //   (their residual: extra blocks of k_sub_fwd; x summed from the
                                 //   partials and added to p': extra blocks of k_sub_bwd)
-    double *energy_part;        // one partial sum per 256 vertices
+    MD_G double *energy_part;   // one partial sum per 256 vertices
     // Jacobi-PCG alternative to the triangular sweeps (solve type vectors, nf entries)
-    float *w32;                 // slots: cotan weights in FP32
-    void *diag;                 // nf: L_ii = sum_j w_ij (solve type scalar)
-    void *cg_x, *cg_p, *cg_ap;  // correction, direction, A * direction (V4)
-    void *cg_part;              // 3 x nblk partial dot products (V4): p.Ap, r.z (two buffers)
+    MD_G float *w32;            // slots: cotan weights in FP32
+    MD_G void *diag;            // nf: L_ii = sum_j w_ij (solve type scalar)
+    MD_G void *cg_x, *cg_p, *cg_ap; // correction, direction, A * direction (V4)
+    MD_G void *cg_part;         // 3 x nblk partial dot products (V4): p.Ap, r.z (two buffers)
     int nblk_free;              // 256-vertex blocks covering [0, nf)
     double pcg_tol2;            // stop (freeze) when r.z <= tol^2 * (r.z)_0, per coordinate
 };
@@ -89,26 +99,26 @@ struct FacDev {
 };
 
 struct FacArgs {
-    const FacDev *sn;
-    const int *hlist;               // supernodes by height
-    const int *offrows, *rel, *child;
-    const int *slice_ptr, *sell_col;
-    const double *w64;              // cotan weights over the one-ring slots
+    const MD_G FacDev *sn;
+    const MD_G int *hlist;          // supernodes by height
+    const MD_G int *offrows, *rel, *child;
+    const MD_G int *slice_ptr, *sell_col;
+    const MD_G double *w64;         // cotan weights over the one-ring slots
     int nf;
-    double *P, *S, *T, *U, *diag;
-    unsigned long long *amax;       // max |L_ii| as the bits of a non-negative double
-    int *flag;                      // 1: a pivot block is not positive definite; 2: structure
+    MD_G double *P, *S, *T, *U, *diag;
+    MD_G unsigned long long *amax;  // max |L_ii| as the bits of a non-negative double
+    MD_G int *flag;                 // 1: a pivot block is not positive definite; 2: structure
 };
 
 // A set of meshes processed by one launch. Block b belongs to mesh k where
 // prefix[k] <= b < prefix[k+1]; `meshes` and `prefix` are device arrays.
 struct LaunchSet {
-    const MeshDev *meshes;
-    const int *prefix;
+    const MD_G MeshDev *meshes;
+    const MD_G int *prefix;
     int nactive;
-    const int *bmesh;  // optional: mesh of every block (static batches), else a binary search
-    const int *tbase;  // per mesh: first tile of the launch's level (level sweeps), else 0
-    const TileDev *tiles;  // one-mesh level launches: the level's first tile (else null)
+    const MD_G int *bmesh;      // optional: mesh of every block (static batches), else a binary search
+    const MD_G int *tbase;      // per mesh: first tile of the launch's level (level sweeps), else 0
+    const MD_G TileDev *tiles;  // one-mesh level launches: the level's first tile (else null)
 };
 
 // Launchers (grid = total blocks of the launch, from the host-side prefix table).

@@ -4,6 +4,7 @@
 // Every kernel works on a LaunchSet (one or many meshes); the mesh of a block is found by a
 // binary search of the block index in the launch's prefix table, so a single mesh, a static
 // batch and a served mix of meshes all run through the same launches.
+#define MD_KERNELS_DEVICE 1
 #include "md_kernels.h"
 #include "md_analysis.h"
 
@@ -44,7 +45,7 @@ __device__ __forceinline__ double ref_max(double a, double b) { return (a < b) ?
 __global__ __launch_bounds__(kBlock) void k_cotan_face(LaunchSet L) {
 #pragma clang fp contract(off)
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int f = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
     if (f >= M.F) return;
     int i0 = M.tri[3 * f], i1 = M.tri[3 * f + 1], i2 = M.tri[3 * f + 2];
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_cotan_face(LaunchSet L) {
 // w_ij and the weighted rest edge a_ij = w_ij (p_i - p_j) in FP64 and FP32.
 __global__ __launch_bounds__(kBlock) void k_cotan_slot(LaunchSet L) {
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (slice >= M.nslices) return;
@@ -206,9 +207,9 @@ __device__ __forceinline__ void quat_to_mat(T qx, T qy, T qz, T qw, T (&R)[9]) {
     R[6] = s * (qx * qz - qw * qy); R[7] = s * (qy * qz + qw * qx); R[8] = T(1) - s * (qx * qx + qy * qy);
 }
 
-template <typename T> __device__ __forceinline__ const T *aplanes(const MeshDev &M);
-template <> __device__ __forceinline__ const float *aplanes<float>(const MeshDev &M) { return M.a32; }
-template <> __device__ __forceinline__ const double *aplanes<double>(const MeshDev &M) { return M.a64; }
+template <typename T> __device__ __forceinline__ const T *aplanes(const MD_G MeshDev &M);
+template <> __device__ __forceinline__ const float *aplanes<float>(const MD_G MeshDev &M) { return M.a32; }
+template <> __device__ __forceinline__ const double *aplanes<double>(const MD_G MeshDev &M) { return M.a64; }
 
 // One lane per vertex (all n, constrained ones too, as arap.h:358 loops over every vertex):
 // covariance sum_j w_ij (p_i - p_j)(p'_i - p'_j)^T (arap.h:366-374) from the slice's slots,
@@ -217,7 +218,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_local(LaunchSet L) {
     using V = typename V4T<T>::type;
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (slice >= M.nslices) return;
@@ -266,7 +267,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_identity_quat(LaunchSet L) {
     using V = typename V4T<T>::type;
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int iv = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
     if (iv >= M.n) return;
     V out;
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_identity_quat(LaunchSet L) {
 // The rotation term is in T, the weight term in FP64.
 // r_i of one free vertex, slots kU at a time (ids and edges, then the neighbour gathers).
 template <typename T, int kU>
-__device__ __forceinline__ typename V4T<T>::type residual_of(const MeshDev &M, int iv) {
+__device__ __forceinline__ typename V4T<T>::type residual_of(const MD_G MeshDev &M, int iv) {
     using V = typename V4T<T>::type;
     const int slice = iv / kSlice, lane = iv % kSlice;
     const T *A = aplanes<T>(M);
@@ -343,7 +344,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_residual(LaunchSet L) {
     using V = typename V4T<T>::type;
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (slice >= M.nslices_free) return;
@@ -392,7 +393,7 @@ __device__ __forceinline__ V sum_partials(const V *U, int cnt, int stride, int e
 // at k * m + pos; the children's partials fill them in child order, unused ones are zero), so
 // their addresses follow from the tile alone.
 template <typename V, int B = kPartialBatch>
-__device__ __forceinline__ V pull_slots(const MeshDev &M, const SnDev &S, int pos) {
+__device__ __forceinline__ V pull_slots(const MD_G MeshDev &M, const SnDev &S, int pos) {
     return sum_partials<V, B>(reinterpret_cast<const V *>(M.upart) + S.ubase, S.uk, S.w + S.noff, pos);
 }
 
@@ -448,7 +449,7 @@ __device__ __forceinline__ void tile_dot_any(const T *P, long long stride, const
 // a shorter chain of dependent loads for twice the registers (launches of few tiles);
 // fwd_tile_lean loads them after the contributions.
 template <int N, typename T>
-__device__ __forceinline__ void fwd_tile(const MeshDev &M, const SnDev &S, int r0, int q0, int qn, int rn) {
+__device__ __forceinline__ void fwd_tile(const MD_G MeshDev &M, const SnDev &S, int r0, int q0, int qn, int rn) {
     using V = typename V4T<T>::type;
     __shared__ V vd[kTile];
     __shared__ V vo[kTile];
@@ -485,7 +486,7 @@ __device__ __forceinline__ void fwd_tile(const MeshDev &M, const SnDev &S, int r
 }
 
 template <typename T>
-__device__ __forceinline__ void fwd_tile_lean(const MeshDev &M, const SnDev &S, int r0, int q0, int qn, int rn) {
+__device__ __forceinline__ void fwd_tile_lean(const MD_G MeshDev &M, const SnDev &S, int r0, int q0, int qn, int rn) {
     using V = typename V4T<T>::type;
     __shared__ V vd[kTile];
     __shared__ V vo[kTile];
@@ -524,7 +525,7 @@ __device__ __forceinline__ void fwd_tile_lean(const MeshDev &M, const SnDev &S, 
 template <typename T, bool Pre>
 __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
     const int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     const TileDev tl = L.tiles ? L.tiles[blockIdx.x] : M.ftile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
     const SnDev &S = tl.s;
     const int r0 = tl.a, q0 = tl.b;
@@ -541,7 +542,7 @@ __global__ __launch_bounds__(kTile) void k_fwd(LaunchSet L, int level) {
 // the owner's partials (xloc gives where they are). bwd_tile loads the tile's panel rows (N =
 // 16, 32 or 64) while t is gathered; bwd_tile_lean loads them afterwards.
 template <int N, typename T>
-__device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q0, int p0, int qn, int pn) {
+__device__ __forceinline__ void bwd_tile(const MD_G MeshDev &M, const SnDev &S, int q0, int p0, int qn, int pn) {
     using V = typename V4T<T>::type;
     __shared__ V tv[kTile];
     const int lane = threadIdx.x, w = S.w;
@@ -581,7 +582,7 @@ __device__ __forceinline__ void bwd_tile(const MeshDev &M, const SnDev &S, int q
 }
 
 template <typename T>
-__device__ __forceinline__ void bwd_tile_lean(const MeshDev &M, const SnDev &S, int q0, int p0, int qn, int pn) {
+__device__ __forceinline__ void bwd_tile_lean(const MD_G MeshDev &M, const SnDev &S, int q0, int p0, int qn, int pn) {
     using V = typename V4T<T>::type;
     __shared__ V tv[kTile];
     const int lane = threadIdx.x, w = S.w;
@@ -620,7 +621,7 @@ __device__ __forceinline__ void bwd_tile_lean(const MeshDev &M, const SnDev &S, 
 template <typename T, bool Pre>
 __global__ __launch_bounds__(kTile) void k_bwd(LaunchSet L, int level) {
     const int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     const TileDev tl = L.tiles ? L.tiles[blockIdx.x] : M.btile[L.tbase[k] + (blockIdx.x - L.prefix[k])];
     const SnDev &S = tl.s;
     const int q0 = tl.a, p0 = tl.b;
@@ -663,7 +664,7 @@ __device__ __forceinline__ int sub_find(const SubNode *N, int a, int b, int t) {
 }
 
 // Copies the records [sd.x, sd.y) of a subtree to LDS.
-__device__ __forceinline__ void sub_records(const MeshDev &M, const int4 &sd, SubNode *N) {
+__device__ __forceinline__ void sub_records(const MD_G MeshDev &M, const int4 &sd, SubNode *N) {
     int4 *dst = reinterpret_cast<int4 *>(N);
     for (int i = threadIdx.x; i < 4 * (sd.y - sd.x); i += kTile) dst[i] = M.snode[4 * sd.x + i];
     __syncthreads();
@@ -685,7 +686,7 @@ __global__ __launch_bounds__(kTile) void k_sub_fwd(LaunchSet L, int node_cap) {
     SubNode *N = reinterpret_cast<SubNode *>(sub_smem);
     V *F = reinterpret_cast<V *>(sub_smem + sizeof(SubNode) * node_cap);
     const int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     const int lane = threadIdx.x;
     const int bk = blockIdx.x - L.prefix[k];
     if (Fused && bk >= M.nsub) {
@@ -772,7 +773,7 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
     SubNode *N = reinterpret_cast<SubNode *>(sub_smem);
     V *F = reinterpret_cast<V *>(sub_smem + sizeof(SubNode) * node_cap);
     const int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     const int lane = threadIdx.x;
     const V *X = reinterpret_cast<const V *>(M.xpart);
     const int bk = blockIdx.x - L.prefix[k];
@@ -860,9 +861,9 @@ __global__ __launch_bounds__(kTile) void k_sub_bwd(LaunchSet L, int node_cap) {
 // on 256 consecutive free vertices; dot products are per-block partials that each block of
 // the next kernel reduces in the same fixed order (deterministic, no atomics).
 
-template <typename T> __device__ __forceinline__ const T *wts(const MeshDev &M);
-template <> __device__ __forceinline__ const float *wts<float>(const MeshDev &M) { return M.w32; }
-template <> __device__ __forceinline__ const double *wts<double>(const MeshDev &M) { return M.w64; }
+template <typename T> __device__ __forceinline__ const T *wts(const MD_G MeshDev &M);
+template <> __device__ __forceinline__ const float *wts<float>(const MD_G MeshDev &M) { return M.w32; }
+template <> __device__ __forceinline__ const double *wts<double>(const MD_G MeshDev &M) { return M.w64; }
 
 // Sums the 3-component partials part[0 .. nb) in a fixed order; result broadcast via LDS.
 template <typename T, typename V>
@@ -910,7 +911,7 @@ __device__ __forceinline__ void block_partial(T a, T b, T c, V *part, int blk, T
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_diag(LaunchSet L) {
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (slice >= M.nslices_free) return;
@@ -928,7 +929,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(LaunchSet L) {
     using V = typename V4T<T>::type;
     __shared__ T red[3][kBlock];
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int blk = blockIdx.x - L.prefix[k];
     int i = blk * kBlock + threadIdx.x;
     T a = 0, b = 0, c = 0;
@@ -954,7 +955,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(LaunchSet L) {
     using V = typename V4T<T>::type;
     __shared__ T red[3][kBlock];
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int blk = blockIdx.x - L.prefix[k];
     int slice = blk * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
@@ -991,7 +992,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(LaunchSet L, int parity) 
     using V = typename V4T<T>::type;
     __shared__ T red[3][kBlock];
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int blk = blockIdx.x - L.prefix[k];
     V *part = reinterpret_cast<V *>(M.cg_part);
     T rz[3], pap[3], rz0[3];
@@ -1026,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_direction(LaunchSet L, int parit
     using V = typename V4T<T>::type;
     __shared__ T red[3][kBlock];
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int blk = blockIdx.x - L.prefix[k];
     const V *part = reinterpret_cast<const V *>(M.cg_part);
     T rn[3], ro[3];
@@ -1049,7 +1050,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_apply(LaunchSet L) {
     using V = typename V4T<T>::type;
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int i = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
     if (i >= M.nf) return;
     V x = reinterpret_cast<const V *>(M.cg_x)[i];
@@ -1064,7 +1065,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_apply(LaunchSet L) {
 // three doubles per face into fn.
 __global__ __launch_bounds__(kBlock) void k_face_normals(LaunchSet L, double *fn) {
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int f = (blockIdx.x - L.prefix[k]) * kBlock + threadIdx.x;
     if (f >= M.F) return;
     d4 a = M.pp[M.tri[3 * f]], b = M.pp[M.tri[3 * f + 1]], c = M.pp[M.tri[3 * f + 2]];
@@ -1080,7 +1081,7 @@ __global__ __launch_bounds__(kBlock) void k_face_normals(LaunchSet L, double *fn
 // face normals; normalised (zero stays zero). Output in internal order.
 __global__ __launch_bounds__(kBlock) void k_vertex_normals(LaunchSet L, const double *fn, d4 *out) {
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int slice = (blockIdx.x - L.prefix[k]) * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (slice >= M.nslices) return;
@@ -1111,7 +1112,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(LaunchSet L) {
     using V = typename V4T<T>::type;
     __shared__ double red[kBlock];
     int k = find_mesh(L, blockIdx.x);
-    const MeshDev &M = L.meshes[k];
+    const MD_G MeshDev &M = L.meshes[k];
     int lb = blockIdx.x - L.prefix[k];
     int slice = lb * kSlicesPerBlock + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
