@@ -363,8 +363,8 @@ template <typename T> __device__ __forceinline__ const T *tptr(const void *p) {
 // one load; more are loaded 8 at a time (clamped addresses, masked values) so their
 // latencies overlap.
 constexpr int kPartialBatch = 8;
-// The same in the panel-preloading backward tiles (few waves: registers to spare), so the 9-16
-// partials of a wide owner's x arrive in one round.
+// The same in the panel-preloading tiles (few waves: registers to spare), so the 9-16 partials
+// of a wide owner's x, or the 9-16 update slots of a wide front's position, arrive in one round.
 constexpr int kWidePartialBatch = 16;
 // Level launches of at most this many tiles load their panel entries ahead of the vector
 // (latency-bound: few waves); larger ones keep the register count low for occupancy.
@@ -464,11 +464,11 @@ __device__ __forceinline__ void fwd_tile(const MD_G MeshDev &M, const SnDev &S, 
     if (rn > 0) panel_load<N, T>(tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1), no, qn, g);
     const long long dst = lane < rn ? M.udst[S.roff + r0 + lane] : 0;  // this row's slot in the parent
     if (lane < qn) {
-        const V c = pull_slots<V>(M, S, q0 + lane);
+        const V c = pull_slots<V, kWidePartialBatch>(M, S, q0 + lane);
         b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
     }
     vd[lane] = b;
-    if (kcb == 0 && lane < rn) o = pull_slots<V>(M, S, w + r0 + lane);
+    if (kcb == 0 && lane < rn) o = pull_slots<V, kWidePartialBatch>(M, S, w + r0 + lane);
     vo[lane] = o;
     __syncthreads();
     if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
