@@ -397,6 +397,43 @@ __device__ __forceinline__ V pull_slots(const MD_G MeshDev &M, const SnDev &S, i
     return sum_partials<V, B>(reinterpret_cast<const V *>(M.upart) + S.ubase, S.uk, S.w + S.noff, pos);
 }
 
+// The update-slot sums of two front positions (a: ona lanes, b: onb lanes), their loads
+// issued together, B slots of each per round: the first column tile of a forward tile needs
+// both its columns' and its off rows' sums, and one round trip instead of two brings them. The
+// sums run in slot order, as pull_slots' do (inactive lanes read position 0 and add nothing).
+template <typename V, int B>
+__device__ __forceinline__ void pull_slots_pair(const MD_G MeshDev &M, const SnDev &S, int pa, bool ona, int pb,
+                                                bool onb, V &a, V &b) {
+    using T = decltype(V{}.x);
+    const MD_G V *U = reinterpret_cast<const MD_G V *>(M.upart) + S.ubase;
+    const int cnt = S.uk, stride = S.w + S.noff;
+    const int ea = ona ? pa : 0, eb = onb ? pb : 0;
+    a.x = 0; a.y = 0; a.z = 0; a.w = 0;
+    b = a;
+    if (cnt == 1) {
+        const V ua = U[ea], ub = U[eb];
+        if (ona) a = ua;
+        if (onb) b = ub;
+        return;
+    }
+    for (int k0 = 0; k0 < cnt; k0 += B) {
+        V ua[B], ub[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const long long row = (long long)min(k0 + u, cnt - 1) * stride;
+            ua[u] = U[row + ea];
+            ub[u] = U[row + eb];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const T m = (k0 + u < cnt) ? T(1) : T(0);
+            const T ma = ona ? m : T(0), mb = onb ? m : T(0);
+            a.x += ma * ua[u].x; a.y += ma * ua[u].y; a.z += ma * ua[u].z;
+            b.x += mb * ub[u].x; b.y += mb * ub[u].y; b.z += mb * ub[u].z;
+        }
+    }
+}
+
 // g[i] = P[min(i, n - 1) * stride] for i < N (n >= 1): a column of a panel tile loaded into
 // registers ahead of the vector it multiplies; entries past n are valid panel values that meet
 // zero vector entries.
@@ -463,12 +500,16 @@ __device__ __forceinline__ void fwd_tile(const MD_G MeshDev &M, const SnDev &S, 
     T g[N];
     if (rn > 0) panel_load<N, T>(tptr<T>(M.pc) + S.pc_off + (long long)q0 * no + r0 + min(lane, rn - 1), no, qn, g);
     const long long dst = lane < rn ? M.udst[S.roff + r0 + lane] : 0;  // this row's slot in the parent
-    if (lane < qn) {
+    if (kcb == 0) {
+        // columns and off rows: their slots in one round trip
+        V c;
+        pull_slots_pair<V, kPartialBatch>(M, S, q0 + lane, lane < qn, w + r0 + lane, lane < rn, c, o);
+        if (lane < qn) { b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0; }
+    } else if (lane < qn) {
         const V c = pull_slots<V, kWidePartialBatch>(M, S, q0 + lane);
         b.x += c.x; b.y += c.y; b.z += c.z; b.w = 0;
     }
     vd[lane] = b;
-    if (kcb == 0 && lane < rn) o = pull_slots<V, kWidePartialBatch>(M, S, w + r0 + lane);
     vo[lane] = o;
     __syncthreads();
     if (r0 == 0 && lane < qn) reinterpret_cast<V *>(M.z)[S.c0 + q0 + lane] = vd[lane];
