@@ -11,7 +11,7 @@ Legs:
           K timed iterations replayed from hipGraphs; iterations/s.
   check   position error of that path (and of the FP64 path) against the FP64 CPU oracle
           after the same number of iterations from the same start (100, as many as fit the
-          CPU time cap), and the ARAP energy of each; the same for the Jacobi-PCG alternative
+          CPU time cap), the ARAP energy and iterations/s of each; the same for the Jacobi-PCG alternative
           of the global step (FP32, 1000 CG iterations per solve), with its own time per
           iteration.
   cpu     the oracle (single-threaded C restatement of inc/deform/arap.h) on the same mesh:
@@ -116,7 +116,7 @@ def check_leg(ctx, pr, iters, cpu_seconds_cap, budget, ratio):
     assert a.deform(n_it - 1)
     rest_s = time.perf_counter() - t0
     cpu_iter_s = (one + rest_s) / n_it
-    errs, energies = {}, {"oracle": a.energy()}
+    errs, energies, rates = {}, {"oracle": a.energy()}, {}
     for name, prec in (("fp32_refine", mdarap.PREC_FP32_REFINE), ("fp64", mdarap.PREC_FP64)):
         # the plain leg's factor and sweep layout
         m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=prec, subtree_budget=budget, amalg_ratio=ratio))
@@ -124,6 +124,8 @@ def check_leg(ctx, pr, iters, cpu_seconds_cap, budget, ratio):
         assert m.deform(n_it)
         errs[name] = meshgen.relative_error(m.mesh, a.mesh)
         energies[name] = m.energy()
+        ev_ms, _ = m.time_iterations(1, 20)  # HIP events over 20 further iterations
+        rates[name] = round(2e4 / ev_ms, 2)
         m.close()
     # the Jacobi-PCG alternative of the global step (FP32, warm-started from the previous
     # iterate, FP64 residual): CG iterations per solve for the 1e-5 bound on this mesh
@@ -137,7 +139,7 @@ def check_leg(ctx, pr, iters, cpu_seconds_cap, budget, ratio):
     m.close()
     pcg = {"cg_iterations_per_solve": PCG_ITERS, "ms_per_iteration": round(pcg_ms / 10, 3),
            "iters_per_s": round(1e4 / pcg_ms, 2)}
-    return dict(setup_s=setup_s, iters=n_it, iter_s=cpu_iter_s, errors=errs, energy=energies, pcg=pcg)
+    return dict(setup_s=setup_s, iters=n_it, iter_s=cpu_iter_s, errors=errs, energy=energies, pcg=pcg, rates=rates)
 
 
 def served_problems(count, lo, hi, seed):
@@ -465,6 +467,7 @@ def main():
         out["rel_err_iters"] = ck["iters"]
         out["arap_energy"] = {k: float("%.10e" % v) for k, v in ck["energy"].items()}
         out["pcg_alternative"] = ck["pcg"]
+        out["iters_per_s_by_precision"] = ck["rates"]  # the check meshes, same factor and layout
         out["cpu_baseline"] = {"value": round(1.0 / ck["iter_s"], 3), "unit": "iters/s", "cores": 1,
                                "kind": "port",
                                "sample": "oracle (FP64 C restatement of arap.h, up-looking sparse LDL^T, "
