@@ -21,7 +21,8 @@ Legs:
           trajectory and runs 5 warm-started iterations; p50/p99 latency (arrival to result
           on the host) and aggregate iterations/s.
   batched (--batched) 256 meshes of 1k-200k vertices iterated together (20 iterations)
-          against the same meshes iterated one by one.
+          against the same meshes iterated one by one; each side with its own factor shape
+          (the batch without level pairs, the one-by-one meshes with them).
 
 Prints one JSON line.
 """
@@ -213,24 +214,29 @@ def served_leg(ctx, count, seconds, rate, iters, seed):
 
 def batched_leg(ctx, count, iters, seed, tune=True, baseline=True):
     probs = meshgen.mixed_batch(count, 1000, 200000, seed=seed)
-    meshes = []
-    for pr in probs:
-        idx, tgt = pr.constraints()
-        m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE))
-        m.set_constraints(idx, tgt)
-        meshes.append(m)
-    t0 = time.perf_counter()
-    mdarap.prepare_many(ctx, meshes)
-    setup_s = time.perf_counter() - t0
-    # plain: each mesh with its own tuned sweep layout, iterated alone (its own graph), back
-    # to back
-    t0 = time.perf_counter()
-    if tune and baseline:
-        for m in meshes:
-            m.autotune(iters=3)
-    tune_plain_s = time.perf_counter() - t0
-    plain_s = None
+
+    def prepared(level_pairs):
+        out = []
+        for pr in probs:
+            idx, tgt = pr.constraints()
+            m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=mdarap.PREC_FP32_REFINE,
+                                                               level_pairs=level_pairs))
+            m.set_constraints(idx, tgt)
+            out.append(m)
+        mdarap.prepare_many(ctx, out)
+        return out
+
+    # plain: each mesh with its own tuned sweep layout and the library's default factor shape
+    # (level pairs: fewer launches, the better choice for a mesh iterated alone), iterated alone
+    # (its own graphs), back to back
+    tune_plain_s, plain_s = 0.0, None
     if baseline:
+        meshes = prepared(None)
+        t0 = time.perf_counter()
+        if tune:
+            for m in meshes:
+                m.autotune(iters=3)
+        tune_plain_s = time.perf_counter() - t0
         for m in meshes:
             m.iterate(1)  # graph capture
         ctx.synchronize()
@@ -238,6 +244,13 @@ def batched_leg(ctx, count, iters, seed, tune=True, baseline=True):
         for m in meshes:
             m.iterate(iters)
         plain_s = time.perf_counter() - t0
+        for m in meshes:
+            m.close()
+    # batched: the same meshes prepared for batch use (no level pairs: a batch is bound by
+    # bytes rather than launches, and pairs add panel bytes), one launch per kernel for all
+    t0 = time.perf_counter()
+    meshes = prepared(False)
+    setup_s = time.perf_counter() - t0
     b = mdarap.Batch(ctx, meshes)
     t0 = time.perf_counter()
     budget = b.autotune()[0] if tune else meshes[0].stats()["subtree_budget"]
@@ -255,7 +268,7 @@ def batched_leg(ctx, count, iters, seed, tune=True, baseline=True):
                 achieved_GBs=round(achieved, 1), hbm_frac=round(achieved / PEAK_HBM_GBS, 4),
                 plain_iters_per_s=None if plain_s is None else total_iters / plain_s,
                 speedup=None if plain_s is None else (total_iters / (ms / 1e3)) / (total_iters / plain_s),
-                batch_subtree_budget=budget,
+                batch_subtree_budget=budget, batch_level_pairs=0, plain_level_pairs=mdarap.config().level_pairs,
                 setup_s=setup_s, autotune_plain_s=tune_plain_s, autotune_batch_s=tune_batch_s)
 
 
