@@ -22,7 +22,7 @@ REST_REFERENCE, REST_FIXED = 0, 1
 # candidate bottom-subtree budgets of the sweep-layout autotuner (LDS vector entries; 0: none)
 AUTOTUNE_BUDGETS = (0, 128, 256, 512, 1024)
 # candidate relaxed-amalgamation ratios of the factor-layout autotuner
-AUTOTUNE_RATIOS = (0.3, 0.5, 0.7, 0.9)
+AUTOTUNE_RATIOS = (0.3, 0.5, 0.7, 0.8, 0.9)
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
