@@ -78,7 +78,7 @@ def plain_leg(ctx, pr, steps, warmup, tune=True, budget=None, fixed_ratio=None):
     # this mesh; the mesh is left freshly prepared with them
     ratio = fixed_ratio
     if tune and budget is None and fixed_ratio is None:
-        ratio, budget, _ = m.autotune_factor()
+        ratio, budget, _ = m.autotune_factor(iters=30)
     else:
         budget = m.stats()["subtree_budget"]
     tune_s = time.perf_counter() - t0
