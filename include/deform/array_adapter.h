@@ -17,30 +17,30 @@ namespace deform {
 
 template <class S = double> class ArrayAdapter {
 public:
-    /** Floating point precision type of mesh. Required. */
+    /** Scalar type of the stored positions (concept member). */
     typedef S Scalar;
     typedef Vector3<S> VertexType;
     typedef Vector3<int> FaceType;
 
     ArrayAdapter(std::vector<S> &xyz, const std::vector<int> &tri) : _xyz(xyz), _tri(tri) {}
 
-    /** Retrieve a specific vertex position. Required. */
+    /** Position of vertex idx (concept member). */
     VertexType vertexLocation(int idx) const {
         return VertexType(_xyz[3 * (size_t)idx], _xyz[3 * (size_t)idx + 1], _xyz[3 * (size_t)idx + 2]);
     }
-    /** Set a specific vertex position. Required. */
+    /** Overwrites the position of vertex idx (concept member). */
     void vertexLocation(int idx, const VertexType &v) {
         _xyz[3 * (size_t)idx] = v[0];
         _xyz[3 * (size_t)idx + 1] = v[1];
         _xyz[3 * (size_t)idx + 2] = v[2];
     }
-    /** Retrieve triangular vertex indices of face (counter-clockwise). Required. */
+    /** The three vertex indices of face idx, counter-clockwise (concept member). */
     FaceType face(int idx) const {
         return FaceType(_tri[3 * (size_t)idx], _tri[3 * (size_t)idx + 1], _tri[3 * (size_t)idx + 2]);
     }
-    /** Return the total number of faces. Required. */
+    /** Face count (concept member). */
     int numberOfFaces() const { return (int)(_tri.size() / 3); }
-    /** Return the total number of vertices. Required. */
+    /** Vertex count (concept member). */
     int numberOfVertices() const { return (int)(_xyz.size() / 3); }
 
 private:

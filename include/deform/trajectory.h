@@ -1,13 +1,20 @@
 /*
  * deform/trajectory.h -- 6-DoF keyframe trajectory, counterpart of inc/deform/trajectory.h
- * of cheind/mesh-deform (TrajectorySE3, trajectory.h:33-83), without Eigen or Sophus.
+ * of cheind/mesh-deform (TrajectorySE3, trajectory.h:33-83), computed without Eigen or Sophus.
  *
- * Keyframes are 4x4 rigid transforms (row-major, deform::Transform). Evaluation follows the
- * reference: each keyframe is mapped to se(3) by the SE3 logarithm (tangent (upsilon, omega),
- * Sophus' convention), a cubic B-spline interpolates the tangents at normalised chord-length
- * parameters with averaged knots (Eigen's SplineFitting<..>::Interpolate(points, 3)), and the
- * spline value is mapped back by the SE3 exponential. With four keyframes the spline is one
- * cubic Bezier segment.
+ * Evaluation follows the reference: each keyframe is mapped to se(3) by the SE3 logarithm
+ * (tangent (upsilon, omega), Sophus' convention), a cubic B-spline interpolates the tangents at
+ * normalised chord-length parameters with averaged knots (Eigen's
+ * SplineFitting<..>::Interpolate(points, 3)), and the spline value is mapped back by the SE3
+ * exponential. With four keyframes the spline is one cubic Bezier segment.
+ *
+ * Transform types. The reference takes and returns Eigen::Transform<Scalar, 3, Affine>
+ * (trajectory.h:46,55-73). Here the transform type is a template parameter: anything that
+ * reads and writes as a 4x4 affine matrix -- t.matrix()(r, c) (Eigen::Transform and Eigen
+ * matrices), t.m[r][c] (deform::Transform below) or t(r, c) -- and has a static Identity().
+ * The default is Eigen::Transform<Scalar, 3, Eigen::Affine> where Eigen is on the include path,
+ * so code written against the reference compiles unchanged, and deform::Transform<Scalar>
+ * otherwise. Only the affine rows 0-2 are read and written; the transforms are rigid.
  */
 #ifndef DEFORM_TRAJECTORY_H
 #define DEFORM_TRAJECTORY_H
@@ -16,7 +23,14 @@
 #include <array>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
+#include <utility>
 #include <vector>
+
+#if __has_include(<Eigen/Geometry>)
+#include <Eigen/Geometry>
+#define DEFORM_HAVE_EIGEN_TRANSFORM 1
+#endif
 
 namespace deform {
 
@@ -53,6 +67,49 @@ template <class S> struct Transform {
         return q;
     }
 };
+
+namespace detail {
+
+// Priority tags: the first matching access form wins.
+struct xf_p0 {};
+struct xf_p1 : xf_p0 {};
+struct xf_p2 : xf_p1 {};
+
+template <class T> auto xf_at(T &t, int r, int c, xf_p2) -> decltype(t.matrix()(r, c)) { return t.matrix()(r, c); }
+template <class T> auto xf_at(T &t, int r, int c, xf_p1) -> decltype(t.m[r][c]) { return t.m[r][c]; }
+template <class T> auto xf_at(T &t, int r, int c, xf_p0) -> decltype(t(r, c)) { return t(r, c); }
+
+/** Entry (r, c) of a 4x4 affine transform of the caller's type (a reference when t is mutable). */
+template <class T> auto xf_at(T &t, int r, int c) -> decltype(xf_at(t, r, c, xf_p2())) {
+    return xf_at(t, r, c, xf_p2());
+}
+
+/** The caller's transform as a deform::Transform (affine rows 0-2; row 3 is [0 0 0 1]). */
+template <class S, class T> Transform<S> to_rigid(const T &t) {
+    Transform<S> o = Transform<S>::Identity();
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) o.m[r][c] = (S)xf_at(t, r, c);
+    return o;
+}
+
+/** A deform::Transform as the caller's transform type (T::Identity() with rows 0-2 set). */
+template <class T, class S> T from_rigid(const Transform<S> &x) {
+    T t = T::Identity();
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) {
+            auto &e = xf_at(t, r, c);
+            e = (typename std::decay<decltype(e)>::type)x.m[r][c];
+        }
+    return t;
+}
+
+}  // namespace detail
+
+#ifdef DEFORM_HAVE_EIGEN_TRANSFORM
+template <class S> using DefaultTransform = Eigen::Transform<S, 3, Eigen::Affine>;
+#else
+template <class S> using DefaultTransform = Transform<S>;
+#endif
 
 namespace se3 {
 
@@ -160,23 +217,28 @@ inline std::vector<double> basis(const std::vector<double> &knots, int p, double
 
 }  // namespace spline
 
-/** Keyframe trajectory interpolated in se(3) (trajectory.h:33-83). */
-template <class PrecisionType> class TrajectorySE3 {
+/** Keyframe trajectory interpolated in se(3) (trajectory.h:33-83); key poses and results are of
+ *  the caller's transform type (Eigen::Transform<Scalar, 3, Affine> by default where Eigen is
+ *  available, as in the reference). */
+template <class PrecisionType, class TransformType = DefaultTransform<PrecisionType>> class TrajectorySE3 {
 public:
     typedef PrecisionType Scalar;
-    typedef deform::Transform<Scalar> Transform;
+    typedef TransformType Transform;
 
     TrajectorySE3() : _dirty(false), _p(3) {}
 
-    /** addKeyPose (trajectory.h:55-59). */
+    /** addKeyPose (trajectory.h:55-59): appends a key pose and returns it. */
     Transform addKeyPose(const Transform &t) {
-        _poses.push_back(t);
+        _poses.push_back(detail::to_rigid<Scalar>(t));
         _dirty = true;
         return t;
     }
 
-    /** operator()(time) (trajectory.h:61-73). */
-    Transform operator()(Scalar time) {
+    /** operator()(time) (trajectory.h:61-73): the pose at time in [0, 1]. */
+    Transform operator()(Scalar time) { return detail::from_rigid<Transform>(rigid(time)); }
+
+    /** The pose at time as a deform::Transform. */
+    deform::Transform<Scalar> rigid(Scalar time) {
         if (_dirty) fit();
         std::vector<double> N = spline::basis(_knots, _p, (double)time);
         Scalar xi[6];
@@ -240,7 +302,7 @@ private:
         _dirty = false;
     }
 
-    std::vector<Transform> _poses;
+    std::vector<deform::Transform<Scalar>> _poses;
     bool _dirty;
     int _p;
     std::vector<double> _knots;

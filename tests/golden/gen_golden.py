@@ -141,7 +141,8 @@ def main(ref):
     # sphere: anchor 37 at rest, handle 32 lifted by 0.5 (deform_sphere.cpp:60-91), 5 iterations
     cases.append(("sphere_obj", sph_xyz, sph_tri, np.array([37, 32], np.int32),
                   np.stack([sph_xyz[37], sph_xyz[32] + [0, 0, 0.5]]), 5))
-    # bar: anchors at rest, handles rotated 90 degrees about x (deform_bar.cpp:146-152), 20 iterations
+    # bar: anchors at rest (deform_bar.cpp:70-73), handles rotated about x as applyRotationToHandles
+    # does (deform_bar.cpp:47-53,77-78; here by 90 degrees in one step), 20 iterations (:94)
     rx = meshgen.rot_x(np.pi / 2)
     cases.append(("bar_obj", bar_xyz, bar_tri, np.concatenate([bar_anchors, bar_handles]),
                   np.concatenate([bar_xyz[bar_anchors], bar_xyz[bar_handles] @ rx.T]), 20))

@@ -131,3 +131,18 @@ def test_cpp_header_matches_python_restatement():
         vals = [float(v) for v in line.split()[1:]]
         t, T = vals[0], np.array(vals[1:]).reshape(4, 4)
         np.testing.assert_allclose(T, path(t), atol=1e-12)
+
+
+def test_cpp_transform_types_like_eigen():
+    """trajectory.h and deformation_util.h with a transform type read as t.matrix()(r, c) (as
+    Eigen::Transform is): the calls of tests/test_trajectory.cpp:17-38 and of
+    examples/deform_trajectory.cpp:53-56,84 compile as written there, the endpoint test
+    passes, and DeformationUtil's handle targets equal O T O^-1 p0."""
+    import os
+    import subprocess
+
+    cpp = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp")
+    subprocess.check_call(["make", "-s", "-C", cpp, "test_transform_types"])
+    out = subprocess.run([os.path.join(cpp, "test_transform_types")], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().endswith("transform types: ok")
