@@ -19,10 +19,16 @@ Legs:
   served  64 resident meshes (log-uniform 5k-164k vertices), Poisson drag requests at
           2000/s; each request moves its handles to a point of a seeded 4-keyframe SE3
           trajectory and runs 5 warm-started iterations; p50/p99 latency (arrival to result
-          on the host) and aggregate iterations/s.
-  batched (--batched) 256 meshes of 1k-200k vertices iterated together (20 iterations)
-          against the same meshes iterated one by one; each side with its own factor shape
-          (the batch without level pairs, the one-by-one meshes with them).
+          on the host) and aggregate iterations/s; then a saturation sweep (4k-32k requests/s,
+          until p99 passes 16 ms: the capacity), compared with the same meshes iterated one by
+          one, and one stream under an HBM budget of half the meshes' footprint.
+  frames  the reference's interactive loop, setConstraint + deform(5) per frame in the
+          reference's rest mode (each frame re-runs the cotan weights and the refactorization),
+          on the 67k bar (FP64) and the 164k icosphere (FP64, FP32 + refinement): frames/s
+          and the library's milliseconds per frame by phase.
+  batched 256 meshes of 1k-200k vertices iterated together (20 iterations) against the
+          same meshes iterated one by one; each side with its own factor shape (the batch
+          without level pairs, the one-by-one meshes with them); --no-batched skips it.
 
 Prints one JSON line.
 """
@@ -143,35 +149,54 @@ def check_leg(ctx, pr, iters, cpu_seconds_cap, budget, ratio):
     return dict(setup_s=setup_s, iters=n_it, iter_s=cpu_iter_s, errors=errs, energy=energies, pcg=pcg, rates=rates)
 
 
+def frames_leg(ctx, pr, frames, iters, precision):
+    """The reference's interactive loop (examples/deform_sphere.cpp:71-91,
+    examples/deform_bar.cpp:84-94): every frame moves the handles with setConstraint and calls
+    deform(iters). In the reference's rest mode every such call is dirty: it takes the current
+    mesh as the rest pose, recomputes the cotan weights and refactors (arap.h:84,102-120); the
+    constrained set does not change, so the host analysis of the first call is kept. Frames/s
+    of the whole loop (host copies of the Python binding included) and the library's own
+    milliseconds per frame by phase (medians)."""
+    idx, _ = pr.constraints()
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(precision=precision))
+    m.set_constraints(idx, pr.xyz[idx])
+    assert m.deform(0)  # the first call analyses the constrained set
+    first = m.stats()["deform_ms"]
+    h = pr.handles
+    start = pr.xyz[h].copy()
+    ph = []
+    t0 = time.perf_counter()
+    for f in range(frames):
+        s = (f + 1.0) / frames
+        m.set_constraints(h, start + s * (pr.handle_targets - start))
+        assert m.deform(iters)
+        ph.append(m.stats()["deform_ms"])
+    wall = time.perf_counter() - t0
+    m.close()
+    ph = np.median(np.array(ph), axis=0)
+    return dict(workload=pr.name, n=pr.n, precision="fp64" if precision == mdarap.PREC_FP64 else "fp32_refine",
+                frames=frames, iterations_per_frame=iters, frames_per_s=round(frames / wall, 2),
+                library_ms_per_frame=round(float(ph[5]), 3),
+                phases_ms={"upload_and_cotan": round(float(ph[1]), 3), "factor": round(float(ph[2]), 3),
+                           "iterations": round(float(ph[3]), 3), "download": round(float(ph[4]), 3)},
+                first_call_analysis_ms=round(float(first[0]), 1))
+
+
 def served_problems(count, lo, hi, seed):
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), size=count))
     return [meshgen.mixed_problem(k, int(s), rng) for k, s in enumerate(sizes)]
 
 
-def served_leg(ctx, count, seconds, rate, iters, seed):
-    rng = np.random.default_rng(seed)
-    probs = served_problems(count, 5000, 164000, seed)
-    meshes, utils, trajs = [], [], []
-    t0 = time.perf_counter()
-    for pr in probs:
-        idx, _ = pr.constraints()
-        m = mdarap.Mesh(ctx, pr.xyz, pr.tri,
-                        mdarap.config(precision=mdarap.PREC_FP32_REFINE, rest_mode=mdarap.REST_FIXED))
-        m.set_constraints(idx, pr.xyz[idx])
-        meshes.append(m)
-        origin = np.eye(4)
-        origin[:3, 3] = pr.xyz[pr.handles].mean(axis=0)  # handles pivot about their centroid
-        utils.append(tj.DeformationUtil(pr.xyz, pr.handles, origin=origin))
-        trajs.append(tj.random_trajectory(rng))
-    mdarap.prepare_many(ctx, meshes)
-    for m in meshes:
-        m.autotune(iters=3)
-    setup_s = time.perf_counter() - t0
-    srv = mdarap.Server(ctx)
+def served_stream(ctx, probs, meshes, utils, trajs, rng, rate, seconds, iters, budget=0):
+    """One Poisson stream of drag requests at `rate` per second for `seconds` against a fresh
+    server holding the meshes (their state carries over from earlier streams). Latency is
+    arrival to result on the host; the aggregate is ARAP iterations per second of wall time
+    from the first arrival to the last result."""
+    srv = mdarap.Server(ctx, hbm_budget_bytes=int(budget))
     ids = [srv.add_mesh(m) for m in meshes]
     # Poisson arrivals; targets precomputed so the submit loop only submits
-    nreq = int(rate * seconds)
+    nreq = max(1, int(rate * seconds))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
     picks = rng.integers(len(probs), size=nreq)
     times = rng.uniform(0.0, 1.0, size=nreq)
@@ -200,16 +225,74 @@ def served_leg(ctx, count, seconds, rate, iters, seed):
         tickets[i] = srv.submit(ids[k], h, t, iters)
     done.wait(timeout=600)
     elapsed = time.perf_counter() - start
+    st = srv.stats()
     srv.close()
     total = lat + lag * 1e3
+    return dict(rate_per_s=rate, requests=nreq, p50_ms=round(float(np.percentile(total, 50)), 4),
+                p99_ms=round(float(np.percentile(total, 99)), 4), max_ms=round(float(total.max()), 4),
+                iters_per_s=round(float(nreq * iters / elapsed), 1),
+                submit_lag_p99_ms=round(float(np.percentile(lag, 99) * 1e3), 4),
+                launches=st["iterations"], mean_requests_per_launch=round(st["mesh_iterations"] / max(1, st["iterations"]), 2),
+                evictions=st["evictions"], restores=st["restores"], peak_resident_bytes=st["peak_resident_bytes"])
+
+
+def served_leg(ctx, count, seconds, rate, iters, seed, sweep=(4000, 8000, 16000, 32000), p99_bound_ms=16.0):
+    """The served workload at `rate` (the headline), then a saturation sweep over higher rates
+    until p99 exceeds the bound, then one stream at `rate` under an HBM budget of half the
+    meshes' footprint (evictions to host memory)."""
+    rng = np.random.default_rng(seed)
+    probs = served_problems(count, 5000, 164000, seed)
+    meshes, utils, trajs = [], [], []
+    t0 = time.perf_counter()
+    for pr in probs:
+        idx, _ = pr.constraints()
+        m = mdarap.Mesh(ctx, pr.xyz, pr.tri,
+                        mdarap.config(precision=mdarap.PREC_FP32_REFINE, rest_mode=mdarap.REST_FIXED))
+        m.set_constraints(idx, pr.xyz[idx])
+        meshes.append(m)
+        origin = np.eye(4)
+        origin[:3, 3] = pr.xyz[pr.handles].mean(axis=0)  # handles pivot about their centroid
+        utils.append(tj.DeformationUtil(pr.xyz, pr.handles, origin=origin))
+        trajs.append(tj.random_trajectory(rng))
+    mdarap.prepare_many(ctx, meshes)
+    for m in meshes:
+        m.autotune(iters=3)
+    setup_s = time.perf_counter() - t0
+    footprint = sum(m.stats()["device_bytes"] for m in meshes)
+    # the same meshes iterated one by one (each alone, its own graphs), `iters` iterations each
+    for m in meshes:
+        m.iterate(1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for m in meshes:
+        m.iterate(iters)
+    plain_same = count * iters / (time.perf_counter() - t0)
+    head = served_stream(ctx, probs, meshes, utils, trajs, rng, rate, seconds, iters)
+    sweep_out, capacity = [], head if head["p99_ms"] < p99_bound_ms else None
+    for r in sweep:
+        log("served sweep: %.0f requests/s" % r)
+        x = served_stream(ctx, probs, meshes, utils, trajs, rng, r, min(seconds, 1.0), iters)
+        sweep_out.append(x)
+        if x["p99_ms"] >= p99_bound_ms:
+            break
+        capacity = x
+    log("served: %.0f requests/s under a budget of half the footprint" % rate)
+    budgeted = served_stream(ctx, probs, meshes, utils, trajs, rng, rate, min(seconds, 1.0), iters,
+                             budget=footprint // 2)
+    budgeted["hbm_budget_bytes"] = int(footprint // 2)
     sizes = [p.n for p in probs]
     for m in meshes:
         m.close()
-    return dict(meshes=count, total_vertices=int(sum(sizes)), requests=nreq, rate_per_s=rate,
-                iters_per_request=iters, p50_ms=float(np.percentile(total, 50)),
-                p99_ms=float(np.percentile(total, 99)), max_ms=float(total.max()),
-                iters_per_s=float(nreq * iters / elapsed), setup_s=setup_s,
-                submit_lag_p99_ms=float(np.percentile(lag, 99) * 1e3))
+    out = dict(meshes=count, total_vertices=int(sum(sizes)), iters_per_request=iters, setup_s=round(setup_s, 3),
+               device_bytes=int(footprint), plain_one_by_one_iters_per_s=round(plain_same, 1))
+    out.update(head)
+    out["sweep"] = sweep_out
+    out["capacity"] = None if capacity is None else dict(rate_per_s=capacity["rate_per_s"],
+                                                          p99_ms=capacity["p99_ms"],
+                                                          iters_per_s=capacity["iters_per_s"],
+                                                          vs_plain_one_by_one=round(capacity["iters_per_s"] / plain_same, 3))
+    out["budgeted"] = budgeted
+    return out
 
 
 def batched_leg(ctx, count, iters, seed, tune=True, baseline=True):
@@ -389,8 +472,11 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--served-rate", type=float, default=2000.0)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-frames", action="store_true", help="skip the interactive-frames leg")
+    ap.add_argument("--frames", type=int, default=20)
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--batched", action="store_true", help="(default) kept for older command lines")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (for profiling)")
     ap.add_argument("--no-plain-baseline", action="store_true",
@@ -482,17 +568,22 @@ def main():
         out["pcg_alternative"] = ck["pcg"]
         out["iters_per_s_by_precision"] = ck["rates"]  # the check meshes, same factor and layout
         out["cpu_baseline"] = {"value": round(1.0 / ck["iter_s"], 3), "unit": "iters/s", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", "host_cpu": cpu_model(), "host_nproc": os.cpu_count(),
                                "sample": "oracle (FP64 C restatement of arap.h, up-looking sparse LDL^T, "
                                          "Jacobi SVD) on the same 163842-vertex mesh: %d iterations after a "
                                          "%.2f s setup, host %s" % (ck["iters"], ck["setup_s"], cpu_model()),
                                "setup_s": round(ck["setup_s"], 3)}
+    if not args.no_frames:
+        log("frames leg: setConstraint + deform(5) per frame, reference rest mode")
+        out["frames"] = [frames_leg(ctx, meshgen.bar_problem(), args.frames, 5, mdarap.PREC_FP64),
+                         frames_leg(ctx, pr, args.frames, 5, mdarap.PREC_FP64),
+                         frames_leg(ctx, pr, args.frames, 5, mdarap.PREC_FP32_REFINE)]
     if not args.no_served:
         log("served leg: %d meshes, %.1f s of arrivals at %.0f/s" %
             (args.served_meshes, args.served_seconds, args.served_rate))
         out["served"] = served_leg(ctx, args.served_meshes, args.served_seconds, args.served_rate, 5, 777)
         out["served"]["plain_iters_per_s_at_164k"] = round(value, 2)
-    if args.batched:
+    if not args.no_batched:
         log("batched leg: %d meshes" % args.batched_meshes)
         out["batched"] = batched_leg(ctx, args.batched_meshes, 20, 4242, not args.no_autotune)
     ctx.close()

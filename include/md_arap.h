@@ -10,7 +10,10 @@
  *
  * Threading: a context and everything created from it belong to one host thread at a time
  * (the reference class is single-threaded too). A server (md_server_*) owns an internal
- * worker thread; its submit/poll/wait calls may come from any thread.
+ * worker thread and a stream of its own; its submit/poll/wait/stats calls may come from any
+ * thread. A mesh registered with a server belongs to it: md_mesh_* calls that change or read
+ * the mesh's device state, md_mesh_destroy and batches holding the mesh return MD_ERR_STATE
+ * until the server is destroyed.
  * Ownership: the caller owns every host buffer it passes; the library owns device memory.
  */
 #ifndef MD_ARAP_H
@@ -98,6 +101,11 @@ typedef struct {
     int subtrees;            /* bottom subtrees swept by one wavefront each */
     int64_t factor_nnz_exact;/* strictly-lower nonzeros of the exact factor (no explicit zeros) */
     double factor_ms;        /* wall time of the last numeric factorization (device or host) */
+    /* wall milliseconds of the last md_mesh_deform call by phase: [0] host analysis (a new
+     * constrained set only), [1] rest pose and initial guess upload with the cotan weights,
+     * [2] numeric factorization, [3] iterations, [4] position download and write-back,
+     * [5] the whole call */
+    double deform_ms[6];
 } md_mesh_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -197,16 +205,33 @@ int md_batch_autotune(md_batch *batch, const int *budgets, int count, int iterat
 int md_batch_destroy(md_batch *batch);
 
 /* ---- serving: resident meshes, continuous batching of drag requests ----------------- */
+/* A server iterates the admitted drag requests of many meshes in the same launches (one ARAP
+ * iteration per boundary for the whole set) and admits queued requests at every iteration
+ * boundary. HBM-budgeted residency: a mesh's device footprint (md_mesh_stats.device_bytes)
+ * counts against hbm_budget_bytes while it is resident; a request on a mesh that is not
+ * resident is admitted once the mesh fits, after the least recently used idle meshes are
+ * evicted to pinned host memory (their state is copied back bit for bit when they are next
+ * needed), and otherwise waits until iterations finish and memory frees up. Requests on one
+ * mesh run in submission order, each warm-started from the previous result. */
 typedef struct md_server md_server;
 typedef struct {
     size_t hbm_budget_bytes; /* device bytes the resident meshes may hold (0 = unlimited) */
     int max_active;          /* requests iterated together (0 = unlimited) */
 } md_server_config;
 
+typedef struct {
+    uint64_t submitted, completed, failed;
+    uint64_t iterations;      /* launches: one ARAP iteration of every request in the set */
+    uint64_t mesh_iterations; /* ARAP iterations summed over the requests of every launch */
+    uint64_t evictions, restores;
+    size_t resident_bytes, peak_resident_bytes;
+    int meshes, resident_meshes, queued;
+} md_server_stats;
+
 int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_server **out);
-/* Registers a prepared mesh (md_mesh_prepare done, MD_REST_FIXED); fails with
- * MD_ERR_BUDGET when it does not fit the budget. The server takes no ownership: the mesh
- * must outlive the server. */
+/* Registers a prepared mesh (md_mesh_prepare done, MD_REST_FIXED). MD_ERR_BUDGET when the
+ * mesh alone exceeds the budget; a mesh that does not fit next to the resident ones is
+ * registered evicted. The server takes no ownership: destroy the server before the mesh. */
 int md_server_add_mesh(md_server *srv, md_mesh *mesh, int *mesh_id);
 /* Queues a drag: move the k constrained vertices idx to xyz, then run `iterations`
  * warm-started iterations. Indices must already be constrained on that mesh. */
@@ -215,7 +240,12 @@ int md_server_submit(md_server *srv, int mesh_id, int k, const int *idx, const d
 /* done = 1 when finished; positions (3n, original order, may be NULL) and the latency in
  * milliseconds from submit to completion are returned once done. */
 int md_server_poll(md_server *srv, uint64_t ticket, int *done, double *xyz, double *latency_ms);
+/* poll and wait hand a finished request's result out once: a second call for the ticket (or
+ * a wait whose ticket another call took meanwhile) is MD_ERR_INVALID. */
 int md_server_wait(md_server *srv, uint64_t ticket, double timeout_ms, double *xyz, double *latency_ms);
+int md_server_get_stats(md_server *srv, md_server_stats *out);
+/* Waits for the launches in flight; requests still queued fail with MD_ERR_STATE. Meshes the
+ * server evicted are restored by their next md_mesh_* call. */
 int md_server_destroy(md_server *srv);
 
 /* ---- device diagnostics ------------------------------------------------------------- */
@@ -237,6 +267,15 @@ int md_host_factor_solve(int n, const int *rowptr, const int *col, const double 
                          const double *xyz, int nrhs, const double *b, double *x,
                          int64_t *panel_entries, int *levels, int subtree_budget,
                          int device_factor_emulation, int level_pairs);
+
+/* The local step's rotation (arap.h:376-382) for count covariances cov (row-major 3x3, 9
+ * per entry: sum_j w_ij (p_i - p_j)(p'_i - p'_j)^T) by the kernel's own code compiled for the
+ * host: unit quaternions (x, y, z, w) with w >= 0 into quat (4 per entry). prev (4 per entry,
+ * may be NULL: identity) is the previous rotation, the warm start of the Jacobi fallback and
+ * the result for a zero covariance. fp64 = 0 runs it in FP32, as MD_PREC_FP32_REFINE does.
+ * fallback (may be NULL) receives 1 per entry where the top eigenvalue of Horn's matrix was
+ * too close to the next for the characteristic-polynomial path and Jacobi sweeps decided. */
+int md_host_best_rotation(int fp64, int count, const double *cov, const double *prev, double *quat, int *fallback);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,10 @@
 #define MD_KERNELS_DEVICE 1
 #include "md_kernels.h"
 #include "md_analysis.h"
+#include "md_arap.h"
+
+#include <cmath>
+#include <string>
 
 namespace md {
 
@@ -25,15 +29,35 @@ __device__ __forceinline__ int find_mesh(const LaunchSet &L, int b) {
 // Reciprocal, square root and reciprocal square root of the local step's Jacobi rotations:
 // single hardware instructions in FP32 (about 1 ulp; the rotation is renormalised at the
 // end), correctly rounded in FP64.
-template <typename T> __device__ __forceinline__ T jrcp(T x);
-template <> __device__ __forceinline__ float jrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-template <> __device__ __forceinline__ double jrcp<double>(double x) { return 1.0 / x; }
-template <typename T> __device__ __forceinline__ T jsqrt(T x);
-template <> __device__ __forceinline__ float jsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
-template <> __device__ __forceinline__ double jsqrt<double>(double x) { return sqrt(x); }
-template <typename T> __device__ __forceinline__ T jrsqrt(T x);
-template <> __device__ __forceinline__ float jrsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
-template <> __device__ __forceinline__ double jrsqrt<double>(double x) { return 1.0 / sqrt(x); }
+// (host builds of the rotation code, for the host diagnostic md_host_best_rotation, use the
+// correctly rounded operations)
+template <typename T> __host__ __device__ __forceinline__ T jrcp(T x);
+template <> __host__ __device__ __forceinline__ float jrcp<float>(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+template <> __host__ __device__ __forceinline__ double jrcp<double>(double x) { return 1.0 / x; }
+template <typename T> __host__ __device__ __forceinline__ T jsqrt(T x);
+template <> __host__ __device__ __forceinline__ float jsqrt<float>(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return std::sqrt(x);
+#endif
+}
+template <> __host__ __device__ __forceinline__ double jsqrt<double>(double x) { return sqrt(x); }
+template <typename T> __host__ __device__ __forceinline__ T jrsqrt(T x);
+template <> __host__ __device__ __forceinline__ float jrsqrt<float>(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / std::sqrt(x);
+#endif
+}
+template <> __host__ __device__ __forceinline__ double jrsqrt<double>(double x) { return 1.0 / sqrt(x); }
 
 // std::max<Scalar>(a, b) = (a < b) ? b : a; arap.h:199-218 pass the clamp first, so NaN
 // yields the clamp.
@@ -101,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_cotan_slot(LaunchSet L) {
 // One Jacobi rotation of the symmetric 4x4 matrix a on the pair (P, Q); v accumulates the
 // eigenvectors (columns).
 template <typename T, int P, int Q>
-__device__ __forceinline__ void jrot(T (&a)[4][4], T (&v)[4][4]) {
+__host__ __device__ __forceinline__ void jrot(T (&a)[4][4], T (&v)[4][4]) {
     T apq = a[P][Q];
     T scale = fabs(a[P][P]) + fabs(a[Q][Q]);
     if (!(fabs(apq) > (sizeof(T) == 4 ? T(1e-9) : T(1e-18)) * scale) ) {
@@ -134,15 +158,15 @@ __device__ __forceinline__ void jrot(T (&a)[4][4], T (&v)[4][4]) {
 // R = argmax_{R in SO(3)} tr(R M) for the covariance M = sum_j a_j e'_j^T, returned as the
 // unit quaternion (x, y, z, w). This is the rotation V diag(1,1,det(V U^T)) U^T of the
 // reference's JacobiSVD step (arap.h:376-382), computed as the dominant eigenvector of
-// Horn's symmetric 4x4 matrix (J. Opt. Soc. Am. A 4(4), 1987) by cyclic Jacobi sweeps.
+// Horn's symmetric 4x4 matrix N (J. Opt. Soc. Am. A 4(4), 1987).
 //
-// The sweeps start from the basis of the previous rotation (p = previous quaternion): the
-// left-multiplication matrix L(p) is orthogonal with first column p, so L^T N L is already
-// nearly diagonal when the rotation changed little and one or two sweeps converge. The
-// converged eigenvector does not depend on the start.
+// best_rotation_jacobi: cyclic Jacobi sweeps that start from the basis of the previous
+// rotation (p = previous quaternion): the left-multiplication matrix L(p) is orthogonal with
+// first column p, so L^T N L is already nearly diagonal when the rotation changed little and
+// one or two sweeps converge. The converged eigenvector does not depend on the start.
 template <typename T>
-__device__ __forceinline__ void best_rotation(const T (&m)[9], T px, T py, T pz, T pw, T &qx, T &qy, T &qz,
-                                              T &qw) {
+__host__ __device__ __forceinline__ void best_rotation_jacobi(const T (&m)[9], T px, T py, T pz, T pw, T &qx, T &qy,
+                                                              T &qz, T &qw) {
     const T Sxx = m[0], Sxy = m[1], Sxz = m[2], Syx = m[3], Syy = m[4], Syz = m[5], Szx = m[6],
             Szy = m[7], Szz = m[8];
     const T N[4][4] = {{Sxx + Syy + Szz, Syz - Szy, Szx - Sxz, Sxy - Syx},
@@ -199,8 +223,123 @@ __device__ __forceinline__ void best_rotation(const T (&m)[9], T px, T py, T pz,
     qw = q0 * nrm; qx = q1 * nrm; qy = q2 * nrm; qz = q3 * nrm;
 }
 
+// Cofactor (-1)^(I+J) det(a without row I and column J) of a 4x4 matrix (indices fixed at
+// compile time, so a stays in registers).
+template <typename T, int I, int J>
+__host__ __device__ __forceinline__ T cofactor(const T (&a)[4][4]) {
+    constexpr int r0 = I == 0 ? 1 : 0, r1 = I <= 1 ? 2 : 1, r2 = I <= 2 ? 3 : 2;
+    constexpr int k0 = J == 0 ? 1 : 0, k1 = J <= 1 ? 2 : 1, k2 = J <= 2 ? 3 : 2;
+    const T d = a[r0][k0] * (a[r1][k1] * a[r2][k2] - a[r1][k2] * a[r2][k1]) -
+                a[r0][k1] * (a[r1][k0] * a[r2][k2] - a[r1][k2] * a[r2][k0]) +
+                a[r0][k2] * (a[r1][k0] * a[r2][k1] - a[r1][k1] * a[r2][k0]);
+    return ((I + J) & 1) ? -d : d;
+}
+
+// Column col of adj(a) for a symmetric 4x4 a.
 template <typename T>
-__device__ __forceinline__ void quat_to_mat(T qx, T qy, T qz, T qw, T (&R)[9]) {
+__host__ __device__ __forceinline__ void adj_column(const T (&a)[4][4], int col, T &v0, T &v1, T &v2, T &v3) {
+    if (col == 0) {
+        v0 = cofactor<T, 0, 0>(a); v1 = cofactor<T, 1, 0>(a); v2 = cofactor<T, 2, 0>(a); v3 = cofactor<T, 3, 0>(a);
+    } else if (col == 1) {
+        v0 = cofactor<T, 0, 1>(a); v1 = cofactor<T, 1, 1>(a); v2 = cofactor<T, 2, 1>(a); v3 = cofactor<T, 3, 1>(a);
+    } else if (col == 2) {
+        v0 = cofactor<T, 0, 2>(a); v1 = cofactor<T, 1, 2>(a); v2 = cofactor<T, 2, 2>(a); v3 = cofactor<T, 3, 2>(a);
+    } else {
+        v0 = cofactor<T, 0, 3>(a); v1 = cofactor<T, 1, 3>(a); v2 = cofactor<T, 2, 3>(a); v3 = cofactor<T, 3, 3>(a);
+    }
+}
+
+// best_rotation: the largest eigenvalue lambda of N by Newton's method on its characteristic
+// polynomial P(x) = x^4 + c2 x^2 + c1 x + c0 with c2 = -2 |M|_F^2, c1 = -8 det M, c0 = det N
+// (N is traceless; Theobald, Acta Cryst. A61 (2005) 478-480, "quaternion characteristic
+// polynomial"), started at sqrt(3/2 |c2|) = sqrt(3) |M|_F, an upper bound of lambda (the sum
+// of M's singular values), from where Newton's iterates decrease monotonically to the largest
+// root of a polynomial with real roots. The eigenvector is then the largest column of
+// adj(N - lambda I) = prod_{j>1}(lambda_j - lambda) v v^T, taken at its largest diagonal entry.
+// When that entry is small against |N|^3 the top eigenvalue is (nearly) repeated -- a rank-1
+// or reflection-ambiguous covariance -- the adjugate loses accuracy, and the warm-started
+// Jacobi sweeps decide instead. A zero covariance keeps the previous rotation.
+template <typename T>
+__host__ __device__ __forceinline__ bool best_rotation(const T (&m)[9], T px, T py, T pz, T pw, T &qx, T &qy, T &qz,
+                                                       T &qw) {
+    const T Sxx = m[0], Sxy = m[1], Sxz = m[2], Syx = m[3], Syy = m[4], Syz = m[5], Szx = m[6],
+            Szy = m[7], Szz = m[8];
+    const T f2 = Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz + Szx * Szx + Szy * Szy +
+                 Szz * Szz;
+    const T n00 = Sxx + Syy + Szz, n01 = Syz - Szy, n02 = Szx - Sxz, n03 = Sxy - Syx;
+    const T n11 = Sxx - Syy - Szz, n12 = Sxy + Syx, n13 = Szx + Sxz;
+    const T n22 = -Sxx + Syy - Szz, n23 = Syz + Szy;
+    const T n33 = -Sxx - Syy + Szz;
+    const T c2 = T(-2) * f2;
+    const T c1 = T(-8) * (Sxx * (Syy * Szz - Syz * Szy) - Sxy * (Syx * Szz - Syz * Szx) + Sxz * (Syx * Szy - Syy * Szx));
+    // det N from the 2x2 minors of rows (0, 1) and rows (2, 3)
+    const T s0 = n00 * n11 - n01 * n01, s1 = n00 * n12 - n01 * n02, s2 = n00 * n13 - n01 * n03;
+    const T s3 = n01 * n12 - n11 * n02, s4 = n01 * n13 - n11 * n03, s5 = n02 * n13 - n12 * n03;
+    const T t5 = n22 * n33 - n23 * n23, t4 = n12 * n33 - n13 * n23, t3 = n12 * n23 - n13 * n22;
+    const T t2 = n02 * n33 - n03 * n23, t1 = n02 * n23 - n03 * n22, t0 = n02 * n13 - n03 * n12;
+    const T c0 = s0 * t5 - s1 * t4 + s2 * t3 + s3 * t2 - s4 * t1 + s5 * t0;
+    if (!(f2 > T(0))) {  // no edges carry weight: keep the previous rotation
+        T pn = px * px + py * py + pz * pz + pw * pw;
+        T inv = jrsqrt<T>(pn > T(0) ? pn : T(1));
+        if (!(pn > T(0))) { px = 0; py = 0; pz = 0; pw = 1; inv = 1; }
+        if (pw < T(0)) inv = -inv;
+        qx = px * inv; qy = py * inv; qz = pz * inv; qw = pw * inv;
+        return true;
+    }
+    const T scale = jsqrt<T>(T(1.5) * (-c2));
+    T lam = scale;
+    constexpr T kTol = sizeof(T) == 4 ? T(1e-6) : T(1e-14);
+    for (int it = 0; it < 60; ++it) {
+        const T l2 = lam * lam;
+        const T p = (l2 + c2) * l2 + c1 * lam + c0;
+        const T dp = T(4) * l2 * lam + T(2) * c2 * lam + c1;
+        if (!(dp > T(0))) break;  // at (or beyond) a multiple root: the fallback decides
+        const T step = p / dp;
+        lam -= step;
+        if (!(fabs(step) > kTol * scale)) break;
+    }
+    // a = N - lambda I; the diagonal of adj(a) (principal 3x3 minors) picks the column
+    const T a[4][4] = {{n00 - lam, n01, n02, n03},
+                       {n01, n11 - lam, n12, n13},
+                       {n02, n12, n22 - lam, n23},
+                       {n03, n13, n23, n33 - lam}};
+    const T d0 = cofactor<T, 0, 0>(a), d1 = cofactor<T, 1, 1>(a), d2 = cofactor<T, 2, 2>(a),
+            d3 = cofactor<T, 3, 3>(a);
+    T best = fabs(d0);
+    int col = 0;
+    if (fabs(d1) > best) { best = fabs(d1); col = 1; }
+    if (fabs(d2) > best) { best = fabs(d2); col = 2; }
+    if (fabs(d3) > best) { best = fabs(d3); col = 3; }
+    constexpr T kGap = sizeof(T) == 4 ? T(1e-2) : T(1e-4);
+    if (!(best > kGap * scale * scale * scale)) {
+        best_rotation_jacobi<T>(m, px, py, pz, pw, qx, qy, qz, qw);
+        return false;
+    }
+    // column `col` of adj(a) (a is symmetric, so adj(a) is too); then once more at the
+    // Rayleigh quotient of that vector, whose error is the square of the vector's: the rounding
+    // of det N moves the polynomial's root by up to eps |N|^4 / prod(gaps), and the vector's
+    // error is that shift over the gap
+    T v0, v1, v2, v3;
+    adj_column<T>(a, col, v0, v1, v2, v3);
+    {
+        const T nv = v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3;
+        const T w0 = n00 * v0 + n01 * v1 + n02 * v2 + n03 * v3, w1 = n01 * v0 + n11 * v1 + n12 * v2 + n13 * v3;
+        const T w2 = n02 * v0 + n12 * v1 + n22 * v2 + n23 * v3, w3 = n03 * v0 + n13 * v1 + n23 * v2 + n33 * v3;
+        const T rq = (v0 * w0 + v1 * w1 + v2 * w2 + v3 * w3) / nv;
+        const T b[4][4] = {{n00 - rq, n01, n02, n03},
+                           {n01, n11 - rq, n12, n13},
+                           {n02, n12, n22 - rq, n23},
+                           {n03, n13, n23, n33 - rq}};
+        adj_column<T>(b, col, v0, v1, v2, v3);
+    }
+    T nrm = jrsqrt<T>(v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3);
+    if (v0 < T(0)) nrm = -nrm;  // canonical sign: scalar part >= 0
+    qw = v0 * nrm; qx = v1 * nrm; qy = v2 * nrm; qz = v3 * nrm;
+    return true;
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void quat_to_mat(T qx, T qy, T qz, T qw, T (&R)[9]) {
     T s = T(2) / (qx * qx + qy * qy + qz * qz + qw * qw);
     R[0] = T(1) - s * (qy * qy + qz * qz); R[1] = s * (qx * qy - qw * qz); R[2] = s * (qx * qz + qw * qy);
     R[3] = s * (qx * qy + qw * qz); R[4] = T(1) - s * (qx * qx + qz * qz); R[5] = s * (qy * qz - qw * qx);
@@ -1656,3 +1795,36 @@ void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64) {
 }
 
 }  // namespace md
+
+namespace md {
+void set_error(const std::string &msg);
+}
+
+// Host diagnostic: the local step's rotation code compiled for the host (correctly rounded
+// reciprocal and square roots instead of the device's approximate ones).
+extern "C" int md_host_best_rotation(int fp64, int count, const double *cov, const double *prev, double *quat,
+                                     int *fallback) {
+    if (count < 0 || (count > 0 && (!cov || !quat))) {
+        md::set_error("md_host_best_rotation: invalid argument");
+        return MD_ERR_INVALID;
+    }
+    for (int k = 0; k < count; ++k) {
+        const double *c = cov + 9 * (size_t)k;
+        const double p[4] = {prev ? prev[4 * k] : 0.0, prev ? prev[4 * k + 1] : 0.0, prev ? prev[4 * k + 2] : 0.0,
+                             prev ? prev[4 * k + 3] : 1.0};
+        bool qcp;
+        if (fp64) {
+            double m[9], q[4];
+            for (int i = 0; i < 9; ++i) m[i] = c[i];
+            qcp = md::best_rotation<double>(m, p[0], p[1], p[2], p[3], q[0], q[1], q[2], q[3]);
+            for (int i = 0; i < 4; ++i) quat[4 * k + i] = q[i];
+        } else {
+            float m[9], q[4];
+            for (int i = 0; i < 9; ++i) m[i] = (float)c[i];
+            qcp = md::best_rotation<float>(m, (float)p[0], (float)p[1], (float)p[2], (float)p[3], q[0], q[1], q[2], q[3]);
+            for (int i = 0; i < 4; ++i) quat[4 * k + i] = q[i];
+        }
+        if (fallback) fallback[k] = qcp ? 0 : 1;
+    }
+    return MD_OK;
+}

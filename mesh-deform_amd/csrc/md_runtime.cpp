@@ -11,6 +11,7 @@
 #include <atomic>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 
 namespace md {
 
@@ -376,6 +377,10 @@ extern "C" int md_mesh_create(md_ctx *ctx, int n, int F, const int *tri, const m
 
 extern "C" int md_mesh_destroy(md_mesh *m) {
     if (!m) return MD_OK;
+    if (m->servers > 0) {
+        set_error("md_mesh_destroy: the mesh is registered with a server (destroy the server first)");
+        return MD_ERR_STATE;
+    }
     md_ctx *ctx = m->ctx;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
@@ -386,6 +391,10 @@ extern "C" int md_mesh_destroy(md_mesh *m) {
 
 extern "C" int md_mesh_set_constraint(md_mesh *m, int vidx, const double *xyz) {
     if (!m || !xyz) { set_error("md_mesh_set_constraint: null pointer"); return MD_ERR_INVALID; }
+    if (m->servers > 0) {
+        set_error("md_mesh_set_constraint: the mesh is registered with a server (drags go through md_server_submit)");
+        return MD_ERR_STATE;
+    }
     if (vidx < 0 || vidx >= m->n) {
         set_error("md_mesh_set_constraint: vertex index " + std::to_string(vidx) +
                   " outside [0, " + std::to_string(m->n) + ")");
@@ -401,6 +410,10 @@ extern "C" int md_mesh_set_constraints(md_mesh *m, int k, const int *idx, const 
     if (!m || k < 0 || (k > 0 && (!idx || !xyz))) {
         set_error("md_mesh_set_constraints: invalid argument");
         return MD_ERR_INVALID;
+    }
+    if (m->servers > 0) {
+        set_error("md_mesh_set_constraints: the mesh is registered with a server (drags go through md_server_submit)");
+        return MD_ERR_STATE;
     }
     for (int i = 0; i < k; ++i)
         if (idx[i] < 0 || idx[i] >= m->n) {
@@ -939,21 +952,29 @@ int prep_device_b(md_mesh *m, Prep &p) {
 
 // Returns MD_OK with *early = true when no vertex is constrained (arap.h:113-114).
 int setup(md_mesh *m, const double *xyz, bool *early) {
+    auto ms_since = [](Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); };
     Prep p;
+    auto t0 = Clock::now();
     prep_begin(m, xyz, p);
+    m->setup_phase_ms[0] = m->setup_phase_ms[1] = m->setup_phase_ms[2] = 0.0;
     if (p.need_analysis) {
-        auto t0 = Clock::now();
         analyze_host(m);
-        m->setup_ms_host = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+        m->setup_ms_host = ms_since(t0);
     }
+    m->setup_phase_ms[0] = ms_since(t0);
+    t0 = Clock::now();
     int rc = prep_device_a(m, p);
+    m->setup_phase_ms[1] = ms_since(t0);
     *early = p.early;
     if (rc || p.early) return rc;
+    t0 = Clock::now();
     if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT && !m->cfg.factor_device) {
         std::string err;
         if (!factor_host(m, err)) { set_error(err); return MD_ERR_FACTOR; }
     }
-    return prep_device_b(m, p);
+    rc = prep_device_b(m, p);
+    m->setup_phase_ms[2] = ms_since(t0);
+    return rc;
 }
 
 // Runs fn(k) for k in [0, count) on up to nthreads threads.
@@ -1038,6 +1059,153 @@ hipError_t time_plan(Plan &plan, hipStream_t st, int iters, bool use_graph, doub
     return e;
 }
 
+// ---- residency: eviction to a host image and restoration -----------------------------------
+
+void HostImage::release() {
+    if (a) hipHostFree(a);
+    if (l) hipHostFree(l);
+    a = l = nullptr;
+    asz = lsz = 0;
+    complete = false;
+}
+
+namespace {
+
+// The launch plan of a single mesh over its current device arrays (drops captured graphs).
+hipError_t mesh_plan(md_mesh *m, hipStream_t st) {
+    std::vector<const MeshDev *> ms{&m->dev};
+    std::vector<const Factor *> fs{&m->fac};
+    m->plan.pcg = m->cfg.solver == MD_SOLVER_PCG;
+    m->plan.pcg_iters = m->cfg.pcg_max_iters;
+    return build_plan(m->plan, ms, fs, m->fp64(), m->cfg.refine_steps, st);
+}
+
+// Moves every device pointer of the mesh from the old arena bases to the new ones.
+void rebind(md_mesh *m, const char *oa, size_t asz, char *na, const char *ol, size_t lsz, char *nl) {
+    auto fix = [&](auto &p) {
+        using P = std::remove_reference_t<decltype(p)>;
+        const char *c = reinterpret_cast<const char *>(p);
+        if (!c) return;
+        if (c >= oa && c < oa + asz) p = reinterpret_cast<P>(na + (c - oa));
+        else if (c >= ol && c < ol + lsz) p = reinterpret_cast<P>(nl + (c - ol));
+    };
+    MeshDev &d = m->dev;
+    fix(d.slice_ptr); fix(d.sell_col); fix(d.a32); fix(d.a64); fix(d.w64); fix(d.tri); fix(d.fe_val);
+    fix(d.slot_fe_ptr); fix(d.slot_fe); fix(d.p); fix(d.pp); fix(d.quat); fix(d.r); fix(d.z);
+    fix(d.upart); fix(d.xpart); fix(d.child_ptr); fix(d.child); fix(d.udst); fix(d.rowinfo); fix(d.xloc);
+    fix(d.pc); fix(d.pr); fix(d.ftile); fix(d.btile); fix(d.sub); fix(d.sub_h); fix(d.snode); fix(d.scp);
+    fix(d.sfc); fix(d.sbo); fix(d.srow); fix(d.sfi); fix(d.lrows); fix(d.energy_part); fix(d.w32);
+    fix(d.diag); fix(d.cg_x); fix(d.cg_p); fix(d.cg_ap); fix(d.cg_part);
+    fix(m->d_fac); fix(m->d_hlist); fix(m->d_offrows); fix(m->d_rel); fix(m->d_tgt_idx); fix(m->d_tgt_xyz);
+}
+
+// True when an 8-byte word of the mesh's device record still addresses an old arena (a
+// pointer field rebind() does not know): checked whenever the old and new ranges are apart.
+bool stale_pointer(const md_mesh *m, const char *oa, size_t asz, const char *ol, size_t lsz) {
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(&m->dev);
+    for (size_t o = 0; o + sizeof(uintptr_t) <= sizeof(MeshDev); o += sizeof(uintptr_t)) {
+        uintptr_t w;
+        std::memcpy(&w, b + o, sizeof(w));
+        const char *c = reinterpret_cast<const char *>(w);
+        if ((c >= oa && c < oa + asz) || (c >= ol && c < ol + lsz)) return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+int mesh_evict(md_mesh *m, hipStream_t st) {
+    if (m->evicted) return MD_OK;
+    if (!m->analyzed || !m->arena.allocated()) { set_error("mesh_evict: the mesh has no device state"); return MD_ERR_STATE; }
+    HostImage &I = m->img;
+    const size_t asz = m->arena.size(), lsz = m->larena.size();
+    if (I.a && (I.asz != asz || I.lsz != lsz)) I.release();
+    if (!I.a) {
+        if (hipHostMalloc(&I.a, std::max<size_t>(asz, 1), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&I.l, std::max<size_t>(lsz, 1), hipHostMallocDefault) != hipSuccess) {
+            I.release();
+            set_error("mesh_evict: pinned host image of " + std::to_string(asz + lsz) + " bytes: allocation failed");
+            return MD_ERR_NOMEM;
+        }
+        I.asz = asz;
+        I.lsz = lsz;
+        I.complete = false;
+    }
+    char *base = m->arena.base();
+    hipError_t e = hipSuccess;
+    if (!I.complete) {
+        e = hipMemcpyAsync(I.a, base, asz, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && lsz) e = hipMemcpyAsync(I.l, m->larena.base(), lsz, hipMemcpyDeviceToHost, st);
+    } else {
+        // the image holds every static array already: only the iteration state moved since
+        const size_t vsz = m->fp64() ? sizeof(d4) : sizeof(f4);
+        const size_t opp = reinterpret_cast<char *>(m->dev.pp) - base;
+        const size_t oq = reinterpret_cast<char *>(m->dev.quat) - base;
+        e = hipMemcpyAsync(static_cast<char *>(I.a) + opp, m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(static_cast<char *>(I.a) + oq, m->dev.quat, vsz * m->n, hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { set_error(std::string("mesh_evict: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    I.complete = true;
+    m->img_old_a = base;
+    m->img_old_l = m->larena.base();
+    m->plan.drop_graph();
+    m->arena.release();
+    m->larena.release();
+    m->evicted = true;
+    ++m->gen;
+    return MD_OK;
+}
+
+int mesh_restore(md_mesh *m, hipStream_t st) {
+    if (!m->evicted) return MD_OK;
+    HostImage &I = m->img;
+    if (m->arena.allocate() != hipSuccess || m->larena.allocate() != hipSuccess) {
+        m->arena.release();
+        m->larena.release();
+        set_error("mesh_restore: device allocation of " + std::to_string(I.asz + I.lsz) + " bytes failed");
+        return MD_ERR_NOMEM;
+    }
+    char *na = m->arena.base(), *nl = m->larena.base();
+    hipError_t e = hipMemcpyAsync(na, I.a, I.asz, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && I.lsz) e = hipMemcpyAsync(nl, I.l, I.lsz, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        m->arena.release();
+        m->larena.release();
+        set_error(std::string("mesh_restore: ") + hipGetErrorString(e));
+        return MD_ERR_HIP;
+    }
+    const char *oa = m->img_old_a, *ol = m->img_old_l;
+    rebind(m, oa, I.asz, na, ol, I.lsz, nl);
+    const bool apart = (na + I.asz <= oa || oa + I.asz <= na) && (!I.lsz || nl + I.lsz <= ol || ol + I.lsz <= nl);
+    if (apart && stale_pointer(m, oa, I.asz, ol, I.lsz)) {
+        m->arena.release();
+        m->larena.release();
+        set_error("mesh_restore: a device pointer of the mesh was not re-pointed");
+        return MD_ERR_STATE;
+    }
+    m->evicted = false;
+    ++m->gen;
+    e = mesh_plan(m, st);
+    if (e != hipSuccess) { set_error(std::string("mesh_restore: launch plan: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
+    return MD_OK;
+}
+
+int ensure_resident(md_mesh *m, const char *who) {
+    if (m->servers > 0) {
+        set_error(std::string(who) + ": the mesh is registered with a server (its results come from "
+                  "md_server_wait / md_server_poll until the server is destroyed)");
+        return MD_ERR_STATE;
+    }
+    if (!m->evicted) return MD_OK;
+    int rc = mesh_restore(m, m->ctx->stream);
+    if (rc) return rc;
+    m->img.release();  // a mesh outside a server may be re-prepared: the image would go stale
+    return MD_OK;
+}
+
 }  // namespace md
 
 namespace {
@@ -1068,6 +1236,7 @@ extern "C" int md_mesh_autotune(md_mesh *m, const int *budgets, int count, int i
     if (m->cfg.solver != MD_SOLVER_LDLT) { set_error("md_mesh_autotune: only the LDLT solver has a sweep layout"); return MD_ERR_INVALID; }
     if (m->servers > 0) { set_error("md_mesh_autotune: the mesh is registered with a server"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if ((rc = ensure_resident(m, "md_mesh_autotune"))) return rc;
     hipStream_t st = m->ctx->stream;
     std::vector<md_mesh *> ms{m};
     StateSnapshot snap;
@@ -1110,6 +1279,7 @@ extern "C" int md_mesh_autotune_factor(md_mesh *m, const double *ratios, int nra
     if (m->cfg.solver != MD_SOLVER_LDLT) { set_error("md_mesh_autotune_factor: only the LDLT solver has a factor layout"); return MD_ERR_INVALID; }
     if (m->servers > 0) { set_error("md_mesh_autotune_factor: the mesh is registered with a server"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if ((rc = ensure_resident(m, "md_mesh_autotune_factor"))) return rc;
     // every candidate re-runs the dirty setup from the rest pose: the state it leaves is the
     // freshly prepared one, whatever the candidate
     const std::vector<double> rest = m->rest;
@@ -1144,6 +1314,7 @@ extern "C" int md_mesh_autotune_factor(md_mesh *m, const double *ratios, int nra
 extern "C" int md_mesh_prepare(md_mesh *m, const double *xyz) {
     if (!m || !xyz) { set_error("md_mesh_prepare: null pointer"); return MD_ERR_INVALID; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_prepare")) return rc;
     if (!m->dirty) return MD_OK;
     bool early = false;
     int rc = setup(m, xyz, &early);
@@ -1166,6 +1337,8 @@ extern "C" int md_meshes_prepare(md_ctx *ctx, int count, md_mesh *const *meshes,
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
     nthreads = std::min(nthreads, 16);
     hipSetDevice(ctx->device);
+    for (int k = 0; k < count; ++k)
+        if (int rc = ensure_resident(meshes[k], "md_meshes_prepare")) return rc;
     std::vector<md_mesh *> todo;
     std::vector<Prep> preps;
     for (int k = 0; k < count; ++k)
@@ -1209,17 +1382,27 @@ extern "C" int md_mesh_deform(md_mesh *m, int iterations, double *xyz, int *ok) 
     if (ok) *ok = 0;
     if (!m || !xyz || iterations < 0) { set_error("md_mesh_deform: invalid argument"); return MD_ERR_INVALID; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_deform")) return rc;
+    auto ms_since = [](Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); };
+    const auto t_call = Clock::now();
+    for (double &x : m->deform_ms) x = 0.0;
     if (m->dirty) {
         bool early = false;
         int rc = setup(m, xyz, &early);
+        for (int k = 0; k < 3; ++k) m->deform_ms[k] = m->setup_phase_ms[k];
         if (rc == MD_ERR_FACTOR) return rc;   // deform() == false (arap.h:116-117)
         if (rc) return rc;
-        if (early) { if (ok) *ok = 1; return MD_OK; }
+        if (early) { if (ok) *ok = 1; m->deform_ms[5] = ms_since(t_call); return MD_OK; }
     }
+    auto t0 = Clock::now();
     int rc = iterate(m, iterations);  // arap.h:122-129
     if (rc) return rc;
+    m->deform_ms[3] = ms_since(t0);
+    t0 = Clock::now();
     hipError_t e = download_positions(m, xyz);  // write-back, arap.h:133-135
     if (e != hipSuccess) { set_error("position download failed"); return MD_ERR_HIP; }
+    m->deform_ms[4] = ms_since(t0);
+    m->deform_ms[5] = ms_since(t_call);
     if (ok) *ok = 1;
     return MD_OK;
 }
@@ -1228,6 +1411,7 @@ extern "C" int md_mesh_iterate(md_mesh *m, int iterations) {
     if (!m || iterations < 0) { set_error("md_mesh_iterate: invalid argument"); return MD_ERR_INVALID; }
     if (m->dirty) { set_error("md_mesh_iterate: mesh is not prepared"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_iterate")) return rc;
     return iterate(m, iterations);
 }
 
@@ -1235,6 +1419,7 @@ extern "C" int md_mesh_download(md_mesh *m, double *xyz) {
     if (!m || !xyz) { set_error("md_mesh_download: null pointer"); return MD_ERR_INVALID; }
     if (!m->analyzed) { set_error("md_mesh_download: no device state yet"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_download")) return rc;
     if (download_positions(m, xyz) != hipSuccess) { set_error("position download failed"); return MD_ERR_HIP; }
     return MD_OK;
 }
@@ -1259,6 +1444,7 @@ extern "C" int md_mesh_cotan_csr(md_mesh *m, int *rowptr, int *col, double *val)
     if (!m || !rowptr || !col || !val) { set_error("md_mesh_cotan_csr: null pointer"); return MD_ERR_INVALID; }
     if (!m->analyzed) { set_error("md_mesh_cotan_csr: call deform first"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_cotan_csr")) return rc;
     const Topology &t = m->topo;
     int64_t slots = t.total_slots();
     std::vector<double> w(slots);
@@ -1288,6 +1474,7 @@ extern "C" int md_mesh_rotations(md_mesh *m, double *R) {
     if (!m || !R) { set_error("md_mesh_rotations: null pointer"); return MD_ERR_INVALID; }
     if (!m->analyzed) { set_error("md_mesh_rotations: call deform first"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_rotations")) return rc;
     int n = m->n;
     std::vector<double> q(4 * (size_t)n);
     if (m->fp64()) {
@@ -1316,6 +1503,7 @@ extern "C" int md_mesh_normals(md_mesh *m, double *nrm) {
     if (!m || !nrm) { set_error("md_mesh_normals: null pointer"); return MD_ERR_INVALID; }
     if (!m->analyzed) { set_error("md_mesh_normals: call deform first"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_normals")) return rc;
     hipStream_t st = m->ctx->stream;
     double *fn = nullptr;
     d4 *out = nullptr;
@@ -1344,6 +1532,7 @@ extern "C" int md_mesh_energy(md_mesh *m, double *energy) {
     if (!m || !energy) { set_error("md_mesh_energy: null pointer"); return MD_ERR_INVALID; }
     if (!m->analyzed) { set_error("md_mesh_energy: call deform first"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_energy")) return rc;
     const Plan &P = m->plan;
     launch_energy(m->ctx->stream, P.set(P.off_slice), P.blk_slice, m->fp64());
     int nb = std::max(1, cdiv(m->topo.nslices, kSlicesPerBlock));
@@ -1380,6 +1569,7 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     out->factor_nnz = s.strict_nnz_equiv;
     out->factor_nnz_exact = s.nnz_exact;
     out->factor_ms = m->factor_ms;
+    for (int k = 0; k < 6; ++k) out->deform_ms[k] = m->deform_ms[k];
     out->fwd_items = s.fwd_items;
     out->bwd_items = s.bwd_items;
     out->device_bytes = (int64_t)(m->arena.size() + m->larena.size());
@@ -1437,6 +1627,7 @@ extern "C" int md_mesh_time_iterations(md_mesh *m, int warmup, int iterations, d
     if (!m || iterations < 0 || warmup < 0) { set_error("md_mesh_time_iterations: invalid argument"); return MD_ERR_INVALID; }
     if (m->dirty) { set_error("md_mesh_time_iterations: mesh is not prepared"); return MD_ERR_STATE; }
     hipSetDevice(m->ctx->device);
+    if (int rc = ensure_resident(m, "md_mesh_time_iterations")) return rc;
     hipStream_t st = m->ctx->stream;
     bool g = m->cfg.use_graph != 0;
     MD_HIP(run_iterations(m->plan, st, warmup, g));
@@ -1502,63 +1693,3 @@ extern "C" int md_copy_probe(md_ctx *ctx, size_t bytes, int repeats, double *gbp
     return MD_OK;
 }
 
-// ------------------------------------------------------------------------------------------
-// host-only diagnostics
-// ------------------------------------------------------------------------------------------
-
-extern "C" int md_host_factor_solve(int n, const int *rowptr, const int *col, const double *val,
-                                    const double *xyz, int nrhs, const double *b, double *x,
-                                    int64_t *panel_entries, int *levels, int subtree_budget,
-                                    int device_factor_emulation, int level_pairs) {
-    if (n <= 0 || !rowptr || !col || !val || !xyz || nrhs < 0 || (nrhs > 0 && (!b || !x))) {
-        set_error("md_host_factor_solve: invalid argument");
-        return MD_ERR_INVALID;
-    }
-    const int budget = subtree_budget < 0 ? kSubBudgetDefault : std::min(subtree_budget, kSubBudgetMax);
-    std::vector<int> perm = order_graph(n, rowptr, col, xyz);
-    std::vector<int> group;
-    if (level_pairs) {
-        std::vector<int> ptr0(n + 1, 0), adj0;
-        for (int i = 0; i < n; ++i) {
-            for (int e = rowptr[i]; e < rowptr[i + 1]; ++e)
-                if (col[e] != i) adj0.push_back(col[e]);
-            ptr0[i + 1] = (int)adj0.size();
-        }
-        perm = pair_levels(perm, ptr0, adj0, kPairBudget, kAmalgRatioDefault, group);
-    }
-    std::vector<int> iperm(n);
-    for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
-    std::vector<int> ptr(n + 1, 0), adj;
-    std::vector<double> av, diag(n, 0.0);
-    for (int k = 0; k < n; ++k) {
-        int i = perm[k];
-        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
-            int j = col[e];
-            if (j == i) diag[k] += val[e];
-            else { adj.push_back(iperm[j]); av.push_back(val[e]); }
-        }
-        ptr[k + 1] = (int)adj.size();
-    }
-    Factor f;
-    symbolic(n, ptr, adj, f, budget, kAmalgRatioDefault, level_pairs ? &group : nullptr);
-    std::string err;
-    const bool ok = device_factor_emulation ? numeric_device_emulation(ptr, adj, av, diag, f, err)
-                                            : numeric(ptr, adj, av, diag, f, err);
-    if (!ok) { set_error(err); return err.rfind("factor emulation", 0) == 0 ? MD_ERR_INVALID : MD_ERR_FACTOR; }
-    std::vector<double> bp((size_t)n * nrhs), xp((size_t)n * nrhs);
-    for (int r = 0; r < nrhs; ++r)
-        for (int k = 0; k < n; ++k) bp[(size_t)r * n + k] = b[(size_t)r * n + perm[k]];
-    // three right-hand sides at a time through the tile emulation of the device sweeps
-    for (int r0 = 0; r0 < nrhs; r0 += 3) {
-        std::vector<double> b3(3 * (size_t)n, 0.0), x3(3 * (size_t)n, 0.0);
-        int cnt = std::min(3, nrhs - r0);
-        std::copy(bp.begin() + (size_t)r0 * n, bp.begin() + (size_t)(r0 + cnt) * n, b3.begin());
-        if (!solve_tiles_host(f, b3.data(), x3.data(), err)) { set_error(err); return MD_ERR_INVALID; }
-        std::copy(x3.begin(), x3.begin() + (size_t)cnt * n, xp.begin() + (size_t)r0 * n);
-    }
-    for (int r = 0; r < nrhs; ++r)
-        for (int k = 0; k < n; ++k) x[(size_t)r * n + perm[k]] = xp[(size_t)r * n + k];
-    if (panel_entries) *panel_entries = f.factor_nnz();
-    if (levels) *levels = f.nlevels;
-    return MD_OK;
-}

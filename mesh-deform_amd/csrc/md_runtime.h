@@ -31,6 +31,8 @@ public:
         return hipMalloc(&base_, size_);
     }
     void *at(size_t off) const { return static_cast<char *>(base_) + off; }
+    char *base() const { return static_cast<char *>(base_); }
+    bool allocated() const { return base_ != nullptr; }
     size_t size() const { return size_; }
     void release() {
         if (base_) hipFree(base_);
@@ -112,6 +114,26 @@ hipError_t restore_state(const std::vector<md_mesh *> &meshes, hipStream_t st, c
 // Event-timed replay of `iters` iterations of a plan (ms for all of them).
 hipError_t time_plan(Plan &plan, hipStream_t st, int iters, bool use_graph, double &ms);
 
+// Pinned host copy of a mesh's two device arenas while the mesh is not resident.
+struct HostImage {
+    void *a = nullptr, *l = nullptr;   // main arena, sweep-layout arena
+    size_t asz = 0, lsz = 0;
+    bool complete = false;             // every byte is current (then an eviction copies only
+                                       //   the iteration state: p' and the rotations)
+    void release();
+    ~HostImage() { release(); }
+};
+
+// Device residency of a prepared mesh. mesh_evict copies the device state to the host image
+// (stream-ordered on st, then synchronised) and frees both arenas; mesh_restore allocates them
+// again, copies the image back and re-points every device pointer of the mesh, its launch plan
+// included (the generation is bumped: batches built before must be rebuilt). Iterating a
+// restored mesh gives bitwise what iterating it without the round trip gives.
+int mesh_evict(md_mesh *m, hipStream_t st);
+int mesh_restore(md_mesh *m, hipStream_t st);
+// Restores an evicted mesh on its context's stream (entry points that touch device state).
+int ensure_resident(md_mesh *m, const char *who);
+
 }  // namespace md
 
 struct md_ctx {
@@ -149,7 +171,15 @@ struct md_mesh {
     const md::FacDev *d_fac = nullptr;
     const int *d_hlist = nullptr, *d_offrows = nullptr, *d_rel = nullptr;
     double factor_ms = 0.0;        // last numeric factorization (host or device), wall ms
-    std::atomic<int> servers{0};   // servers this mesh is registered with (its layout is fixed)
+    double deform_ms[6] = {0, 0, 0, 0, 0, 0};  // phases of the last md_mesh_deform (md_mesh_stats)
+    double setup_phase_ms[3] = {0, 0, 0};      // phases of the last setup: analysis, uploads +
+                                               //   cotan weights, factorization
+    std::atomic<int> servers{0};   // servers this mesh is registered with: its device state
+                                   //   belongs to the server (entry points that change or read
+                                   //   it return MD_ERR_STATE until the server is destroyed)
+    bool evicted = false;          // device arenas freed, state held in img
+    md::HostImage img;
+    const char *img_old_a = nullptr, *img_old_l = nullptr;  // arena addresses at the eviction
     md::Plan plan;                 // single-mesh launch plan
     std::vector<int> adj_ptr, adj; // free-free pattern in solve order
     std::vector<int64_t> adj_slot; // slot of each adj entry

@@ -39,13 +39,19 @@ hipError_t batch_plan(md_batch *b) {
 
 // A mesh re-prepared with a new constraint set, or given another sweep layout, since the
 // plan was built has moved its device arrays: launching the old plan would read freed memory.
+// A mesh registered with a server belongs to the server's stream until the server is gone.
 int batch_current(md_batch *b, const char *who) {
-    for (size_t k = 0; k < b->meshes.size(); ++k)
-        if (b->meshes[k]->gen != b->gens[k] || b->meshes[k]->dirty) {
-            set_error(std::string(who) + ": mesh " + std::to_string(k) +
-                      " was re-prepared or re-laid out after the batch was built");
+    for (size_t k = 0; k < b->meshes.size(); ++k) {
+        if (b->meshes[k]->servers > 0) {
+            set_error(std::string(who) + ": mesh " + std::to_string(k) + " is registered with a server");
             return MD_ERR_STATE;
         }
+        if (b->meshes[k]->gen != b->gens[k] || b->meshes[k]->dirty || b->meshes[k]->evicted) {
+            set_error(std::string(who) + ": mesh " + std::to_string(k) +
+                      " was re-prepared, re-laid out or evicted after the batch was built");
+            return MD_ERR_STATE;
+        }
+    }
     return MD_OK;
 }
 
@@ -61,6 +67,8 @@ extern "C" int md_batch_create(md_ctx *ctx, int count, md_mesh *const *meshes, m
         md_mesh *m = meshes[k];
         if (!m || m->ctx != ctx) { set_error("md_batch_create: mesh from another context"); return MD_ERR_INVALID; }
         if (m->dirty || !m->analyzed) { set_error("md_batch_create: mesh " + std::to_string(k) + " is not prepared"); return MD_ERR_STATE; }
+        hipSetDevice(ctx->device);
+        if (int rc = ensure_resident(m, "md_batch_create")) return rc;
         if (m->cfg.precision != prec || m->cfg.refine_steps != refine) {
             set_error("md_batch_create: meshes differ in precision or refinement steps");
             return MD_ERR_INVALID;
@@ -179,62 +187,79 @@ extern "C" int md_batch_destroy(md_batch *b) {
 
 // ---- server (continuous batching) ---------------------------------------------------------
 //
-// One worker thread owns the context's stream while the server runs. Requests queue in
-// submission order; at every iteration boundary the worker admits queued requests whose mesh
-// is idle (one request in flight per mesh, at most max_active in flight), rebuilds the launch
-// plan when the set changed, and runs one ARAP iteration for every admitted request in the
-// same launches. A request leaves the set when its iterations are done; its positions are
-// downloaded and its waiters woken. Meshes are resident from md_server_add_mesh on, within the
-// HBM budget.
+// One worker thread owns the server's stream. Requests queue in submission order; at every
+// iteration boundary the worker admits queued requests whose mesh is idle (one request in
+// flight per mesh, at most max_active in flight) and resident. A request whose mesh was
+// evicted is admitted when its footprint fits the HBM budget, after the least recently used
+// idle meshes are evicted to host memory if that makes room; it waits in the queue otherwise,
+// until iterations finish and memory frees up. Every admitted request is iterated in the same
+// launches (one ARAP iteration per boundary for the whole set); a request leaves the set when
+// its iterations are done and its positions are copied to a pinned buffer. Launches run one
+// iteration ahead of the host: the host admits, plans and launches iteration i+1 before it waits
+// for iteration i, so the device does not idle at a boundary.
 
 namespace {
 using SClock = std::chrono::steady_clock;
 
+struct ServedMesh {
+    md_mesh *m = nullptr;
+    int id = 0;
+    size_t bytes = 0;             // device footprint while resident (both arenas)
+    bool busy = false;            // a request admitted and not completed
+    uint64_t last_use = 0;        // admission stamp (least recently used eviction)
+    std::vector<d4 *> pinned_free;  // reusable pinned result buffers
+};
+
 struct Request {
     uint64_t ticket = 0;
-    int mesh_id = -1;
+    ServedMesh *sm = nullptr;     // fixed at submission (the server's mesh table may grow)
     std::vector<int> idx;
     std::vector<double> xyz;
     int iters = 0, left = 0;
     SClock::time_point t_submit, t_done;
-    bool admitted = false, done = false;
+    bool done = false;
     int status = MD_OK;
     std::string error;
-    std::vector<int> iv;        // internal ids of the moved targets (upload source)
-    d4 *pinned = nullptr;       // positions in internal order, copied at completion
-    int n = 0;
+    std::vector<int> iv;          // internal ids of the moved targets (upload source)
+    d4 *pinned = nullptr;         // positions in internal order, copied at completion
+    uint64_t finish_seq = 0;      // launch after which the copy of its positions is complete
 };
 }  // namespace
 
 struct md_server {
     md_ctx *ctx = nullptr;
     md_server_config cfg{};
-    std::vector<md_mesh *> meshes;
-    size_t resident = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<ServedMesh>> meshes;  // guarded by mu; the entries never move
+    size_t resident = 0, peak_resident = 0;
+    uint64_t lru_clock = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> queue;
     std::map<uint64_t, std::shared_ptr<Request>> tickets;
-    std::vector<std::shared_ptr<Request>> active;
-    std::vector<char> busy;  // per mesh
     uint64_t next_ticket = 1;
     bool stop = false;
     std::thread worker;
-    Plan plan;
-    std::vector<int> plan_meshes;
-    std::string fatal;
-    std::vector<std::vector<d4 *>> pinned_free;  // per mesh: reusable pinned result buffers
-    std::vector<d4 *> pinned_all;
+    md_server_stats stats{};      // guarded by mu
+    // worker thread only
+    std::vector<std::shared_ptr<Request>> active;     // iterated at the next boundary
+    std::vector<std::shared_ptr<Request>> finishing;  // positions being copied
+    Plan plans[2];
+    std::vector<std::pair<int, uint64_t>> keys[2];    // (mesh id, generation) of each plan's set
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    uint64_t seq = 0;                                 // launches so far
+    std::vector<d4 *> pinned_all;                     // guarded by mu
 
-    d4 *take_pinned(int mesh_id, int n) {  // caller holds mu
-        auto &fl = pinned_free[mesh_id];
+    d4 *take_pinned(ServedMesh *sm) {  // caller holds mu
+        auto &fl = sm->pinned_free;
         if (!fl.empty()) {
             d4 *p = fl.back();
             fl.pop_back();
             return p;
         }
         void *p = nullptr;
-        if (hipHostMalloc(&p, sizeof(d4) * (size_t)std::max(1, n), hipHostMallocDefault) != hipSuccess) return nullptr;
+        if (hipHostMalloc(&p, sizeof(d4) * (size_t)std::max(1, sm->m->n), hipHostMallocDefault) != hipSuccess)
+            return nullptr;
         pinned_all.push_back((d4 *)p);
         return (d4 *)p;
     }
@@ -245,7 +270,7 @@ namespace {
 // Pastes the request's targets into p' (fixed rest pose: the previous result is the initial
 // guess): one upload and one scatter launch, stream-ordered, no host synchronisation. The
 // sources live in the request until it completes.
-int apply_targets(md_mesh *m, Request &r) {
+int apply_targets(md_mesh *m, Request &r, hipStream_t st) {
     int k = (int)r.idx.size();
     if (k == 0) return MD_OK;
     r.iv.resize(k);
@@ -253,7 +278,6 @@ int apply_targets(md_mesh *m, Request &r) {
         r.iv[i] = m->topo.internal_of[r.idx[i]];
         std::memcpy(&m->con[3 * (size_t)r.idx[i]], &r.xyz[3 * (size_t)i], 3 * sizeof(double));
     }
-    hipStream_t st = m->ctx->stream;
     hipError_t e = hipMemcpyAsync(m->d_tgt_idx, r.iv.data(), sizeof(int) * k, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(m->d_tgt_xyz, r.xyz.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, st);
@@ -264,96 +288,239 @@ int apply_targets(md_mesh *m, Request &r) {
     return e == hipSuccess ? MD_OK : MD_ERR_HIP;
 }
 
+void fail(Request &r, int status, const std::string &msg) {
+    if (r.status == MD_OK) {
+        r.status = status;
+        r.error = msg;
+    }
+    r.left = 0;
+}
+
+// Admission at an iteration boundary (S->mu held). Fills the evictions and restorations it
+// needs; admitted requests join S->active with their meshes marked busy.
+void admit(md_server *S, std::vector<ServedMesh *> &evict, std::vector<ServedMesh *> &restore,
+           std::vector<std::shared_ptr<Request>> &admitted) {
+    const size_t budget = S->cfg.hbm_budget_bytes;
+    size_t resident = S->resident;
+    bool blocked = false;  // a request waits for memory: later evicted meshes wait behind it
+    for (auto it = S->queue.begin(); it != S->queue.end();) {
+        std::shared_ptr<Request> r = *it;
+        ServedMesh *sm = r->sm;
+        if (S->cfg.max_active > 0 && (int)S->active.size() >= S->cfg.max_active) break;
+        if (sm->busy) { ++it; continue; }
+        if (sm->m->evicted) {
+            if (blocked) { ++it; continue; }
+            std::vector<ServedMesh *> victims;
+            size_t after = resident;
+            if (budget > 0) {
+                // least recently used idle resident meshes, until the request fits
+                std::vector<ServedMesh *> idle;
+                for (auto &x : S->meshes)
+                    if (!x->m->evicted && !x->busy &&
+                        std::find(evict.begin(), evict.end(), x.get()) == evict.end())
+                        idle.push_back(x.get());
+                std::sort(idle.begin(), idle.end(),
+                          [](const ServedMesh *a, const ServedMesh *b) { return a->last_use < b->last_use; });
+                for (ServedMesh *v : idle) {
+                    if (after + sm->bytes <= budget) break;
+                    victims.push_back(v);
+                    after -= v->bytes;
+                }
+                if (after + sm->bytes > budget) { blocked = true; ++it; continue; }
+            }
+            for (ServedMesh *v : victims) evict.push_back(v);
+            restore.push_back(sm);
+            resident = after + sm->bytes;
+        }
+        sm->busy = true;
+        sm->last_use = ++S->lru_clock;
+        S->active.push_back(r);
+        admitted.push_back(r);
+        it = S->queue.erase(it);
+    }
+}
+
+// Marks requests done (S->mu not held) and wakes their waiters.
+void complete(md_server *S, std::vector<std::shared_ptr<Request>> &done) {
+    if (done.empty()) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto now = SClock::now();
+        for (auto &r : done) {
+            r->sm->m->iterated = true;
+            r->done = true;
+            r->t_done = now;
+            r->sm->busy = false;
+            if (r->status == MD_OK) ++S->stats.completed;
+            else ++S->stats.failed;
+        }
+    }
+    done.clear();
+    S->cv_done.notify_all();
+}
+
+// Waits for the launches up to `upto` (all queued work when upto is 0) and completes the
+// requests whose positions have arrived.
+void drain(md_server *S, uint64_t upto, hipError_t &err) {
+    hipError_t e = upto ? hipEventSynchronize(S->ev[upto % 2]) : hipStreamSynchronize(S->stream);
+    if (e != hipSuccess && err == hipSuccess) err = e;
+    std::vector<std::shared_ptr<Request>> done;
+    for (auto &r : S->finishing)
+        if (upto == 0 || r->finish_seq <= upto) {
+            if (err != hipSuccess) fail(*r, MD_ERR_HIP, hipGetErrorString(err));
+            done.push_back(r);
+        }
+    S->finishing.erase(std::remove_if(S->finishing.begin(), S->finishing.end(),
+                                      [&](const std::shared_ptr<Request> &r) {
+                                          return std::find(done.begin(), done.end(), r) != done.end();
+                                      }),
+                       S->finishing.end());
+    complete(S, done);
+}
+
 void server_loop(md_server *S) {
     hipSetDevice(S->ctx->device);
-    hipStream_t st = S->ctx->stream;
+    hipStream_t st = S->stream;
+    int cur = 0;  // plan of the last launch
     for (;;) {
-        std::vector<std::shared_ptr<Request>> admitted_now;
+        std::vector<ServedMesh *> evict, restore;
+        std::vector<std::shared_ptr<Request>> admitted;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
-                if (S->stop) return;
-                // admission at the iteration boundary
-                for (auto it = S->queue.begin(); it != S->queue.end();) {
-                    auto &r = *it;
-                    bool slot = S->cfg.max_active <= 0 || (int)S->active.size() < S->cfg.max_active;
-                    if (slot && !S->busy[r->mesh_id]) {
-                        S->busy[r->mesh_id] = 1;
-                        r->admitted = true;
-                        S->active.push_back(r);
-                        admitted_now.push_back(r);
-                        it = S->queue.erase(it);
-                    } else {
-                        ++it;
-                    }
-                }
-                if (!S->active.empty()) break;
+                if (S->stop) break;
+                admit(S, evict, restore, admitted);
+                if (!S->active.empty() || !S->finishing.empty()) break;
                 S->cv_work.wait(lk);
             }
-            // pinned result buffers for the requests that finish after this iteration
-            for (auto &r : S->active)
-                if (r->left <= 1 && !r->pinned && r->status == MD_OK) {
-                    r->n = S->meshes[r->mesh_id]->n;
-                    r->pinned = S->take_pinned(r->mesh_id, r->n);
-                    if (!r->pinned) { r->status = MD_ERR_NOMEM; r->error = "pinned result buffer"; r->left = 0; }
+            if (S->stop) {
+                lk.unlock();
+                hipError_t e = hipSuccess;
+                drain(S, 0, e);
+                std::vector<std::shared_ptr<Request>> left;
+                {
+                    std::lock_guard<std::mutex> g(S->mu);
+                    for (auto &r : S->queue) left.push_back(r);
+                    S->queue.clear();
                 }
-        }
-        for (auto &r : admitted_now) {
-            int rc = apply_targets(S->meshes[r->mesh_id], *r);
-            if (rc) { r->status = rc; r->error = "target upload failed"; r->left = 0; }
-        }
-        // plan over the active meshes (rebuilt only when the set changed; stream-ordered)
-        std::vector<int> ids;
-        for (auto &r : S->active) ids.push_back(r->mesh_id);
-        if (ids != S->plan_meshes) {
-            std::vector<const MeshDev *> devs;
-            std::vector<const Factor *> facs;
-            for (int id : ids) { devs.push_back(&S->meshes[id]->dev); facs.push_back(&S->meshes[id]->fac); }
-            md_mesh *m0 = S->meshes[ids[0]];
-            S->plan.pcg = m0->cfg.solver == MD_SOLVER_PCG;
-            S->plan.pcg_iters = m0->cfg.pcg_max_iters;
-            hipError_t e = build_plan(S->plan, devs, facs, m0->fp64(), m0->cfg.refine_steps, st, false);
-            if (e != hipSuccess) { S->fatal = hipGetErrorString(e); }
-            S->plan_meshes = ids;
-        }
-        bool any_left = false;
-        for (auto &r : S->active) any_left |= r->left > 0;
-        hipError_t e = hipSuccess;
-        if (any_left && S->fatal.empty()) {
-            enqueue_iteration(S->plan, st, nullptr);
-            e = hipGetLastError();
-        }
-        // results of the requests finishing now: device-to-host into their pinned buffers
-        for (auto &r : S->active)
-            if (e == hipSuccess && r->left <= 1 && r->pinned && r->status == MD_OK) {
-                md_mesh *m = S->meshes[r->mesh_id];
-                e = hipMemcpyAsync(r->pinned, m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+                for (auto &r : S->active) left.push_back(r);
+                S->active.clear();
+                for (auto &r : left) fail(*r, MD_ERR_STATE, "the server was destroyed");
+                complete(S, left);
+                return;
             }
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        std::vector<std::shared_ptr<Request>> finished;
-        for (auto &r : S->active) {
-            if (r->left > 0) r->left--;
-            if (e != hipSuccess || !S->fatal.empty()) {
-                r->status = MD_ERR_HIP;
-                r->error = e != hipSuccess ? hipGetErrorString(e) : S->fatal;
-                r->left = 0;
-            }
-            if (r->left == 0) finished.push_back(r);
         }
-        {
+        // residency changes: the victims are idle, so no launch in flight reads them
+        for (ServedMesh *v : evict) {
+            int rc = mesh_evict(v->m, st);
             std::lock_guard<std::mutex> lk(S->mu);
-            auto now = SClock::now();
-            for (auto &r : finished) {
-                S->meshes[r->mesh_id]->iterated = true;
-                r->done = true;
-                r->t_done = now;
-                S->busy[r->mesh_id] = 0;
+            if (rc == MD_OK) {
+                S->resident -= v->bytes;
+                ++S->stats.evictions;
             }
-            S->active.erase(std::remove_if(S->active.begin(), S->active.end(),
-                                           [](const std::shared_ptr<Request> &r) { return r->done; }),
-                            S->active.end());
         }
-        if (!finished.empty()) S->cv_done.notify_all();
+        for (ServedMesh *sm : restore) {
+            int rc = mesh_restore(sm->m, st);
+            std::string msg = rc ? md_last_error() : "";
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (rc == MD_OK) {
+                S->resident += sm->bytes;
+                S->peak_resident = std::max(S->peak_resident, S->resident);
+                ++S->stats.restores;
+            } else {
+                for (auto &r : admitted)
+                    if (r->sm == sm) fail(*r, rc, "restoring the mesh: " + msg);
+            }
+        }
+        for (auto &r : admitted)
+            if (r->status == MD_OK && apply_targets(r->sm->m, *r, st) != MD_OK)
+                fail(*r, MD_ERR_HIP, "target upload failed");
+        // one iteration for every request with iterations left, in the same launches
+        std::vector<std::shared_ptr<Request>> it_set;
+        for (auto &r : S->active)
+            if (r->left > 0 && r->status == MD_OK) it_set.push_back(r);
+        hipError_t e = hipSuccess;
+        bool launched = false;
+        if (!it_set.empty()) {
+            std::vector<std::pair<int, uint64_t>> key;
+            for (auto &r : it_set) key.emplace_back(r->sm->id, r->sm->m->gen);
+            int p = cur;
+            if (S->keys[cur] != key) {
+                // the other plan's last launch was waited for at the previous boundary
+                p = 1 - cur;
+                std::vector<const MeshDev *> devs;
+                std::vector<const Factor *> facs;
+                for (auto &r : it_set) {
+                    devs.push_back(&r->sm->m->dev);
+                    facs.push_back(&r->sm->m->fac);
+                }
+                md_mesh *m0 = it_set[0]->sm->m;
+                S->plans[p].pcg = m0->cfg.solver == MD_SOLVER_PCG;
+                S->plans[p].pcg_iters = m0->cfg.pcg_max_iters;
+                e = build_plan(S->plans[p], devs, facs, m0->fp64(), m0->cfg.refine_steps, st, false);
+                S->keys[p] = e == hipSuccess ? key : std::vector<std::pair<int, uint64_t>>();
+            }
+            if (e == hipSuccess) {
+                enqueue_iteration(S->plans[p], st, nullptr);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) {
+                launched = true;
+                cur = p;
+                ++S->seq;
+                std::lock_guard<std::mutex> lk(S->mu);
+                ++S->stats.iterations;
+                S->stats.mesh_iterations += it_set.size();
+            } else {
+                // this boundary's requests fail; the server goes on with the next ones
+                for (auto &r : it_set) fail(*r, MD_ERR_HIP, std::string("launch failed: ") + hipGetErrorString(e));
+                S->keys[0].clear();
+                S->keys[1].clear();
+            }
+            if (launched)
+                for (auto &r : it_set) r->left--;
+        }
+        // requests without iterations left: their positions to the pinned buffers
+        std::vector<std::shared_ptr<Request>> ending;
+        for (auto &r : S->active)
+            if (r->left == 0) ending.push_back(r);
+        if (!ending.empty()) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto &r : ending)
+                if (r->status == MD_OK && !(r->pinned = S->take_pinned(r->sm)))
+                    fail(*r, MD_ERR_NOMEM, "pinned result buffer");
+        }
+        for (auto &r : ending) {
+            if (r->status == MD_OK) {
+                md_mesh *m = r->sm->m;
+                hipError_t x = hipMemcpyAsync(r->pinned, m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+                if (x != hipSuccess) fail(*r, MD_ERR_HIP, hipGetErrorString(x));
+            }
+            r->finish_seq = launched ? S->seq : 0;
+            S->finishing.push_back(r);
+        }
+        S->active.erase(std::remove_if(S->active.begin(), S->active.end(),
+                                       [](const std::shared_ptr<Request> &r) { return r->left == 0; }),
+                        S->active.end());
+        // complete what the previous launch finished (this one keeps the device busy), or all
+        // queued work when nothing was launched
+        hipError_t err = hipSuccess;
+        if (launched) {
+            hipEventRecord(S->ev[S->seq % 2], st);
+            if (S->seq > 1) drain(S, S->seq - 1, err);
+            else if (S->seq == 1) {
+                // first launch: nothing earlier to wait for, except copies of requests that
+                // ended before any launch
+                bool early = false;
+                for (auto &r : S->finishing) early |= r->finish_seq == 0;
+                if (early) drain(S, 0, err);
+            }
+        } else {
+            drain(S, 0, err);
+        }
+        if (err != hipSuccess) {
+            for (auto &r : S->active) fail(*r, MD_ERR_HIP, hipGetErrorString(err));
+        }
     }
 }
 
@@ -364,6 +531,16 @@ extern "C" int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_ser
     std::unique_ptr<md_server> S(new md_server());
     S->ctx = ctx;
     if (cfg) S->cfg = *cfg;
+    hipSetDevice(ctx->device);
+    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t &x : S->ev)
+            if (x) hipEventDestroy(x);
+        if (S->stream) hipStreamDestroy(S->stream);
+        set_error("md_server_create: stream or event creation failed");
+        return MD_ERR_HIP;
+    }
     ctx_retain(ctx);
     S->worker = std::thread(server_loop, S.get());
     *out = S.release();
@@ -373,30 +550,44 @@ extern "C" int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_ser
 extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
     if (!S || !m || !mesh_id) { set_error("md_server_add_mesh: invalid argument"); return MD_ERR_INVALID; }
     if (m->ctx != S->ctx) { set_error("md_server_add_mesh: mesh from another context"); return MD_ERR_INVALID; }
+    if (m->servers > 0) { set_error("md_server_add_mesh: the mesh is registered with a server already"); return MD_ERR_STATE; }
     if (m->dirty || !m->analyzed || m->cfg.rest_mode != MD_REST_FIXED) {
         set_error("md_server_add_mesh: the mesh must be prepared (md_mesh_prepare) with MD_REST_FIXED");
         return MD_ERR_STATE;
     }
+    hipSetDevice(S->ctx->device);
+    if (int rc = ensure_resident(m, "md_server_add_mesh")) return rc;
+    const size_t bytes = m->arena.size() + m->larena.size();
+    if (S->cfg.hbm_budget_bytes > 0 && bytes > S->cfg.hbm_budget_bytes) {
+        set_error("md_server_add_mesh: mesh needs " + std::to_string(bytes) + " device bytes, more than the HBM budget of " +
+                  std::to_string(S->cfg.hbm_budget_bytes));
+        return MD_ERR_BUDGET;
+    }
+    // its preparation on the context's stream is complete before the server's stream reads it
+    if (hipStreamSynchronize(S->ctx->stream) != hipSuccess) { set_error("md_server_add_mesh: synchronisation failed"); return MD_ERR_HIP; }
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->meshes.empty()) {
-        md_mesh *m0 = S->meshes[0];
+        md_mesh *m0 = S->meshes[0]->m;
         if (m0->cfg.precision != m->cfg.precision || m0->cfg.refine_steps != m->cfg.refine_steps ||
             m0->cfg.solver != m->cfg.solver || m0->cfg.pcg_max_iters != m->cfg.pcg_max_iters) {
             set_error("md_server_add_mesh: meshes of one server share precision and refinement steps");
             return MD_ERR_INVALID;
         }
     }
-    size_t bytes = m->arena.size() + m->larena.size();
     if (S->cfg.hbm_budget_bytes > 0 && S->resident + bytes > S->cfg.hbm_budget_bytes) {
-        set_error("md_server_add_mesh: mesh needs " + std::to_string(bytes) + " device bytes, " +
-                  std::to_string(S->cfg.hbm_budget_bytes - S->resident) + " left in the HBM budget");
-        return MD_ERR_BUDGET;
+        // registered non-resident: restored when a request for it is admitted
+        if (int rc = mesh_evict(m, S->ctx->stream)) return rc;
+        ++S->stats.evictions;
+    } else {
+        S->resident += bytes;
+        S->peak_resident = std::max(S->peak_resident, S->resident);
     }
-    S->resident += bytes;
-    S->meshes.push_back(m);
+    std::unique_ptr<ServedMesh> sm(new ServedMesh());
+    sm->m = m;
+    sm->id = (int)S->meshes.size();
+    sm->bytes = bytes;
+    S->meshes.push_back(std::move(sm));
     ++m->servers;
-    S->busy.push_back(0);
-    S->pinned_free.emplace_back();
     *mesh_id = (int)S->meshes.size() - 1;
     return MD_OK;
 }
@@ -411,7 +602,8 @@ extern "C" int md_server_submit(md_server *S, int mesh_id, int k, const int *idx
     {
         std::lock_guard<std::mutex> lk(S->mu);
         if (mesh_id < 0 || mesh_id >= (int)S->meshes.size()) { set_error("md_server_submit: unknown mesh"); return MD_ERR_INVALID; }
-        md_mesh *m = S->meshes[mesh_id];
+        ServedMesh *sm = S->meshes[mesh_id].get();
+        md_mesh *m = sm->m;
         if (k > m->n - m->topo.nf) { set_error("md_server_submit: more targets than constrained vertices"); return MD_ERR_INVALID; }
         for (int i = 0; i < k; ++i)
             if (idx[i] < 0 || idx[i] >= m->n || !m->is_con[idx[i]]) {
@@ -419,23 +611,26 @@ extern "C" int md_server_submit(md_server *S, int mesh_id, int k, const int *idx
                 return MD_ERR_INVALID;
             }
         r->ticket = S->next_ticket++;
-        r->mesh_id = mesh_id;
+        r->sm = sm;
         r->idx.assign(idx, idx + k);
         r->xyz.assign(xyz, xyz + 3 * (size_t)k);
         r->iters = r->left = iterations;
         r->t_submit = SClock::now();
         S->queue.push_back(r);
         S->tickets[r->ticket] = r;
+        ++S->stats.submitted;
         *ticket = r->ticket;
     }
     S->cv_work.notify_one();
     return MD_OK;
 }
 
-// Called with S->mu held: takes the finished request out of the ticket map. The permutation
-// of its positions into the original vertex order runs afterwards without the lock.
+// Called with S->mu held: takes the finished request out of the ticket map (null when another
+// caller took it first). The permutation of its positions into the original vertex order runs
+// afterwards without the lock.
 static std::shared_ptr<Request> take_finished(md_server *S, uint64_t ticket) {
     auto it = S->tickets.find(ticket);
+    if (it == S->tickets.end()) return nullptr;
     std::shared_ptr<Request> r = it->second;
     S->tickets.erase(it);
     return r;
@@ -446,7 +641,7 @@ static int deliver(md_server *S, std::shared_ptr<Request> r, double *xyz, double
     int rc = r->status;
     if (rc != MD_OK) set_error("request failed: " + r->error);
     else if (xyz && r->pinned) {
-        md_mesh *m = S->meshes[r->mesh_id];
+        md_mesh *m = r->sm->m;
         for (int iv = 0; iv < m->n; ++iv) {
             int v = m->topo.original_of[iv];
             xyz[3 * (size_t)v] = r->pinned[iv].x;
@@ -456,7 +651,7 @@ static int deliver(md_server *S, std::shared_ptr<Request> r, double *xyz, double
     }
     if (r->pinned) {
         std::lock_guard<std::mutex> lk(S->mu);
-        S->pinned_free[r->mesh_id].push_back(r->pinned);
+        r->sm->pinned_free.push_back(r->pinned);
         r->pinned = nullptr;
     }
     return rc;
@@ -468,7 +663,7 @@ extern "C" int md_server_poll(md_server *S, uint64_t ticket, int *done, double *
     {
         std::lock_guard<std::mutex> lk(S->mu);
         auto it = S->tickets.find(ticket);
-        if (it == S->tickets.end()) { set_error("md_server_poll: unknown ticket"); return MD_ERR_INVALID; }
+        if (it == S->tickets.end()) { set_error("md_server_poll: unknown ticket (or already taken)"); return MD_ERR_INVALID; }
         *done = it->second->done ? 1 : 0;
         if (!*done) return MD_OK;
         r = take_finished(S, ticket);
@@ -482,13 +677,27 @@ extern "C" int md_server_wait(md_server *S, uint64_t ticket, double timeout_ms, 
     {
         std::unique_lock<std::mutex> lk(S->mu);
         auto it = S->tickets.find(ticket);
-        if (it == S->tickets.end()) { set_error("md_server_wait: unknown ticket"); return MD_ERR_INVALID; }
+        if (it == S->tickets.end()) { set_error("md_server_wait: unknown ticket (or already taken)"); return MD_ERR_INVALID; }
         r = it->second;
         bool ok = S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), [&] { return r->done; });
         if (!ok) { set_error("md_server_wait: timed out"); return MD_ERR_TIMEOUT; }
-        take_finished(S, ticket);
+        // another poll or wait may have taken the ticket while this one slept
+        if (!take_finished(S, ticket)) { set_error("md_server_wait: the ticket was taken by another call"); return MD_ERR_INVALID; }
     }
     return deliver(S, r, xyz, latency_ms);
+}
+
+extern "C" int md_server_get_stats(md_server *S, md_server_stats *out) {
+    if (!S || !out) { set_error("md_server_get_stats: invalid argument"); return MD_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->stats;
+    out->resident_bytes = S->resident;
+    out->peak_resident_bytes = S->peak_resident;
+    out->meshes = (int)S->meshes.size();
+    out->resident_meshes = 0;
+    for (auto &x : S->meshes) out->resident_meshes += x->m->evicted ? 0 : 1;
+    out->queued = (int)S->queue.size();
+    return MD_OK;
 }
 
 extern "C" int md_server_destroy(md_server *S) {
@@ -501,9 +710,12 @@ extern "C" int md_server_destroy(md_server *S) {
     if (S->worker.joinable()) S->worker.join();
     md_ctx *ctx = S->ctx;
     hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    hipStreamSynchronize(S->stream);
     for (d4 *p : S->pinned_all) hipHostFree(p);
-    for (md_mesh *m : S->meshes) --m->servers;
+    // meshes evicted by the server stay on the host until their next use (md_mesh_* restores)
+    for (auto &x : S->meshes) --x->m->servers;
+    for (hipEvent_t &x : S->ev) hipEventDestroy(x);
+    hipStreamDestroy(S->stream);
     delete S;
     ctx_release(ctx);
     return MD_OK;

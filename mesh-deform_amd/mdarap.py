@@ -46,14 +46,27 @@ class MeshStats(ctypes.Structure):
                 ("bytes_local", ctypes.c_int64), ("bytes_residual", ctypes.c_int64),
                 ("bytes_forward", ctypes.c_int64), ("bytes_backward", ctypes.c_int64),
                 ("subtree_budget", ctypes.c_int), ("subtrees", ctypes.c_int), ("factor_nnz_exact", ctypes.c_int64),
-                ("factor_ms", ctypes.c_double)]
+                ("factor_ms", ctypes.c_double), ("deform_ms", ctypes.c_double * 6)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["deform_ms"] = list(self.deform_ms)
+        return d
 
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_active", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint64), ("mesh_iterations", ctypes.c_uint64),
+                ("evictions", ctypes.c_uint64), ("restores", ctypes.c_uint64),
+                ("resident_bytes", ctypes.c_size_t), ("peak_resident_bytes", ctypes.c_size_t),
+                ("meshes", ctypes.c_int), ("resident_meshes", ctypes.c_int), ("queued", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class MDError(RuntimeError):
@@ -109,8 +122,10 @@ _SIGS = [
     ("md_server_poll", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), _vp,
                                       ctypes.POINTER(ctypes.c_double)]),
     ("md_server_wait", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_double, _vp, ctypes.POINTER(ctypes.c_double)]),
+    ("md_server_get_stats", ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
     ("md_server_destroy", ctypes.c_int, [_vp]),
     ("md_copy_probe", ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    ("md_host_best_rotation", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _f64p, _vp, _f64p, _vp]),
     ("md_host_factor_solve", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _f64p, _f64p,
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int]),
@@ -399,6 +414,11 @@ class Server:
                                    out.ctypes.data if out is not None else None, ctypes.byref(lat)))
         return (out.reshape(-1, 3) if out is not None else None), lat.value
 
+    def stats(self):
+        st = ServerStats()
+        check(lib().md_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
     def close(self):
         if getattr(self, "_h", None):
             lib().md_server_destroy(self._h)
@@ -435,3 +455,26 @@ def host_factor_solve(rp, ci, v, xyz, b, subtree_budget=-1, device_factor_emulat
                                      ctypes.byref(lv), int(subtree_budget), 1 if device_factor_emulation else 0,
                                      1 if level_pairs else 0))
     return (x[0] if np.ndim(b) == 1 else x), pe.value, lv.value
+
+
+def host_best_rotation(cov, prev=None, fp64=True):
+    """Host-only: the local step's rotation code on the CPU for covariances (k, 3, 3); returns
+    (quaternions (k, 4) as x, y, z, w, per-entry flag of the Jacobi fallback)."""
+    cov = np.ascontiguousarray(cov, np.float64).reshape(-1, 9)
+    k = cov.shape[0]
+    q = np.zeros((k, 4), np.float64)
+    fb = np.zeros(k, np.int32)
+    pv = None if prev is None else np.ascontiguousarray(prev, np.float64).reshape(k, 4)
+    check(lib().md_host_best_rotation(1 if fp64 else 0, k, cov.ravel(), None if pv is None else pv.ctypes.data,
+                                      q.reshape(-1), fb.ctypes.data))
+    return q, fb.astype(bool)
+
+
+def quat_to_matrix(q):
+    """Rotation matrices (k, 3, 3) of quaternions (k, 4) as x, y, z, w."""
+    q = np.asarray(q, np.float64).reshape(-1, 4)
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    s = 2.0 / (x * x + y * y + z * z + w * w)
+    return np.stack([1 - s * (y * y + z * z), s * (x * y - w * z), s * (x * z + w * y),
+                     s * (x * y + w * z), 1 - s * (x * x + z * z), s * (y * z - w * x),
+                     s * (x * z - w * y), s * (y * z + w * x), 1 - s * (x * x + y * y)], axis=1).reshape(-1, 3, 3)

@@ -302,12 +302,12 @@ def test_interactive_frames_match_oracle(ctx, golden_dir, rest_mode):
 
 # ---- full-size workloads ------------------------------------------------------------------
 
-@pytest.mark.parametrize("which,iters", [("bar", 50), ("ico", 5)])
+@pytest.mark.parametrize("which,iters", [("bar", 50)])
 def test_large_workloads_fp64_and_fp32(ctx, which, iters):
     """The 67k-vertex bar (40 x 40 x 400 quads, one end anchored, the other rotated 90 degrees
-    about the long axis and lifted 0.3) over 50 iterations, and the 164k noisy icosphere, FP64
-    and FP32 + refinement against the oracle."""
-    pr = {"bar": meshgen.bar_problem, "ico": meshgen.noisy_icosphere_problem}[which]()
+    about the long axis and lifted 0.3) over 50 iterations, FP64 and FP32 + refinement against
+    the oracle (the 164k icosphere over 100 iterations is in test_gpu_configs.py)."""
+    pr = {"bar": meshgen.bar_problem}[which]()
     say("case: full size", pr.name, "n", pr.n)
     idx, tgt = pr.constraints()
     a = oracle.Arap(pr.xyz, pr.tri)

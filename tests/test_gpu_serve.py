@@ -6,7 +6,11 @@
 * Served results match the FP64 oracle run in fixed-rest mode (the rest pose and factor of
   the first deform are kept; each request moves the handles and warm-starts from the last
   result), within the FP32 + refinement tolerance 1e-5.
-* HBM-budgeted residency: a mesh that does not fit the budget is rejected.
+* HBM-budgeted residency: a mesh larger than the budget is rejected; under a budget smaller
+  than the meshes' total footprint, requests wait for least recently used meshes to be evicted
+  to host memory and give bitwise the results of an unlimited server.
+* A served mesh's own entry points refuse (MD_ERR_STATE) until the server is destroyed; a
+  ticket's result is handed out once.
 """
 import sys
 
@@ -155,3 +159,82 @@ def test_prepare_many_equals_prepare(ctx):
         outs.append([m.download() for m in meshes])
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+def test_budgeted_admission_evicts_and_matches_unlimited(ctx):
+    """HBM-budgeted admission: with a budget that holds the two largest of six meshes but not
+    all of them, requests on evicted meshes wait until least recently used idle meshes are evicted to host
+    memory and theirs is restored; every result is bitwise the one an unlimited server gives
+    (eviction and restoration copy the device state exactly), and the resident bytes never
+    exceed the budget."""
+    say("case: server budget-driven admission")
+    probs = meshgen.mixed_batch(6, 1500, 9000, seed=31)
+    reqs = request_stream(probs, 30, seed=17)
+    iters = 4
+    meshes_u = make_meshes(ctx, probs)
+    srv = mdarap.Server(ctx)
+    ids = [srv.add_mesh(m) for m in meshes_u]
+    ref = [srv.wait(srv.submit(ids[k], h, t, iters), probs[k].n)[0] for k, h, t in reqs]
+    srv.close()
+    meshes = make_meshes(ctx, probs)
+    sizes = sorted(m.stats()["device_bytes"] for m in meshes)
+    budget = sizes[-1] + sizes[-2]  # the two largest fit together, not three meshes
+    assert budget < sum(sizes)
+    srv = mdarap.Server(ctx, hbm_budget_bytes=budget)
+    ids = [srv.add_mesh(m) for m in meshes]
+    st0 = srv.stats()
+    assert st0["resident_bytes"] <= budget and st0["resident_meshes"] < len(meshes)
+    tickets = [srv.submit(ids[k], h, t, iters) for k, h, t in reqs]
+    got = [srv.wait(tk, probs[k].n)[0] for tk, (k, _, _) in zip(tickets, reqs)]
+    st = srv.stats()
+    say("  stats", st)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["evictions"] > 0 and st["restores"] > 0
+    assert st["peak_resident_bytes"] <= budget
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), i
+    # a served mesh belongs to the server: its own entry points refuse
+    with pytest.raises(mdarap.MDError) as e:
+        meshes[0].iterate(1)
+    assert e.value.code == mdarap.MD_ERR_STATE
+    with pytest.raises(mdarap.MDError) as e:
+        meshes[0].set_constraint(int(probs[0].handles[0]), [0.0, 0.0, 0.0])
+    assert e.value.code == mdarap.MD_ERR_STATE
+    with pytest.raises(mdarap.MDError) as e:
+        mdarap.Batch(ctx, [meshes[1]])
+    assert e.value.code == mdarap.MD_ERR_STATE
+    assert mdarap.lib().md_mesh_destroy(meshes[2].handle) == mdarap.MD_ERR_STATE
+    srv.close()
+    # after the server: evicted meshes come back on their next call, with the last result
+    last = {}
+    for (k, _, _), pos in zip(reqs, got):
+        last[k] = pos
+    for k, m in enumerate(meshes):
+        if k in last:
+            assert np.array_equal(m.download(), last[k]), k
+    m = meshes[0]
+    m.iterate(1)
+    assert np.all(np.isfinite(m.download()))
+
+
+def test_ticket_taken_once(ctx):
+    say("case: server ticket taken once")
+    probs = meshgen.mixed_batch(2, 400, 900, seed=41)
+    meshes = make_meshes(ctx, probs)
+    srv = mdarap.Server(ctx)
+    ids = [srv.add_mesh(m) for m in meshes]
+    (k, h, t), = request_stream(probs, 1, seed=2)
+    tk = srv.submit(ids[k], h, t, 2)
+    pos, _ = srv.wait(tk, probs[k].n)
+    assert np.all(np.isfinite(pos))
+    with pytest.raises(mdarap.MDError) as e:
+        srv.poll(tk, probs[k].n)
+    assert e.value.code == mdarap.MD_ERR_INVALID
+    with pytest.raises(mdarap.MDError) as e:
+        srv.wait(tk, probs[k].n)
+    assert e.value.code == mdarap.MD_ERR_INVALID
+    # zero-iteration requests return the pasted targets without iterating
+    tk = srv.submit(ids[k], h, t, 0)
+    pos0, _ = srv.wait(tk, probs[k].n)
+    np.testing.assert_array_equal(pos0[h], t)
+    srv.close()

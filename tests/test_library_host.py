@@ -140,3 +140,51 @@ def test_host_block_ldlt_rejects_singular():
     with pytest.raises(mdarap.MDError) as e:
         mdarap.host_factor_solve(L.indptr, L.indices, L.data, x1, np.ones(n))
     assert e.value.code == mdarap.MD_ERR_FACTOR
+
+
+# ---- the local step's rotation (kernel code compiled for the host) ------------------------
+
+def test_rotation_matches_svd_golden():
+    """The characteristic-polynomial rotation (Newton on Horn's quartic, adjugate eigenvector,
+    one Rayleigh-quotient refinement) against numpy.linalg.svd's V diag(1,1,det) U^T on the
+    fixture's covariances (reflections and near-rank-2 included): 1e-12 in FP64, 2e-5 in FP32."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "svd_rotations.npz"))
+    for fp64, tol in ((True, 1e-12), (False, 2e-5)):
+        q, fb = mdarap.host_best_rotation(g["cov"], fp64=fp64)
+        R = mdarap.quat_to_matrix(q)
+        np.testing.assert_allclose(R, g["R"], atol=tol)
+        assert np.all(q[:, 3] >= 0)
+        assert fb.sum() <= 2
+
+
+def test_rotation_planar_rank1_and_zero_covariances():
+    rng = np.random.default_rng(12)
+    # planar neighbourhoods: rank-2 covariances, a well separated top eigenvalue
+    C = []
+    for _ in range(200):
+        a = rng.standard_normal((6, 3))
+        a[:, 2] = 0.0
+        Q = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+        e = a @ Q.T + 1e-3 * rng.standard_normal((6, 3))
+        C.append(a.T @ e)
+    C = np.array(C)
+    ref = np.array([oracle.svd_rotation(c) for c in C])
+    q, fb = mdarap.host_best_rotation(C, fp64=True)
+    assert not fb.any()
+    np.testing.assert_allclose(mdarap.quat_to_matrix(q), ref, atol=1e-12)
+    # rank 1 (collinear edges): the top eigenvalue is double, the Jacobi sweeps decide; any
+    # rotation taking the edge direction onto the deformed one maximises the trace
+    a = rng.standard_normal(3)
+    e = rng.standard_normal(3)
+    c = np.outer(a, e)
+    q, fb = mdarap.host_best_rotation(c[None], fp64=True)
+    assert fb[0]
+    R = mdarap.quat_to_matrix(q)[0]
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+    assert np.trace(R @ c) == pytest.approx(np.linalg.norm(a) * np.linalg.norm(e), rel=1e-10)
+    # zero covariance: the previous rotation stays
+    prev = np.array([[0.1, -0.2, 0.3, 0.9]])
+    prev /= np.linalg.norm(prev)
+    q, _ = mdarap.host_best_rotation(np.zeros((1, 3, 3)), prev=prev, fp64=True)
+    np.testing.assert_allclose(q, prev, atol=1e-15)
