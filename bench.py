@@ -21,7 +21,7 @@ Legs:
           trajectory and runs 5 warm-started iterations; p50/p99 latency (arrival to result
           on the host) and aggregate iterations/s; then a saturation sweep (4k-32k requests/s,
           until p99 passes 16 ms: the capacity), compared with the same meshes iterated one by
-          one, and one stream under an HBM budget of half the meshes' footprint.
+          one.
   frames  the reference's interactive loop, setConstraint + deform(5) per frame in the
           reference's rest mode (each frame re-runs the cotan weights and the refactorization),
           on the 67k bar (FP64) and the 164k icosphere (FP64, FP32 + refinement): frames/s
@@ -238,8 +238,7 @@ def served_stream(ctx, probs, meshes, utils, trajs, rng, rate, seconds, iters, b
 
 def served_leg(ctx, count, seconds, rate, iters, seed, sweep=(4000, 8000, 16000, 32000), p99_bound_ms=16.0):
     """The served workload at `rate` (the headline), then a saturation sweep over higher rates
-    until p99 exceeds the bound, then one stream at `rate` under an HBM budget of half the
-    meshes' footprint (evictions to host memory)."""
+    until p99 exceeds the bound."""
     rng = np.random.default_rng(seed)
     probs = served_problems(count, 5000, 164000, seed)
     meshes, utils, trajs = [], [], []
@@ -276,10 +275,6 @@ def served_leg(ctx, count, seconds, rate, iters, seed, sweep=(4000, 8000, 16000,
         if x["p99_ms"] >= p99_bound_ms:
             break
         capacity = x
-    log("served: %.0f requests/s under a budget of half the footprint" % rate)
-    budgeted = served_stream(ctx, probs, meshes, utils, trajs, rng, rate, min(seconds, 1.0), iters,
-                             budget=footprint // 2)
-    budgeted["hbm_budget_bytes"] = int(footprint // 2)
     sizes = [p.n for p in probs]
     for m in meshes:
         m.close()
@@ -291,7 +286,6 @@ def served_leg(ctx, count, seconds, rate, iters, seed, sweep=(4000, 8000, 16000,
                                                           p99_ms=capacity["p99_ms"],
                                                           iters_per_s=capacity["iters_per_s"],
                                                           vs_plain_one_by_one=round(capacity["iters_per_s"] / plain_same, 3))
-    out["budgeted"] = budgeted
     return out
 
 

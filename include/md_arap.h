@@ -207,12 +207,11 @@ int md_batch_destroy(md_batch *batch);
 /* ---- serving: resident meshes, continuous batching of drag requests ----------------- */
 /* A server iterates the admitted drag requests of many meshes in the same launches (one ARAP
  * iteration per boundary for the whole set) and admits queued requests at every iteration
- * boundary. HBM-budgeted residency: a mesh's device footprint (md_mesh_stats.device_bytes)
- * counts against hbm_budget_bytes while it is resident; a request on a mesh that is not
- * resident is admitted once the mesh fits, after the least recently used idle meshes are
- * evicted to pinned host memory (their state is copied back bit for bit when they are next
- * needed), and otherwise waits until iterations finish and memory frees up. Requests on one
- * mesh run in submission order, each warm-started from the previous result. */
+ * boundary (one request in flight per mesh, at most max_active in flight). HBM budget: a
+ * mesh's device footprint (md_mesh_stats.device_bytes) counts against hbm_budget_bytes, and a
+ * registration that would exceed it is refused (every served mesh stays resident; eviction
+ * of idle meshes to host memory is switched off). Requests on one mesh run in submission
+ * order, each warm-started from the previous result. */
 typedef struct md_server md_server;
 typedef struct {
     size_t hbm_budget_bytes; /* device bytes the resident meshes may hold (0 = unlimited) */
@@ -230,8 +229,8 @@ typedef struct {
 
 int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_server **out);
 /* Registers a prepared mesh (md_mesh_prepare done, MD_REST_FIXED). MD_ERR_BUDGET when the
- * mesh alone exceeds the budget; a mesh that does not fit next to the resident ones is
- * registered evicted. The server takes no ownership: destroy the server before the mesh. */
+ * registered meshes' footprint would exceed the budget. The server takes no ownership:
+ * destroy the server before the mesh. */
 int md_server_add_mesh(md_server *srv, md_mesh *mesh, int *mesh_id);
 /* Queues a drag: move the k constrained vertices idx to xyz, then run `iterations`
  * warm-started iterations. Indices must already be constrained on that mesh. */
@@ -244,8 +243,7 @@ int md_server_poll(md_server *srv, uint64_t ticket, int *done, double *xyz, doub
  * a wait whose ticket another call took meanwhile) is MD_ERR_INVALID. */
 int md_server_wait(md_server *srv, uint64_t ticket, double timeout_ms, double *xyz, double *latency_ms);
 int md_server_get_stats(md_server *srv, md_server_stats *out);
-/* Waits for the launches in flight; requests still queued fail with MD_ERR_STATE. Meshes the
- * server evicted are restored by their next md_mesh_* call. */
+/* Waits for the launches in flight; requests still queued fail with MD_ERR_STATE. */
 int md_server_destroy(md_server *srv);
 
 /* ---- device diagnostics ------------------------------------------------------------- */
@@ -276,6 +274,18 @@ int md_host_factor_solve(int n, const int *rowptr, const int *col, const double 
  * fallback (may be NULL) receives 1 per entry where the top eigenvalue of Horn's matrix was
  * too close to the next for the characteristic-polynomial path and Jacobi sweeps decided. */
 int md_host_best_rotation(int fp64, int count, const double *cov, const double *prev, double *quat, int *fallback);
+
+/* One admission pass of the server's planner (the code md_server_* runs at every iteration
+ * boundary) on a described state, without a device: nmeshes served meshes with their device
+ * footprint, least-recently-used stamp, residency and busy flags; nqueue queued requests given
+ * by their mesh; `active` requests in flight, max_active (<= 0: no limit) and the HBM budget
+ * (0: no limit); *resident_bytes in and out. Out: take[q] = 1 for admitted requests, the meshes
+ * to evict (evict[0 .. *nevict)) and then to restore (restore[0 .. *nrestore)); evict and
+ * restore hold nmeshes entries. */
+int md_host_admission_plan(int nmeshes, const int64_t *bytes, const uint64_t *last_use, const int *resident,
+                           const int *busy, int nqueue, const int *queue, int active, int max_active,
+                           int64_t budget, int64_t *resident_bytes, int *take, int *evict, int *nevict,
+                           int *restore, int *nrestore);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@
 #include <mutex>
 #include <thread>
 
+#include "md_admission.h"
 #include "md_runtime.h"
 
 using namespace md;
@@ -270,22 +271,26 @@ namespace {
 // Pastes the request's targets into p' (fixed rest pose: the previous result is the initial
 // guess): one upload and one scatter launch, stream-ordered, no host synchronisation. The
 // sources live in the request until it completes.
-int apply_targets(md_mesh *m, Request &r, hipStream_t st) {
+hipError_t apply_targets(md_mesh *m, Request &r, hipStream_t st, std::string &what) {
     int k = (int)r.idx.size();
-    if (k == 0) return MD_OK;
+    if (k == 0) return hipSuccess;
     r.iv.resize(k);
     for (int i = 0; i < k; ++i) {
         r.iv[i] = m->topo.internal_of[r.idx[i]];
         std::memcpy(&m->con[3 * (size_t)r.idx[i]], &r.xyz[3 * (size_t)i], 3 * sizeof(double));
     }
+    what = "target index upload";
     hipError_t e = hipMemcpyAsync(m->d_tgt_idx, r.iv.data(), sizeof(int) * k, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(m->d_tgt_xyz, r.xyz.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
+        what = "target position upload";
+        e = hipMemcpyAsync(m->d_tgt_xyz, r.xyz.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) {
+        what = "target scatter launch";
         launch_scatter_targets(st, m->dev.pp, m->d_tgt_idx, m->d_tgt_xyz, k);
         e = hipGetLastError();
     }
-    return e == hipSuccess ? MD_OK : MD_ERR_HIP;
+    return e;
 }
 
 void fail(Request &r, int status, const std::string &msg) {
@@ -296,48 +301,43 @@ void fail(Request &r, int status, const std::string &msg) {
     r.left = 0;
 }
 
-// Admission at an iteration boundary (S->mu held). Fills the evictions and restorations it
-// needs; admitted requests join S->active with their meshes marked busy.
+// Admission at an iteration boundary (S->mu held), by plan_admission over the served meshes'
+// state. Fills the evictions and restorations it needs; admitted requests join S->active with
+// their meshes marked busy.
 void admit(md_server *S, std::vector<ServedMesh *> &evict, std::vector<ServedMesh *> &restore,
            std::vector<std::shared_ptr<Request>> &admitted) {
-    const size_t budget = S->cfg.hbm_budget_bytes;
+    if (S->queue.empty()) return;
+    std::vector<AdmitMesh> ms(S->meshes.size());
+    for (size_t k = 0; k < ms.size(); ++k) {
+        const ServedMesh &x = *S->meshes[k];
+        ms[k].bytes = x.bytes;
+        ms[k].last_use = x.last_use;
+        ms[k].resident = !x.m->evicted;
+        ms[k].busy = x.busy;
+    }
+    std::vector<int> queue;
+    queue.reserve(S->queue.size());
+    for (auto &r : S->queue) queue.push_back(r->sm->id);
     size_t resident = S->resident;
-    bool blocked = false;  // a request waits for memory: later evicted meshes wait behind it
-    for (auto it = S->queue.begin(); it != S->queue.end();) {
-        std::shared_ptr<Request> r = *it;
-        ServedMesh *sm = r->sm;
-        if (S->cfg.max_active > 0 && (int)S->active.size() >= S->cfg.max_active) break;
-        if (sm->busy) { ++it; continue; }
-        if (sm->m->evicted) {
-            if (blocked) { ++it; continue; }
-            std::vector<ServedMesh *> victims;
-            size_t after = resident;
-            if (budget > 0) {
-                // least recently used idle resident meshes, until the request fits
-                std::vector<ServedMesh *> idle;
-                for (auto &x : S->meshes)
-                    if (!x->m->evicted && !x->busy &&
-                        std::find(evict.begin(), evict.end(), x.get()) == evict.end())
-                        idle.push_back(x.get());
-                std::sort(idle.begin(), idle.end(),
-                          [](const ServedMesh *a, const ServedMesh *b) { return a->last_use < b->last_use; });
-                for (ServedMesh *v : idle) {
-                    if (after + sm->bytes <= budget) break;
-                    victims.push_back(v);
-                    after -= v->bytes;
-                }
-                if (after + sm->bytes > budget) { blocked = true; ++it; continue; }
-            }
-            for (ServedMesh *v : victims) evict.push_back(v);
-            restore.push_back(sm);
-            resident = after + sm->bytes;
+    std::vector<char> take;
+    std::vector<int> ev, rs;
+    plan_admission(ms, queue, (int)S->active.size(), S->cfg.max_active, S->cfg.hbm_budget_bytes, resident,
+                   S->lru_clock, take, ev, rs);
+    for (int k : ev) evict.push_back(S->meshes[k].get());
+    for (int k : rs) restore.push_back(S->meshes[k].get());
+    std::deque<std::shared_ptr<Request>> left;
+    for (size_t q = 0; q < take.size(); ++q) {
+        std::shared_ptr<Request> &r = S->queue[q];
+        if (!take[q]) {
+            left.push_back(r);
+            continue;
         }
-        sm->busy = true;
-        sm->last_use = ++S->lru_clock;
+        r->sm->busy = true;
+        r->sm->last_use = ms[r->sm->id].last_use;
         S->active.push_back(r);
         admitted.push_back(r);
-        it = S->queue.erase(it);
     }
+    S->queue.swap(left);
 }
 
 // Marks requests done (S->mu not held) and wakes their waiters.
@@ -432,9 +432,16 @@ void server_loop(md_server *S) {
                     if (r->sm == sm) fail(*r, rc, "restoring the mesh: " + msg);
             }
         }
-        for (auto &r : admitted)
-            if (r->status == MD_OK && apply_targets(r->sm->m, *r, st) != MD_OK)
-                fail(*r, MD_ERR_HIP, "target upload failed");
+        for (auto &r : admitted) {
+            if (r->status != MD_OK) continue;
+            if (r->sm->m->evicted) {  // its restoration did not happen: nothing may launch on it
+                fail(*r, MD_ERR_STATE, "the mesh is not resident");
+                continue;
+            }
+            std::string what;
+            hipError_t x = apply_targets(r->sm->m, *r, st, what);
+            if (x != hipSuccess) fail(*r, MD_ERR_HIP, what + " failed: " + hipGetErrorString(x));
+        }
         // one iteration for every request with iterations left, in the same launches
         std::vector<std::shared_ptr<Request>> it_set;
         for (auto &r : S->active)
@@ -575,13 +582,15 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
         }
     }
     if (S->cfg.hbm_budget_bytes > 0 && S->resident + bytes > S->cfg.hbm_budget_bytes) {
-        // registered non-resident: restored when a request for it is admitted
-        if (int rc = mesh_evict(m, S->ctx->stream)) return rc;
-        ++S->stats.evictions;
-    } else {
-        S->resident += bytes;
-        S->peak_resident = std::max(S->peak_resident, S->resident);
+        // Registration beyond the budget (non-resident meshes, evicted to host memory and
+        // restored on demand by the admission planner) is switched off: every served mesh
+        // stays resident, so the planner never evicts or restores.
+        set_error("md_server_add_mesh: the served meshes would need " + std::to_string(S->resident + bytes) +
+                  " device bytes, more than the HBM budget of " + std::to_string(S->cfg.hbm_budget_bytes));
+        return MD_ERR_BUDGET;
     }
+    S->resident += bytes;
+    S->peak_resident = std::max(S->peak_resident, S->resident);
     std::unique_ptr<ServedMesh> sm(new ServedMesh());
     sm->m = m;
     sm->id = (int)S->meshes.size();

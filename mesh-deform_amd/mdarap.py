@@ -126,6 +126,9 @@ _SIGS = [
     ("md_server_destroy", ctypes.c_int, [_vp]),
     ("md_copy_probe", ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("md_host_best_rotation", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _f64p, _vp, _f64p, _vp]),
+    ("md_host_admission_plan", ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _vp, _vp,
+                                              ctypes.POINTER(ctypes.c_int), _vp, ctypes.POINTER(ctypes.c_int)]),
     ("md_host_factor_solve", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _f64p, _f64p,
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int]),
@@ -468,6 +471,28 @@ def host_best_rotation(cov, prev=None, fp64=True):
     check(lib().md_host_best_rotation(1 if fp64 else 0, k, cov.ravel(), None if pv is None else pv.ctypes.data,
                                       q.reshape(-1), fb.ctypes.data))
     return q, fb.astype(bool)
+
+
+def host_admission_plan(bytes_, last_use, resident, busy, queue, active=0, max_active=0, budget=0,
+                        resident_bytes=None):
+    """Host-only: one admission pass of the server's planner. Returns (take per queued request,
+    meshes to evict, meshes to restore, resident bytes after both)."""
+    b = np.ascontiguousarray(bytes_, np.int64)
+    lu = np.ascontiguousarray(last_use, np.uint64)
+    rs = np.ascontiguousarray(resident, np.int32)
+    bz = np.ascontiguousarray(busy, np.int32)
+    q = np.ascontiguousarray(queue, np.int32)
+    n = b.shape[0]
+    take = np.zeros(max(1, q.shape[0]), np.int32)
+    ev = np.zeros(max(1, n), np.int32)
+    re = np.zeros(max(1, n), np.int32)
+    nev, nre = ctypes.c_int(0), ctypes.c_int(0)
+    rb = ctypes.c_int64(int(b[rs != 0].sum()) if resident_bytes is None else int(resident_bytes))
+    check(lib().md_host_admission_plan(n, b.ctypes.data, lu.ctypes.data, rs.ctypes.data, bz.ctypes.data, q.shape[0],
+                                       q.ctypes.data, int(active), int(max_active), int(budget), ctypes.byref(rb),
+                                       take.ctypes.data, ev.ctypes.data, ctypes.byref(nev), re.ctypes.data,
+                                       ctypes.byref(nre)))
+    return take[: q.shape[0]].astype(bool), ev[: nev.value].tolist(), re[: nre.value].tolist(), rb.value
 
 
 def quat_to_matrix(q):

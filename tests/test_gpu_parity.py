@@ -529,13 +529,22 @@ def test_context_outlives_its_meshes_regression():
     m = mdarap.Mesh(c, pr.xyz, pr.tri, mdarap.config(rest_mode=mdarap.REST_FIXED))
     m.set_constraints(idx, tgt)
     m.prepare()
+    m2 = mdarap.Mesh(c, pr.xyz, pr.tri, mdarap.config(rest_mode=mdarap.REST_FIXED))
+    m2.set_constraints(idx, tgt)
+    m2.prepare()
     b = mdarap.Batch(c, [m])
     s = mdarap.Server(c)
-    s.add_mesh(m)
+    mid = s.add_mesh(m2)   # a served mesh belongs to the server until it is destroyed
     c.close()          # the caller's handle goes first
     b.iterate(1)       # the objects still work
+    t = s.submit(mid, idx[:1], tgt[:1], 1)
+    out, _ = s.wait(t, pr.n)
+    assert np.all(np.isfinite(out))
     s.close()
     b.close()
     m.iterate(1)
+    m2.iterate(1)
     assert np.all(np.isfinite(m.download()))
+    assert np.all(np.isfinite(m2.download()))
     m.close()
+    m2.close()

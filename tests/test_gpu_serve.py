@@ -103,6 +103,7 @@ def test_served_alone_equals_in_mix_and_oracle(ctx):
 
 
 def test_server_rejects_over_budget(ctx):
+    """A registration beyond the HBM budget is refused: alone, and next to registered meshes."""
     say("case: server budget")
     pr = meshgen.sphere_problem(10, 16)
     (m,) = make_meshes(ctx, [pr])
@@ -114,6 +115,12 @@ def test_server_rejects_over_budget(ctx):
     srv.close()
     srv = mdarap.Server(ctx, hbm_budget_bytes=need)
     assert srv.add_mesh(m) == 0
+    (m2,) = make_meshes(ctx, [pr])
+    with pytest.raises(mdarap.MDError) as e:
+        srv.add_mesh(m2)
+    assert e.value.code == mdarap.MD_ERR_BUDGET
+    st = srv.stats()
+    assert st["meshes"] == 1 and st["resident_bytes"] == need and st["evictions"] == 0
     # a request may only move constrained vertices
     free_vertex = int(np.setdiff1d(np.arange(pr.n), pr.constraints()[0])[0])
     with pytest.raises(mdarap.MDError) as e:
@@ -161,6 +168,10 @@ def test_prepare_many_equals_prepare(ctx):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.skip(reason="eviction of idle served meshes to host memory is switched off: two GPU runs of "
+                         "this path ended in a GPU fault; the admission bug found (a mesh chosen for eviction "
+                         "admitted as resident in the same pass) is fixed and pinned on the CPU "
+                         "(test_server_admission.py), the second fault is not explained")
 def test_budgeted_admission_evicts_and_matches_unlimited(ctx):
     """HBM-budgeted admission: with a budget that holds the two largest of six meshes but not
     all of them, requests on evicted meshes wait until least recently used idle meshes are evicted to host
