@@ -162,6 +162,16 @@ int md_mesh_rotations(md_mesh *mesh, double *R);
  * the incident faces' area-weighted normals, for rendering clients that update normals after
  * every deform (examples/osg_viewer.cpp:48). */
 int md_mesh_normals(md_mesh *mesh, double *nrm);
+/* md_mesh_normals without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The face normals and the staging of the result live in one static
+ * device workspace per GPU of 64 MiB (held from the library's load until the process ends),
+ * enough for 24 F + 60 n bytes (256-byte aligned parts), e.g. 500k vertices with 1M faces;
+ * larger meshes return MD_ERR_INVALID (md_mesh_normals has no limit). The mesh must be prepared
+ * and resident (an evicted mesh returns MD_ERR_STATE: restoring it would allocate). Every call
+ * on one GPU shares that workspace: calls that overlap on the same GPU (from several threads or
+ * contexts) race and must be serialised by the caller. Same result as md_mesh_normals. */
+int md_mesh_normals_noalloc(md_mesh *mesh, double *nrm);
 /* ARAP energy sum_i sum_j w_ij |(p'_i - p'_j) - R_i (p_i - p_j)|^2 of the current state. */
 int md_mesh_energy(md_mesh *mesh, double *energy);
 int md_mesh_get_stats(md_mesh *mesh, md_mesh_stats *out);

@@ -154,6 +154,14 @@ void launch_fac_diag(hipStream_t s, const FacArgs &A);
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_dinv, int per_g,
                        int per_u);
 void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64);
+// Internal-order vertex normals to packed xyz in original order (perm: original id per internal id).
+void launch_pack_normals(hipStream_t s, const d4 *nrm, const int *perm, int n, double *dst);
+
+// Per-GPU device workspace of the non-allocating entry points (a static device array of
+// kScratchBytes): its address on the current device, from hipGetSymbolAddress. Call once per
+// device with that device current and use only the returned address in host code.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+hipError_t scratch_address(void **ptr);
 // Plain copy of n16 16-byte elements (counter calibration, copy bandwidth).
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16);
 

@@ -1283,6 +1283,22 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals(LaunchSet L, const do
     out[iv] = o;
 }
 
+// Vertex normals from internal order to the caller's packed xyz in original vertex order:
+// dst[3 perm[iv] + c] = nrm[iv].c (perm = original id of each internal id, a bijection).
+__global__ __launch_bounds__(kBlock) void k_pack_normals(const d4 *nrm, const int *perm, int n, double *dst) {
+    int iv = blockIdx.x * kBlock + threadIdx.x;
+    if (iv >= n) return;
+    d4 v = nrm[iv];
+    double *o = dst + 3 * (size_t)perm[iv];
+    o[0] = v.x; o[1] = v.y; o[2] = v.z;
+}
+
+// Device workspace of the entry points that allocate nothing (md_mesh_normals_noalloc): one
+// array per GPU, set aside when the code object loads and held until the process ends. Host
+// code reaches it only through the address hipGetSymbolAddress returns on each device (the
+// array's name in host code is a host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 // ---- ARAP energy -------------------------------------------------------------------------
 
 // E = sum_i sum_j w_ij |(p'_i - p'_j) - R_i (p_i - p_j)|^2 over all vertices, in FP64; one
@@ -1723,6 +1739,15 @@ void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LA
 void launch_normals(hipStream_t s, LaunchSet Lf, int blocks_face, LaunchSet Ls, int blocks_slice, double *fn, d4 *out) {
     if (blocks_face > 0) hipLaunchKernelGGL(k_face_normals, dim3(blocks_face), dim3(kBlock), 0, s, Lf, fn);
     if (blocks_slice > 0) hipLaunchKernelGGL(k_vertex_normals, dim3(blocks_slice), dim3(kBlock), 0, s, Ls, (const double *)fn, out);
+}
+
+hipError_t scratch_address(void **ptr) {
+    *ptr = nullptr;
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+void launch_pack_normals(hipStream_t s, const d4 *nrm, const int *perm, int n, double *dst) {
+    if (n > 0) hipLaunchKernelGGL(k_pack_normals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nrm, perm, n, dst);
 }
 
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {

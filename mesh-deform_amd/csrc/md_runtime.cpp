@@ -308,6 +308,12 @@ extern "C" int md_ctx_create(int device, md_ctx **out) {
     ctx->device = device;
     ctx->arch = arch;
     MD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // the static workspace of the non-allocating entry points, at its address on this device
+    if (hipError_t e = scratch_address(&ctx->scratch); e != hipSuccess || !ctx->scratch) {
+        hipStreamDestroy(ctx->stream);
+        set_error(std::string("md_ctx_create: address of the device workspace: ") + hipGetErrorString(e));
+        return MD_ERR_HIP;
+    }
     *out = ctx.release();
     return MD_OK;
 }
@@ -1525,6 +1531,58 @@ extern "C" int md_mesh_normals(md_mesh *m, double *nrm) {
         double *o = nrm + 3 * (size_t)m->topo.original_of[iv];
         o[0] = h[iv].x; o[1] = h[iv].y; o[2] = h[iv].z;
     }
+    return MD_OK;
+}
+
+namespace {
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+// Workspace of md_mesh_normals_noalloc: face normals, vertex normals (internal order), the
+// internal-to-original permutation and the packed result.
+size_t normals_scratch_bytes(size_t n, size_t F) {
+    return align256(sizeof(double) * 3 * F) + align256(sizeof(d4) * n) + align256(sizeof(int) * n) +
+           align256(sizeof(double) * 3 * n);
+}
+}  // namespace
+
+extern "C" int md_mesh_normals_noalloc(md_mesh *m, double *nrm) {
+    if (!m || !nrm) { set_error("md_mesh_normals_noalloc: null pointer"); return MD_ERR_INVALID; }
+    const size_t need = normals_scratch_bytes((size_t)m->n, (size_t)m->F);
+    if (need > kScratchBytes) {
+        set_error("md_mesh_normals_noalloc: a mesh of " + std::to_string(m->n) + " vertices and " +
+                  std::to_string(m->F) + " faces needs " + std::to_string(need) +
+                  " bytes of device workspace, more than its maximum of " + std::to_string(kScratchBytes) +
+                  " (md_mesh_normals has no limit)");
+        return MD_ERR_INVALID;
+    }
+    if (!m->analyzed) { set_error("md_mesh_normals_noalloc: call deform first"); return MD_ERR_STATE; }
+    if (m->servers > 0) {
+        set_error("md_mesh_normals_noalloc: the mesh is registered with a server");
+        return MD_ERR_STATE;
+    }
+    if (m->evicted) {
+        set_error("md_mesh_normals_noalloc: the mesh is evicted to host memory (restoring it allocates)");
+        return MD_ERR_STATE;
+    }
+    hipSetDevice(m->ctx->device);
+    hipStream_t st = m->ctx->stream;
+    char *base = static_cast<char *>(m->ctx->scratch);
+    const size_t n = (size_t)m->n, F = (size_t)m->F;
+    double *fn = reinterpret_cast<double *>(base);
+    d4 *vn = reinterpret_cast<d4 *>(base + align256(sizeof(double) * 3 * F));
+    int *perm = reinterpret_cast<int *>(reinterpret_cast<char *>(vn) + align256(sizeof(d4) * n));
+    double *dst = reinterpret_cast<double *>(reinterpret_cast<char *>(perm) + align256(sizeof(int) * n));
+    const Plan &P = m->plan;
+    launch_normals(st, P.set(P.off_face), P.blk_face, P.set(P.off_slice), P.blk_slice, fn, vn);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && n)
+        e = hipMemcpyAsync(perm, m->topo.original_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        launch_pack_normals(st, vn, perm, m->n, dst);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && n) e = hipMemcpyAsync(nrm, dst, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { set_error(std::string("md_mesh_normals_noalloc: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     return MD_OK;
 }
 

@@ -140,6 +140,7 @@ struct md_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string arch;
+    void *scratch = nullptr;  // this device's address of the static workspace (kScratchBytes)
     // the caller's handle plus one reference per mesh, batch and server created from it:
     // the stream is destroyed with the last holder, whatever the order of destruction
     std::atomic<int> refs{1};

@@ -100,6 +100,7 @@ _SIGS = [
     ("md_mesh_cotan_csr", ctypes.c_int, [_vp, _i32p, _i32p, _f64p]),
     ("md_mesh_rotations", ctypes.c_int, [_vp, _f64p]),
     ("md_mesh_normals", ctypes.c_int, [_vp, _f64p]),
+    ("md_mesh_normals_noalloc", ctypes.c_int, [_vp, _f64p]),
     ("md_mesh_energy", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     ("md_mesh_get_stats", ctypes.c_int, [_vp, ctypes.POINTER(MeshStats)]),
     ("md_mesh_time_iterations", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -292,6 +293,12 @@ class Mesh:
     def normals(self):
         out = np.zeros(3 * self.n, np.float64)
         check(lib().md_mesh_normals(self._h, out))
+        return out.reshape(-1, 3)
+
+    def normals_noalloc(self):
+        """md_mesh_normals_noalloc: the same normals from the per-GPU static workspace."""
+        out = np.zeros(3 * self.n, np.float64)
+        check(lib().md_mesh_normals_noalloc(self._h, out))
         return out.reshape(-1, 3)
 
     def energy(self):
