@@ -504,6 +504,8 @@ def main():
         ctx.close()
         return
     if args.batched_only:
+        # calibration for counter passes: the plain copy kernel moving 256 MiB each way, 4 times
+        mdarap.copy_probe(ctx, 256 << 20, 3)
         log("batched leg: %d meshes" % args.batched_meshes)
         print(json.dumps({"batched": batched_leg(ctx, args.batched_meshes, 20, 4242, not args.no_autotune,
                                                  not args.no_plain_baseline)}), flush=True)
