@@ -1356,6 +1356,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
 // children's update matrices are added in child order, so the factor is deterministic.
 
 constexpr int kFacBlock = 256;
+constexpr int kFacCols = 64;  // front columns per assembly workgroup
 
 // L_jj = sum of the weights of every neighbour of j, in slot order, and their maximum.
 __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
@@ -1373,17 +1374,26 @@ __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
 
 // Front of each supernode of one height: the pivot block and G rows (panel), F_os (copy),
 // the update matrix; original entries (L_jj and -w_ij for free i > j, slot order) first,
-// then the children's update matrices in child order.
-__global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
+// then the children's update matrices in child order. `per` workgroups per supernode, each
+// owning 64 columns of the front (pivot columns: P and F; later ones: U), so no two write
+// the same entry and every entry receives its additions in that fixed order.
+__global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
+    const int t0 = (int)(blockIdx.x % per) * kFacCols, t1 = min(m, t0 + kFacCols);
+    if (t0 >= m) return;
     double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
-    for (long long i = tid; i < (long long)m * w; i += kFacBlock) P[i] = 0.0;
-    for (long long i = tid; i < (long long)no * w; i += kFacBlock) F[i] = 0.0;
-    for (long long i = tid; i < (long long)no * no; i += kFacBlock) U[i] = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        if (t < w) {
+            for (int i = tid; i < m; i += kFacBlock) P[(long long)t * m + i] = 0.0;
+            for (int i = tid; i < no; i += kFacBlock) F[(long long)t * no + i] = 0.0;
+        } else {
+            for (int i = tid; i < no; i += kFacBlock) U[(long long)(t - w) * no + i] = 0.0;
+        }
+    }
     __syncthreads();
     const int *rows = A.offrows + S.roff;
-    for (int k = tid; k < w; k += kFacBlock) {
+    for (int k = t0 + tid; k < min(t1, w); k += kFacBlock) {
         const int j = S.c0 + k, sl = j / kSlice, l = j % kSlice;
         P[(long long)k * m + k] = A.diag[j];
         for (int q = A.slice_ptr[sl]; q < A.slice_ptr[sl + 1]; ++q) {
@@ -1411,17 +1421,25 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0) {
         const int nc = C.noff;
         const int *rel = A.rel + C.roff;
         const double *Uc = A.U + C.uoff;
-        for (long long idx = tid; idx < (long long)nc * nc; idx += kFacBlock) {
-            const int b = (int)(idx / nc), a = (int)(idx % nc);
-            if (a < b) continue;
-            const double v = Uc[(long long)b * nc + a];
-            const int ra = rel[a], rb = rel[b];
-            if (ra >= m || rb > ra) { *A.flag = 2; continue; }
-            if (rb < w) {
-                if (ra < w) P[(long long)rb * m + ra] += v;
-                else F[(long long)rb * no + (ra - w)] += v;
-            } else {
-                U[(long long)(rb - w) * no + (ra - w)] += v;
+        // the child's update columns b landing in [t0, t1): rel is increasing
+        int blo = 0, bhi = nc;
+        while (blo < bhi) {
+            const int mid = (blo + bhi) >> 1;
+            if (rel[mid] < t0) blo = mid + 1;
+            else bhi = mid;
+        }
+        for (int b = blo; b < nc && rel[b] < t1; ++b) {
+            const int rb = rel[b];
+            for (int a = b + tid; a < nc; a += kFacBlock) {
+                const double v = Uc[(long long)b * nc + a];
+                const int ra = rel[a];
+                if (ra >= m || rb > ra) { *A.flag = 2; continue; }
+                if (rb < w) {
+                    if (ra < w) P[(long long)rb * m + ra] += v;
+                    else F[(long long)rb * no + (ra - w)] += v;
+                } else {
+                    U[(long long)(rb - w) * no + (ra - w)] += v;
+                }
             }
         }
         __syncthreads();
@@ -1605,58 +1623,171 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int 
             if (ti + 16 * x < rb) T[(long long)(r0 + ti + 16 * x) * w + c0 + tj + 16 * y] = -out[x][y];
 }
 
-// D^{-1} = X^T X into the pivot block of the panel (symmetric), `per` workgroups per supernode.
+// ---- FP64 dense products of the factorization on the matrix cores ----------------------
+//
+// v_mfma_f64_16x16x4f64 (one wavefront): D(16x16) = A(16x4) B(4x16) + C with A(i, k) from
+// lane i + 16 k, B(k, j) from lane j + 16 k, and D(i, j) in lane j + 16 (i % 4), register
+// i / 4. A 64 x 64 output tile per workgroup of 256 threads: wave v computes rows
+// [32 (v / 2), +32) and columns [32 (v % 2), +32) as 2 x 2 such blocks, with the operands
+// staged in LDS kMk values of k at a time (As[k][i] = A(i, k), Bs[k][j] = B(k, j)).
+typedef double dv4 __attribute__((ext_vector_type(4)));
+constexpr int kMk = 32;
+constexpr int kMpad = 66;  // row stride of the staged operands (doubles)
+
+struct MmaAcc {
+    dv4 c[2][2];
+};
+
+__device__ __forceinline__ void mma_zero(MmaAcc &acc) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc.c[x][y] = dv4{0.0, 0.0, 0.0, 0.0};
+}
+
+// acc += As^T Bs over the staged kMk values of k.
+__device__ __forceinline__ void mma_chunk(const double (*As)[kMpad], const double (*Bs)[kMpad], MmaAcc &acc) {
+    const int wv = threadIdx.x / 64, lane = threadIdx.x & 63;
+    const int rb = 32 * (wv >> 1) + (lane & 15), cb = 32 * (wv & 1) + (lane & 15);
+#pragma unroll
+    for (int k4 = 0; k4 < kMk; k4 += 4) {
+        const int kk = k4 + (lane >> 4);
+        const double a0 = As[kk][rb], a1 = As[kk][rb + 16];
+        const double b0 = Bs[kk][cb], b1 = Bs[kk][cb + 16];
+        acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.c[0][0], 0, 0, 0);
+        acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.c[0][1], 0, 0, 0);
+        acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.c[1][0], 0, 0, 0);
+        acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc.c[1][1], 0, 0, 0);
+    }
+}
+
+// Tile coordinates (i, j) within the 64 x 64 output of this thread's element (x, y, r).
+__device__ __forceinline__ int mma_row(int x, int r) {
+    return 32 * ((threadIdx.x / 64) >> 1) + 16 * x + 4 * r + ((threadIdx.x & 63) >> 4);
+}
+__device__ __forceinline__ int mma_col(int y) { return 32 * ((threadIdx.x / 64) & 1) + 16 * y + (threadIdx.x & 15); }
+
+// Lower block (I, J), I >= J, of the q-th tile of a lower-triangular tile grid.
+__device__ __forceinline__ void lower_tile(int q, int &I, int &J) {
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= q) ++i;
+    I = i;
+    J = q - i * (i + 1) / 2;
+}
+
+// D^{-1} = X^T X into the pivot block of the panel (symmetric): lower 64 x 64 tile (I, J)
+// of D^{-1}(r, c) = sum_{k >= r} X(k, r) X(k, c), X = L^{-1} row-major in T (zero above its
+// diagonal inside the diagonal blocks), `per` workgroups per supernode.
 __global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int per) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m;
+    const int w = S.w, m = S.m, nb = (w + 63) / 64, q = (int)(blockIdx.x % per);
+    if (q >= nb * (nb + 1) / 2) return;
+    int I, J;
+    lower_tile(q, I, J);
+    const int r0 = 64 * I, c0 = 64 * J;
     double *P = A.P + S.poff;
     const double *X = A.T + S.toff;
-    const long long n = (long long)w * w;
-    for (long long idx = (long long)(blockIdx.x % per) * kFacBlock + threadIdx.x; idx < n;
-         idx += (long long)per * kFacBlock) {
-        const int r = (int)(idx / w), c = (int)(idx % w);
-        if (c > r) continue;
-        double acc = 0.0;
-        for (int k = r; k < w; ++k) acc += X[(long long)k * w + r] * X[(long long)k * w + c];
-        P[(long long)c * m + r] = acc;
-        P[(long long)r * m + c] = acc;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    for (int k0 = r0; k0 < w; k0 += kMk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+            const int kk = e / 64, x = e % 64, k = k0 + kk;
+            const bool kin = k < w;
+            As[kk][x] = kin && r0 + x < w ? X[(long long)k * w + r0 + x] : 0.0;
+            Bs[kk][x] = kin && c0 + x < w ? X[(long long)k * w + c0 + x] : 0.0;
+        }
+        __syncthreads();
+        mma_chunk(As, Bs, acc);
     }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
+                if (rr < w && cc < w && cc <= rr) {
+                    const double v = acc.c[x][y][r];
+                    P[(long long)cc * m + rr] = v;
+                    P[(long long)rr * m + cc] = v;
+                }
+            }
 }
 
-// G = F_os D^{-1} into panel rows [w, m), `per` workgroups per supernode.
+// G = F_os D^{-1} into panel rows [w, m): 64 x 64 tile (rows of G, columns), `per`
+// workgroups per supernode. A(i, k) = F_os(r0 + i, k) (column-major), B(k, j) = D^{-1}(k, q0 + j).
 __global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, no = S.noff, m = S.m;
+    const int w = S.w, no = S.noff, m = S.m, nr = (no + 63) / 64, nq = (w + 63) / 64;
+    const int q = (int)(blockIdx.x % per);
+    if (q >= nr * nq) return;
+    const int r0 = 64 * (q % nr), q0 = 64 * (q / nr);
     double *P = A.P + S.poff;
     const double *F = A.S + S.soff;
-    const long long n = (long long)no * w;
-    for (long long idx = (long long)(blockIdx.x % per) * kFacBlock + threadIdx.x; idx < n;
-         idx += (long long)per * kFacBlock) {
-        const int q = (int)(idx / no), r = (int)(idx % no);
-        double acc = 0.0;
-        for (int k = 0; k < w; ++k) acc += F[(long long)k * no + r] * P[(long long)q * m + k];
-        P[(long long)q * m + w + r] = acc;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    for (int k0 = 0; k0 < w; k0 += kMk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+            const int kk = e / 64, x = e % 64, k = k0 + kk;
+            As[kk][x] = k < w && r0 + x < no ? F[(long long)k * no + r0 + x] : 0.0;
+            // B(k, q0 + x): consecutive threads read consecutive k of one column
+            const int kb = e % kMk, xb = e / kMk, kg = k0 + kb;
+            Bs[kb][xb] = kg < w && q0 + xb < w ? P[(long long)(q0 + xb) * m + kg] : 0.0;
+        }
+        __syncthreads();
+        mma_chunk(As, Bs, acc);
     }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = r0 + mma_row(x, r), qq = q0 + mma_col(y);
+                if (rr < no && qq < w) P[(long long)qq * m + w + rr] = acc.c[x][y][r];
+            }
 }
 
-// U -= G F_os^T (lower part), `per` workgroups per supernode.
+// U -= G F_os^T (lower part): lower 64 x 64 tile (I, J) of U(a, b), a >= b, `per` workgroups
+// per supernode. A(i, k) = G(a0 + i, k) (panel rows [w, m)), B(k, j) = F_os(b0 + j, k).
 __global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, no = S.noff, m = S.m;
+    const int w = S.w, no = S.noff, m = S.m, nb = (no + 63) / 64, q = (int)(blockIdx.x % per);
+    if (q >= nb * (nb + 1) / 2) return;
+    int I, J;
+    lower_tile(q, I, J);
+    const int a0 = 64 * I, b0 = 64 * J;
     const double *P = A.P + S.poff, *F = A.S + S.soff;
     double *U = A.U + S.uoff;
-    const long long n = (long long)no * no;
-    for (long long idx = (long long)(blockIdx.x % per) * kFacBlock + threadIdx.x; idx < n;
-         idx += (long long)per * kFacBlock) {
-        const int b = (int)(idx / no), a = (int)(idx % no);
-        if (a < b) continue;
-        double acc = U[idx];
-        for (int k = 0; k < w; ++k) {
-            const double fb = F[(long long)k * no + b];
-            if (fb != 0.0) acc -= P[(long long)k * m + w + a] * fb;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    for (int k0 = 0; k0 < w; k0 += kMk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+            const int kk = e / 64, x = e % 64, k = k0 + kk;
+            As[kk][x] = k < w && a0 + x < no ? P[(long long)k * m + w + a0 + x] : 0.0;
+            Bs[kk][x] = k < w && b0 + x < no ? F[(long long)k * no + b0 + x] : 0.0;
         }
-        U[idx] = acc;
+        __syncthreads();
+        mma_chunk(As, Bs, acc);
     }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int aa = a0 + mma_row(x, r), bb = b0 + mma_col(y);
+                if (aa < no && bb < no && bb <= aa) U[(long long)bb * no + aa] -= acc.c[x][y][r];
+            }
 }
 
 // FP64 panel -> the sweeps' layouts in the solve precision: column-major G (noff x w) and
@@ -1686,10 +1817,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ sr
 void launch_fac_diag(hipStream_t s, const FacArgs &A) {
     if (A.nf > 0) hipLaunchKernelGGL(k_fac_diag, dim3((A.nf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A);
 }
-void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_dinv, int per_g,
-                       int per_u) {
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_asm, int per_dinv,
+                       int per_g, int per_u) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_fac_assemble, dim3(count), dim3(kFacBlock), 0, s, A, h0);
+    hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm);
     for (int b = 0; b < nb; ++b) {
         hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
         const int below = nb - 1 - b;

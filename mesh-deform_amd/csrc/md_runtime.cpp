@@ -845,19 +845,21 @@ int factor_device(md_mesh *m) {
         launch_fac_diag(st, A);
         const int nh = (int)f.hptr.size() - 1;
         for (int h = 0; h < nh; ++h) {
-            // workgroups per supernode of the multi-workgroup stages: enough for the height's
-            // largest block, at most 64
-            int64_t mw2 = 0, mnw = 0, mno2 = 0, mw = 0;
+            // workgroups per supernode of the multi-workgroup stages, for the height's largest
+            // front: assembly one per 64 front columns, the products one per 64 x 64 output tile
+            int per_asm = 1, per_dinv = 1, per_g = 0, per_u = 0, mw = 0;
+            auto lower = [](int b) { return b * (b + 1) / 2; };
             for (int i = f.hptr[h]; i < f.hptr[h + 1]; ++i) {
-                const int64_t w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
+                const int w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
+                const int bw = (w + 63) / 64, bo = (no + 63) / 64;
                 mw = std::max(mw, w);
-                mw2 = std::max(mw2, w * w);
-                mnw = std::max(mnw, no * w);
-                mno2 = std::max(mno2, no * no);
+                per_asm = std::max(per_asm, (w + no + 63) / 64);
+                per_dinv = std::max(per_dinv, lower(bw));
+                per_g = std::max(per_g, bo * bw);
+                per_u = std::max(per_u, lower(bo));
             }
-            auto per = [](int64_t n) { return (int)std::min<int64_t>(64, std::max<int64_t>(1, (n + 255) / 256)); };
-            launch_fac_height(st, A, f.hptr[h], f.hptr[h + 1] - f.hptr[h], (int)((mw + 63) / 64), per(mw2),
-                              mnw > 0 ? per(mnw) : 0, per(mno2));
+            launch_fac_height(st, A, f.hptr[h], f.hptr[h + 1] - f.hptr[h], (mw + 63) / 64, per_asm, per_dinv, per_g,
+                              per_u);
         }
         launch_fac_convert(st, A, ns, const_cast<void *>(m->dev.pc), const_cast<void *>(m->dev.pr), m->fp64());
         e = hipGetLastError();
