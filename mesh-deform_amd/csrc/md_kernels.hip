@@ -1446,181 +1446,95 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     }
 }
 
-// Pivot block of each front of one height (one workgroup per supernode): Cholesky in place
-// and its inverse X = L^{-1} (row-major), both right-looking. At step k the Cholesky scales
-// column k and updates the trailing triangle; the inverse scales row k and updates the later
-// rows. Every entry receives the same subtractions, in the same order, as in the left-looking
-// and column-by-column forms.
-// The pivot block's Cholesky factor and its inverse, blocked by kFb columns (right-looking):
-// for each column block s, k_fac_potrf factors the diagonal block in LDS and inverts it (the
-// diagonal block of X = L^{-1}, row-major in the T workspace), k_fac_trsm solves the rows
-// below it, k_fac_syrk updates the trailing lower blocks; then k_fac_trtri forms the
-// off-diagonal blocks of X one block diagonal at a time:
-// X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ.
+// The pivot block's Cholesky factor and its inverse X = L^{-1}, blocked by kFb columns
+// (right-looking): for each column block s, k_fac_potrf factors the diagonal block and inverts
+// it (the diagonal block of X, row-major in the T workspace), k_fac_trsm forms the rows below
+// it (times that inverse), k_fac_syrk updates the trailing lower blocks; then k_fac_trtri forms
+// the off-diagonal blocks of X one block diagonal at a time: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ.
 constexpr int kFb = 64;
-constexpr int kFbPad = kFb + 1;
 
-// Diagonal block s: unblocked right-looking Cholesky in LDS, then its inverse (right-looking,
-// row-major, the upper part zero).
-__global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int s) {
+// Diagonal block s: unblocked right-looking Cholesky, then its inverse (right-looking, row-major,
+// the upper part zero), by one wavefront with row i of the block in the registers of lane i
+// (a[c] = A(kb + i, kb + c), then L; x[c] = X(kb + i, kb + c)); the values other rows need
+// come from v_readlane (lane index fixed at compile time) as scalar operands, so no LDS and no
+// barriers. The right-looking forms: column k times 1 / sqrt(pivot), trailing entries
+// A(i, c) -= L(i, k) L(c, k); X row k times 1 / L(k, k), later rows X(i, j) -= L(i, k) X(k, j).
+// (v_readlane_b32 moves 32 bits: a double takes two, low and high halves.)
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int K>
+__device__ __forceinline__ double lane_value(double v) {
+    return readlane_f64(v, K);
+}
+
+template <int K>
+__device__ __forceinline__ void chol_step(double (&a)[kFb], int lane, int b, double amax, bool &bad) {
+    if constexpr (K < kFb) {
+        if (K < b) {
+            const double piv = lane_value<K>(a[K]);
+            if (!(piv > 1e-14 * amax)) bad = true;
+            const double d = sqrt(piv), rd = 1.0 / d;
+            a[K] = lane == K ? d : (lane > K ? a[K] * rd : a[K]);
+            // trailing lower triangle: A(i, c) -= L(i, k) L(c, k), k < c <= i
+#pragma unroll
+            for (int c = K + 1; c < kFb; ++c) {
+                const double lck = readlane_f64(a[K], c);
+                if (lane >= c) a[c] -= a[K] * lck;
+            }
+            chol_step<K + 1>(a, lane, b, amax, bad);
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void inv_step(const double (&a)[kFb], double (&x)[kFb], int lane, int b) {
+    if constexpr (K < kFb) {
+        if (K < b) {
+            const double rl = 1.0 / lane_value<K>(a[K]);
+            if (lane == K) {
+#pragma unroll
+                for (int j = 0; j <= K; ++j) x[j] = j == K ? rl : x[j] * rl;
+            }
+            // later rows: X(i, j) -= L(i, k) X(k, j), j <= k < i
+#pragma unroll
+            for (int j = 0; j <= K; ++j) {
+                const double xkj = readlane_f64(x[j], K);
+                if (lane > K) x[j] -= a[K] * xkj;
+            }
+            inv_step<K + 1>(a, x, lane, b);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kFb) void k_fac_potrf(FacArgs A, int h0, int s) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
-    const int w = S.w, m = S.m, kb = s * kFb, tid = threadIdx.x;
+    const int w = S.w, m = S.m, kb = s * kFb, lane = threadIdx.x;
     if (kb >= w) return;
-    const int b = min(kFb, w - kb), lane = tid & (kFb - 1), wv = tid / kFb;
-    __shared__ double D[kFb][kFbPad];  // D[c][r] = L(kb + r, kb + c)
-    __shared__ double X[kFb][kFbPad];  // X[r][c] = Linv(kb + r, kb + c)
+    const int b = min(kFb, w - kb);
     double *P = A.P + S.poff, *T = A.T + S.toff;
-    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
-        const int c = idx / kFb, r = idx % kFb;
-        D[c][r] = (r < b && c < b && r >= c) ? P[(long long)(kb + c) * m + kb + r] : 0.0;
-        X[c][r] = c == r ? 1.0 : 0.0;
+    double a[kFb], x[kFb];
+#pragma unroll
+    for (int c = 0; c < kFb; ++c) {
+        // column-major panel: A(kb + lane, kb + c) at (kb + c) m + kb + lane (lower part only)
+        a[c] = (lane < b && c < b && c <= lane) ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
+        x[c] = c == lane ? 1.0 : 0.0;
     }
     const double amax = __longlong_as_double((long long)*A.amax);
-    __syncthreads();
-    for (int k = 0; k < b; ++k) {
-        const double d0 = D[k][k];
-        if (tid == 0 && !(d0 > 1e-14 * amax)) *A.flag = 1;
-        const double d = sqrt(d0);
-        __syncthreads();  // every thread has read the pivot
-        if (tid < kFb && lane > k && lane < b) D[k][lane] /= d;
-        if (tid == 0) D[k][k] = d;
-        __syncthreads();
-        for (int c = k + 1 + wv; c < b; c += kFacBlock / kFb)
-            if (lane >= c && lane < b) D[c][lane] -= D[k][lane] * D[k][c];
-        __syncthreads();
-    }
-    for (int k = 0; k < b; ++k) {
-        const double lkk = D[k][k];
-        if (tid < kFb && lane <= k) X[k][lane] = lane == k ? 1.0 / lkk : X[k][lane] / lkk;
-        __syncthreads();
-        for (int r = k + 1 + wv; r < b; r += kFacBlock / kFb)
-            if (lane <= k) X[r][lane] -= D[k][r] * X[k][lane];
-        __syncthreads();
-    }
-    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
-        const int c = idx / kFb, r = idx % kFb;
-        if (r < b && c < b) {
-            if (r >= c) P[(long long)(kb + c) * m + kb + r] = D[c][r];
-            T[(long long)(kb + c) * w + kb + r] = X[c][r];  // row kb + c of X
+    bool bad = false;
+    chol_step<0>(a, lane, b, amax, bad);
+    inv_step<0>(a, x, lane, b);
+    if (bad && lane == 0) *A.flag = 1;
+    if (lane < b) {
+#pragma unroll
+        for (int c = 0; c < kFb; ++c) {
+            if (c < b && c <= lane) P[(long long)(kb + c) * m + kb + lane] = a[c];
+            if (c < b) T[(long long)(kb + lane) * w + kb + c] = x[c];  // row kb + lane of X
         }
     }
-}
-
-// Rows of row block s + 1 + (blockIdx.x % per) below diagonal block s:
-// L(r, kb + j) = (A(r, kb + j) - sum_{t<j} L(r, kb + t) L(kb + j, kb + t)) / L(kb + j, kb + j),
-// one row per lane.
-__global__ __launch_bounds__(kFb) void k_fac_trsm(FacArgs A, int h0, int s, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m, kb = s * kFb, r0 = (s + 1 + (int)(blockIdx.x % per)) * kFb;
-    if (r0 >= w) return;  // also kb + kFb <= r0 < w: the diagonal block is full
-    const int lane = threadIdx.x, rb = min(kFb, w - r0);
-    __shared__ double D[kFb][kFbPad];  // D[t][j] = L(kb + j, kb + t)
-    __shared__ double R[kFb][kFbPad];  // R[t][i] = A(r0 + i, kb + t)
-    double *P = A.P + S.poff;
-    for (int c = 0; c < kFb; ++c) {
-        D[c][lane] = lane >= c ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
-        R[c][lane] = lane < rb ? P[(long long)(kb + c) * m + r0 + lane] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < kFb; ++j) {
-        double v = R[j][lane];
-        for (int t = 0; t < j; ++t) v -= R[t][lane] * D[t][j];
-        R[j][lane] = v / D[j][j];
-    }
-    __syncthreads();
-    if (lane < rb)
-        for (int c = 0; c < kFb; ++c) P[(long long)(kb + c) * m + r0 + lane] = R[c][lane];
-}
-
-// 64 x 64 product in LDS by 256 threads: acc[x][y] += sum_k As[k][ti + 16x] Bs[k][tj + 16y].
-__device__ __forceinline__ void tile_mma(const double (*As)[kFbPad], const double (*Bs)[kFbPad], int ti, int tj,
-                                         double acc[4][4]) {
-    for (int k = 0; k < kFb; ++k) {
-        double a[4], bb[4];
-#pragma unroll
-        for (int x = 0; x < 4; ++x) a[x] = As[k][ti + 16 * x];
-#pragma unroll
-        for (int y = 0; y < 4; ++y) bb[y] = Bs[k][tj + 16 * y];
-#pragma unroll
-        for (int x = 0; x < 4; ++x)
-#pragma unroll
-            for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * bb[y];
-    }
-}
-
-// Lower tile q = (I, J), s < J <= I, of the trailing block:
-// A(r, c) -= sum_{t<64} L(r, kb + t) L(c, kb + t).
-__global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
-    const int nt = (w + kFb - 1) / kFb - 1 - s;  // row blocks below diagonal block s
-    if (q >= nt * (nt + 1) / 2) return;
-    int i = 0;
-    while ((i + 1) * (i + 2) / 2 <= q) ++i;
-    const int r0 = (s + 1 + i) * kFb, c0 = (s + 1 + q - i * (i + 1) / 2) * kFb;
-    const int tid = threadIdx.x, ti = tid % 16, tj = tid / 16;
-    __shared__ double As[kFb][kFbPad];  // As[t][x] = L(r0 + x, kb + t)
-    __shared__ double Bs[kFb][kFbPad];  // Bs[t][y] = L(c0 + y, kb + t)
-    double *P = A.P + S.poff;
-    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
-        const int t = idx / kFb, x = idx % kFb;
-        const long long col = (long long)(kb + t) * m;
-        As[t][x] = r0 + x < w ? P[col + r0 + x] : 0.0;
-        Bs[t][x] = c0 + x < w ? P[col + c0 + x] : 0.0;
-    }
-    __syncthreads();
-    double acc[4][4] = {};
-    tile_mma(As, Bs, ti, tj, acc);
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            const int r = r0 + ti + 16 * x, c = c0 + tj + 16 * y;
-            if (r < w && c < w && r >= c) P[(long long)c * m + r] -= acc[x][y];
-        }
-}
-
-// Off-diagonal blocks of X on block diagonal d: I = d + (blockIdx.x % per), J = I - d.
-__global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int d, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m, I = d + (int)(blockIdx.x % per), J = I - d;
-    const int r0 = I * kFb, c0 = J * kFb;
-    if (r0 >= w) return;  // every block K in [J, I) is full
-    const int tid = threadIdx.x, ti = tid % 16, tj = tid / 16, rb = min(kFb, w - r0);
-    __shared__ double As[kFb][kFbPad];
-    __shared__ double Bs[kFb][kFbPad];
-    const double *P = A.P + S.poff;
-    double *T = A.T + S.toff;
-    double acc[4][4] = {};
-    // Y = sum_K L_IK X_KJ: As[k][x] = L(r0 + x, 64K + k), Bs[k][y] = X(64K + k, c0 + y)
-    for (int K = J; K < I; ++K) {
-        __syncthreads();
-        for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
-            const int k = idx / kFb, x = idx % kFb;
-            As[k][x] = x < rb ? P[(long long)(K * kFb + k) * m + r0 + x] : 0.0;
-            Bs[k][x] = T[(long long)(K * kFb + k) * w + c0 + x];
-        }
-        __syncthreads();
-        tile_mma(As, Bs, ti, tj, acc);
-    }
-    __syncthreads();
-    // X_IJ = -X_II Y: As[k][x] = X(r0 + x, r0 + k), Bs[k][y] = Y(k, y)
-    for (int idx = tid; idx < kFb * kFb; idx += kFacBlock) {
-        const int x = idx / kFb, k = idx % kFb;
-        As[k][x] = (x < rb && k < rb) ? T[(long long)(r0 + x) * w + r0 + k] : 0.0;
-    }
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) Bs[ti + 16 * x][tj + 16 * y] = acc[x][y];
-    __syncthreads();
-    double out[4][4] = {};
-    tile_mma(As, Bs, ti, tj, out);
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-            if (ti + 16 * x < rb) T[(long long)(r0 + ti + 16 * x) * w + c0 + tj + 16 * y] = -out[x][y];
 }
 
 // ---- FP64 dense products of the factorization on the matrix cores ----------------------
@@ -1790,18 +1704,168 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per)
             }
 }
 
+// Rows of row block s + 1 + (blockIdx.x % per) below diagonal block s, as a product with the
+// diagonal block's inverse that k_fac_potrf left in T: L(r, kb + j) = sum_t A(r, kb + t)
+// X(kb + j, kb + t) (L_below = A_below L_kk^{-T}). A(i, t) from the panel (column-major),
+// B(t, j) = X(kb + j, kb + t) (row-major X); the tile is read whole before it is written.
+__global__ __launch_bounds__(kFacBlock) void k_fac_trsm(FacArgs A, int h0, int s, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, kb = s * kFb, r0 = (s + 1 + (int)(blockIdx.x % per)) * kFb;
+    if (r0 >= w) return;  // also kb + kFb <= r0 < w: the diagonal block is full
+    double *P = A.P + S.poff;
+    const double *X = A.T + S.toff;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    for (int k0 = 0; k0 < kFb; k0 += kMk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+            const int kk = e / 64, x = e % 64;
+            As[kk][x] = r0 + x < w ? P[(long long)(kb + k0 + kk) * m + r0 + x] : 0.0;
+            // B(t, j) = X(kb + j, kb + t): consecutive threads read consecutive t of one row
+            const int tb = e % kMk, jb = e / kMk;
+            Bs[tb][jb] = X[(long long)(kb + jb) * w + kb + k0 + tb];
+        }
+        __syncthreads();
+        mma_chunk(As, Bs, acc);
+    }
+    __syncthreads();  // every wave has staged its last reads of the tile before any write
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = r0 + mma_row(x, r), jj = kb + mma_col(y);
+                if (rr < w) P[(long long)jj * m + rr] = acc.c[x][y][r];
+            }
+}
+
+// Lower tile q = (I, J), s < J <= I, of the trailing block:
+// A(r, c) -= sum_{t<64} L(r, kb + t) L(c, kb + t).
+__global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
+    const int nt = (w + kFb - 1) / kFb - 1 - s;  // row blocks below diagonal block s
+    if (q >= nt * (nt + 1) / 2) return;
+    int I, J;
+    lower_tile(q, I, J);
+    const int r0 = (s + 1 + I) * kFb, c0 = (s + 1 + J) * kFb;
+    double *P = A.P + S.poff;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    for (int k0 = 0; k0 < kFb; k0 += kMk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+            const int kk = e / 64, x = e % 64;
+            const long long col = (long long)(kb + k0 + kk) * m;
+            As[kk][x] = r0 + x < w ? P[col + r0 + x] : 0.0;
+            Bs[kk][x] = c0 + x < w ? P[col + c0 + x] : 0.0;
+        }
+        __syncthreads();
+        mma_chunk(As, Bs, acc);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
+                if (rr < w && cc < w && rr >= cc) P[(long long)cc * m + rr] -= acc.c[x][y][r];
+            }
+}
+
+// Off-diagonal blocks of X on block diagonal d: I = d + (blockIdx.x % per), J = I - d:
+// Y = sum_{K=J}^{I-1} L_IK X_KJ, then X_IJ = -X_II Y (Y staged through LDS 32 rows at a time).
+__global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int d, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, I = d + (int)(blockIdx.x % per), J = I - d;
+    const int r0 = I * kFb, c0 = J * kFb;
+    if (r0 >= w) return;  // every block K in [J, I) is full
+    const double *P = A.P + S.poff;
+    double *T = A.T + S.toff;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    // Y: A(i, k) = L(r0 + i, 64 K + k) (panel, column-major), B(k, j) = X(64 K + k, c0 + j)
+    for (int K = J; K < I; ++K)
+        for (int k0 = 0; k0 < kFb; k0 += kMk) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+                const int kk = e / 64, x = e % 64, k = K * kFb + k0 + kk;
+                As[kk][x] = r0 + x < w ? P[(long long)k * m + r0 + x] : 0.0;
+                Bs[kk][x] = T[(long long)k * w + c0 + x];
+            }
+            __syncthreads();
+            mma_chunk(As, Bs, acc);
+        }
+    // X_IJ = -X_II Y: A(i, k) = X(r0 + i, r0 + k) (row-major), B(k, j) = Y(k, j)
+    MmaAcc out;
+    mma_zero(out);
+    for (int k0 = 0; k0 < kFb; k0 += kMk) {
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = mma_row(x, r) - k0;
+                    if (k >= 0 && k < kMk) Bs[k][mma_col(y)] = acc.c[x][y][r];
+                }
+        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
+            // consecutive threads read consecutive k of one row of X
+            const int kk = e % kMk, x = e / kMk;
+            As[kk][x] = (r0 + x < w && r0 + k0 + kk < w) ? T[(long long)(r0 + x) * w + r0 + k0 + kk] : 0.0;
+        }
+        __syncthreads();
+        mma_chunk(As, Bs, out);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
+                if (rr < w) T[(long long)rr * w + cc] = -out.c[x][y][r];
+            }
+}
+
 // FP64 panel -> the sweeps' layouts in the solve precision: column-major G (noff x w) and
-// row-major [D^{-1}; G] (m x w).
+// row-major [D^{-1}; G] (m x w). Workgroup (s, y) converts the 64 x 64 tiles y, y + gridDim.y,
+// ... of supernode s: read down the panel's columns, the row-major copy written across rows
+// from an LDS transpose.
+constexpr int kConvertSplit = 16;
 template <typename T>
 __global__ __launch_bounds__(kFacBlock) void k_fac_convert(FacArgs A, T *pc, T *pr) {
     const FacDev S = A.sn[blockIdx.x];
     const int w = S.w, m = S.m, no = S.noff;
+    const int np = (m + 63) / 64, nq = (w + 63) / 64;
     const double *P = A.P + S.poff;
-    for (long long idx = threadIdx.x; idx < (long long)m * w; idx += kFacBlock) {
-        const int q = (int)(idx / m), p = (int)(idx % m);
-        const T v = (T)P[idx];
-        pr[S.pr_off + (long long)p * w + q] = v;
-        if (p >= w) pc[S.pc_off + (long long)q * no + (p - w)] = v;
+    __shared__ T tile[64][65];
+    for (int t = blockIdx.y; t < np * nq; t += gridDim.y) {
+        const int p0 = (t % np) * 64, q0 = (t / np) * 64;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += kFacBlock) {
+            const int qq = e / 64, pp = e % 64, p = p0 + pp, q = q0 + qq;
+            T v = 0;
+            if (p < m && q < w) {
+                v = (T)P[(long long)q * m + p];
+                if (p >= w) pc[S.pc_off + (long long)q * no + (p - w)] = v;
+            }
+            tile[qq][pp] = v;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += kFacBlock) {
+            const int pp = e / 64, qq = e % 64, p = p0 + pp, q = q0 + qq;
+            if (p < m && q < w) pr[S.pr_off + (long long)p * w + q] = tile[qq][pp];
+        }
     }
 }
 
@@ -1822,10 +1886,10 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm);
     for (int b = 0; b < nb; ++b) {
-        hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
+        hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFb), 0, s, A, h0, b);
         const int below = nb - 1 - b;
         if (below > 0) {
-            hipLaunchKernelGGL(k_fac_trsm, dim3(count * below), dim3(kFb), 0, s, A, h0, b, below);
+            hipLaunchKernelGGL(k_fac_trsm, dim3(count * below), dim3(kFacBlock), 0, s, A, h0, b, below);
             const int tiles = below * (below + 1) / 2;
             hipLaunchKernelGGL(k_fac_syrk, dim3(count * tiles), dim3(kFacBlock), 0, s, A, h0, b, tiles);
         }
@@ -1840,8 +1904,9 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
 }
 void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64) {
     if (ns <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_fac_convert<double>, dim3(ns), dim3(kFacBlock), 0, s, A, (double *)pc, (double *)pr);
-    else hipLaunchKernelGGL(k_fac_convert<float>, dim3(ns), dim3(kFacBlock), 0, s, A, (float *)pc, (float *)pr);
+    const dim3 grid(ns, kConvertSplit);
+    if (fp64) hipLaunchKernelGGL(k_fac_convert<double>, grid, dim3(kFacBlock), 0, s, A, (double *)pc, (double *)pr);
+    else hipLaunchKernelGGL(k_fac_convert<float>, grid, dim3(kFacBlock), 0, s, A, (float *)pc, (float *)pr);
 }
 
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16) {
