@@ -92,7 +92,8 @@ typedef struct {
     int64_t factor_nnz;      /* strictly-lower entries of the supernodal factor as stored
                                 (explicit zeros of relaxed amalgamation included) */
     int64_t fwd_items, bwd_items;
-    int64_t device_bytes;    /* device memory held by the mesh */
+    int64_t device_bytes;    /* device memory held by the mesh (with the FP64 factorization
+                                workspace that a reference-rest-mode mesh keeps between calls) */
     double setup_ms_host;    /* last analysis + factorization time on the host */
     /* algorithmic bytes of one ARAP iteration with one refinement step, per phase: every
      * array element the phase must read or write counted once */

@@ -156,8 +156,12 @@ void launch_fac_diag(hipStream_t s, const FacArgs &A);
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_asm, int per_dinv,
                        int per_g, int per_u);
 void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64);
-// Internal-order vertex normals to packed xyz in original order (perm: original id per internal id).
-void launch_pack_normals(hipStream_t s, const d4 *nrm, const int *perm, int n, double *dst);
+// Internal-order vectors (positions, normals) to packed xyz in original order (perm: original
+// id per internal id), and packed xyz to internal order (p and/or pp; pp takes the targets tgt,
+// internal order, for the constrained ids [nf, n)).
+void launch_pack_xyz(hipStream_t s, const d4 *v, const int *perm, int n, double *dst);
+void launch_unpack_xyz(hipStream_t s, const double *xyz, const int *perm, int n, int nf, const double *tgt, d4 *p,
+                       d4 *pp);
 
 // Per-GPU device workspace of the non-allocating entry points (a static device array of
 // kScratchBytes): its address on the current device, from hipGetSymbolAddress. Call once per

@@ -1283,9 +1283,9 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals(LaunchSet L, const do
     out[iv] = o;
 }
 
-// Vertex normals from internal order to the caller's packed xyz in original vertex order:
-// dst[3 perm[iv] + c] = nrm[iv].c (perm = original id of each internal id, a bijection).
-__global__ __launch_bounds__(kBlock) void k_pack_normals(const d4 *nrm, const int *perm, int n, double *dst) {
+// Internal-order vectors (positions or normals) to the caller's packed xyz in original vertex
+// order: dst[3 perm[iv] + c] = v[iv].c (perm = original id of each internal id, a bijection).
+__global__ __launch_bounds__(kBlock) void k_pack_xyz(const d4 *nrm, const int *perm, int n, double *dst) {
     int iv = blockIdx.x * kBlock + threadIdx.x;
     if (iv >= n) return;
     d4 v = nrm[iv];
@@ -1342,6 +1342,25 @@ __global__ __launch_bounds__(kBlock) void k_energy(LaunchSet L) {
 }
 
 // p'[iv[k]] = xyz[k] for k targets (internal ids), used to move handles without a re-setup.
+// The caller's packed xyz (original order) into internal order: p[iv] = xyz[perm[iv]] (p may
+// be null), and pp[iv] the same for free vertices, the constrained ones (iv >= nf) taking their
+// targets tgt[iv - nf] (internal order) instead (pp may be null).
+__global__ __launch_bounds__(kBlock) void k_unpack_xyz(const double *xyz, const int *perm, int n, int nf,
+                                                       const double *tgt, d4 *p, d4 *pp) {
+    const int iv = blockIdx.x * kBlock + threadIdx.x;
+    if (iv >= n) return;
+    const double *s = xyz + 3 * (size_t)perm[iv];
+    const d4 v{s[0], s[1], s[2], 0.0};
+    if (p) p[iv] = v;
+    if (pp) {
+        if (iv < nf) pp[iv] = v;
+        else {
+            const double *t = tgt + 3 * (size_t)(iv - nf);
+            pp[iv] = d4{t[0], t[1], t[2], 0.0};
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *iv, const double *xyz, int k) {
     int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= k) return;
@@ -1453,59 +1472,54 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
 // the off-diagonal blocks of X one block diagonal at a time: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ.
 constexpr int kFb = 64;
 
-// Diagonal block s: unblocked right-looking Cholesky, then its inverse (right-looking, row-major,
-// the upper part zero), by one wavefront with row i of the block in the registers of lane i
-// (a[c] = A(kb + i, kb + c), then L; x[c] = X(kb + i, kb + c)); the values other rows need
-// come from v_readlane (lane index fixed at compile time) as scalar operands, so no LDS and no
-// barriers. The right-looking forms: column k times 1 / sqrt(pivot), trailing entries
-// A(i, c) -= L(i, k) L(c, k); X row k times 1 / L(k, k), later rows X(i, j) -= L(i, k) X(k, j).
-// (v_readlane_b32 moves 32 bits: a double takes two, low and high halves.)
+// Diagonal block s, by one wavefront: unblocked right-looking Cholesky with row i of the block in
+// the registers of lane i (a[c] = A(kb + i, kb + c), then L(i, c)); at step k the pivot comes
+// from lane k (v_readlane, two 32-bit halves per double), every lane scales its L(i, k) and puts
+// it in LDS, and the rank-1 update A(i, c) -= L(i, k) L(c, k) reads column k back as broadcast
+// LDS reads (all lanes the same address). Entries above the diagonal are updated too and never
+// used. Then the inverse X = L^{-1} by columns, lane i forming column i by forward substitution,
+// X(k, i) = (delta_ki - sum_{t<k} L(k, t) X(t, i)) / L(k, k), from L's rows staged in LDS
+// (broadcast reads again); X is zero above the diagonal, exactly.
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
-template <int K>
-__device__ __forceinline__ double lane_value(double v) {
-    return readlane_f64(v, K);
-}
+
+// L's rows in LDS, packed: row k holds k + 1 values padded to an even count (16-byte aligned rows).
+__host__ __device__ constexpr int tri_row_off(int k) { return k * (k + 1) / 2 + (k + 1) / 2; }
+constexpr int kTriLds = tri_row_off(kFb);
 
 template <int K>
-__device__ __forceinline__ void chol_step(double (&a)[kFb], int lane, int b, double amax, bool &bad) {
+__device__ __forceinline__ void chol_step(double (&a)[kFb], double (*col)[kFb], double &rdiag, int lane, int b,
+                                          double amax, bool &bad) {
     if constexpr (K < kFb) {
         if (K < b) {
-            const double piv = lane_value<K>(a[K]);
+            const double piv = readlane_f64(a[K], K);
             if (!(piv > 1e-14 * amax)) bad = true;
             const double d = sqrt(piv), rd = 1.0 / d;
-            a[K] = lane == K ? d : (lane > K ? a[K] * rd : a[K]);
-            // trailing lower triangle: A(i, c) -= L(i, k) L(c, k), k < c <= i
+            if (lane == K) rdiag = rd;
+            a[K] = lane == K ? d : a[K] * rd;
+            col[K & 1][lane] = a[K];
+            __syncthreads();  // one wavefront: orders the LDS write before the reads below
 #pragma unroll
-            for (int c = K + 1; c < kFb; ++c) {
-                const double lck = readlane_f64(a[K], c);
-                if (lane >= c) a[c] -= a[K] * lck;
-            }
-            chol_step<K + 1>(a, lane, b, amax, bad);
+            for (int c = K + 1; c < kFb; ++c) a[c] -= a[K] * col[K & 1][c];
+            chol_step<K + 1>(a, col, rdiag, lane, b, amax, bad);
         }
     }
 }
 
 template <int K>
-__device__ __forceinline__ void inv_step(const double (&a)[kFb], double (&x)[kFb], int lane, int b) {
+__device__ __forceinline__ void inv_col(const double *Lsh, double (&x)[kFb], int lane, int b) {
     if constexpr (K < kFb) {
         if (K < b) {
-            const double rl = 1.0 / lane_value<K>(a[K]);
-            if (lane == K) {
+            constexpr int o = tri_row_off(K);
+            double acc = lane == K ? 1.0 : 0.0;
 #pragma unroll
-                for (int j = 0; j <= K; ++j) x[j] = j == K ? rl : x[j] * rl;
-            }
-            // later rows: X(i, j) -= L(i, k) X(k, j), j <= k < i
-#pragma unroll
-            for (int j = 0; j <= K; ++j) {
-                const double xkj = readlane_f64(x[j], K);
-                if (lane > K) x[j] -= a[K] * xkj;
-            }
-            inv_step<K + 1>(a, x, lane, b);
+            for (int t = 0; t < K; ++t) acc -= Lsh[o + t] * x[t];
+            x[K] = acc * Lsh[o + K];  // the diagonal slot holds 1 / L(k, k)
+            inv_col<K + 1>(Lsh, x, lane, b);
         }
     }
 }
@@ -1516,24 +1530,38 @@ __global__ __launch_bounds__(kFb) void k_fac_potrf(FacArgs A, int h0, int s) {
     if (kb >= w) return;
     const int b = min(kFb, w - kb);
     double *P = A.P + S.poff, *T = A.T + S.toff;
-    double a[kFb], x[kFb];
+    __shared__ double col[2][kFb];
+    __shared__ __attribute__((aligned(16))) double Lsh[kTriLds];
+    double a[kFb];
 #pragma unroll
-    for (int c = 0; c < kFb; ++c) {
+    for (int c = 0; c < kFb; ++c)
         // column-major panel: A(kb + lane, kb + c) at (kb + c) m + kb + lane (lower part only)
         a[c] = (lane < b && c < b && c <= lane) ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
-        x[c] = c == lane ? 1.0 : 0.0;
-    }
     const double amax = __longlong_as_double((long long)*A.amax);
     bool bad = false;
-    chol_step<0>(a, lane, b, amax, bad);
-    inv_step<0>(a, x, lane, b);
+    double rdiag = 0.0;
+    chol_step<0>(a, col, rdiag, lane, b, amax, bad);
     if (bad && lane == 0) *A.flag = 1;
+    // L: lower part to the panel; row `lane` to LDS (pairs, the last one may fill the row's pad)
+    const int ro = tri_row_off(lane);
+#pragma unroll
+    for (int c = 0; c < kFb; c += 2) {
+        if (lane < b && c < b && c <= lane) {
+            P[(long long)(kb + c) * m + kb + lane] = a[c];
+            if (c + 1 < b && c + 1 <= lane) P[(long long)(kb + c + 1) * m + kb + lane] = a[c + 1];
+        }
+        if (c <= lane) *reinterpret_cast<double2 *>(&Lsh[ro + c]) = double2{a[c], a[c + 1]};
+    }
+    Lsh[ro + lane] = rdiag;
+    __syncthreads();
+    double x[kFb];
+#pragma unroll
+    for (int k = 0; k < kFb; ++k) x[k] = 0.0;
+    inv_col<0>(Lsh, x, lane, b);
     if (lane < b) {
 #pragma unroll
-        for (int c = 0; c < kFb; ++c) {
-            if (c < b && c <= lane) P[(long long)(kb + c) * m + kb + lane] = a[c];
-            if (c < b) T[(long long)(kb + lane) * w + kb + c] = x[c];  // row kb + lane of X
-        }
+        for (int k = 0; k < kFb; ++k)
+            if (k < b) T[(long long)(kb + k) * w + kb + lane] = x[k];  // X(kb + k, kb + lane), row-major
     }
 }
 
@@ -1942,8 +1970,14 @@ hipError_t scratch_address(void **ptr) {
     return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
 }
 
-void launch_pack_normals(hipStream_t s, const d4 *nrm, const int *perm, int n, double *dst) {
-    if (n > 0) hipLaunchKernelGGL(k_pack_normals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nrm, perm, n, dst);
+void launch_pack_xyz(hipStream_t s, const d4 *v, const int *perm, int n, double *dst) {
+    if (n > 0) hipLaunchKernelGGL(k_pack_xyz, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, v, perm, n, dst);
+}
+void launch_unpack_xyz(hipStream_t s, const double *xyz, const int *perm, int n, int nf, const double *tgt, d4 *p,
+                       d4 *pp) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_unpack_xyz, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, xyz, perm, n, nf, tgt, p,
+                           pp);
 }
 
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {

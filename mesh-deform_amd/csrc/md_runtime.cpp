@@ -520,6 +520,8 @@ int analyze_device(md_mesh *m) {
     size_t o_cpt = A.reserve(vsz * (pcg ? 4 * (size_t)nblk : 1));
     size_t o_ti = A.reserve(sizeof(int) * std::max(1, n - nf));
     size_t o_tx = A.reserve(sizeof(double) * 3 * std::max(1, n - nf));
+    size_t o_orig = A.reserve(sizeof(int) * std::max(1, n));
+    size_t o_xyz = A.reserve(sizeof(double) * 3 * std::max(1, n));
     if (A.allocate() != hipSuccess) {
         set_error("device allocation of " + std::to_string(A.size()) + " bytes failed");
         A.reset();
@@ -557,6 +559,8 @@ int analyze_device(md_mesh *m) {
     d.pcg_tol2 = m->cfg.pcg_tol * m->cfg.pcg_tol;
     m->d_tgt_idx = (int *)A.at(o_ti);
     m->d_tgt_xyz = (double *)A.at(o_tx);
+    m->d_orig = (const int *)A.at(o_orig);
+    m->d_xyz = (double *)A.at(o_xyz);
     hipStream_t st = m->ctx->stream;
     auto up = [&](const void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -569,6 +573,7 @@ int analyze_device(md_mesh *m) {
     chk(up(d.tri, t.tri_internal.data(), sizeof(int) * 3 * (size_t)m->F));
     chk(up(d.slot_fe_ptr, t.slot_fe_ptr.data(), sizeof(int) * (slots + 1)));
     chk(up(d.slot_fe, t.slot_fe.data(), sizeof(int) * t.slot_fe.size()));
+    chk(up(m->d_orig, t.original_of.data(), sizeof(int) * n));
     std::vector<FacDev> fh(ns);
     for (int k = 0; k < ns; ++k) {
         FacDev &x = fh[k];
@@ -702,42 +707,52 @@ int upload_layout(md_mesh *m) {
     return MD_OK;
 }
 
-// Uploads positions (original order, 3 per vertex) as d4 in internal order.
-hipError_t upload_positions(md_mesh *m, const double *xyz, d4 *dst) {
-    std::vector<d4> h(m->n);
-    for (int iv = 0; iv < m->n; ++iv) {
-        int v = m->topo.original_of[iv];
-        h[iv] = d4{xyz[3 * (size_t)v], xyz[3 * (size_t)v + 1], xyz[3 * (size_t)v + 2], 0.0};
-    }
-    hipError_t e = hipMemcpyAsync(dst, h.data(), sizeof(d4) * m->n, hipMemcpyHostToDevice, m->ctx->stream);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(m->ctx->stream);
+// A copy ordered after the work queued on the mesh's stream (a non-blocking stream, which a
+// plain hipMemcpy on the null stream would not wait for), then waited for.
+hipError_t copy_on_stream(md_mesh *m, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, m->ctx->stream);
+    return e == hipSuccess ? hipStreamSynchronize(m->ctx->stream) : e;
 }
 
+// Uploads the caller's positions (original order, 3 per vertex) and spreads them to internal
+// order on the device: into p (rest, may be null) and pp (current positions with the targets
+// of the constrained vertices pasted, may be null). Stream-ordered, no synchronisation.
+hipError_t upload_positions(md_mesh *m, const double *xyz, d4 *p, d4 *pp) {
+    hipStream_t st = m->ctx->stream;
+    const int n = m->n, nf = m->topo.nf;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(m->d_xyz, xyz, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && pp && n > nf) {
+        m->tgt_stage.resize(3 * (size_t)(n - nf));
+        for (int iv = nf; iv < n; ++iv)
+            std::memcpy(&m->tgt_stage[3 * (size_t)(iv - nf)], &m->con[3 * (size_t)m->topo.original_of[iv]],
+                        3 * sizeof(double));
+        e = hipMemcpyAsync(m->d_tgt_xyz, m->tgt_stage.data(), sizeof(double) * m->tgt_stage.size(),
+                           hipMemcpyHostToDevice, st);
+    }
+    if (e != hipSuccess) return e;
+    launch_unpack_xyz(st, m->d_xyz, m->d_orig, n, nf, m->d_tgt_xyz, p, pp);
+    return hipGetLastError();
+}
+
+// Current positions to the caller's packed xyz (original order), packed on the device.
 hipError_t download_positions(md_mesh *m, double *xyz) {
-    std::vector<d4> h(m->n);
-    hipError_t e = hipMemcpyAsync(h.data(), m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, m->ctx->stream);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(m->ctx->stream);
-    if (e != hipSuccess) return e;
-    for (int iv = 0; iv < m->n; ++iv) {
-        int v = m->topo.original_of[iv];
-        xyz[3 * (size_t)v] = h[iv].x;
-        xyz[3 * (size_t)v + 1] = h[iv].y;
-        xyz[3 * (size_t)v + 2] = h[iv].z;
-    }
-    return hipSuccess;
+    hipStream_t st = m->ctx->stream;
+    if (m->n == 0) return hipSuccess;
+    launch_pack_xyz(st, m->dev.pp, m->d_orig, m->n, m->d_xyz);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(xyz, m->d_xyz, sizeof(double) * 3 * (size_t)m->n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
 }
 
-// Cotan weights of the uploaded rest pose (per-face kernel + per-slot deterministic sum).
 hipError_t compute_cotan(md_mesh *m) {
     const Plan &P = m->plan;
     hipStream_t st = m->ctx->stream;
     launch_cotan_face(st, P.set(P.off_face), P.blk_face);
     launch_cotan_slot(st, P.set(P.off_slice), P.blk_slice);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(st);
+    return hipGetLastError();
 }
 
 // FP64 numeric factorization on the host from the device cotan weights; uploads panels.
@@ -756,7 +771,7 @@ int factor_pcg(md_mesh *m) {
 int weights_download(md_mesh *m) {
     int64_t slots = m->topo.total_slots();
     m->w_host.resize(slots);
-    if (hipMemcpy(m->w_host.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (copy_on_stream(m, m->w_host.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("download of cotan weights failed");
         return MD_ERR_HIP;
     }
@@ -828,11 +843,17 @@ int factor_device(md_mesh *m) {
     const size_t nP = (size_t)f.poff[ns], nS = (size_t)f.woff_s[ns], nT = (size_t)f.woff_t[ns],
                  nU = (size_t)f.woff_u[ns];
     const size_t bytes = sizeof(double) * (nP + nS + nT + nU + (size_t)nf) + 2 * sizeof(unsigned long long);
-    void *ws = nullptr;
-    if (hipMalloc(&ws, bytes) != hipSuccess) {
-        set_error("device factorization workspace of " + std::to_string(bytes) + " bytes: allocation failed");
-        return MD_ERR_NOMEM;
+    const bool keep = m->cfg.rest_mode == MD_REST_REFERENCE;  // refactors on every deform call
+    if (m->fac_ws && m->fac_ws_bytes != bytes) m->release_fac_ws();
+    if (!m->fac_ws) {
+        if (hipMalloc(&m->fac_ws, bytes) != hipSuccess) {
+            m->fac_ws = nullptr;
+            set_error("device factorization workspace of " + std::to_string(bytes) + " bytes: allocation failed");
+            return MD_ERR_NOMEM;
+        }
+        m->fac_ws_bytes = bytes;
     }
+    void *ws = m->fac_ws;
     FacArgs A{};
     A.sn = m->d_fac; A.hlist = m->d_hlist; A.offrows = m->d_offrows; A.rel = m->d_rel; A.child = m->dev.child;
     A.slice_ptr = m->dev.slice_ptr; A.sell_col = m->dev.sell_col; A.w64 = m->dev.w64; A.nf = nf;
@@ -867,7 +888,7 @@ int factor_device(md_mesh *m) {
     int flag = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&flag, A.flag, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(ws);
+    if (!keep || e != hipSuccess) m->release_fac_ws();
     if (e != hipSuccess) {
         set_error(std::string("device factorization: ") + hipGetErrorString(e));
         return MD_ERR_HIP;
@@ -889,9 +910,11 @@ int factor_device(md_mesh *m) {
 }
 
 int factor_upload(md_mesh *m) {
-    hipError_t e = hipMemcpy(const_cast<void *>(m->dev.pc), m->pc_host.data(), m->pc_host.size(), hipMemcpyHostToDevice);
+    hipError_t e = copy_on_stream(m, const_cast<void *>(m->dev.pc), m->pc_host.data(), m->pc_host.size(),
+                                  hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(const_cast<void *>(m->dev.pr), m->pr_host.data(), m->pr_host.size(), hipMemcpyHostToDevice);
+        e = copy_on_stream(m, const_cast<void *>(m->dev.pr), m->pr_host.data(), m->pr_host.size(),
+                           hipMemcpyHostToDevice);
     std::vector<char>().swap(m->pc_host);
     std::vector<char>().swap(m->pr_host);
     if (e != hipSuccess) {
@@ -924,22 +947,27 @@ int prep_device_a(md_mesh *m, Prep &p) {
         int rc = analyze_device(m);
         if (rc) return rc;
     }
-    hipError_t e;
+    hipError_t e = hipSuccess;
+    // rest pose (when it changes) and the initial guess: the current mesh with the targets
+    // pasted (arap.h:162-168, 277-281); in reference rest mode both come from the same upload
+    if (p.full) e = upload_positions(m, p.xyz, m->dev.p, m->dev.pp);
+    else {
+        if (p.need_analysis) e = upload_positions(m, m->rest.data(), m->dev.p, nullptr);
+        if (e == hipSuccess) e = upload_positions(m, p.xyz, nullptr, m->dev.pp);
+    }
+    if (e != hipSuccess) { set_error(std::string("position upload failed: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     if (p.full || p.need_analysis) {
-        e = upload_positions(m, m->rest.data(), m->dev.p);
-        if (e == hipSuccess) e = compute_cotan(m);
+        e = compute_cotan(m);
         if (e != hipSuccess) { set_error(std::string("cotan weights: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     }
-    // initial guess: the current mesh with the targets pasted (arap.h:162-168, 277-281)
-    std::vector<double> init(p.xyz, p.xyz + 3 * (size_t)m->n);
-    for (int v = 0; v < m->n; ++v)
-        if (m->is_con[v]) std::memcpy(&init[3 * (size_t)v], &m->con[3 * (size_t)v], 3 * sizeof(double));
-    e = upload_positions(m, init.data(), m->dev.pp);
-    if (e != hipSuccess) { set_error("position upload failed"); return MD_ERR_HIP; }
     // identity rotations (arap.h:246-249)
     launch_identity_quat(m->ctx->stream, m->plan.set(m->plan.off_ident), m->plan.blk_ident, m->fp64());
     m->initialized = true;
     m->iterated = false;
+    // the caller's buffers have been read once this returns (the copies above may be
+    // asynchronous), and the cotan weights are complete for the host factorization
+    e = hipStreamSynchronize(m->ctx->stream);
+    if (e != hipSuccess) { set_error(std::string("position upload failed: ") + hipGetErrorString(e)); return MD_ERR_HIP; }
     if (m->ncon == 0) { p.early = true; return MD_OK; }  // arap.h:113-114
     p.need_factor = m->topo.nf > 0 && (p.full || p.need_analysis || !m->factored);
     if (p.need_factor && m->cfg.solver == MD_SOLVER_LDLT && !m->cfg.factor_device) return weights_download(m);
@@ -1105,6 +1133,7 @@ void rebind(md_mesh *m, const char *oa, size_t asz, char *na, const char *ol, si
     fix(d.sfc); fix(d.sbo); fix(d.srow); fix(d.sfi); fix(d.lrows); fix(d.energy_part); fix(d.w32);
     fix(d.diag); fix(d.cg_x); fix(d.cg_p); fix(d.cg_ap); fix(d.cg_part);
     fix(m->d_fac); fix(m->d_hlist); fix(m->d_offrows); fix(m->d_rel); fix(m->d_tgt_idx); fix(m->d_tgt_xyz);
+    fix(m->d_orig); fix(m->d_xyz);
 }
 
 // True when an 8-byte word of the mesh's device record still addresses an old arena (a
@@ -1161,6 +1190,7 @@ int mesh_evict(md_mesh *m, hipStream_t st) {
     m->plan.drop_graph();
     m->arena.release();
     m->larena.release();
+    m->release_fac_ws();
     m->evicted = true;
     ++m->gen;
     return MD_OK;
@@ -1456,7 +1486,7 @@ extern "C" int md_mesh_cotan_csr(md_mesh *m, int *rowptr, int *col, double *val)
     const Topology &t = m->topo;
     int64_t slots = t.total_slots();
     std::vector<double> w(slots);
-    if (hipMemcpy(w.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (copy_on_stream(m, w.data(), m->dev.w64, sizeof(double) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("download of cotan weights failed");
         return MD_ERR_HIP;
     }
@@ -1486,12 +1516,12 @@ extern "C" int md_mesh_rotations(md_mesh *m, double *R) {
     int n = m->n;
     std::vector<double> q(4 * (size_t)n);
     if (m->fp64()) {
-        if (hipMemcpy(q.data(), m->dev.quat, sizeof(d4) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (copy_on_stream(m, q.data(), m->dev.quat, sizeof(d4) * n, hipMemcpyDeviceToHost) != hipSuccess) {
             set_error("rotation download failed"); return MD_ERR_HIP;
         }
     } else {
         std::vector<float> qf(4 * (size_t)n);
-        if (hipMemcpy(qf.data(), m->dev.quat, sizeof(f4) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (copy_on_stream(m, qf.data(), m->dev.quat, sizeof(f4) * n, hipMemcpyDeviceToHost) != hipSuccess) {
             set_error("rotation download failed"); return MD_ERR_HIP;
         }
         for (size_t k = 0; k < qf.size(); ++k) q[k] = qf[k];
@@ -1579,7 +1609,7 @@ extern "C" int md_mesh_normals_noalloc(md_mesh *m, double *nrm) {
     if (e == hipSuccess && n)
         e = hipMemcpyAsync(perm, m->topo.original_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        launch_pack_normals(st, vn, perm, m->n, dst);
+        launch_pack_xyz(st, vn, perm, m->n, dst);
         e = hipGetLastError();
     }
     if (e == hipSuccess && n) e = hipMemcpyAsync(nrm, dst, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st);
@@ -1632,7 +1662,7 @@ extern "C" int md_mesh_get_stats(md_mesh *m, md_mesh_stats *out) {
     for (int k = 0; k < 6; ++k) out->deform_ms[k] = m->deform_ms[k];
     out->fwd_items = s.fwd_items;
     out->bwd_items = s.bwd_items;
-    out->device_bytes = (int64_t)(m->arena.size() + m->larena.size());
+    out->device_bytes = (int64_t)(m->arena.size() + m->larena.size() + m->fac_ws_bytes);
     out->setup_ms_host = m->setup_ms_host;
     // algorithmic bytes per iteration: each array element a phase must touch, counted once
     const Topology &t = m->topo;

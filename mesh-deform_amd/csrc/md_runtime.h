@@ -189,5 +189,19 @@ struct md_mesh {
     std::vector<char> pc_host, pr_host;  // device panel layouts staged on the host
     int *d_tgt_idx = nullptr;      // staging for moved targets (ncon entries)
     double *d_tgt_xyz = nullptr;
+    const int *d_orig = nullptr;   // original id of each internal id (topo.original_of)
+    double *d_xyz = nullptr;       // packed xyz in original order (3n): positions in and out
+    std::vector<double> tgt_stage; // targets of the constrained ids in internal order (host
+                                   //   source of an upload; kept until the stream has passed it)
+    // FP64 workspace of the device factorization, kept between calls by meshes that refactor
+    // on every deform call (reference rest mode); others free it after each factorization
+    void *fac_ws = nullptr;
+    size_t fac_ws_bytes = 0;
+    void release_fac_ws() {
+        if (fac_ws) hipFree(fac_ws);
+        fac_ws = nullptr;
+        fac_ws_bytes = 0;
+    }
     bool fp64() const { return cfg.precision == MD_PREC_FP64; }
+    ~md_mesh() { release_fac_ws(); }
 };
