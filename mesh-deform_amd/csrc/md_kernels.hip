@@ -1376,6 +1376,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
 
 constexpr int kFacBlock = 256;
 constexpr int kFacCols = 64;  // front columns per assembly workgroup
+constexpr int kAsmU = 4;      // child update entries per assembly thread in flight
 
 // L_jj = sum of the weights of every neighbour of j, in slot order, and their maximum.
 __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
@@ -1401,7 +1402,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
     const int t0 = (int)(blockIdx.x % per) * kFacCols, t1 = min(m, t0 + kFacCols);
     if (t0 >= m) return;
-    double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
+    MD_G double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
     for (int t = t0; t < t1; ++t) {
         if (t < w) {
             for (int i = tid; i < m; i += kFacBlock) P[(long long)t * m + i] = 0.0;
@@ -1435,31 +1436,50 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
         }
     }
     __syncthreads();
+    // children's update matrices, child by child: the child's lower entries (a >= b) of its
+    // columns b landing in [t0, t1) (at most 64: rel is increasing), flattened over the
+    // workgroup, kAsmU entries per thread in flight (their loads, then the read-modify-writes:
+    // distinct entries of one child land on distinct front entries)
     for (int ci = S.cb; ci < S.ce; ++ci) {
         const FacDev C = A.sn[A.child[ci]];
         const int nc = C.noff;
         const int *rel = A.rel + C.roff;
-        const double *Uc = A.U + C.uoff;
-        // the child's update columns b landing in [t0, t1): rel is increasing
+        const MD_G double *Uc = A.U + C.uoff;
         int blo = 0, bhi = nc;
         while (blo < bhi) {
             const int mid = (blo + bhi) >> 1;
             if (rel[mid] < t0) blo = mid + 1;
             else bhi = mid;
         }
-        for (int b = blo; b < nc && rel[b] < t1; ++b) {
-            const int rb = rel[b];
-            for (int a = b + tid; a < nc; a += kFacBlock) {
-                const double v = Uc[(long long)b * nc + a];
-                const int ra = rel[a];
-                if (ra >= m || rb > ra) { *A.flag = 2; continue; }
-                if (rb < w) {
-                    if (ra < w) P[(long long)rb * m + ra] += v;
-                    else F[(long long)rb * no + (ra - w)] += v;
-                } else {
-                    U[(long long)(rb - w) * no + (ra - w)] += v;
+        int bend = blo, hi = nc;
+        while (bend < hi) {
+            const int mid = (bend + hi) >> 1;
+            if (rel[mid] < t1) bend = mid + 1;
+            else hi = mid;
+        }
+        const int total = (bend - blo) * nc;
+        for (int e0 = tid; e0 < total; e0 += kFacBlock * kAsmU) {
+            double v[kAsmU];
+            MD_G double *dst[kAsmU];
+#pragma unroll
+            for (int u = 0; u < kAsmU; ++u) {
+                const int e = e0 + u * kFacBlock, bb = e / nc, a = e - bb * nc, b = blo + bb;
+                dst[u] = nullptr;
+                v[u] = 0.0;
+                if (e < total && a >= b) {
+                    const int ra = rel[a], rb = rel[b];
+                    v[u] = Uc[(long long)b * nc + a];
+                    if (ra >= m || rb > ra) *A.flag = 2;
+                    else if (rb < w) dst[u] = ra < w ? P + (long long)rb * m + ra : F + (long long)rb * no + (ra - w);
+                    else dst[u] = U + (long long)(rb - w) * no + (ra - w);
                 }
             }
+            double old[kAsmU];
+#pragma unroll
+            for (int u = 0; u < kAsmU; ++u) old[u] = dst[u] ? *dst[u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < kAsmU; ++u)
+                if (dst[u]) *dst[u] = old[u] + v[u];
         }
         __syncthreads();
     }
@@ -1609,6 +1629,25 @@ __device__ __forceinline__ int mma_row(int x, int r) {
 }
 __device__ __forceinline__ int mma_col(int y) { return 32 * ((threadIdx.x / 64) & 1) + 16 * y + (threadIdx.x & 15); }
 
+// acc += sum over nchunks staged chunks of kMk values of k, software-pipelined: each thread
+// holds its kPer values of both operands of the next chunk in registers, loaded (fetch(c, ra,
+// rb)) while the matrix cores work on the current one, and put into As / Bs (put(ra, rb)) once
+// every wave is done reading them. Thread t stages the entries e = t + kFacBlock i, i < kPer.
+constexpr int kPer = kMk * 64 / kFacBlock;
+template <class Fetch, class Put>
+__device__ __forceinline__ void mma_pipeline(int nchunks, Fetch fetch, Put put, double (*As)[kMpad],
+                                             double (*Bs)[kMpad], MmaAcc &acc) {
+    double ra[kPer], rb[kPer];
+    if (nchunks > 0) fetch(0, ra, rb);
+    for (int c = 0; c < nchunks; ++c) {
+        __syncthreads();
+        put(ra, rb);
+        __syncthreads();
+        if (c + 1 < nchunks) fetch(c + 1, ra, rb);
+        mma_chunk(As, Bs, acc);
+    }
+}
+
 // Lower block (I, J), I >= J, of the q-th tile of a lower-triangular tile grid.
 __device__ __forceinline__ void lower_tile(int q, int &I, int &J) {
     int i = 0;
@@ -1633,17 +1672,26 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int p
     __shared__ double Bs[kMk][kMpad];
     MmaAcc acc;
     mma_zero(acc);
-    for (int k0 = r0; k0 < w; k0 += kMk) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-            const int kk = e / 64, x = e % 64, k = k0 + kk;
-            const bool kin = k < w;
-            As[kk][x] = kin && r0 + x < w ? X[(long long)k * w + r0 + x] : 0.0;
-            Bs[kk][x] = kin && c0 + x < w ? X[(long long)k * w + c0 + x] : 0.0;
-        }
-        __syncthreads();
-        mma_chunk(As, Bs, acc);
-    }
+    mma_pipeline(
+        (w - r0 + kMk - 1) / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = r0 + c * kMk + e / 64;
+                const bool kin = k < w;
+                ra[i] = kin && r0 + x < w ? X[(long long)k * w + r0 + x] : 0.0;
+                rb[i] = kin && c0 + x < w ? X[(long long)k * w + c0 + x] : 0.0;
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e / 64][e % 64] = rb[i];
+            }
+        },
+        As, Bs, acc);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1673,18 +1721,27 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per)
     __shared__ double Bs[kMk][kMpad];
     MmaAcc acc;
     mma_zero(acc);
-    for (int k0 = 0; k0 < w; k0 += kMk) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-            const int kk = e / 64, x = e % 64, k = k0 + kk;
-            As[kk][x] = k < w && r0 + x < no ? F[(long long)k * no + r0 + x] : 0.0;
-            // B(k, q0 + x): consecutive threads read consecutive k of one column
-            const int kb = e % kMk, xb = e / kMk, kg = k0 + kb;
-            Bs[kb][xb] = kg < w && q0 + xb < w ? P[(long long)(q0 + xb) * m + kg] : 0.0;
-        }
-        __syncthreads();
-        mma_chunk(As, Bs, acc);
-    }
+    mma_pipeline(
+        (w + kMk - 1) / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = c * kMk + e / 64;
+                ra[i] = k < w && r0 + x < no ? F[(long long)k * no + r0 + x] : 0.0;
+                // B(k, q0 + x): consecutive threads read consecutive k of one column
+                const int xb = e / kMk, kg = c * kMk + e % kMk;
+                rb[i] = kg < w && q0 + xb < w ? P[(long long)(q0 + xb) * m + kg] : 0.0;
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e % kMk][e / kMk] = rb[i];
+            }
+        },
+        As, Bs, acc);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1711,16 +1768,25 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per)
     __shared__ double Bs[kMk][kMpad];
     MmaAcc acc;
     mma_zero(acc);
-    for (int k0 = 0; k0 < w; k0 += kMk) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-            const int kk = e / 64, x = e % 64, k = k0 + kk;
-            As[kk][x] = k < w && a0 + x < no ? P[(long long)k * m + w + a0 + x] : 0.0;
-            Bs[kk][x] = k < w && b0 + x < no ? F[(long long)k * no + b0 + x] : 0.0;
-        }
-        __syncthreads();
-        mma_chunk(As, Bs, acc);
-    }
+    mma_pipeline(
+        (w + kMk - 1) / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = c * kMk + e / 64;
+                ra[i] = k < w && a0 + x < no ? P[(long long)k * m + w + a0 + x] : 0.0;
+                rb[i] = k < w && b0 + x < no ? F[(long long)k * no + b0 + x] : 0.0;
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e / 64][e % 64] = rb[i];
+            }
+        },
+        As, Bs, acc);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1746,18 +1812,26 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_trsm(FacArgs A, int h0, int s
     __shared__ double Bs[kMk][kMpad];
     MmaAcc acc;
     mma_zero(acc);
-    for (int k0 = 0; k0 < kFb; k0 += kMk) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-            const int kk = e / 64, x = e % 64;
-            As[kk][x] = r0 + x < w ? P[(long long)(kb + k0 + kk) * m + r0 + x] : 0.0;
-            // B(t, j) = X(kb + j, kb + t): consecutive threads read consecutive t of one row
-            const int tb = e % kMk, jb = e / kMk;
-            Bs[tb][jb] = X[(long long)(kb + jb) * w + kb + k0 + tb];
-        }
-        __syncthreads();
-        mma_chunk(As, Bs, acc);
-    }
+    mma_pipeline(
+        kFb / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k0 = c * kMk;
+                ra[i] = r0 + x < w ? P[(long long)(kb + k0 + e / 64) * m + r0 + x] : 0.0;
+                // B(t, j) = X(kb + j, kb + t): consecutive threads read consecutive t of one row
+                rb[i] = X[(long long)(kb + e / kMk) * w + kb + k0 + e % kMk];
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e % kMk][e / kMk] = rb[i];
+            }
+        },
+        As, Bs, acc);
     __syncthreads();  // every wave has staged its last reads of the tile before any write
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -1785,17 +1859,26 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s
     __shared__ double Bs[kMk][kMpad];
     MmaAcc acc;
     mma_zero(acc);
-    for (int k0 = 0; k0 < kFb; k0 += kMk) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-            const int kk = e / 64, x = e % 64;
-            const long long col = (long long)(kb + k0 + kk) * m;
-            As[kk][x] = r0 + x < w ? P[col + r0 + x] : 0.0;
-            Bs[kk][x] = c0 + x < w ? P[col + c0 + x] : 0.0;
-        }
-        __syncthreads();
-        mma_chunk(As, Bs, acc);
-    }
+    mma_pipeline(
+        kFb / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64;
+                const long long col = (long long)(kb + c * kMk + e / 64) * m;
+                ra[i] = r0 + x < w ? P[col + r0 + x] : 0.0;
+                rb[i] = c0 + x < w ? P[col + c0 + x] : 0.0;
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e / 64][e % 64] = rb[i];
+            }
+        },
+        As, Bs, acc);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1821,17 +1904,26 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int 
     MmaAcc acc;
     mma_zero(acc);
     // Y: A(i, k) = L(r0 + i, 64 K + k) (panel, column-major), B(k, j) = X(64 K + k, c0 + j)
-    for (int K = J; K < I; ++K)
-        for (int k0 = 0; k0 < kFb; k0 += kMk) {
-            __syncthreads();
-            for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-                const int kk = e / 64, x = e % 64, k = K * kFb + k0 + kk;
-                As[kk][x] = r0 + x < w ? P[(long long)k * m + r0 + x] : 0.0;
-                Bs[kk][x] = T[(long long)k * w + c0 + x];
+    // (chunk c: k = 64 J + kMk c + kk, over blocks K = J .. I - 1)
+    mma_pipeline(
+        (I - J) * (kFb / kMk),
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = J * kFb + c * kMk + e / 64;
+                ra[i] = r0 + x < w ? P[(long long)k * m + r0 + x] : 0.0;
+                rb[i] = T[(long long)k * w + c0 + x];
             }
-            __syncthreads();
-            mma_chunk(As, Bs, acc);
-        }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e / 64][e % 64] = rb[i];
+            }
+        },
+        As, Bs, acc);
     // X_IJ = -X_II Y: A(i, k) = X(r0 + i, r0 + k) (row-major), B(k, j) = Y(k, j)
     MmaAcc out;
     mma_zero(out);
