@@ -1492,14 +1492,17 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
 // the off-diagonal blocks of X one block diagonal at a time: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ.
 constexpr int kFb = 64;
 
-// Diagonal block s, by one wavefront: unblocked right-looking Cholesky with row i of the block in
-// the registers of lane i (a[c] = A(kb + i, kb + c), then L(i, c)); at step k the pivot comes
-// from lane k (v_readlane, two 32-bit halves per double), every lane scales its L(i, k) and puts
-// it in LDS, and the rank-1 update A(i, c) -= L(i, k) L(c, k) reads column k back as broadcast
-// LDS reads (all lanes the same address). Entries above the diagonal are updated too and never
-// used. Then the inverse X = L^{-1} by columns, lane i forming column i by forward substitution,
-// X(k, i) = (delta_ki - sum_{t<k} L(k, t) X(t, i)) / L(k, k), from L's rows staged in LDS
-// (broadcast reads again); X is zero above the diagonal, exactly.
+// Diagonal block s, by four wavefronts (unrolled over the block's columns: a loop over them with
+// the same work per step measured almost twice as slow). Cholesky (unblocked, right-looking):
+// lane i of wave v holds row i of the block's columns [16 v, 16 v + 16) (a[j] = A(kb + i, kb +
+// 16 v + j), then L); at step k the wave owning column k takes the pivot from lane k
+// (v_readlane, two 32-bit halves per double), scales the column by 1 / sqrt(pivot) (v_rsq_f64 and
+// two Newton steps) and puts it in LDS; after one barrier every wave applies A(i, c) -= L(i, k)
+// L(c, k) to its columns c > k (entries above the diagonal too, never used). Inverse X = L^{-1}
+// by columns: wave v forms columns [16 v, 16 v + 16) with four lanes per column, lane r of a
+// column holding X(t, i) for t = r mod 4; X(k, i) = (delta_ki - sum_{t<k} L(k, t) X(t, i)) /
+// L(k, k), the sum split over the four lanes and joined by a DPP butterfly, so the inverse needs
+// no barrier; X is zero above the diagonal, exactly.
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
@@ -1507,81 +1510,106 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// L's rows in LDS, packed: row k holds k + 1 values padded to an even count (16-byte aligned rows).
-__host__ __device__ constexpr int tri_row_off(int k) { return k * (k + 1) / 2 + (k + 1) / 2; }
-constexpr int kTriLds = tri_row_off(kFb);
+// v from the lane whose index is this lane's xor Q (Q = 1, 2: within a quad), by DPP.
+template <int Q>
+__device__ __forceinline__ double quad_xor(double v) {
+    constexpr int ctrl = Q == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, ctrl, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+constexpr int kPotrfCols = kFb / 4;  // columns per wave
+constexpr int kLdsPad = kFb + 1;
 
 template <int K>
-__device__ __forceinline__ void chol_step(double (&a)[kFb], double (*col)[kFb], double &rdiag, int lane, int b,
-                                          double amax, bool &bad) {
+__device__ __forceinline__ void chol_step(double (&a)[kPotrfCols], double (*col)[kFb], double &rdiag, int wv,
+                                          int lane, int b, double amax, bool &bad) {
     if constexpr (K < kFb) {
         if (K < b) {
-            const double piv = readlane_f64(a[K], K);
-            if (!(piv > 1e-14 * amax)) bad = true;
-            const double d = sqrt(piv), rd = 1.0 / d;
-            if (lane == K) rdiag = rd;
-            a[K] = lane == K ? d : a[K] * rd;
-            col[K & 1][lane] = a[K];
-            __syncthreads();  // one wavefront: orders the LDS write before the reads below
+            constexpr int jk = K % kPotrfCols;
+            if (wv == K / kPotrfCols) {
+                const double piv = readlane_f64(a[jk], K);
+                if (!(piv > 1e-14 * amax)) bad = true;
+                double rd = __builtin_amdgcn_rsq(piv);
+                rd = rd * (1.5 - 0.5 * piv * rd * rd);
+                rd = rd * (1.5 - 0.5 * piv * rd * rd);
+                if (lane == K) rdiag = rd;
+                a[jk] = lane == K ? piv * rd : a[jk] * rd;
+                col[K & 1][lane] = a[jk];
+            }
+            __syncthreads();
+            const double lik = col[K & 1][lane];
 #pragma unroll
-            for (int c = K + 1; c < kFb; ++c) a[c] -= a[K] * col[K & 1][c];
-            chol_step<K + 1>(a, col, rdiag, lane, b, amax, bad);
+            for (int j = 0; j < kPotrfCols; ++j) {
+                const int c = kPotrfCols * wv + j;
+                if (c > K) a[j] -= lik * col[K & 1][c];
+            }
+            chol_step<K + 1>(a, col, rdiag, wv, lane, b, amax, bad);
         }
     }
 }
 
 template <int K>
-__device__ __forceinline__ void inv_col(const double *Lsh, double (&x)[kFb], int lane, int b) {
+__device__ __forceinline__ void inv_col(const double (*Lsh)[kLdsPad], double (&x)[kFb / 4], int r, int i, int b) {
     if constexpr (K < kFb) {
         if (K < b) {
-            constexpr int o = tri_row_off(K);
-            double acc = lane == K ? 1.0 : 0.0;
+            double part = 0.0;
 #pragma unroll
-            for (int t = 0; t < K; ++t) acc -= Lsh[o + t] * x[t];
-            x[K] = acc * Lsh[o + K];  // the diagonal slot holds 1 / L(k, k)
-            inv_col<K + 1>(Lsh, x, lane, b);
+            for (int q = 0; q < (K + 3) / 4; ++q) {
+                const int t = 4 * q + r;
+                part += t < K ? Lsh[K][t] * x[q] : 0.0;
+            }
+            part += quad_xor<1>(part);
+            part += quad_xor<2>(part);
+            const double xk = ((i == K ? 1.0 : 0.0) - part) * Lsh[K][K];  // the diagonal slot holds 1 / L(k, k)
+            if (r == (K & 3)) x[K / 4] = xk;
+            inv_col<K + 1>(Lsh, x, r, i, b);
         }
     }
 }
 
-__global__ __launch_bounds__(kFb) void k_fac_potrf(FacArgs A, int h0, int s) {
+__global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int s) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
-    const int w = S.w, m = S.m, kb = s * kFb, lane = threadIdx.x;
+    const int w = S.w, m = S.m, kb = s * kFb;
     if (kb >= w) return;
-    const int b = min(kFb, w - kb);
-    double *P = A.P + S.poff, *T = A.T + S.toff;
+    const int b = min(kFb, w - kb), wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    MD_G double *P = A.P + S.poff;
+    MD_G double *T = A.T + S.toff;
     __shared__ double col[2][kFb];
-    __shared__ __attribute__((aligned(16))) double Lsh[kTriLds];
-    double a[kFb];
+    __shared__ double Lsh[kFb][kLdsPad];
+    double a[kPotrfCols];
 #pragma unroll
-    for (int c = 0; c < kFb; ++c)
+    for (int j = 0; j < kPotrfCols; ++j) {
         // column-major panel: A(kb + lane, kb + c) at (kb + c) m + kb + lane (lower part only)
-        a[c] = (lane < b && c < b && c <= lane) ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
+        const int c = kPotrfCols * wv + j;
+        a[j] = (lane < b && c < b && c <= lane) ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
+    }
     const double amax = __longlong_as_double((long long)*A.amax);
     bool bad = false;
     double rdiag = 0.0;
-    chol_step<0>(a, col, rdiag, lane, b, amax, bad);
-    if (bad && lane == 0) *A.flag = 1;
-    // L: lower part to the panel; row `lane` to LDS (pairs, the last one may fill the row's pad)
-    const int ro = tri_row_off(lane);
+    chol_step<0>(a, col, rdiag, wv, lane, b, amax, bad);
+    if (bad && threadIdx.x == 0) *A.flag = 1;
+    // L: lower part to the panel, all of the wave's columns to LDS (the diagonal as 1 / L(i, i))
 #pragma unroll
-    for (int c = 0; c < kFb; c += 2) {
-        if (lane < b && c < b && c <= lane) {
-            P[(long long)(kb + c) * m + kb + lane] = a[c];
-            if (c + 1 < b && c + 1 <= lane) P[(long long)(kb + c + 1) * m + kb + lane] = a[c + 1];
-        }
-        if (c <= lane) *reinterpret_cast<double2 *>(&Lsh[ro + c]) = double2{a[c], a[c + 1]};
+    for (int j = 0; j < kPotrfCols; ++j) {
+        const int c = kPotrfCols * wv + j;
+        if (lane < b && c < b && c <= lane) P[(long long)(kb + c) * m + kb + lane] = a[j];
+        Lsh[lane][c] = c == lane ? rdiag : a[j];
     }
-    Lsh[ro + lane] = rdiag;
     __syncthreads();
-    double x[kFb];
+    const int r = lane & 3, i = kPotrfCols * wv + (lane >> 2);
+    double x[kFb / 4];
 #pragma unroll
-    for (int k = 0; k < kFb; ++k) x[k] = 0.0;
-    inv_col<0>(Lsh, x, lane, b);
-    if (lane < b) {
+    for (int q = 0; q < kFb / 4; ++q) x[q] = 0.0;
+    inv_col<0>(Lsh, x, r, i, b);
+    if (i < b) {
 #pragma unroll
-        for (int k = 0; k < kFb; ++k)
-            if (k < b) T[(long long)(kb + k) * w + kb + lane] = x[k];  // X(kb + k, kb + lane), row-major
+        for (int q = 0; q < kFb / 4; ++q) {
+            const int k = 4 * q + r;
+            if (k < b) T[(long long)(kb + k) * w + kb + i] = x[q];  // X(kb + k, kb + i), row-major
+        }
     }
 }
 
@@ -2006,7 +2034,7 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm);
     for (int b = 0; b < nb; ++b) {
-        hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFb), 0, s, A, h0, b);
+        hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
         const int below = nb - 1 - b;
         if (below > 0) {
             hipLaunchKernelGGL(k_fac_trsm, dim3(count * below), dim3(kFacBlock), 0, s, A, h0, b, below);
