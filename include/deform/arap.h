@@ -83,11 +83,11 @@ inline md_ctx *context() {
 template <class MeshType, class PrecisionType = typename MeshType::Scalar>
 class AsRigidAsPossibleDeformation {
 public:
-    /** Floating point precision used in calculations. */
+    /** Scalar type of the positions and targets the class takes and returns. */
     typedef PrecisionType Scalar;
-    /** Mesh type we are working on. */
+    /** Adapter the class reads vertices and faces through. */
     typedef MeshType Mesh;
-    /** Type for indices. */
+    /** Vertex index type. */
     typedef int Index;
 
     /** Construct from mesh (arap.h:66-70). The mesh must outlive this object. */
@@ -159,7 +159,8 @@ public:
         return true;
     }
 
-    /** The C ABI handle, for callers that batch or serve this mesh (md_batch_*, md_server_*). */
+    /** The C ABI handle, for callers that batch this mesh (md_batch_*). A server (md_server_*)
+     *  takes meshes in fixed rest mode only; this class creates them in reference rest mode. */
     md_mesh *handle() const { return _handle; }
 
 private:
