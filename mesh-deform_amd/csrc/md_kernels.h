@@ -151,10 +151,10 @@ void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_fac_diag(hipStream_t s, const FacArgs &A);
 // One height: assembly, blocked Cholesky and inverse of the pivot blocks (nb = column blocks
 // of 64 in the widest one), D^{-1}, G and the update matrices.
-// per_*: workgroups per supernode of the stages (the height's most): assembly, one per 64 front
-// columns; D^{-1}, G and U, one per 64 x 64 output tile (lower tiles for the symmetric ones).
-void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_asm, int per_dinv,
-                       int per_g, int per_u);
+// per_*: workgroups per supernode of the stages (the height's most): assembly, one per asm_cols
+// front columns; D^{-1}, G and U, one per 64 x 64 output tile (lower tiles for the symmetric ones).
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
+                       int per_dinv, int per_g, int per_u);
 void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64);
 // Internal-order vectors (positions, normals) to packed xyz in original order (perm: original
 // id per internal id), and packed xyz to internal order (p and/or pp; pp takes the targets tgt,

@@ -1375,8 +1375,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
 // children's update matrices are added in child order, so the factor is deterministic.
 
 constexpr int kFacBlock = 256;
-constexpr int kFacCols = 64;  // front columns per assembly workgroup
-constexpr int kAsmU = 4;      // child update entries per assembly thread in flight
+constexpr int kAsmU = 8;      // child update entries per assembly thread in flight
 
 // L_jj = sum of the weights of every neighbour of j, in slot order, and their maximum.
 __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
@@ -1395,12 +1394,12 @@ __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
 // Front of each supernode of one height: the pivot block and G rows (panel), F_os (copy),
 // the update matrix; original entries (L_jj and -w_ij for free i > j, slot order) first,
 // then the children's update matrices in child order. `per` workgroups per supernode, each
-// owning 64 columns of the front (pivot columns: P and F; later ones: U), so no two write
+// owning `cols` columns of the front (pivot columns: P and F; later ones: U), so no two write
 // the same entry and every entry receives its additions in that fixed order.
-__global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, int per) {
+__global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, int per, int cols) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
-    const int t0 = (int)(blockIdx.x % per) * kFacCols, t1 = min(m, t0 + kFacCols);
+    const int t0 = (int)(blockIdx.x % per) * cols, t1 = min(m, t0 + cols);
     if (t0 >= m) return;
     MD_G double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
     for (int t = t0; t < t1; ++t) {
@@ -1437,7 +1436,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     }
     __syncthreads();
     // children's update matrices, child by child: the child's lower entries (a >= b) of its
-    // columns b landing in [t0, t1) (at most 64: rel is increasing), flattened over the
+    // columns b landing in [t0, t1) (at most `cols`: rel is increasing), flattened over the
     // workgroup, kAsmU entries per thread in flight (their loads, then the read-modify-writes:
     // distinct entries of one child land on distinct front entries)
     for (int ci = S.cb; ci < S.ce; ++ci) {
@@ -2029,10 +2028,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ sr
 void launch_fac_diag(hipStream_t s, const FacArgs &A) {
     if (A.nf > 0) hipLaunchKernelGGL(k_fac_diag, dim3((A.nf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A);
 }
-void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int per_asm, int per_dinv,
-                       int per_g, int per_u) {
+void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
+                       int per_dinv, int per_g, int per_u) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm);
+    hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm, asm_cols);
     for (int b = 0; b < nb; ++b) {
         hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
         const int below = nb - 1 - b;

@@ -867,20 +867,25 @@ int factor_device(md_mesh *m) {
         const int nh = (int)f.hptr.size() - 1;
         for (int h = 0; h < nh; ++h) {
             // workgroups per supernode of the multi-workgroup stages, for the height's largest
-            // front: assembly one per 64 front columns, the products one per 64 x 64 output tile
-            int per_asm = 1, per_dinv = 1, per_g = 0, per_u = 0, mw = 0;
+            // front: assembly one per asm_cols front columns (64, halved down to 8 while the
+            // height has fewer than 2048 assembly workgroups: the top fronts take the largest
+            // children), the products one per 64 x 64 output tile
+            int per_dinv = 1, per_g = 0, per_u = 0, mw = 0, mm = 1;
+            const int count = f.hptr[h + 1] - f.hptr[h];
             auto lower = [](int b) { return b * (b + 1) / 2; };
             for (int i = f.hptr[h]; i < f.hptr[h + 1]; ++i) {
                 const int w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
                 const int bw = (w + 63) / 64, bo = (no + 63) / 64;
                 mw = std::max(mw, w);
-                per_asm = std::max(per_asm, (w + no + 63) / 64);
+                mm = std::max(mm, w + no);
                 per_dinv = std::max(per_dinv, lower(bw));
                 per_g = std::max(per_g, bo * bw);
                 per_u = std::max(per_u, lower(bo));
             }
-            launch_fac_height(st, A, f.hptr[h], f.hptr[h + 1] - f.hptr[h], (mw + 63) / 64, per_asm, per_dinv, per_g,
-                              per_u);
+            int asm_cols = 64;
+            while (asm_cols > 8 && (long long)count * ((mm + asm_cols - 1) / asm_cols) < 2048) asm_cols /= 2;
+            launch_fac_height(st, A, f.hptr[h], count, (mw + 63) / 64, asm_cols, (mm + asm_cols - 1) / asm_cols,
+                              per_dinv, per_g, per_u);
         }
         launch_fac_convert(st, A, ns, const_cast<void *>(m->dev.pc), const_cast<void *>(m->dev.pr), m->fp64());
         e = hipGetLastError();
