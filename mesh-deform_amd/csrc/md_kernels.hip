@@ -1496,8 +1496,8 @@ constexpr int kFb = 64;
 // lane i of wave v holds row i of the block's columns [16 v, 16 v + 16) (a[j] = A(kb + i, kb +
 // 16 v + j), then L); at step k the wave owning column k takes the pivot from lane k
 // (v_readlane, two 32-bit halves per double), scales the column by 1 / sqrt(pivot) (v_rsq_f64 and
-// two Newton steps) and puts it in LDS; after one barrier every wave applies A(i, c) -= L(i, k)
-// L(c, k) to its columns c > k (entries above the diagonal too, never used). Inverse X = L^{-1}
+// two Newton steps), and every column applies A(i, c) -= L(i, k) L(c, k) for c > k (entries
+// above the diagonal too, never used), four steps per barrier (chol_panel). Inverse X = L^{-1}
 // by columns: wave v forms columns [16 v, 16 v + 16) with four lanes per column, lane r of a
 // column holding X(t, i) for t = r mod 4; X(k, i) = (delta_ki - sum_{t<k} L(k, t) X(t, i)) /
 // L(k, k), the sum split over the four lanes and joined by a DPP butterfly, so the inverse needs
@@ -1522,30 +1522,45 @@ __device__ __forceinline__ double quad_xor(double v) {
 constexpr int kPotrfCols = kFb / 4;  // columns per wave
 constexpr int kLdsPad = kFb + 1;
 
+// Four columns per barrier: the owning wave factors the panel k .. k + 3 in its registers
+// (pivots and the panel's own updates through v_readlane), puts the four columns in LDS, and
+// after one barrier every wave applies the rank-4 update to its columns c > k + 3.
 template <int K>
-__device__ __forceinline__ void chol_step(double (&a)[kPotrfCols], double (*col)[kFb], double &rdiag, int wv,
-                                          int lane, int b, double amax, bool &bad) {
+__device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col)[4][kFb], double &rdiag, int wv,
+                                           int lane, int b, double amax, bool &bad) {
     if constexpr (K < kFb) {
         if (K < b) {
-            constexpr int jk = K % kPotrfCols;
+            constexpr int j0 = K % kPotrfCols, buf = (K / 4) & 1;
             if (wv == K / kPotrfCols) {
-                const double piv = readlane_f64(a[jk], K);
-                if (!(piv > 1e-14 * amax)) bad = true;
-                double rd = __builtin_amdgcn_rsq(piv);
-                rd = rd * (1.5 - 0.5 * piv * rd * rd);
-                rd = rd * (1.5 - 0.5 * piv * rd * rd);
-                if (lane == K) rdiag = rd;
-                a[jk] = lane == K ? piv * rd : a[jk] * rd;
-                col[K & 1][lane] = a[jk];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    if (K + kk < b) {
+                        const double piv = readlane_f64(a[j0 + kk], K + kk);
+                        if (!(piv > 1e-14 * amax)) bad = true;
+                        double rd = __builtin_amdgcn_rsq(piv);
+                        rd = rd * (1.5 - 0.5 * piv * rd * rd);
+                        rd = rd * (1.5 - 0.5 * piv * rd * rd);
+                        if (lane == K + kk) rdiag = rd;
+                        a[j0 + kk] = lane == K + kk ? piv * rd : a[j0 + kk] * rd;
+#pragma unroll
+                        for (int c = kk + 1; c < 4; ++c) a[j0 + c] -= a[j0 + kk] * readlane_f64(a[j0 + kk], K + c);
+                    }
+                    col[buf][kk][lane] = a[j0 + kk];  // zero for columns past the block
+                }
             }
             __syncthreads();
-            const double lik = col[K & 1][lane];
+            double lik[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) lik[kk] = col[buf][kk][lane];
 #pragma unroll
             for (int j = 0; j < kPotrfCols; ++j) {
                 const int c = kPotrfCols * wv + j;
-                if (c > K) a[j] -= lik * col[K & 1][c];
+                if (c > K + 3) {
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) a[j] -= lik[kk] * col[buf][kk][c];
+                }
             }
-            chol_step<K + 1>(a, col, rdiag, wv, lane, b, amax, bad);
+            chol_panel<K + 4>(a, col, rdiag, wv, lane, b, amax, bad);
         }
     }
 }
@@ -1576,7 +1591,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
     const int b = min(kFb, w - kb), wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     MD_G double *P = A.P + S.poff;
     MD_G double *T = A.T + S.toff;
-    __shared__ double col[2][kFb];
+    __shared__ double col[2][4][kFb];
     __shared__ double Lsh[kFb][kLdsPad];
     double a[kPotrfCols];
 #pragma unroll
@@ -1588,7 +1603,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
     const double amax = __longlong_as_double((long long)*A.amax);
     bool bad = false;
     double rdiag = 0.0;
-    chol_step<0>(a, col, rdiag, wv, lane, b, amax, bad);
+    chol_panel<0>(a, col, rdiag, wv, lane, b, amax, bad);
     if (bad && threadIdx.x == 0) *A.flag = 1;
     // L: lower part to the panel, all of the wave's columns to LDS (the diagonal as 1 / L(i, i))
 #pragma unroll
