@@ -1497,7 +1497,7 @@ constexpr int kFb = 64;
 // 16 v + j), then L); at step k the wave owning column k takes the pivot from lane k
 // (v_readlane, two 32-bit halves per double), scales the column by 1 / sqrt(pivot) (v_rsq_f64 and
 // two Newton steps), and every column applies A(i, c) -= L(i, k) L(c, k) for c > k (entries
-// above the diagonal too, never used), four steps per barrier (chol_panel). Inverse X = L^{-1}
+// above the diagonal too, never used), kPanel steps per barrier (chol_panel). Inverse X = L^{-1}
 // by columns: wave v forms columns [16 v, 16 v + 16) with four lanes per column, lane r of a
 // column holding X(t, i) for t = r mod 4; X(k, i) = (delta_ki - sum_{t<k} L(k, t) X(t, i)) /
 // L(k, k), the sum split over the four lanes and joined by a DPP butterfly, so the inverse needs
@@ -1522,18 +1522,20 @@ __device__ __forceinline__ double quad_xor(double v) {
 constexpr int kPotrfCols = kFb / 4;  // columns per wave
 constexpr int kLdsPad = kFb + 1;
 
-// Four columns per barrier: the owning wave factors the panel k .. k + 3 in its registers
-// (pivots and the panel's own updates through v_readlane), puts the four columns in LDS, and
-// after one barrier every wave applies the rank-4 update to its columns c > k + 3.
+// kPanel columns per barrier: the owning wave factors the panel k .. k + kPanel - 1 in its
+// registers (pivots and the panel's own updates through v_readlane), puts the panel in LDS, and
+// after one barrier every wave applies the rank-kPanel update to its columns past the panel.
+constexpr int kPanel = 8;
+static_assert(kPotrfCols % kPanel == 0, "a panel lies within one wave's columns");
 template <int K>
-__device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col)[4][kFb], double &rdiag, int wv,
+__device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col)[kPanel][kFb], double &rdiag, int wv,
                                            int lane, int b, double amax, bool &bad) {
     if constexpr (K < kFb) {
         if (K < b) {
-            constexpr int j0 = K % kPotrfCols, buf = (K / 4) & 1;
+            constexpr int j0 = K % kPotrfCols, buf = (K / kPanel) & 1;
             if (wv == K / kPotrfCols) {
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
+                for (int kk = 0; kk < kPanel; ++kk) {
                     if (K + kk < b) {
                         const double piv = readlane_f64(a[j0 + kk], K + kk);
                         if (!(piv > 1e-14 * amax)) bad = true;
@@ -1543,24 +1545,25 @@ __device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col
                         if (lane == K + kk) rdiag = rd;
                         a[j0 + kk] = lane == K + kk ? piv * rd : a[j0 + kk] * rd;
 #pragma unroll
-                        for (int c = kk + 1; c < 4; ++c) a[j0 + c] -= a[j0 + kk] * readlane_f64(a[j0 + kk], K + c);
+                        for (int c = kk + 1; c < kPanel; ++c)
+                            a[j0 + c] -= a[j0 + kk] * readlane_f64(a[j0 + kk], K + c);
                     }
                     col[buf][kk][lane] = a[j0 + kk];  // zero for columns past the block
                 }
             }
             __syncthreads();
-            double lik[4];
+            double lik[kPanel];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) lik[kk] = col[buf][kk][lane];
+            for (int kk = 0; kk < kPanel; ++kk) lik[kk] = col[buf][kk][lane];
 #pragma unroll
             for (int j = 0; j < kPotrfCols; ++j) {
                 const int c = kPotrfCols * wv + j;
-                if (c > K + 3) {
+                if (c >= K + kPanel) {
 #pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) a[j] -= lik[kk] * col[buf][kk][c];
+                    for (int kk = 0; kk < kPanel; ++kk) a[j] -= lik[kk] * col[buf][kk][c];
                 }
             }
-            chol_panel<K + 4>(a, col, rdiag, wv, lane, b, amax, bad);
+            chol_panel<K + kPanel>(a, col, rdiag, wv, lane, b, amax, bad);
         }
     }
 }
@@ -1591,7 +1594,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
     const int b = min(kFb, w - kb), wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     MD_G double *P = A.P + S.poff;
     MD_G double *T = A.T + S.toff;
-    __shared__ double col[2][4][kFb];
+    __shared__ double col[2][kPanel][kFb];
     __shared__ double Lsh[kFb][kLdsPad];
     double a[kPotrfCols];
 #pragma unroll
