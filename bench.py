@@ -19,9 +19,9 @@ Legs:
   served  64 resident meshes (log-uniform 5k-164k vertices), Poisson drag requests at
           2000/s; each request moves its handles to a point of a seeded 4-keyframe SE3
           trajectory and runs 5 warm-started iterations; p50/p99 latency (arrival to result
-          on the host) and aggregate iterations/s; then a saturation sweep (4k-32k requests/s,
-          until p99 passes 16 ms: the capacity), compared with the same meshes iterated one by
-          one.
+          on the host) and aggregate iterations/s; then a saturation sweep (4k-32k requests/s
+          until p99 passes 16 ms, then three bisection steps: the capacity is the highest rate
+          measured within 16 ms), compared with the same meshes iterated one by one.
   frames  the reference's interactive loop, setConstraint + deform(5) per frame in the
           reference's rest mode (each frame re-runs the cotan weights and the refactorization),
           on the 67k bar (FP64) and the 164k icosphere (FP64, FP32 + refinement): frames/s
@@ -238,7 +238,8 @@ def served_stream(ctx, probs, meshes, utils, trajs, rng, rate, seconds, iters, b
 
 def served_leg(ctx, count, seconds, rate, iters, seed, sweep=(4000, 8000, 16000, 32000), p99_bound_ms=16.0):
     """The served workload at `rate` (the headline), then a saturation sweep over higher rates
-    until p99 exceeds the bound."""
+    until p99 exceeds the bound, then three bisection steps between the last rate that met the
+    bound and the first that did not (the capacity: the highest rate measured within it)."""
     rng = np.random.default_rng(seed)
     probs = served_problems(count, 5000, 164000, seed)
     meshes, utils, trajs = [], [], []
@@ -268,13 +269,28 @@ def served_leg(ctx, count, seconds, rate, iters, seed, sweep=(4000, 8000, 16000,
     plain_same = count * iters / (time.perf_counter() - t0)
     head = served_stream(ctx, probs, meshes, utils, trajs, rng, rate, seconds, iters)
     sweep_out, capacity = [], head if head["p99_ms"] < p99_bound_ms else None
+    lo, hi = (rate if capacity is not None else None), None
     for r in sweep:
         log("served sweep: %.0f requests/s" % r)
         x = served_stream(ctx, probs, meshes, utils, trajs, rng, r, min(seconds, 1.0), iters)
         sweep_out.append(x)
         if x["p99_ms"] >= p99_bound_ms:
+            hi = r
             break
-        capacity = x
+        capacity, lo = x, r
+    if lo is not None and hi is not None:
+        for _ in range(3):
+            r = round((lo + hi) / 2 / 250) * 250
+            if r <= lo or r >= hi:
+                break
+            log("served sweep (bisection): %.0f requests/s" % r)
+            x = served_stream(ctx, probs, meshes, utils, trajs, rng, r, min(seconds, 1.0), iters)
+            sweep_out.append(x)
+            if x["p99_ms"] >= p99_bound_ms:
+                hi = r
+            else:
+                capacity, lo = x, r
+        sweep_out.sort(key=lambda x: x["rate_per_s"])
     sizes = [p.n for p in probs]
     for m in meshes:
         m.close()

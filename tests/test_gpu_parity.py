@@ -300,6 +300,29 @@ def test_interactive_frames_match_oracle(ctx, golden_dir, rest_mode):
         assert err < POS_TOL_FP64, (frame, err)
 
 
+def test_fixed_rest_new_constraint_set_matches_oracle(ctx, golden_dir):
+    """Fixed rest mode with the constrained set changed between frames: the rest pose and its
+    weights stay those of the first call while the current mesh is the initial guess (the
+    library uploads the kept rest pose and the caller's positions separately in that case)."""
+    say("case: fixed rest, constrained set changed after the first frame")
+    g = gold(golden_dir, "sphere_obj")
+    xyz, tri = g["xyz"], g["tri"]
+    m = mdarap.Mesh(ctx, xyz, tri, mdarap.config(rest_mode=mdarap.REST_FIXED))
+    a = oracle.Arap(xyz, tri)
+    for obj in (m, a):
+        obj.set_constraint(37, xyz[37])
+        obj.set_constraint(32, xyz[32] + [0.0, 0.0, 0.3])
+    assert m.deform(5)
+    assert a.deform(5, rest_mode=1)
+    for frame, v in enumerate((10, 200, 400)):
+        for obj in (m, a):
+            obj.set_constraint(v, xyz[v] + [0.05 * (frame + 1), 0.0, 0.0])
+        assert m.deform(5)
+        assert a.deform(5, rest_mode=1)
+        err = meshgen.relative_error(m.mesh, a.mesh)
+        assert err < POS_TOL_FP64, (frame, err)
+
+
 # ---- full-size workloads ------------------------------------------------------------------
 
 @pytest.mark.parametrize("which,iters", [("bar", 50)])
