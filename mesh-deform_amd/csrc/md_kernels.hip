@@ -1487,8 +1487,9 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
 // The pivot block's Cholesky factor and its inverse X = L^{-1}, blocked by kFb columns
 // (right-looking): for each column block s, k_fac_potrf factors the diagonal block and inverts
 // it (the diagonal block of X, row-major in the T workspace), k_fac_trsm forms the rows below
-// it (times that inverse), k_fac_syrk updates the trailing lower blocks; then k_fac_trtri forms
-// the off-diagonal blocks of X one block diagonal at a time: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ.
+// it (times that inverse) and row block s of X, k_fac_syrk updates the trailing lower blocks and
+// accumulates the X blocks below: X_IJ = -X_II W_IJ with W_IJ = sum_{K=J}^{I-1} L_IK X_KJ, the
+// term K = s added at step s (in X's place), so W_sJ is complete when step s begins.
 constexpr int kFb = 64;
 
 // Diagonal block s, by four wavefronts (unrolled over the block's columns: a loop over them with
@@ -1843,154 +1844,166 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per)
             }
 }
 
-// Rows of row block s + 1 + (blockIdx.x % per) below diagonal block s, as a product with the
-// diagonal block's inverse that k_fac_potrf left in T: L(r, kb + j) = sum_t A(r, kb + t)
-// X(kb + j, kb + t) (L_below = A_below L_kk^{-T}). A(i, t) from the panel (column-major),
-// B(t, j) = X(kb + j, kb + t) (row-major X); the tile is read whole before it is written.
-__global__ __launch_bounds__(kFacBlock) void k_fac_trsm(FacArgs A, int h0, int s, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m, kb = s * kFb, r0 = (s + 1 + (int)(blockIdx.x % per)) * kFb;
-    if (r0 >= w) return;  // also kb + kFb <= r0 < w: the diagonal block is full
-    double *P = A.P + S.poff;
-    const double *X = A.T + S.toff;
-    __shared__ double As[kMk][kMpad];
-    __shared__ double Bs[kMk][kMpad];
-    MmaAcc acc;
-    mma_zero(acc);
-    mma_pipeline(
-        kFb / kMk,
-        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
-#pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k0 = c * kMk;
-                ra[i] = r0 + x < w ? P[(long long)(kb + k0 + e / 64) * m + r0 + x] : 0.0;
-                // B(t, j) = X(kb + j, kb + t): consecutive threads read consecutive t of one row
-                rb[i] = X[(long long)(kb + e / kMk) * w + kb + k0 + e % kMk];
-            }
-        },
-        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
-#pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i;
-                As[e / 64][e % 64] = ra[i];
-                Bs[e % kMk][e / kMk] = rb[i];
-            }
-        },
-        As, Bs, acc);
-    __syncthreads();  // every wave has staged its last reads of the tile before any write
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = r0 + mma_row(x, r), jj = kb + mma_col(y);
-                if (rr < w) P[(long long)jj * m + rr] = acc.c[x][y][r];
-            }
-}
-
-// Lower tile q = (I, J), s < J <= I, of the trailing block:
-// A(r, c) -= sum_{t<64} L(r, kb + t) L(c, kb + t).
-__global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s, int per) {
+// Step s of the right-looking loop, first launch (`per` tiles per supernode; tiles q < split
+// are row blocks, the others X blocks). Row block s + 1 + q below diagonal block s, as a product
+// with the diagonal block's inverse that k_fac_potrf left in T: L(r, kb + j) = sum_t A(r, kb + t)
+// X(kb + j, kb + t) (L_below = A_below L_ss^{-T}); A(i, t) from the panel (column-major), B(t, j)
+// = X(kb + j, kb + t) (row-major X). X block (s, J), J = q - split < s: X_sJ = -X_ss W_sJ, where
+// W_sJ = sum_{K=J}^{s-1} L_sK X_KJ, accumulated in X's place by the second launches of the
+// earlier steps. Each tile is read whole before it is written.
+__global__ __launch_bounds__(kFacBlock) void k_fac_trsm(FacArgs A, int h0, int s, int split, int per) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
-    const int nt = (w + kFb - 1) / kFb - 1 - s;  // row blocks below diagonal block s
-    if (q >= nt * (nt + 1) / 2) return;
-    int I, J;
-    lower_tile(q, I, J);
-    const int r0 = (s + 1 + I) * kFb, c0 = (s + 1 + J) * kFb;
-    double *P = A.P + S.poff;
+    MD_G double *P = A.P + S.poff;
+    MD_G double *T = A.T + S.toff;
     __shared__ double As[kMk][kMpad];
     __shared__ double Bs[kMk][kMpad];
     MmaAcc acc;
     mma_zero(acc);
-    mma_pipeline(
-        kFb / kMk,
-        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+    if (q < split) {
+        const int r0 = (s + 1 + q) * kFb;
+        if (r0 >= w) return;  // also kb + kFb <= r0 < w: the diagonal block is full
+        mma_pipeline(
+            kFb / kMk,
+            [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i, x = e % 64;
-                const long long col = (long long)(kb + c * kMk + e / 64) * m;
-                ra[i] = r0 + x < w ? P[col + r0 + x] : 0.0;
-                rb[i] = c0 + x < w ? P[col + c0 + x] : 0.0;
-            }
-        },
-        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i, x = e % 64, k0 = c * kMk;
+                    ra[i] = r0 + x < w ? P[(long long)(kb + k0 + e / 64) * m + r0 + x] : 0.0;
+                    // B(t, j) = X(kb + j, kb + t): consecutive threads read consecutive t of one row
+                    rb[i] = T[(long long)(kb + e / kMk) * w + kb + k0 + e % kMk];
+                }
+            },
+            [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i;
-                As[e / 64][e % 64] = ra[i];
-                Bs[e / 64][e % 64] = rb[i];
-            }
-        },
-        As, Bs, acc);
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
-                if (rr < w && cc < w && rr >= cc) P[(long long)cc * m + rr] -= acc.c[x][y][r];
-            }
-}
-
-// Off-diagonal blocks of X on block diagonal d: I = d + (blockIdx.x % per), J = I - d:
-// Y = sum_{K=J}^{I-1} L_IK X_KJ, then X_IJ = -X_II Y (Y staged through LDS 32 rows at a time).
-__global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int d, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m, I = d + (int)(blockIdx.x % per), J = I - d;
-    const int r0 = I * kFb, c0 = J * kFb;
-    if (r0 >= w) return;  // every block K in [J, I) is full
-    const double *P = A.P + S.poff;
-    double *T = A.T + S.toff;
-    __shared__ double As[kMk][kMpad];
-    __shared__ double Bs[kMk][kMpad];
-    MmaAcc acc;
-    mma_zero(acc);
-    // Y: A(i, k) = L(r0 + i, 64 K + k) (panel, column-major), B(k, j) = X(64 K + k, c0 + j)
-    // (chunk c: k = 64 J + kMk c + kk, over blocks K = J .. I - 1)
-    mma_pipeline(
-        (I - J) * (kFb / kMk),
-        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
-#pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = J * kFb + c * kMk + e / 64;
-                ra[i] = r0 + x < w ? P[(long long)k * m + r0 + x] : 0.0;
-                rb[i] = T[(long long)k * w + c0 + x];
-            }
-        },
-        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
-#pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i;
-                As[e / 64][e % 64] = ra[i];
-                Bs[e / 64][e % 64] = rb[i];
-            }
-        },
-        As, Bs, acc);
-    // X_IJ = -X_II Y: A(i, k) = X(r0 + i, r0 + k) (row-major), B(k, j) = Y(k, j)
-    MmaAcc out;
-    mma_zero(out);
-    for (int k0 = 0; k0 < kFb; k0 += kMk) {
-        __syncthreads();
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i;
+                    As[e / 64][e % 64] = ra[i];
+                    Bs[e % kMk][e / kMk] = rb[i];
+                }
+            },
+            As, Bs, acc);
+        __syncthreads();  // every wave has staged its last reads of the tile before any write
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int k = mma_row(x, r) - k0;
-                    if (k >= 0 && k < kMk) Bs[k][mma_col(y)] = acc.c[x][y][r];
+                    const int rr = r0 + mma_row(x, r), jj = kb + mma_col(y);
+                    if (rr < w) P[(long long)jj * m + rr] = acc.c[x][y][r];
                 }
-        for (int e = threadIdx.x; e < kMk * 64; e += kFacBlock) {
-            // consecutive threads read consecutive k of one row of X
-            const int kk = e % kMk, x = e / kMk;
-            As[kk][x] = (r0 + x < w && r0 + k0 + kk < w) ? T[(long long)(r0 + x) * w + r0 + k0 + kk] : 0.0;
-        }
-        __syncthreads();
-        mma_chunk(As, Bs, out);
+        return;
     }
+    const int J = q - split, c0 = J * kFb;
+    if (J >= s || kb >= w) return;  // block J < s is full
+    const int bs = min(kFb, w - kb);
+    mma_pipeline(
+        kFb / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, k0 = c * kMk;
+                // A(i, t) = X(kb + i, kb + t): consecutive threads read consecutive t of one row
+                const int ia = e / kMk, ta = k0 + e % kMk;
+                ra[i] = ia < bs && ta < bs ? T[(long long)(kb + ia) * w + kb + ta] : 0.0;
+                // B(t, j) = W(kb + t, c0 + j)
+                const int tb = k0 + e / 64;
+                rb[i] = tb < bs ? T[(long long)(kb + tb) * w + c0 + e % 64] : 0.0;
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e % kMk][e / kMk] = ra[i];
+                Bs[e / 64][e % 64] = rb[i];
+            }
+        },
+        As, Bs, acc);
+    __syncthreads();  // W_sJ is read whole before X_sJ replaces it
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = kb + mma_row(x, r), cc = c0 + mma_col(y);
+                if (rr < w) T[(long long)rr * w + cc] = -acc.c[x][y][r];
+            }
+}
+
+// Step s, second launch (`per` tiles per supernode; tiles q < split are trailing tiles, the
+// others W tiles). Lower trailing tile q = (I, J), s < J <= I: A(r, c) -= sum_{t<64} L(r, kb + t)
+// L(c, kb + t). W tile (I, J), I = s + 1 + (q - split) / (s + 1), J = (q - split) % (s + 1):
+// W_IJ += L_Is X_sJ (stored by the first contribution, J = s), for the X blocks of row block I.
+__global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s, int split, int per) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+    const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
+    MD_G double *P = A.P + S.poff;
+    MD_G double *T = A.T + S.toff;
+    __shared__ double As[kMk][kMpad];
+    __shared__ double Bs[kMk][kMpad];
+    MmaAcc acc;
+    mma_zero(acc);
+    if (q < split) {
+        const int nt = (w + kFb - 1) / kFb - 1 - s;  // row blocks below diagonal block s
+        if (q >= nt * (nt + 1) / 2) return;
+        int I, J;
+        lower_tile(q, I, J);
+        const int r0 = (s + 1 + I) * kFb, c0 = (s + 1 + J) * kFb;
+        mma_pipeline(
+            kFb / kMk,
+            [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i, x = e % 64;
+                    const long long col = (long long)(kb + c * kMk + e / 64) * m;
+                    ra[i] = r0 + x < w ? P[col + r0 + x] : 0.0;
+                    rb[i] = c0 + x < w ? P[col + c0 + x] : 0.0;
+                }
+            },
+            [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i;
+                    As[e / 64][e % 64] = ra[i];
+                    Bs[e / 64][e % 64] = rb[i];
+                }
+            },
+            As, Bs, acc);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
+                    if (rr < w && cc < w && rr >= cc) P[(long long)cc * m + rr] -= acc.c[x][y][r];
+                }
+        return;
+    }
+    const int qq = q - split, r0 = (s + 1 + qq / (s + 1)) * kFb, J = qq % (s + 1), c0 = J * kFb;
+    if (r0 >= w) return;  // also block s (and every block J <= s) is full
+    mma_pipeline(
+        kFb / kMk,
+        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i, x = e % 64, t = c * kMk + e / 64;
+                ra[i] = r0 + x < w ? P[(long long)(kb + t) * m + r0 + x] : 0.0;  // L(r0 + x, kb + t)
+                rb[i] = T[(long long)(kb + t) * w + c0 + x];                       // X(kb + t, c0 + x)
+            }
+        },
+        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + kFacBlock * i;
+                As[e / 64][e % 64] = ra[i];
+                Bs[e / 64][e % 64] = rb[i];
+            }
+        },
+        As, Bs, acc);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1998,7 +2011,10 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_trtri(FacArgs A, int h0, int 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
-                if (rr < w) T[(long long)rr * w + cc] = -out.c[x][y][r];
+                if (rr < w) {
+                    MD_G double *dst = T + (long long)rr * w + cc;
+                    *dst = (J == s ? 0.0 : *dst) + acc.c[x][y][r];
+                }
             }
 }
 
@@ -2050,17 +2066,18 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
                        int per_dinv, int per_g, int per_u) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm, asm_cols);
+    // per step b: the diagonal block, then its row blocks and X blocks (X_bJ, J < b), then the
+    // trailing tiles and the W tiles of the row blocks below (W_IJ, I > b, J <= b)
     for (int b = 0; b < nb; ++b) {
         hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
         const int below = nb - 1 - b;
-        if (below > 0) {
-            hipLaunchKernelGGL(k_fac_trsm, dim3(count * below), dim3(kFacBlock), 0, s, A, h0, b, below);
-            const int tiles = below * (below + 1) / 2;
-            hipLaunchKernelGGL(k_fac_syrk, dim3(count * tiles), dim3(kFacBlock), 0, s, A, h0, b, tiles);
-        }
+        if (below + b > 0)
+            hipLaunchKernelGGL(k_fac_trsm, dim3(count * (below + b)), dim3(kFacBlock), 0, s, A, h0, b, below, below + b);
+        const int tiles = below * (below + 1) / 2, wtiles = below * (b + 1);
+        if (tiles + wtiles > 0)
+            hipLaunchKernelGGL(k_fac_syrk, dim3(count * (tiles + wtiles)), dim3(kFacBlock), 0, s, A, h0, b, tiles,
+                               tiles + wtiles);
     }
-    for (int d = 1; d < nb; ++d)
-        hipLaunchKernelGGL(k_fac_trtri, dim3(count * (nb - d)), dim3(kFacBlock), 0, s, A, h0, d, nb - d);
     hipLaunchKernelGGL(k_fac_dinv, dim3(count * per_dinv), dim3(kFacBlock), 0, s, A, h0, per_dinv);
     if (per_g > 0) {
         hipLaunchKernelGGL(k_fac_g, dim3(count * per_g), dim3(kFacBlock), 0, s, A, h0, per_g);
