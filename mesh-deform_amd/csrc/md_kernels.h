@@ -108,6 +108,8 @@ struct FacArgs {
     MD_G double *P, *S, *T, *U, *diag;
     MD_G unsigned long long *amax;  // max |L_ii| as the bits of a non-negative double
     MD_G int *flag;                 // 1: a pivot block is not positive definite; 2: structure
+    MD_G void *pc, *pr;             // the sweeps' panels (float or double), written by the D^{-1}
+    int solve64;                    //   and G kernels in the solve precision (1: double)
 };
 
 // A set of meshes processed by one launch. Block b belongs to mesh k where
@@ -147,7 +149,8 @@ void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 // Device numeric factorization: diagonal and its maximum, then per height (the `count`
 // supernodes from hlist[h0]) the front assembly and the pivot block's Cholesky and inverse (one
 // workgroup per supernode), D^{-1}, G and the update matrix (per_* workgroups per supernode;
-// per_g = 0 when no supernode of the height has off rows), then the solve-precision panels.
+// per_g = 0 when no supernode of the height has off rows); D^{-1} and G also go to the
+// solve-precision panels.
 void launch_fac_diag(hipStream_t s, const FacArgs &A);
 // One height: assembly, blocked Cholesky and inverse of the pivot blocks (nb = column blocks
 // of 64 in the widest one), D^{-1}, G and the update matrices.
@@ -155,7 +158,6 @@ void launch_fac_diag(hipStream_t s, const FacArgs &A);
 // front columns; D^{-1}, G and U, one per 64 x 64 output tile (lower tiles for the symmetric ones).
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
                        int per_dinv, int per_g, int per_u);
-void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64);
 // Internal-order vectors (positions, normals) to packed xyz in original order (perm: original
 // id per internal id), and packed xyz to internal order (p and/or pp; pp takes the targets tgt,
 // internal order, for the constrained ids [nf, n)).

@@ -1749,6 +1749,15 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int p
                     const double v = acc.c[x][y][r];
                     P[(long long)cc * m + rr] = v;
                     P[(long long)rr * m + cc] = v;
+                    // the backward panel [D^{-1}; G] (row-major) in the solve precision
+                    const long long o1 = S.pr_off + (long long)rr * w + cc, o2 = S.pr_off + (long long)cc * w + rr;
+                    if (A.solve64) {
+                        static_cast<MD_G double *>(A.pr)[o1] = v;
+                        static_cast<MD_G double *>(A.pr)[o2] = v;
+                    } else {
+                        static_cast<MD_G float *>(A.pr)[o1] = (float)v;
+                        static_cast<MD_G float *>(A.pr)[o2] = (float)v;
+                    }
                 }
             }
 }
@@ -1795,7 +1804,19 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rr = r0 + mma_row(x, r), qq = q0 + mma_col(y);
-                if (rr < no && qq < w) P[(long long)qq * m + w + rr] = acc.c[x][y][r];
+                if (rr < no && qq < w) {
+                    const double v = acc.c[x][y][r];
+                    P[(long long)qq * m + w + rr] = v;
+                    // the forward panel G (column-major) and the backward panel's G rows
+                    const long long oc = S.pc_off + (long long)qq * no + rr, orr = S.pr_off + (long long)(w + rr) * w + qq;
+                    if (A.solve64) {
+                        static_cast<MD_G double *>(A.pc)[oc] = v;
+                        static_cast<MD_G double *>(A.pr)[orr] = v;
+                    } else {
+                        static_cast<MD_G float *>(A.pc)[oc] = (float)v;
+                        static_cast<MD_G float *>(A.pr)[orr] = (float)v;
+                    }
+                }
             }
 }
 
@@ -2018,38 +2039,6 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s
             }
 }
 
-// FP64 panel -> the sweeps' layouts in the solve precision: column-major G (noff x w) and
-// row-major [D^{-1}; G] (m x w). Workgroup (s, y) converts the 64 x 64 tiles y, y + gridDim.y,
-// ... of supernode s: read down the panel's columns, the row-major copy written across rows
-// from an LDS transpose.
-constexpr int kConvertSplit = 16;
-template <typename T>
-__global__ __launch_bounds__(kFacBlock) void k_fac_convert(FacArgs A, T *pc, T *pr) {
-    const FacDev S = A.sn[blockIdx.x];
-    const int w = S.w, m = S.m, no = S.noff;
-    const int np = (m + 63) / 64, nq = (w + 63) / 64;
-    const double *P = A.P + S.poff;
-    __shared__ T tile[64][65];
-    for (int t = blockIdx.y; t < np * nq; t += gridDim.y) {
-        const int p0 = (t % np) * 64, q0 = (t / np) * 64;
-        __syncthreads();
-        for (int e = threadIdx.x; e < 64 * 64; e += kFacBlock) {
-            const int qq = e / 64, pp = e % 64, p = p0 + pp, q = q0 + qq;
-            T v = 0;
-            if (p < m && q < w) {
-                v = (T)P[(long long)q * m + p];
-                if (p >= w) pc[S.pc_off + (long long)q * no + (p - w)] = v;
-            }
-            tile[qq][pp] = v;
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < 64 * 64; e += kFacBlock) {
-            const int pp = e / 64, qq = e % 64, p = p0 + pp, q = q0 + qq;
-            if (p < m && q < w) pr[S.pr_off + (long long)p * w + q] = tile[qq][pp];
-        }
-    }
-}
-
 // Plain device copy of n 16-byte elements (grid-stride): the known-size reference kernel for
 // HBM counters and the achievable copy bandwidth.
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ src, f4 *__restrict__ dst, long long n) {
@@ -2083,12 +2072,6 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
         hipLaunchKernelGGL(k_fac_g, dim3(count * per_g), dim3(kFacBlock), 0, s, A, h0, per_g);
         hipLaunchKernelGGL(k_fac_u, dim3(count * per_u), dim3(kFacBlock), 0, s, A, h0, per_u);
     }
-}
-void launch_fac_convert(hipStream_t s, const FacArgs &A, int ns, void *pc, void *pr, bool fp64) {
-    if (ns <= 0) return;
-    const dim3 grid(ns, kConvertSplit);
-    if (fp64) hipLaunchKernelGGL(k_fac_convert<double>, grid, dim3(kFacBlock), 0, s, A, (double *)pc, (double *)pr);
-    else hipLaunchKernelGGL(k_fac_convert<float>, grid, dim3(kFacBlock), 0, s, A, (float *)pc, (float *)pr);
 }
 
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16) {

@@ -861,6 +861,9 @@ int factor_device(md_mesh *m) {
     A.P = base; A.S = A.P + nP; A.T = A.S + nS; A.U = A.T + nT; A.diag = A.U + nU;
     A.amax = reinterpret_cast<unsigned long long *>(A.diag + nf);
     A.flag = reinterpret_cast<int *>(A.amax + 1);
+    A.pc = const_cast<void *>(m->dev.pc);
+    A.pr = const_cast<void *>(m->dev.pr);
+    A.solve64 = m->fp64() ? 1 : 0;
     hipError_t e = hipMemsetAsync(A.amax, 0, 2 * sizeof(unsigned long long), st);
     if (e == hipSuccess) {
         launch_fac_diag(st, A);
@@ -887,7 +890,6 @@ int factor_device(md_mesh *m) {
             launch_fac_height(st, A, f.hptr[h], count, (mw + 63) / 64, asm_cols, (mm + asm_cols - 1) / asm_cols,
                               per_dinv, per_g, per_u);
         }
-        launch_fac_convert(st, A, ns, const_cast<void *>(m->dev.pc), const_cast<void *>(m->dev.pr), m->fp64());
         e = hipGetLastError();
     }
     int flag = 0;
