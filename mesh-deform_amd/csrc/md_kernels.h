@@ -139,6 +139,13 @@ void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_e
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k);
+// A target of a served request: the device address of its vertex's p' entry and the position.
+struct alignas(16) TargetEntry {
+    unsigned long long addr;
+    double x, y, z;
+};
+// *entry.addr = (x, y, z) for k entries (targets of several meshes in one launch).
+void launch_scatter_entries(hipStream_t s, const TargetEntry *e, int k);
 void launch_normals(hipStream_t s, LaunchSet Lf, int blocks_face, LaunchSet Ls, int blocks_slice, double *fn, d4 *out);
 void launch_pcg_diag(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_pcg_init(hipStream_t s, LaunchSet L, int blocks, bool fp64);

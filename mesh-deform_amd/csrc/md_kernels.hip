@@ -1369,6 +1369,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *i
     pp[iv[t]] = v;
 }
 
+// Targets of several meshes: entry t writes its position into the p' entry at its address.
+__global__ __launch_bounds__(kBlock) void k_scatter_entries(const TargetEntry *e, int k) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= k) return;
+    const TargetEntry x = e[t];
+    *reinterpret_cast<MD_G d4 *>(x.addr) = d4{x.x, x.y, x.z, 0.0};
+}
+
 // ---- numeric factorization (arap.h:337, SimplicialLDLT::compute) ------------------------
 // Multifrontal block LDL^T in FP64, one height of the supernodal tree at a time, one
 // workgroup per supernode. Every supernode's front lives in its own workspace blocks; the
@@ -2117,6 +2125,9 @@ void launch_unpack_xyz(hipStream_t s, const double *xyz, const int *perm, int n,
                            pp);
 }
 
+void launch_scatter_entries(hipStream_t s, const TargetEntry *e, int k) {
+    if (k > 0) hipLaunchKernelGGL(k_scatter_entries, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, e, k);
+}
 void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {
     if (k > 0) hipLaunchKernelGGL(k_scatter_targets, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pp, iv, xyz, k);
 }

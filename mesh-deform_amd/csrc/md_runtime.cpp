@@ -48,9 +48,8 @@ void ctx_release(md_ctx *c) {
 
 Plan::~Plan() {
     drop_graph();
-    if (d_meshes) hipFree(d_meshes);
-    if (d_prefix) hipFree(d_prefix);
-    if (d_tbase) hipFree(d_tbase);
+    if (d_tables) hipFree(d_tables);
+    if (h_tables) hipHostFree(h_tables);
     if (d_bmesh) hipFree(d_bmesh);
 }
 
@@ -143,29 +142,29 @@ hipError_t build_plan(Plan &plan, const std::vector<const MeshDev *> &meshes,
     }
     plan.h_meshes.resize(na);
     for (int k = 0; k < na; ++k) plan.h_meshes[k] = *meshes[k];
-    if ((size_t)na > plan.cap_meshes) {
-        if (plan.d_meshes) hipFree(plan.d_meshes);
-        plan.d_meshes = nullptr;
-        MD_HIP_E(hipMalloc(&plan.d_meshes, sizeof(MeshDev) * (size_t)na));
-        plan.cap_meshes = na;
+    // the mesh records, prefix tables and tile bases in one buffer: staged in pinned host
+    // memory and uploaded with one asynchronous copy (the mirror is rewritten only after the
+    // stream has passed the previous upload: the callers synchronise or alternate two plans)
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_pre = up(sizeof(MeshDev) * (size_t)na), o_tb = o_pre + up(sizeof(int) * plan.h_prefix.size());
+    const size_t total = std::max<size_t>(256, o_tb + up(sizeof(int) * plan.h_tbase.size()));
+    if (total > plan.cap_tables) {
+        if (plan.d_tables) hipFree(plan.d_tables);
+        if (plan.h_tables) hipHostFree(plan.h_tables);
+        plan.d_tables = plan.h_tables = nullptr;
+        plan.cap_tables = 0;
+        const size_t cap = total + total / 2;
+        MD_HIP_E(hipMalloc((void **)&plan.d_tables, cap));
+        MD_HIP_E(hipHostMalloc((void **)&plan.h_tables, cap, hipHostMallocDefault));
+        plan.cap_tables = cap;
     }
-    if (plan.h_prefix.size() > plan.cap_prefix) {
-        if (plan.d_prefix) hipFree(plan.d_prefix);
-        if (plan.d_tbase) hipFree(plan.d_tbase);
-        plan.d_prefix = nullptr;
-        plan.d_tbase = nullptr;
-        MD_HIP_E(hipMalloc(&plan.d_prefix, sizeof(int) * plan.h_prefix.size()));
-        MD_HIP_E(hipMalloc(&plan.d_tbase, sizeof(int) * plan.h_prefix.size()));
-        plan.cap_prefix = plan.h_prefix.size();
-    }
-    if (na > 0) MD_HIP_E(hipMemcpyAsync(plan.d_meshes, plan.h_meshes.data(), sizeof(MeshDev) * na,
-                                        hipMemcpyHostToDevice, stream));
-    if (!plan.h_prefix.empty()) {
-        MD_HIP_E(hipMemcpyAsync(plan.d_prefix, plan.h_prefix.data(), sizeof(int) * plan.h_prefix.size(),
-                                hipMemcpyHostToDevice, stream));
-        MD_HIP_E(hipMemcpyAsync(plan.d_tbase, plan.h_tbase.data(), sizeof(int) * plan.h_tbase.size(),
-                                hipMemcpyHostToDevice, stream));
-    }
+    std::memcpy(plan.h_tables, plan.h_meshes.data(), sizeof(MeshDev) * (size_t)na);
+    std::memcpy(plan.h_tables + o_pre, plan.h_prefix.data(), sizeof(int) * plan.h_prefix.size());
+    std::memcpy(plan.h_tables + o_tb, plan.h_tbase.data(), sizeof(int) * plan.h_tbase.size());
+    plan.d_meshes = reinterpret_cast<MeshDev *>(plan.d_tables);
+    plan.d_prefix = reinterpret_cast<int *>(plan.d_tables + o_pre);
+    plan.d_tbase = reinterpret_cast<int *>(plan.d_tables + o_tb);
+    MD_HIP_E(hipMemcpyAsync(plan.d_tables, plan.h_tables, total, hipMemcpyHostToDevice, stream));
     return sync ? hipStreamSynchronize(stream) : hipSuccess;
 }
 

@@ -56,11 +56,14 @@ struct Plan {
     int sub_lds_fwd = 0, sub_lds_bwd = 0;  // LDS vector entries of the largest bottom subtree
     int sub_nodes = 0;                 // supernode records of the largest bottom subtree
     std::vector<int> off_fwd, off_bwd;
-    MeshDev *d_meshes = nullptr;
-    int *d_prefix = nullptr;
-    int *d_tbase = nullptr;            // per table and mesh: first tile of the level (level sweeps)
+    MeshDev *d_meshes = nullptr;       // views into d_tables: the mesh records,
+    int *d_prefix = nullptr;           //   the prefix tables,
+    int *d_tbase = nullptr;            //   per table and mesh: first tile of the level (level sweeps)
+    char *d_tables = nullptr;          // one device buffer for the three, uploaded in one copy
+    char *h_tables = nullptr;          //   from this pinned mirror
+    size_t cap_tables = 0;
     int *d_bmesh = nullptr;            // block -> mesh maps of every table (static batches)
-    size_t cap_meshes = 0, cap_prefix = 0, cap_bmesh = 0;
+    size_t cap_bmesh = 0;
     std::vector<int> h_bmesh;
     std::vector<size_t> bmesh_off;     // per table: offset into d_bmesh (SIZE_MAX: no map)
     std::vector<int> h_prefix, h_tbase;

@@ -221,7 +221,6 @@ struct Request {
     bool done = false;
     int status = MD_OK;
     std::string error;
-    std::vector<int> iv;          // internal ids of the moved targets (upload source)
     d4 *pinned = nullptr;         // positions in internal order, copied at completion
     uint64_t finish_seq = 0;      // launch after which the copy of its positions is complete
 };
@@ -231,6 +230,7 @@ struct md_server {
     md_ctx *ctx = nullptr;
     md_server_config cfg{};
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // result copies (default priority, like every stream here)
     std::vector<std::unique_ptr<ServedMesh>> meshes;  // guarded by mu; the entries never move
     size_t resident = 0, peak_resident = 0;
     uint64_t lru_clock = 0;
@@ -247,8 +247,18 @@ struct md_server {
     std::vector<std::shared_ptr<Request>> finishing;  // positions being copied
     Plan plans[2];
     std::vector<std::pair<int, uint64_t>> keys[2];    // (mesh id, generation) of each plan's set
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};            // after launch seq (by parity)
+    hipEvent_t evc[2] = {nullptr, nullptr};           // after the result copies of launch seq
+    hipEvent_t ev_flush = nullptr;                    // copies at a boundary without a launch
     uint64_t seq = 0;                                 // launches so far
+    uint64_t boundaries = 0;                          // iteration boundaries so far
+    // targets of the requests admitted at a boundary: pinned staging by boundary parity (the
+    // copy from the buffer of boundary b has completed when boundary b + 2 writes it), one
+    // device buffer (stream-ordered)
+    TargetEntry *tgt_host[2] = {nullptr, nullptr};
+    size_t tgt_cap_host[2] = {0, 0};
+    TargetEntry *tgt_dev = nullptr;
+    size_t tgt_cap_dev = 0;
     std::vector<d4 *> pinned_all;                     // guarded by mu
 
     d4 *take_pinned(ServedMesh *sm) {  // caller holds mu
@@ -268,26 +278,56 @@ struct md_server {
 
 namespace {
 
-// Pastes the request's targets into p' (fixed rest pose: the previous result is the initial
-// guess): one upload and one scatter launch, stream-ordered, no host synchronisation. The
-// sources live in the request until it completes.
-hipError_t apply_targets(md_mesh *m, Request &r, hipStream_t st, std::string &what) {
-    int k = (int)r.idx.size();
+// Pastes the targets of the requests admitted at this boundary into their meshes' p' (fixed
+// rest pose: the previous result is the initial guess): the entries (address of p'_i, target)
+// of every request gathered into this boundary's pinned staging buffer, one upload and one
+// scatter launch, stream-ordered, no host synchronisation.
+hipError_t apply_targets(md_server *S, const std::vector<std::shared_ptr<Request>> &reqs, hipStream_t st,
+                         std::string &what) {
+    size_t k = 0;
+    for (auto &r : reqs) k += r->idx.size();
     if (k == 0) return hipSuccess;
-    r.iv.resize(k);
-    for (int i = 0; i < k; ++i) {
-        r.iv[i] = m->topo.internal_of[r.idx[i]];
-        std::memcpy(&m->con[3 * (size_t)r.idx[i]], &r.xyz[3 * (size_t)i], 3 * sizeof(double));
+    const int par = (int)(S->boundaries & 1);
+    if (S->tgt_cap_host[par] < k) {
+        what = "pinned target staging";
+        if (S->tgt_host[par]) hipHostFree(S->tgt_host[par]);
+        S->tgt_host[par] = nullptr;
+        S->tgt_cap_host[par] = 0;
+        const size_t cap = k + k / 2;
+        if (hipError_t e = hipHostMalloc((void **)&S->tgt_host[par], sizeof(TargetEntry) * cap, hipHostMallocDefault))
+            return e;
+        S->tgt_cap_host[par] = cap;
     }
-    what = "target index upload";
-    hipError_t e = hipMemcpyAsync(m->d_tgt_idx, r.iv.data(), sizeof(int) * k, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        what = "target position upload";
-        e = hipMemcpyAsync(m->d_tgt_xyz, r.xyz.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, st);
+    if (S->tgt_cap_dev < k) {
+        // the device buffer is read by earlier scatter launches on this stream only
+        what = "device target staging";
+        if (hipError_t e = hipStreamSynchronize(st)) return e;
+        if (S->tgt_dev) hipFree(S->tgt_dev);
+        S->tgt_dev = nullptr;
+        S->tgt_cap_dev = 0;
+        const size_t cap = k + k / 2;
+        if (hipError_t e = hipMalloc((void **)&S->tgt_dev, sizeof(TargetEntry) * cap)) return e;
+        S->tgt_cap_dev = cap;
     }
+    TargetEntry *h = S->tgt_host[par];
+    size_t n = 0;
+    for (auto &r : reqs) {
+        md_mesh *m = r->sm->m;
+        for (size_t i = 0; i < r->idx.size(); ++i, ++n) {
+            const int v = r->idx[i];
+            const double *x = &r->xyz[3 * i];
+            std::memcpy(&m->con[3 * (size_t)v], x, 3 * sizeof(double));
+            h[n].addr = reinterpret_cast<unsigned long long>(m->dev.pp + m->topo.internal_of[v]);
+            h[n].x = x[0];
+            h[n].y = x[1];
+            h[n].z = x[2];
+        }
+    }
+    what = "target upload";
+    hipError_t e = hipMemcpyAsync(S->tgt_dev, h, sizeof(TargetEntry) * k, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         what = "target scatter launch";
-        launch_scatter_targets(st, m->dev.pp, m->d_tgt_idx, m->d_tgt_xyz, k);
+        launch_scatter_entries(st, S->tgt_dev, (int)k);
         e = hipGetLastError();
     }
     return e;
@@ -359,10 +399,17 @@ void complete(md_server *S, std::vector<std::shared_ptr<Request>> &done) {
     S->cv_done.notify_all();
 }
 
-// Waits for the launches up to `upto` (all queued work when upto is 0) and completes the
-// requests whose positions have arrived.
+// Waits for the launches up to `upto` and the copies of their results (all queued work of both
+// streams when upto is 0) and completes the requests whose positions have arrived.
 void drain(md_server *S, uint64_t upto, hipError_t &err) {
-    hipError_t e = upto ? hipEventSynchronize(S->ev[upto % 2]) : hipStreamSynchronize(S->stream);
+    hipError_t e = hipSuccess;
+    if (upto) {
+        e = hipEventSynchronize(S->ev[upto % 2]);
+        if (e == hipSuccess) e = hipEventSynchronize(S->evc[upto % 2]);
+    } else {
+        e = hipStreamSynchronize(S->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+    }
     if (e != hipSuccess && err == hipSuccess) err = e;
     std::vector<std::shared_ptr<Request>> done;
     for (auto &r : S->finishing)
@@ -432,15 +479,20 @@ void server_loop(md_server *S) {
                     if (r->sm == sm) fail(*r, rc, "restoring the mesh: " + msg);
             }
         }
-        for (auto &r : admitted) {
-            if (r->status != MD_OK) continue;
-            if (r->sm->m->evicted) {  // its restoration did not happen: nothing may launch on it
-                fail(*r, MD_ERR_STATE, "the mesh is not resident");
-                continue;
+        {
+            std::vector<std::shared_ptr<Request>> moving;
+            for (auto &r : admitted) {
+                if (r->status != MD_OK) continue;
+                if (r->sm->m->evicted) {  // its restoration did not happen: nothing may launch on it
+                    fail(*r, MD_ERR_STATE, "the mesh is not resident");
+                    continue;
+                }
+                moving.push_back(r);
             }
             std::string what;
-            hipError_t x = apply_targets(r->sm->m, *r, st, what);
-            if (x != hipSuccess) fail(*r, MD_ERR_HIP, what + " failed: " + hipGetErrorString(x));
+            hipError_t x = apply_targets(S, moving, st, what);
+            if (x != hipSuccess)
+                for (auto &r : moving) fail(*r, MD_ERR_HIP, what + " failed: " + hipGetErrorString(x));
         }
         // one iteration for every request with iterations left, in the same launches
         std::vector<std::shared_ptr<Request>> it_set;
@@ -497,15 +549,28 @@ void server_loop(md_server *S) {
                 if (r->status == MD_OK && !(r->pinned = S->take_pinned(r->sm)))
                     fail(*r, MD_ERR_NOMEM, "pinned result buffer");
         }
+        // the positions go to the pinned buffers on the copy stream, after this boundary's
+        // launch (or, when nothing was launched, after everything queued so far), so the copies
+        // overlap the next launches; a mesh is not iterated again before its result is delivered
+        if (launched) hipEventRecord(S->ev[S->seq % 2], st);
+        bool waited = false;
         for (auto &r : ending) {
             if (r->status == MD_OK) {
+                if (!waited) {
+                    hipEvent_t after = launched ? S->ev[S->seq % 2] : S->ev_flush;
+                    if (!launched) hipEventRecord(after, st);
+                    hipStreamWaitEvent(S->copy_stream, after, 0);
+                    waited = true;
+                }
                 md_mesh *m = r->sm->m;
-                hipError_t x = hipMemcpyAsync(r->pinned, m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost, st);
+                hipError_t x = hipMemcpyAsync(r->pinned, m->dev.pp, sizeof(d4) * m->n, hipMemcpyDeviceToHost,
+                                              S->copy_stream);
                 if (x != hipSuccess) fail(*r, MD_ERR_HIP, hipGetErrorString(x));
             }
             r->finish_seq = launched ? S->seq : 0;
             S->finishing.push_back(r);
         }
+        if (launched) hipEventRecord(S->evc[S->seq % 2], S->copy_stream);
         S->active.erase(std::remove_if(S->active.begin(), S->active.end(),
                                        [](const std::shared_ptr<Request> &r) { return r->left == 0; }),
                         S->active.end());
@@ -513,7 +578,6 @@ void server_loop(md_server *S) {
         // queued work when nothing was launched
         hipError_t err = hipSuccess;
         if (launched) {
-            hipEventRecord(S->ev[S->seq % 2], st);
             if (S->seq > 1) drain(S, S->seq - 1, err);
             else if (S->seq == 1) {
                 // first launch: nothing earlier to wait for, except copies of requests that
@@ -528,6 +592,7 @@ void server_loop(md_server *S) {
         if (err != hipSuccess) {
             for (auto &r : S->active) fail(*r, MD_ERR_HIP, hipGetErrorString(err));
         }
+        ++S->boundaries;
     }
 }
 
@@ -540,10 +605,18 @@ extern "C" int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_ser
     if (cfg) S->cfg = *cfg;
     hipSetDevice(ctx->device);
     if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S->evc[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S->evc[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S->ev_flush, hipEventDisableTiming) != hipSuccess) {
         for (hipEvent_t &x : S->ev)
             if (x) hipEventDestroy(x);
+        for (hipEvent_t &x : S->evc)
+            if (x) hipEventDestroy(x);
+        if (S->ev_flush) hipEventDestroy(S->ev_flush);
+        if (S->copy_stream) hipStreamDestroy(S->copy_stream);
         if (S->stream) hipStreamDestroy(S->stream);
         set_error("md_server_create: stream or event creation failed");
         return MD_ERR_HIP;
@@ -720,10 +793,17 @@ extern "C" int md_server_destroy(md_server *S) {
     md_ctx *ctx = S->ctx;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(S->stream);
+    hipStreamSynchronize(S->copy_stream);
     for (d4 *p : S->pinned_all) hipHostFree(p);
+    for (TargetEntry *p : S->tgt_host)
+        if (p) hipHostFree(p);
+    if (S->tgt_dev) hipFree(S->tgt_dev);
     // meshes evicted by the server stay on the host until their next use (md_mesh_* restores)
     for (auto &x : S->meshes) --x->m->servers;
     for (hipEvent_t &x : S->ev) hipEventDestroy(x);
+    for (hipEvent_t &x : S->evc) hipEventDestroy(x);
+    hipEventDestroy(S->ev_flush);
+    hipStreamDestroy(S->copy_stream);
     hipStreamDestroy(S->stream);
     delete S;
     ctx_release(ctx);
