@@ -645,7 +645,16 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
     }
     // its preparation on the context's stream is complete before the server's stream reads it
     if (hipStreamSynchronize(S->ctx->stream) != hipSuccess) { set_error("md_server_add_mesh: synchronisation failed"); return MD_ERR_HIP; }
+    // one pinned result buffer now (pinned allocations take milliseconds: made at a boundary,
+    // they would stall the worker); more are made only while earlier results wait for their
+    // md_server_wait / md_server_poll
+    void *buf = nullptr;
+    if (hipHostMalloc(&buf, sizeof(d4) * (size_t)std::max(1, m->n), hipHostMallocDefault) != hipSuccess) {
+        set_error("md_server_add_mesh: pinned result buffer: allocation failed");
+        return MD_ERR_NOMEM;
+    }
     std::lock_guard<std::mutex> lk(S->mu);
+    S->pinned_all.push_back(static_cast<d4 *>(buf));
     if (!S->meshes.empty()) {
         md_mesh *m0 = S->meshes[0]->m;
         if (m0->cfg.precision != m->cfg.precision || m0->cfg.refine_steps != m->cfg.refine_steps ||
@@ -668,6 +677,7 @@ extern "C" int md_server_add_mesh(md_server *S, md_mesh *m, int *mesh_id) {
     sm->m = m;
     sm->id = (int)S->meshes.size();
     sm->bytes = bytes;
+    sm->pinned_free.push_back(static_cast<d4 *>(buf));
     S->meshes.push_back(std::move(sm));
     ++m->servers;
     *mesh_id = (int)S->meshes.size() - 1;
