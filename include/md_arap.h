@@ -244,7 +244,8 @@ int md_server_create(md_ctx *ctx, const md_server_config *cfg, md_server **out);
  * destroy the server before the mesh. */
 int md_server_add_mesh(md_server *srv, md_mesh *mesh, int *mesh_id);
 /* Queues a drag: move the k constrained vertices idx to xyz, then run `iterations`
- * warm-started iterations. Indices must already be constrained on that mesh. */
+ * warm-started iterations. Indices must already be constrained on that mesh; a vertex listed
+ * more than once takes its last target. */
 int md_server_submit(md_server *srv, int mesh_id, int k, const int *idx, const double *xyz,
                      int iterations, uint64_t *ticket);
 /* done = 1 when finished; positions (3n, original order, may be NULL) and the latency in

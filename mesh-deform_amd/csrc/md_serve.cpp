@@ -209,6 +209,8 @@ struct ServedMesh {
     bool busy = false;            // a request admitted and not completed
     uint64_t last_use = 0;        // admission stamp (least recently used eviction)
     std::vector<d4 *> pinned_free;  // reusable pinned result buffers
+    std::vector<uint32_t> stamp;    // per vertex: the last submission that moved it (dedup)
+    uint32_t stamp_gen = 0;
 };
 
 struct Request {
@@ -696,7 +698,6 @@ extern "C" int md_server_submit(md_server *S, int mesh_id, int k, const int *idx
         if (mesh_id < 0 || mesh_id >= (int)S->meshes.size()) { set_error("md_server_submit: unknown mesh"); return MD_ERR_INVALID; }
         ServedMesh *sm = S->meshes[mesh_id].get();
         md_mesh *m = sm->m;
-        if (k > m->n - m->topo.nf) { set_error("md_server_submit: more targets than constrained vertices"); return MD_ERR_INVALID; }
         for (int i = 0; i < k; ++i)
             if (idx[i] < 0 || idx[i] >= m->n || !m->is_con[idx[i]]) {
                 set_error("md_server_submit: vertex " + std::to_string(idx[i]) + " is not a constrained vertex of the mesh");
@@ -704,8 +705,21 @@ extern "C" int md_server_submit(md_server *S, int mesh_id, int k, const int *idx
             }
         r->ticket = S->next_ticket++;
         r->sm = sm;
-        r->idx.assign(idx, idx + k);
-        r->xyz.assign(xyz, xyz + 3 * (size_t)k);
+        // a vertex listed twice takes its last target (setConstraint called twice, arap.h:81-85);
+        // the kept targets are distinct, so their scatter has no conflicting writes
+        if (sm->stamp.size() != (size_t)m->n) sm->stamp.assign((size_t)m->n, 0);
+        if (++sm->stamp_gen == 0) {
+            std::fill(sm->stamp.begin(), sm->stamp.end(), 0u);
+            sm->stamp_gen = 1;
+        }
+        r->idx.reserve(k);
+        r->xyz.reserve(3 * (size_t)k);
+        for (int i = k - 1; i >= 0; --i) {
+            if (sm->stamp[idx[i]] == sm->stamp_gen) continue;
+            sm->stamp[idx[i]] = sm->stamp_gen;
+            r->idx.push_back(idx[i]);
+            r->xyz.insert(r->xyz.end(), xyz + 3 * (size_t)i, xyz + 3 * (size_t)i + 3);
+        }
         r->iters = r->left = iterations;
         r->t_submit = SClock::now();
         S->queue.push_back(r);

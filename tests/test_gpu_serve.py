@@ -248,4 +248,11 @@ def test_ticket_taken_once(ctx):
     tk = srv.submit(ids[k], h, t, 0)
     pos0, _ = srv.wait(tk, probs[k].n)
     np.testing.assert_array_equal(pos0[h], t)
+    # a vertex listed twice takes its last target (setConstraint called twice)
+    h2 = np.concatenate([h, h[:1]])
+    t2 = np.concatenate([t, t[:1] + 0.25])
+    tk = srv.submit(ids[k], h2, t2, 0)
+    pos2, _ = srv.wait(tk, probs[k].n)
+    np.testing.assert_array_equal(pos2[h[1:]], t[1:])
+    np.testing.assert_array_equal(pos2[h[0]], t[0] + 0.25)
     srv.close()
