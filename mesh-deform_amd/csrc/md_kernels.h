@@ -138,7 +138,6 @@ void launch_sub_fwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_e
 void launch_sub_bwd(hipStream_t s, LaunchSet L, int blocks, int nodes, int lds_entries, bool fp64);
 void launch_energy(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 void launch_identity_quat(hipStream_t s, LaunchSet L, int blocks, bool fp64);
-void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k);
 // A target of a served request: the device address of its vertex's p' entry and the position.
 struct alignas(16) TargetEntry {
     unsigned long long addr;

@@ -1361,14 +1361,6 @@ __global__ __launch_bounds__(kBlock) void k_unpack_xyz(const double *xyz, const 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter_targets(d4 *pp, const int *iv, const double *xyz, int k) {
-    int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= k) return;
-    d4 v;
-    v.x = xyz[3 * t]; v.y = xyz[3 * t + 1]; v.z = xyz[3 * t + 2]; v.w = 0.0;
-    pp[iv[t]] = v;
-}
-
 // Targets of several meshes: entry t writes its position into the p' entry at its address.
 __global__ __launch_bounds__(kBlock) void k_scatter_entries(const TargetEntry *e, int k) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -2127,9 +2119,6 @@ void launch_unpack_xyz(hipStream_t s, const double *xyz, const int *perm, int n,
 
 void launch_scatter_entries(hipStream_t s, const TargetEntry *e, int k) {
     if (k > 0) hipLaunchKernelGGL(k_scatter_entries, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, e, k);
-}
-void launch_scatter_targets(hipStream_t s, d4 *pp, const int *iv, const double *xyz, int k) {
-    if (k > 0) hipLaunchKernelGGL(k_scatter_targets, dim3((k + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pp, iv, xyz, k);
 }
 
 void launch_cotan_face(hipStream_t s, LaunchSet L, int blocks) {

@@ -517,7 +517,6 @@ int analyze_device(md_mesh *m) {
     size_t o_cpp = A.reserve(vsz * std::max(1, pcg ? nf : 1));
     size_t o_cap = A.reserve(vsz * std::max(1, pcg ? nf : 1));
     size_t o_cpt = A.reserve(vsz * (pcg ? 4 * (size_t)nblk : 1));
-    size_t o_ti = A.reserve(sizeof(int) * std::max(1, n - nf));
     size_t o_tx = A.reserve(sizeof(double) * 3 * std::max(1, n - nf));
     size_t o_orig = A.reserve(sizeof(int) * std::max(1, n));
     size_t o_xyz = A.reserve(sizeof(double) * 3 * std::max(1, n));
@@ -556,7 +555,6 @@ int analyze_device(md_mesh *m) {
     d.cg_x = A.at(o_cx); d.cg_p = A.at(o_cpp); d.cg_ap = A.at(o_cap); d.cg_part = A.at(o_cpt);
     d.nblk_free = nblk;
     d.pcg_tol2 = m->cfg.pcg_tol * m->cfg.pcg_tol;
-    m->d_tgt_idx = (int *)A.at(o_ti);
     m->d_tgt_xyz = (double *)A.at(o_tx);
     m->d_orig = (const int *)A.at(o_orig);
     m->d_xyz = (double *)A.at(o_xyz);
@@ -1138,7 +1136,7 @@ void rebind(md_mesh *m, const char *oa, size_t asz, char *na, const char *ol, si
     fix(d.pc); fix(d.pr); fix(d.ftile); fix(d.btile); fix(d.sub); fix(d.sub_h); fix(d.snode); fix(d.scp);
     fix(d.sfc); fix(d.sbo); fix(d.srow); fix(d.sfi); fix(d.lrows); fix(d.energy_part); fix(d.w32);
     fix(d.diag); fix(d.cg_x); fix(d.cg_p); fix(d.cg_ap); fix(d.cg_part);
-    fix(m->d_fac); fix(m->d_hlist); fix(m->d_offrows); fix(m->d_rel); fix(m->d_tgt_idx); fix(m->d_tgt_xyz);
+    fix(m->d_fac); fix(m->d_hlist); fix(m->d_offrows); fix(m->d_rel); fix(m->d_tgt_xyz);
     fix(m->d_orig); fix(m->d_xyz);
 }
 

@@ -190,8 +190,7 @@ struct md_mesh {
     double setup_ms_host = 0.0;
     std::vector<double> w_host;    // cotan weights on the host (factorization input)
     std::vector<char> pc_host, pr_host;  // device panel layouts staged on the host
-    int *d_tgt_idx = nullptr;      // staging for moved targets (ncon entries)
-    double *d_tgt_xyz = nullptr;
+    double *d_tgt_xyz = nullptr;   // targets of the constrained ids, internal order (3 (n - nf))
     const int *d_orig = nullptr;   // original id of each internal id (topo.original_of)
     double *d_xyz = nullptr;       // packed xyz in original order (3n): positions in and out
     std::vector<double> tgt_stage; // targets of the constrained ids in internal order (host
