@@ -164,6 +164,11 @@ void launch_fac_diag(hipStream_t s, const FacArgs &A);
 // front columns; D^{-1}, G and U, one per 64 x 64 output tile (lower tiles for the symmetric ones).
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
                        int per_dinv, int per_g, int per_u);
+// A height whose fronts all have m <= mm <= kFacSmallFront and w <= mw: every step of
+// launch_fac_height for one front in one wavefront (front in mm^2 + mm mw doubles of LDS: under
+// 64 KiB), without the FP64 panel workspace.
+constexpr int kFacSmallFront = 62;
+void launch_fac_small(hipStream_t s, const FacArgs &A, int h0, int count, int mm, int mw);
 // Internal-order vectors (positions, normals) to packed xyz in original order (perm: original
 // id per internal id), and packed xyz to internal order (p and/or pp; pp takes the targets tgt,
 // internal order, for the constrained ids [nf, n)).

@@ -2039,6 +2039,136 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s
             }
 }
 
+// A whole height of small fronts (every m = w + noff <= mm <= kSmallFront) in one launch, one
+// wavefront per front, the front in LDS (2 mm^2 doubles, sized by the launch): assembly (the
+// original entries, then the children's update matrices in child order, as k_fac_assemble adds
+// them; the front's row list staged in LDS for the searches), the pivot block's Cholesky
+// (right-looking), X = L^{-1} by columns (one lane per column), D^{-1} = X^T X, G = F_os D^{-1},
+// U = U - G F_os^T into the update workspace for the parent, and the two panel layouts in the
+// solve precision. The FP64 panel workspace is not used.
+constexpr int kSmallFront = 64;
+constexpr int kSmallBlock = 64;
+__global__ __launch_bounds__(kSmallBlock) void k_fac_small(FacArgs A, int h0, int mm, int mw) {
+    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
+    const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
+    if (m > mm || w > mw || mm > kSmallFront) {  // the host sends only heights of small fronts
+        if (tid == 0) *A.flag = 2;
+        return;
+    }
+    extern __shared__ double smem[];
+    double *Fs = smem;            // the front (lower, row stride mm), then L / D^{-1}
+    double *Xs = smem + mm * mm;  // m x w (row stride mw): X (rows < w), then G (rows >= w)
+    __shared__ int rows[kSmallFront];
+    for (int e = tid; e < no; e += kSmallBlock) rows[e] = A.offrows[S.roff + e];
+    for (int e = tid; e < m * mm; e += kSmallBlock) Fs[e] = 0.0;
+    __syncthreads();
+    // original entries, column by column (one lane per pivot column)
+    for (int k = tid; k < w; k += kSmallBlock) {
+        const int j = S.c0 + k, sl = j / kSlice, l = j % kSlice;
+        Fs[k * mm + k] = A.diag[j];
+        for (int q = A.slice_ptr[sl]; q < A.slice_ptr[sl + 1]; ++q) {
+            const long long g = (long long)q * kSlice + l;
+            const int i = A.sell_col[g];
+            if (i <= j || i >= A.nf) continue;
+            const double v = -A.w64[g];
+            if (i < S.c0 + w) {
+                Fs[(i - S.c0) * mm + k] += v;
+            } else {
+                int lo = 0, hi = no;  // first off row >= i
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rows[mid] < i) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < no && rows[lo] == i) Fs[(w + lo) * mm + k] += v;
+                else *A.flag = 2;
+            }
+        }
+    }
+    __syncthreads();
+    // children's update matrices in child order (distinct entries of one child land on distinct
+    // front entries)
+    for (int ci = S.cb; ci < S.ce; ++ci) {
+        const FacDev C = A.sn[A.child[ci]];
+        const int nc = C.noff;
+        const MD_G int *rel = A.rel + C.roff;
+        const MD_G double *Uc = A.U + C.uoff;
+        for (int e = tid; e < nc * nc; e += kSmallBlock) {
+            const int b = e / nc, a = e - b * nc;
+            if (a < b) continue;
+            const int ra = rel[a], rb = rel[b];
+            if (ra >= m || rb > ra) { *A.flag = 2; continue; }
+            Fs[ra * mm + rb] += Uc[(long long)b * nc + a];
+        }
+        __syncthreads();
+    }
+    // Cholesky of the pivot block
+    const double amax = __longlong_as_double((long long)*A.amax);
+    for (int k = 0; k < w; ++k) {
+        const double piv = Fs[k * mm + k];
+        if (tid == 0 && !(piv > 1e-14 * amax)) *A.flag = 1;
+        const double d = sqrt(piv), rd = 1.0 / d;
+        __syncthreads();  // every lane has read the pivot before it changes
+        if (tid == 0) Fs[k * mm + k] = d;
+        for (int i = k + 1 + tid; i < w; i += kSmallBlock) Fs[i * mm + k] *= rd;
+        __syncthreads();
+        const int t = w - k - 1;  // trailing lower triangle, t x t
+        for (int e = tid; e < t * t; e += kSmallBlock) {
+            const int i = k + 1 + e / t, j = k + 1 + e % t;
+            if (j <= i) Fs[i * mm + j] -= Fs[i * mm + k] * Fs[j * mm + k];
+        }
+        __syncthreads();
+    }
+    // X = L^{-1}, column i by lane i: X(r, i) = (delta_ri - sum_{i<=t<r} L(r, t) X(t, i)) / L(r, r)
+    for (int i = tid; i < w; i += kSmallBlock) {
+        for (int r = 0; r < i; ++r) Xs[r * mw + i] = 0.0;
+        for (int r = i; r < w; ++r) {
+            double acc = r == i ? 1.0 : 0.0;
+            for (int t = i; t < r; ++t) acc -= Fs[r * mm + t] * Xs[t * mw + i];
+            Xs[r * mw + i] = acc / Fs[r * mm + r];
+        }
+    }
+    __syncthreads();
+    // D^{-1}(r, c) = sum_{k >= max(r, c)} X(k, r) X(k, c), over the pivot block (L is done)
+    for (int e = tid; e < w * w; e += kSmallBlock) {
+        const int r = e / w, c = e % w;
+        double acc = 0.0;
+        for (int k = max(r, c); k < w; ++k) acc += Xs[k * mw + r] * Xs[k * mw + c];
+        Fs[r * mm + c] = acc;  // the upper part too: D^{-1} is symmetric
+    }
+    __syncthreads();
+    // G(r, c) = sum_k F_os(r, k) D^{-1}(k, c) into Xs rows w + r (X is no longer needed)
+    for (int e = tid; e < no * w; e += kSmallBlock) {
+        const int r = e / w, c = e % w;
+        double acc = 0.0;
+        for (int k = 0; k < w; ++k) acc += Fs[(w + r) * mm + k] * Fs[k * mm + c];
+        Xs[(w + r) * mw + c] = acc;
+    }
+    __syncthreads();
+    // U(a, b) = U_front(a, b) - sum_k G(a, k) F_os(b, k), lower part, column-major for the parent
+    MD_G double *U = A.U + S.uoff;
+    for (int e = tid; e < no * no; e += kSmallBlock) {
+        const int b = e / no, a = e - b * no;
+        if (a < b) continue;
+        double acc = Fs[(w + a) * mm + w + b];
+        for (int k = 0; k < w; ++k) acc -= Xs[(w + a) * mw + k] * Fs[(w + b) * mm + k];
+        U[(long long)b * no + a] = acc;
+    }
+    // the sweeps' panels: row-major [D^{-1}; G] (m x w) and column-major G (no x w)
+    for (int e = tid; e < m * w; e += kSmallBlock) {
+        const int r = e / w, c = e % w;
+        const double v = r < w ? Fs[r * mm + c] : Xs[r * mw + c];
+        if (A.solve64) static_cast<MD_G double *>(A.pr)[S.pr_off + e] = v;
+        else static_cast<MD_G float *>(A.pr)[S.pr_off + e] = (float)v;
+    }
+    for (int e = tid; e < no * w; e += kSmallBlock) {
+        const int c = e / no, r = e - c * no;
+        const double v = Xs[(w + r) * mw + c];
+        if (A.solve64) static_cast<MD_G double *>(A.pc)[S.pc_off + e] = v;
+        else static_cast<MD_G float *>(A.pc)[S.pc_off + e] = (float)v;
+    }
+}
+
 // Plain device copy of n 16-byte elements (grid-stride): the known-size reference kernel for
 // HBM counters and the achievable copy bandwidth.
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ src, f4 *__restrict__ dst, long long n) {
@@ -2050,6 +2180,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const f4 *__restrict__ sr
 
 void launch_fac_diag(hipStream_t s, const FacArgs &A) {
     if (A.nf > 0) hipLaunchKernelGGL(k_fac_diag, dim3((A.nf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A);
+}
+void launch_fac_small(hipStream_t s, const FacArgs &A, int h0, int count, int mm, int mw) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_fac_small, dim3(count), dim3(kSmallBlock), sizeof(double) * (mm * mm + mm * mw), s, A, h0,
+                           mm, mw);
 }
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
                        int per_dinv, int per_g, int per_u) {

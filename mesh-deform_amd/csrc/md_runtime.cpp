@@ -882,6 +882,10 @@ int factor_device(md_mesh *m) {
                 per_g = std::max(per_g, bo * bw);
                 per_u = std::max(per_u, lower(bo));
             }
+            if (mm <= kFacSmallFront) {  // small fronts only: one fused launch
+                launch_fac_small(st, A, f.hptr[h], count, mm, mw);
+                continue;
+            }
             int asm_cols = 64;
             while (asm_cols > 8 && (long long)count * ((mm + asm_cols - 1) / asm_cols) < 2048) asm_cols /= 2;
             launch_fac_height(st, A, f.hptr[h], count, (mw + 63) / 64, asm_cols, (mm + asm_cols - 1) / asm_cols,
