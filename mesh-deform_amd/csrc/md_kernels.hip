@@ -1376,6 +1376,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_entries(const TargetEntry *e
 
 constexpr int kFacBlock = 256;
 constexpr int kAsmU = 8;      // child update entries per assembly thread in flight
+constexpr int kAsmRows = 2048;  // off rows of a front searched in LDS (longer lists: in memory)
 
 // L_jj = sum of the weights of every neighbour of j, in slot order, and their maximum.
 __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
@@ -1402,6 +1403,12 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     const int t0 = (int)(blockIdx.x % per) * cols, t1 = min(m, t0 + cols);
     if (t0 >= m) return;
     MD_G double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
+    const MD_G int *rows = A.offrows + S.roff;
+    // the front's row list in LDS for the searches of the original entries (pivot columns)
+    __shared__ int rsh[kAsmRows];
+    const bool staged = t0 < w && no <= kAsmRows;
+    if (staged)
+        for (int e = tid; e < no; e += kFacBlock) rsh[e] = rows[e];
     for (int t = t0; t < t1; ++t) {
         if (t < w) {
             for (int i = tid; i < m; i += kFacBlock) P[(long long)t * m + i] = 0.0;
@@ -1411,7 +1418,6 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
         }
     }
     __syncthreads();
-    const int *rows = A.offrows + S.roff;
     for (int k = t0 + tid; k < min(t1, w); k += kFacBlock) {
         const int j = S.c0 + k, sl = j / kSlice, l = j % kSlice;
         P[(long long)k * m + k] = A.diag[j];
@@ -1426,10 +1432,10 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
                 int lo = 0, hi = no;  // first off row >= i
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
-                    if (rows[mid] < i) lo = mid + 1;
+                    if ((staged ? rsh[mid] : rows[mid]) < i) lo = mid + 1;
                     else hi = mid;
                 }
-                if (lo < no && rows[lo] == i) F[(long long)k * no + lo] += v;
+                if (lo < no && (staged ? rsh[lo] : rows[lo]) == i) F[(long long)k * no + lo] += v;
                 else *A.flag = 2;
             }
         }
