@@ -83,8 +83,10 @@ struct MeshDev {
     // Jacobi-PCG alternative to the triangular sweeps (solve type vectors, nf entries)
     MD_G float *w32;            // slots: cotan weights in FP32
     MD_G void *diag;            // nf: L_ii = sum_j w_ij (solve type scalar)
-    MD_G void *cg_x, *cg_p, *cg_ap; // correction, direction, A * direction (V4)
-    MD_G void *cg_part;         // 3 x nblk partial dot products (V4): p.Ap, r.z (two buffers)
+    MD_G void *cg_x, *cg_p;     // correction, direction (V4)
+    MD_G void *cg_v;            // 5 vectors (V4): r (second buffer; the first is r), w and s (two each)
+    MD_G void *cg_part;         // 4 nblk + 5 (V4): partials of (r, u) and (w, u) for two parities,
+                                //   then (gamma, alpha) for two parities and the first gamma
     int nblk_free;              // 256-vertex blocks covering [0, nf)
     double pcg_tol2;            // stop (freeze) when r.z <= tol^2 * (r.z)_0, per coordinate
 };
@@ -147,10 +149,9 @@ struct alignas(16) TargetEntry {
 void launch_scatter_entries(hipStream_t s, const TargetEntry *e, int k);
 void launch_normals(hipStream_t s, LaunchSet Lf, int blocks_face, LaunchSet Ls, int blocks_slice, double *fn, d4 *out);
 void launch_pcg_diag(hipStream_t s, LaunchSet L, int blocks, bool fp64);
+// Jacobi-PCG in the Chronopoulos-Gear form: one launch per CG iteration (step i).
 void launch_pcg_init(hipStream_t s, LaunchSet L, int blocks, bool fp64);
-void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64);
-void launch_pcg_update(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
-void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64);
+void launch_pcg_step(hipStream_t s, LaunchSet L, int blocks, int step, bool fp64);
 void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 // Device numeric factorization: diagonal and its maximum, then per height (the `count`
 // supernodes from hlist[h0]) the front assembly and the pivot block's Cholesky and inverse (one

@@ -1045,46 +1045,58 @@ template <typename T> __device__ __forceinline__ const T *wts(const MD_G MeshDev
 template <> __device__ __forceinline__ const float *wts<float>(const MD_G MeshDev &M) { return M.w32; }
 template <> __device__ __forceinline__ const double *wts<double>(const MD_G MeshDev &M) { return M.w64; }
 
-// Sums the 3-component partials part[0 .. nb) in a fixed order; result broadcast via LDS.
+// Two block sums at once (six values in one tree): a into pa[blk], b into pb[blk].
 template <typename T, typename V>
-__device__ __forceinline__ void reduce_partials(const V *part, int nb, T (&out)[3], T (*red)[kBlock]) {
-    T a = 0, b = 0, c = 0;
-    for (int k = threadIdx.x; k < nb; k += kBlock) {
-        V v = part[k];
-        a += v.x; b += v.y; c += v.z;
+__device__ __forceinline__ void block_partial2(const T (&a)[3], const T (&b)[3], MD_G V *pa, MD_G V *pb, int blk,
+                                               T (*red)[kBlock]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        red[q][threadIdx.x] = a[q];
+        red[3 + q][threadIdx.x] = b[q];
     }
-    red[0][threadIdx.x] = a; red[1][threadIdx.x] = b; red[2][threadIdx.x] = c;
     __syncthreads();
     for (int o = kBlock / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + o];
-            red[1][threadIdx.x] += red[1][threadIdx.x + o];
-            red[2][threadIdx.x] += red[2][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    out[0] = red[0][0]; out[1] = red[1][0]; out[2] = red[2][0];
-    __syncthreads();
-}
-
-// Block-wide sum of a 3-vector into part[blk].
-template <typename T, typename V>
-__device__ __forceinline__ void block_partial(T a, T b, T c, V *part, int blk, T (*red)[kBlock]) {
-    red[0][threadIdx.x] = a; red[1][threadIdx.x] = b; red[2][threadIdx.x] = c;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + o];
-            red[1][threadIdx.x] += red[1][threadIdx.x + o];
-            red[2][threadIdx.x] += red[2][threadIdx.x + o];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        V v;
-        v.x = red[0][0]; v.y = red[1][0]; v.z = red[2][0]; v.w = 0;
-        part[blk] = v;
+        V va, vb;
+        va.x = red[0][0]; va.y = red[1][0]; va.z = red[2][0]; va.w = 0;
+        vb.x = red[3][0]; vb.y = red[4][0]; vb.z = red[5][0]; vb.w = 0;
+        pa[blk] = va;
+        pb[blk] = vb;
     }
+}
+
+// Two partial arrays summed at once (fixed order); the results broadcast through LDS.
+template <typename T, typename V>
+__device__ __forceinline__ void reduce_partials2(const MD_G V *pa, const MD_G V *pb, int nb, T (&oa)[3], T (&ob)[3],
+                                                 T (*red)[kBlock]) {
+    T a[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < nb; k += kBlock) {
+        const V x = pa[k], y = pb[k];
+        a[0] += x.x; a[1] += x.y; a[2] += x.z;
+        a[3] += y.x; a[4] += y.y; a[5] += y.z;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        oa[q] = red[q][0];
+        ob[q] = red[3 + q][0];
+    }
+    __syncthreads();
 }
 
 // L_ii = sum of every neighbour weight (arap.h:332), once per factorisation.
@@ -1103,126 +1115,163 @@ __global__ __launch_bounds__(kBlock) void k_pcg_diag(LaunchSet L) {
     reinterpret_cast<T *>(M.diag)[iv] = (T)d;
 }
 
-// d = 0, p = M^-1 r, partial r.z into buffer 1.
+// Jacobi-PCG in the Chronopoulos-Gear form (Chronopoulos and Gear, J. Comput. Appl. Math. 25
+// (1989) 153-168): with u = M^-1 r and w = A u, step i forms
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r, w = A u,
+//   gamma = (r, u), delta = (w, u),
+// and the next step takes beta = gamma_new / gamma, alpha = gamma_new / (delta - beta gamma_new /
+// alpha): one reduction, so one launch, per CG iteration. The new u of a neighbour is formed from
+// its r, w and s of the previous step, which live in the other buffer of each pair (parity i),
+// so no lane reads what another writes. Per coordinate, a solve whose gamma fell below
+// tol^2 gamma_0 keeps alpha = 0 (its x and r stay).
+template <typename T, typename V> struct CgVecs {
+    MD_G V *r[2], *w[2], *s[2], *x, *p;
+};
+template <typename T, typename V>
+__device__ __forceinline__ CgVecs<T, V> cg_vecs(const MD_G MeshDev &M) {
+    CgVecs<T, V> c;
+    MD_G V *v = reinterpret_cast<MD_G V *>(M.cg_v);
+    const long long nf = M.nf;
+    c.r[0] = reinterpret_cast<MD_G V *>(M.r);
+    c.r[1] = v;
+    c.w[0] = v + nf;
+    c.w[1] = v + 2 * nf;
+    c.s[0] = v + 3 * nf;
+    c.s[1] = v + 4 * nf;
+    c.x = reinterpret_cast<MD_G V *>(M.cg_x);
+    c.p = reinterpret_cast<MD_G V *>(M.cg_p);
+    return c;
+}
+
+// x = p = s_0 = 0, u = M^-1 r, w_0 = A u, partials of (r, u) and (w, u) (parity 0). One lane
+// per free vertex in slice order (block b: vertices [256 b, 256 b + 256)).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(LaunchSet L) {
     using V = typename V4T<T>::type;
-    __shared__ T red[3][kBlock];
-    int k = find_mesh(L, blockIdx.x);
+    __shared__ T red[6][kBlock];
+    const int k = find_mesh(L, blockIdx.x);
     const MD_G MeshDev &M = L.meshes[k];
-    int blk = blockIdx.x - L.prefix[k];
-    int i = blk * kBlock + threadIdx.x;
-    T a = 0, b = 0, c = 0;
-    if (i < M.nf) {
-        V r = reinterpret_cast<const V *>(M.r)[i];
-        T dinv = T(1) / reinterpret_cast<const T *>(M.diag)[i];
-        V z;
-        z.x = r.x * dinv; z.y = r.y * dinv; z.z = r.z * dinv; z.w = 0;
-        V zero;
-        zero.x = 0; zero.y = 0; zero.z = 0; zero.w = 0;
-        reinterpret_cast<V *>(M.cg_x)[i] = zero;
-        reinterpret_cast<V *>(M.cg_p)[i] = z;
-        a = r.x * z.x; b = r.y * z.y; c = r.z * z.z;
-    }
-    V *part = reinterpret_cast<V *>(M.cg_part);
-    block_partial<T, V>(a, b, c, part + M.nblk_free, blk, red);
-    if (threadIdx.x == 0) part[3 * M.nblk_free + blk] = part[M.nblk_free + blk];
-}
-
-// Ap = A p over the one-ring slots (constrained neighbours contribute 0), partial p.Ap.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(LaunchSet L) {
-    using V = typename V4T<T>::type;
-    __shared__ T red[3][kBlock];
-    int k = find_mesh(L, blockIdx.x);
-    const MD_G MeshDev &M = L.meshes[k];
-    int blk = blockIdx.x - L.prefix[k];
-    int slice = blk * kSlicesPerBlock + (threadIdx.x >> 6);
-    int lane = threadIdx.x & 63;
-    int iv = slice * kSlice + lane;
-    T a = 0, b = 0, c = 0;
+    const int blk = blockIdx.x - L.prefix[k];
+    const int slice = blk * kSlicesPerBlock + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int iv = slice * kSlice + lane;
+    const CgVecs<T, V> c = cg_vecs<T, V>(M);
+    const MD_G T *D = reinterpret_cast<const MD_G T *>(M.diag);
+    T gam[3] = {0, 0, 0}, del[3] = {0, 0, 0};
     if (slice < M.nslices_free && iv < M.nf) {
-        const V *P = reinterpret_cast<const V *>(M.cg_p);
         const T *W = wts<T>(M);
-        int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
-        V pi = P[iv];
-        T dg = reinterpret_cast<const T *>(M.diag)[iv];
-        T yx = dg * pi.x, yy = dg * pi.y, yz = dg * pi.z;
+        const int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
+        const V r = c.r[0][iv];
+        const T di = D[iv], rdi = T(1) / di;
+        const T ux = r.x * rdi, uy = r.y * rdi, uz = r.z * rdi;
+        T yx = di * ux, yy = di * uy, yz = di * uz;
         for (int q = 0; q < wd; ++q) {
-            long long g = (long long)(s0 + q) * kSlice + lane;
-            int j = M.sell_col[g];
+            const long long g = (long long)(s0 + q) * kSlice + lane;
+            const int j = M.sell_col[g];
             if (j != iv && j < M.nf) {
-                T w = W[g];
-                V pj = P[j];
-                yx -= w * pj.x; yy -= w * pj.y; yz -= w * pj.z;
+                const V rj = c.r[0][j];
+                const T wq = W[g] / D[j];
+                yx -= wq * rj.x; yy -= wq * rj.y; yz -= wq * rj.z;
             }
         }
-        V ap;
-        ap.x = yx; ap.y = yy; ap.z = yz; ap.w = 0;
-        reinterpret_cast<V *>(M.cg_ap)[iv] = ap;
-        a = pi.x * yx; b = pi.y * yy; c = pi.z * yz;
+        V z;
+        z.x = 0; z.y = 0; z.z = 0; z.w = 0;
+        V w;
+        w.x = yx; w.y = yy; w.z = yz; w.w = 0;
+        c.w[0][iv] = w;
+        c.s[0][iv] = z;
+        c.x[iv] = z;
+        c.p[iv] = z;
+        gam[0] = r.x * ux; gam[1] = r.y * uy; gam[2] = r.z * uz;
+        del[0] = yx * ux; del[1] = yy * uy; del[2] = yz * uz;
     }
-    block_partial<T, V>(a, b, c, reinterpret_cast<V *>(M.cg_part), blk, red);
+    MD_G V *part = reinterpret_cast<MD_G V *>(M.cg_part);
+    const int nb = M.nblk_free;
+    block_partial2<T, V>(gam, del, part, part + nb, blk, red);
 }
 
-// alpha = (r.z) / (p.Ap); d += alpha p; r -= alpha Ap; partial of the new r.z into buffer
-// 1 + (1 - parity) (the old one is buffer 1 + parity).
+// Step i (parity c = i & 1): reads r, w, s and the partials of parity c, writes those of 1 - c.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(LaunchSet L, int parity) {
+__global__ __launch_bounds__(kBlock) void k_pcg_step(LaunchSet L, int step) {
     using V = typename V4T<T>::type;
-    __shared__ T red[3][kBlock];
-    int k = find_mesh(L, blockIdx.x);
+    __shared__ T red[6][kBlock];
+    const int k = find_mesh(L, blockIdx.x);
     const MD_G MeshDev &M = L.meshes[k];
-    int blk = blockIdx.x - L.prefix[k];
-    V *part = reinterpret_cast<V *>(M.cg_part);
-    T rz[3], pap[3], rz0[3];
-    reduce_partials<T, V>(part + (1 + parity) * M.nblk_free, M.nblk_free, rz, red);
-    reduce_partials<T, V>(part, M.nblk_free, pap, red);
-    reduce_partials<T, V>(part + 3 * M.nblk_free, M.nblk_free, rz0, red);
-    T al[3];
-    for (int c = 0; c < 3; ++c) {
-        bool conv = (double)rz[c] <= M.pcg_tol2 * (double)rz0[c];
-        al[c] = (pap[c] != T(0) && !conv) ? rz[c] / pap[c] : T(0);
+    const int blk = blockIdx.x - L.prefix[k];
+    const int slice = blk * kSlicesPerBlock + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int iv = slice * kSlice + lane;
+    const int c = step & 1, n = 1 - c;
+    const CgVecs<T, V> v = cg_vecs<T, V>(M);
+    const MD_G T *D = reinterpret_cast<const MD_G T *>(M.diag);
+    MD_G V *part = reinterpret_cast<MD_G V *>(M.cg_part);
+    const int nb = M.nblk_free;
+    T ga[3], de[3];
+    reduce_partials2<T, V>(part + 2 * c * nb, part + (2 * c + 1) * nb, nb, ga, de, red);
+    MD_G V *scal = part + 4 * nb;  // (gamma, alpha) of each parity, then the first gamma
+    T al[3], be[3];
+    const V gp = scal[2 * c], ap = scal[2 * c + 1], gz = scal[4];
+    const T gprev[3] = {gp.x, gp.y, gp.z}, aprev[3] = {ap.x, ap.y, ap.z};
+    const T g0[3] = {step == 0 ? ga[0] : gz.x, step == 0 ? ga[1] : gz.y, step == 0 ? ga[2] : gz.z};
+    for (int q = 0; q < 3; ++q) {
+        const bool conv = (double)ga[q] <= M.pcg_tol2 * (double)g0[q];
+        if (step == 0) {
+            be[q] = T(0);
+            al[q] = (!conv && de[q] != T(0)) ? ga[q] / de[q] : T(0);
+        } else if (conv || aprev[q] == T(0) || gprev[q] == T(0)) {
+            be[q] = T(0);
+            al[q] = T(0);
+        } else {
+            be[q] = ga[q] / gprev[q];
+            const T den = de[q] - be[q] * ga[q] / aprev[q];
+            al[q] = den != T(0) ? ga[q] / den : T(0);
+        }
     }
-    int i = blk * kBlock + threadIdx.x;
-    T a = 0, b = 0, c = 0;
-    if (i < M.nf) {
-        V p = reinterpret_cast<const V *>(M.cg_p)[i];
-        V ap = reinterpret_cast<const V *>(M.cg_ap)[i];
-        V x = reinterpret_cast<V *>(M.cg_x)[i];
-        V r = reinterpret_cast<V *>(M.r)[i];
+    if (blockIdx.x == L.prefix[k] && threadIdx.x == 0) {  // for the next step
+        V g, a;
+        g.x = ga[0]; g.y = ga[1]; g.z = ga[2]; g.w = 0;
+        a.x = al[0]; a.y = al[1]; a.z = al[2]; a.w = 0;
+        scal[2 * n] = g;
+        scal[2 * n + 1] = a;
+        if (step == 0) scal[4] = g;  // the first gamma, for the stopping test
+    }
+    T gam[3] = {0, 0, 0}, del[3] = {0, 0, 0};
+    if (slice < M.nslices_free && iv < M.nf) {
+        const T *W = wts<T>(M);
+        const int s0 = M.slice_ptr[slice], wd = M.slice_ptr[slice + 1] - s0;
+        const T di = D[iv], rdi = T(1) / di;
+        const V r = v.r[c][iv], w = v.w[c][iv], so = v.s[c][iv];
+        V p = v.p[iv], x = v.x[iv];
+        // p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s
+        p.x = r.x * rdi + be[0] * p.x; p.y = r.y * rdi + be[1] * p.y; p.z = r.z * rdi + be[2] * p.z;
+        V sn;
+        sn.x = w.x + be[0] * so.x; sn.y = w.y + be[1] * so.y; sn.z = w.z + be[2] * so.z; sn.w = 0;
         x.x += al[0] * p.x; x.y += al[1] * p.y; x.z += al[2] * p.z;
-        r.x -= al[0] * ap.x; r.y -= al[1] * ap.y; r.z -= al[2] * ap.z;
-        reinterpret_cast<V *>(M.cg_x)[i] = x;
-        reinterpret_cast<V *>(M.r)[i] = r;
-        T dinv = T(1) / reinterpret_cast<const T *>(M.diag)[i];
-        a = r.x * r.x * dinv; b = r.y * r.y * dinv; c = r.z * r.z * dinv;
+        V rn;
+        rn.x = r.x - al[0] * sn.x; rn.y = r.y - al[1] * sn.y; rn.z = r.z - al[2] * sn.z; rn.w = 0;
+        v.p[iv] = p;
+        v.x[iv] = x;
+        v.s[n][iv] = sn;
+        v.r[n][iv] = rn;
+        // w = A u with u = M^-1 r of the new r; a neighbour's new u from its previous r, w, s
+        const T ux = rn.x * rdi, uy = rn.y * rdi, uz = rn.z * rdi;
+        T yx = di * ux, yy = di * uy, yz = di * uz;
+        for (int q = 0; q < wd; ++q) {
+            const long long g = (long long)(s0 + q) * kSlice + lane;
+            const int j = M.sell_col[g];
+            if (j != iv && j < M.nf) {
+                const V rj = v.r[c][j], wj = v.w[c][j], sj = v.s[c][j];
+                const T wq = W[g] / D[j];
+                yx -= wq * (rj.x - al[0] * (wj.x + be[0] * sj.x));
+                yy -= wq * (rj.y - al[1] * (wj.y + be[1] * sj.y));
+                yz -= wq * (rj.z - al[2] * (wj.z + be[2] * sj.z));
+            }
+        }
+        V wn;
+        wn.x = yx; wn.y = yy; wn.z = yz; wn.w = 0;
+        v.w[n][iv] = wn;
+        gam[0] = rn.x * ux; gam[1] = rn.y * uy; gam[2] = rn.z * uz;
+        del[0] = yx * ux; del[1] = yy * uy; del[2] = yz * uz;
     }
-    block_partial<T, V>(a, b, c, part + (2 - parity) * M.nblk_free, blk, red);
-}
-
-// beta = (r.z)_new / (r.z)_old; p = M^-1 r + beta p.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_direction(LaunchSet L, int parity) {
-    using V = typename V4T<T>::type;
-    __shared__ T red[3][kBlock];
-    int k = find_mesh(L, blockIdx.x);
-    const MD_G MeshDev &M = L.meshes[k];
-    int blk = blockIdx.x - L.prefix[k];
-    const V *part = reinterpret_cast<const V *>(M.cg_part);
-    T rn[3], ro[3];
-    reduce_partials<T, V>(part + (2 - parity) * M.nblk_free, M.nblk_free, rn, red);
-    reduce_partials<T, V>(part + (1 + parity) * M.nblk_free, M.nblk_free, ro, red);
-    T be[3];
-    for (int c = 0; c < 3; ++c) be[c] = (ro[c] != T(0)) ? rn[c] / ro[c] : T(0);
-    int i = blk * kBlock + threadIdx.x;
-    if (i < M.nf) {
-        V r = reinterpret_cast<const V *>(M.r)[i];
-        V p = reinterpret_cast<V *>(M.cg_p)[i];
-        T dinv = T(1) / reinterpret_cast<const T *>(M.diag)[i];
-        p.x = r.x * dinv + be[0] * p.x; p.y = r.y * dinv + be[1] * p.y; p.z = r.z * dinv + be[2] * p.z;
-        reinterpret_cast<V *>(M.cg_p)[i] = p;
-    }
+    block_partial2<T, V>(gam, del, part + 2 * n * nb, part + (2 * n + 1) * nb, blk, red);
 }
 
 // p' += d for the free vertices.
@@ -2229,12 +2278,8 @@ void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long n16)
 
 void launch_pcg_diag(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_diag, blocks, s, L); }
 void launch_pcg_init(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_init, blocks, s, L); }
-void launch_pcg_spmv(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_spmv, blocks, s, L); }
-void launch_pcg_update(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64) {
-    MD_LAUNCH_T(k_pcg_update, blocks, s, L, parity);
-}
-void launch_pcg_direction(hipStream_t s, LaunchSet L, int blocks, int parity, bool fp64) {
-    MD_LAUNCH_T(k_pcg_direction, blocks, s, L, parity);
+void launch_pcg_step(hipStream_t s, LaunchSet L, int blocks, int step, bool fp64) {
+    MD_LAUNCH_T(k_pcg_step, blocks, s, L, step);
 }
 void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64) { MD_LAUNCH_T(k_pcg_apply, blocks, s, L); }
 

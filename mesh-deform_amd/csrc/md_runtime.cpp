@@ -181,13 +181,11 @@ void enqueue_iteration(const Plan &plan, hipStream_t stream, hipEvent_t *ev) {
         if (plan.pcg) {
             launch_residual(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
             if (ev && r == 0) hipEventRecord(ev[2], stream);
-            // Jacobi-PCG on the correction: init, K x (A p, update, direction), apply
-            launch_pcg_init(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
-            for (int k = 0; k < plan.pcg_iters; ++k) {
-                launch_pcg_spmv(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
-                launch_pcg_update(stream, plan.set(plan.off_apply), plan.blk_apply, k & 1, plan.fp64);
-                launch_pcg_direction(stream, plan.set(plan.off_apply), plan.blk_apply, k & 1, plan.fp64);
-            }
+            // Jacobi-PCG on the correction (Chronopoulos-Gear form): init, one launch per CG
+            // iteration, apply
+            launch_pcg_init(stream, plan.set(plan.off_resid), plan.blk_resid, plan.fp64);
+            for (int k = 0; k < plan.pcg_iters; ++k)
+                launch_pcg_step(stream, plan.set(plan.off_resid), plan.blk_resid, k, plan.fp64);
             if (ev && r == 0) hipEventRecord(ev[3], stream);
             launch_pcg_apply(stream, plan.set(plan.off_apply), plan.blk_apply, plan.fp64);
             if (ev && r == 0) hipEventRecord(ev[4], stream);
@@ -515,8 +513,8 @@ int analyze_device(md_mesh *m) {
     size_t o_dg = A.reserve(ssz * std::max(1, pcg ? nf : 1));
     size_t o_cx = A.reserve(vsz * std::max(1, pcg ? nf : 1));
     size_t o_cpp = A.reserve(vsz * std::max(1, pcg ? nf : 1));
-    size_t o_cap = A.reserve(vsz * std::max(1, pcg ? nf : 1));
-    size_t o_cpt = A.reserve(vsz * (pcg ? 4 * (size_t)nblk : 1));
+    size_t o_cv = A.reserve(vsz * (pcg ? 5 * (size_t)std::max(1, nf) : 1));
+    size_t o_cpt = A.reserve(vsz * (pcg ? 4 * (size_t)nblk + 5 : 1));
     size_t o_tx = A.reserve(sizeof(double) * 3 * std::max(1, n - nf));
     size_t o_orig = A.reserve(sizeof(int) * std::max(1, n));
     size_t o_xyz = A.reserve(sizeof(double) * 3 * std::max(1, n));
@@ -552,7 +550,7 @@ int analyze_device(md_mesh *m) {
     d.energy_part = (double *)A.at(o_en);
     d.w32 = (float *)A.at(o_w32);
     d.diag = A.at(o_dg);
-    d.cg_x = A.at(o_cx); d.cg_p = A.at(o_cpp); d.cg_ap = A.at(o_cap); d.cg_part = A.at(o_cpt);
+    d.cg_x = A.at(o_cx); d.cg_p = A.at(o_cpp); d.cg_v = A.at(o_cv); d.cg_part = A.at(o_cpt);
     d.nblk_free = nblk;
     d.pcg_tol2 = m->cfg.pcg_tol * m->cfg.pcg_tol;
     m->d_tgt_xyz = (double *)A.at(o_tx);
@@ -1139,7 +1137,7 @@ void rebind(md_mesh *m, const char *oa, size_t asz, char *na, const char *ol, si
     fix(d.upart); fix(d.xpart); fix(d.child_ptr); fix(d.child); fix(d.udst); fix(d.rowinfo); fix(d.xloc);
     fix(d.pc); fix(d.pr); fix(d.ftile); fix(d.btile); fix(d.sub); fix(d.sub_h); fix(d.snode); fix(d.scp);
     fix(d.sfc); fix(d.sbo); fix(d.srow); fix(d.sfi); fix(d.lrows); fix(d.energy_part); fix(d.w32);
-    fix(d.diag); fix(d.cg_x); fix(d.cg_p); fix(d.cg_ap); fix(d.cg_part);
+    fix(d.diag); fix(d.cg_x); fix(d.cg_p); fix(d.cg_v); fix(d.cg_part);
     fix(m->d_fac); fix(m->d_hlist); fix(m->d_offrows); fix(m->d_rel); fix(m->d_tgt_xyz);
     fix(m->d_orig); fix(m->d_xyz);
 }
