@@ -300,6 +300,33 @@ def test_interactive_frames_match_oracle(ctx, golden_dir, rest_mode):
         assert err < POS_TOL_FP64, (frame, err)
 
 
+def test_reference_mode_keeps_factor_workspace(ctx, golden_dir):
+    """A mesh in reference rest mode refactors on every dirty deform() and keeps the FP64
+    factorization workspace between calls (md_mesh_stats.device_bytes counts it); a fixed-rest
+    mesh frees it after its factorization. Frames that reuse the workspace match the oracle."""
+    say("case: factorization workspace by rest mode")
+    g = gold(golden_dir, "bar_obj")
+    xyz, tri, idx, tgt = g["xyz"], g["tri"], g["con_idx"], g["con_xyz"]
+    ref, ok = run_gpu(ctx, xyz, tri, idx, tgt, 2, rest_mode=mdarap.REST_REFERENCE)
+    assert ok
+    fix, ok = run_gpu(ctx, xyz, tri, idx, tgt, 2, rest_mode=mdarap.REST_FIXED)
+    assert ok
+    extra = ref.stats()["device_bytes"] - fix.stats()["device_bytes"]
+    assert extra > 0, extra
+    a = oracle.Arap(xyz, tri)
+    a.set_constraints(idx, tgt)
+    assert a.deform(2)
+    for frame in range(3):
+        t = tgt + 0.02 * (frame + 1)
+        ref.set_constraints(idx, t)
+        a.set_constraints(idx, t)
+        assert ref.deform(3)
+        assert a.deform(3)
+        assert ref.stats()["device_bytes"] - fix.stats()["device_bytes"] == extra
+        err = meshgen.relative_error(ref.mesh, a.mesh)
+        assert err < POS_TOL_FP64, (frame, err)
+
+
 def test_fixed_rest_new_constraint_set_matches_oracle(ctx, golden_dir):
     """Fixed rest mode with the constrained set changed between frames: the rest pose and its
     weights stay those of the first call while the current mesh is the initial guess (the
