@@ -1550,14 +1550,22 @@ constexpr int kFb = 64;
 // Diagonal block s, by four wavefronts (unrolled over the block's columns: a loop over them with
 // the same work per step measured almost twice as slow). Cholesky (unblocked, right-looking):
 // lane i of wave v holds row i of the block's columns [16 v, 16 v + 16) (a[j] = A(kb + i, kb +
-// 16 v + j), then L); at step k the wave owning column k takes the pivot from lane k
-// (v_readlane, two 32-bit halves per double), scales the column by 1 / sqrt(pivot) (v_rsq_f64 and
-// two Newton steps), and every column applies A(i, c) -= L(i, k) L(c, k) for c > k (entries
-// above the diagonal too, never used), kPanel steps per barrier (chol_panel). Inverse X = L^{-1}
-// by columns: wave v forms columns [16 v, 16 v + 16) with four lanes per column, lane r of a
-// column holding X(t, i) for t = r mod 4; X(k, i) = (delta_ki - sum_{t<k} L(k, t) X(t, i)) /
-// L(k, k), the sum split over the four lanes and joined by a DPP butterfly, so the inverse needs
-// no barrier; X is zero above the diagonal, exactly.
+// 16 v + j), then L). The wave owning a panel of kPanel columns factors it in registers as an
+// unscaled LDL^T: at step k it takes the pivot d_k from lane k (v_readlane, two 32-bit halves per
+// double), forms 1/d_k (v_rcp_f64 and one cubic Newton step) and updates the panel's later columns,
+// a(i, c) -= a(i, k) a(c, k) / d_k (entries above the diagonal too, never used). The next pivot
+// waits only on 1/d_k; 1/sqrt(d_k) (v_rsq_f64 and two Newton steps) scales the panel's columns to
+// L afterwards. The panel goes to LDS, and after one barrier every wave applies the rank-kPanel
+// update to its columns past the panel.
+//
+// Inverse X = L^{-1} by 16 x 16 blocks: wave v inverts diagonal block v by columns (four lanes per
+// column, lane r of column i holding X(t, i) for t = r mod 4; X(k, i) = (delta_ki - sum_{t<k}
+// L(k, t) X(t, i)) / L(k, k), the sum split over the four lanes and joined by a DPP butterfly), and
+// after one barrier wave J forms block column J downwards on the matrix cores, X_IJ = -X_II W_IJ
+// with W_IJ = sum_{K=J}^{I-1} L_IK X_KJ. X is zero above the diagonal, exactly. Rows and columns
+// past a partial block (b < 64) hold zero in L and in the slot of 1 / L(k, k), so X is zero there.
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
@@ -1577,10 +1585,12 @@ __device__ __forceinline__ double quad_xor(double v) {
 
 constexpr int kPotrfCols = kFb / 4;  // columns per wave
 constexpr int kLdsPad = kFb + 1;
+constexpr int kIb = 16;              // blocks of the inverse (one per wave)
+static_assert(kFb == 4 * kIb && kFacBlock == 4 * 64, "four wavefronts, one 16-row block of the inverse each");
 
 // kPanel columns per barrier: the owning wave factors the panel k .. k + kPanel - 1 in its
-// registers (pivots and the panel's own updates through v_readlane), puts the panel in LDS, and
-// after one barrier every wave applies the rank-kPanel update to its columns past the panel.
+// registers, puts it in LDS, and after one barrier every wave applies the rank-kPanel update to
+// its columns past the panel.
 constexpr int kPanel = 8;
 static_assert(kPotrfCols % kPanel == 0, "a panel lies within one wave's columns");
 template <int K>
@@ -1590,21 +1600,32 @@ __device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col
         if (K < b) {
             constexpr int j0 = K % kPotrfCols, buf = (K / kPanel) & 1;
             if (wv == K / kPotrfCols) {
+                double rs[kPanel];
 #pragma unroll
                 for (int kk = 0; kk < kPanel; ++kk) {
+                    rs[kk] = 0.0;  // columns past the block stay zero
                     if (K + kk < b) {
-                        const double piv = readlane_f64(a[j0 + kk], K + kk);
-                        if (!(piv > 1e-14 * amax)) bad = true;
-                        double rd = __builtin_amdgcn_rsq(piv);
-                        rd = rd * (1.5 - 0.5 * piv * rd * rd);
-                        rd = rd * (1.5 - 0.5 * piv * rd * rd);
-                        if (lane == K + kk) rdiag = rd;
-                        a[j0 + kk] = lane == K + kk ? piv * rd : a[j0 + kk] * rd;
+                        const double d = readlane_f64(a[j0 + kk], K + kk);
+                        if (!(d > 1e-14 * amax)) bad = true;
+                        // 1/d: v_rcp_f64 (relative error below 2^-22) and y (1 + e + e^2), e = 1 - d y
+                        double y = __builtin_amdgcn_rcp(d);
+                        const double e = __builtin_fma(-d, y, 1.0);
+                        y = __builtin_fma(y, __builtin_fma(e, e, e), y);
 #pragma unroll
                         for (int c = kk + 1; c < kPanel; ++c)
-                            a[j0 + c] -= a[j0 + kk] * readlane_f64(a[j0 + kk], K + c);
+                            a[j0 + c] -= a[j0 + kk] * (readlane_f64(a[j0 + kk], K + c) * y);
+                        double rd = __builtin_amdgcn_rsq(d);
+                        rd = rd * (1.5 - 0.5 * d * rd * rd);
+                        rd = rd * (1.5 - 0.5 * d * rd * rd);
+                        rs[kk] = rd;
                     }
-                    col[buf][kk][lane] = a[j0 + kk];  // zero for columns past the block
+                }
+                // L(:, k) = a(:, k) / sqrt(d_k); on lane k, d_k / sqrt(d_k) = L(k, k)
+#pragma unroll
+                for (int kk = 0; kk < kPanel; ++kk) {
+                    if (lane == K + kk) rdiag = rs[kk];
+                    a[j0 + kk] *= rs[kk];
+                    col[buf][kk][lane] = a[j0 + kk];
                 }
             }
             __syncthreads();
@@ -1624,22 +1645,58 @@ __device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col
     }
 }
 
+// Diagonal block o / 16 of the inverse: lane (4 ii + r) forms X(o + t, o + ii) for t = r mod 4,
+// x[q] for t = 4 q + r; the slot (k, k) of Lsh holds 1 / L(k, k).
 template <int K>
-__device__ __forceinline__ void inv_col(const double (*Lsh)[kLdsPad], double (&x)[kFb / 4], int r, int i, int b) {
-    if constexpr (K < kFb) {
-        if (K < b) {
-            double part = 0.0;
+__device__ __forceinline__ void inv_diag(const double (*Lsh)[kLdsPad], double (&x)[kIb / 4], int r, int ii, int o) {
+    if constexpr (K < kIb) {
+        double part = 0.0;
 #pragma unroll
-            for (int q = 0; q < (K + 3) / 4; ++q) {
-                const int t = 4 * q + r;
-                part += t < K ? Lsh[K][t] * x[q] : 0.0;
-            }
-            part += quad_xor<1>(part);
-            part += quad_xor<2>(part);
-            const double xk = ((i == K ? 1.0 : 0.0) - part) * Lsh[K][K];  // the diagonal slot holds 1 / L(k, k)
-            if (r == (K & 3)) x[K / 4] = xk;
-            inv_col<K + 1>(Lsh, x, r, i, b);
+        for (int q = 0; q < (K + 3) / 4; ++q) {
+            const int t = 4 * q + r;
+            part += t < K ? Lsh[o + K][o + t] * x[q] : 0.0;
         }
+        part += quad_xor<1>(part);
+        part += quad_xor<2>(part);
+        const double xk = ((ii == K ? 1.0 : 0.0) - part) * Lsh[o + K][o + K];
+        if (r == (K & 3)) x[K / 4] = xk;
+        inv_diag<K + 1>(Lsh, x, r, ii, o);
+    }
+}
+
+// Block column J of the inverse on the matrix cores (v_mfma_f64_16x16x4f64: A(i, k) from lane
+// i + 16 k, B(k, j) from lane j + 16 k, D(i, j) in lane j + 16 (i % 4), register i / 4). Register c
+// of a D block is the B operand of k in [4 c, 4 c + 4), so the blocks X_KJ feed the next products
+// as they are. Writes the block column to T (row-major), zero above the diagonal block.
+template <int J>
+__device__ __forceinline__ void inv_blockcol(const double (*Lsh)[kLdsPad], const double (*Xd)[kIb][kIb + 1], int lane,
+                                             MD_G double *T, int w, int kb, int b) {
+    const int i = lane & 15, m = lane >> 4;
+    dv4 xb[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xb[J][c] = Xd[J][4 * c + m][i];
+#pragma unroll
+    for (int I = J + 1; I < 4; ++I) {
+        dv4 acc = dv4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int K = J; K < I; ++K)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lsh[kIb * I + i][kIb * K + 4 * c + m], xb[K][c], acc, 0, 0, 0);
+        dv4 d = dv4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d = __builtin_amdgcn_mfma_f64_16x16x4f64(Xd[I][i][4 * c + m], acc[c], d, 0, 0, 0);
+        xb[I] = -d;
+    }
+    const int colx = kIb * J + i;
+    if (colx < b) {
+#pragma unroll
+        for (int I = 0; I < 4; ++I)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int row = kIb * I + 4 * c + m;
+                if (row < b) T[(long long)(kb + row) * w + kb + colx] = I < J ? 0.0 : xb[I][c];
+            }
     }
 }
 
@@ -1652,6 +1709,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
     MD_G double *T = A.T + S.toff;
     __shared__ double col[2][kPanel][kFb];
     __shared__ double Lsh[kFb][kLdsPad];
+    __shared__ double Xd[kFb / kIb][kIb][kIb + 1];
     double a[kPotrfCols];
 #pragma unroll
     for (int j = 0; j < kPotrfCols; ++j) {
@@ -1672,18 +1730,18 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
         Lsh[lane][c] = c == lane ? rdiag : a[j];
     }
     __syncthreads();
-    const int r = lane & 3, i = kPotrfCols * wv + (lane >> 2);
-    double x[kFb / 4];
+    {
+        const int r = lane & 3, ii = lane >> 2;
+        double x[kIb / 4] = {0.0, 0.0, 0.0, 0.0};
+        inv_diag<0>(Lsh, x, r, ii, kIb * wv);
 #pragma unroll
-    for (int q = 0; q < kFb / 4; ++q) x[q] = 0.0;
-    inv_col<0>(Lsh, x, r, i, b);
-    if (i < b) {
-#pragma unroll
-        for (int q = 0; q < kFb / 4; ++q) {
-            const int k = 4 * q + r;
-            if (k < b) T[(long long)(kb + k) * w + kb + i] = x[q];  // X(kb + k, kb + i), row-major
-        }
+        for (int q = 0; q < kIb / 4; ++q) Xd[wv][4 * q + r][ii] = x[q];
     }
+    __syncthreads();
+    if (wv == 0) inv_blockcol<0>(Lsh, Xd, lane, T, w, kb, b);
+    else if (wv == 1) inv_blockcol<1>(Lsh, Xd, lane, T, w, kb, b);
+    else if (wv == 2) inv_blockcol<2>(Lsh, Xd, lane, T, w, kb, b);
+    else inv_blockcol<3>(Lsh, Xd, lane, T, w, kb, b);
 }
 
 // ---- FP64 dense products of the factorization on the matrix cores ----------------------
@@ -1693,7 +1751,6 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
 // i / 4. A 64 x 64 output tile per workgroup of 256 threads: wave v computes rows
 // [32 (v / 2), +32) and columns [32 (v % 2), +32) as 2 x 2 such blocks, with the operands
 // staged in LDS kMk values of k at a time (As[k][i] = A(i, k), Bs[k][j] = B(k, j)).
-typedef double dv4 __attribute__((ext_vector_type(4)));
 constexpr int kMk = 32;
 constexpr int kMpad = 66;  // row stride of the staged operands (doubles)
 
