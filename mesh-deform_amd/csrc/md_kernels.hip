@@ -1540,23 +1540,28 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
 }
 
 // The pivot block's Cholesky factor and its inverse X = L^{-1}, blocked by kFb columns
-// (right-looking): for each column block s, k_fac_potrf factors the diagonal block and inverts
-// it (the diagonal block of X, row-major in the T workspace), k_fac_trsm forms the rows below
-// it (times that inverse) and row block s of X, k_fac_syrk updates the trailing lower blocks and
-// accumulates the X blocks below: X_IJ = -X_II W_IJ with W_IJ = sum_{K=J}^{I-1} L_IK X_KJ, the
-// term K = s added at step s (in X's place), so W_sJ is complete when step s begins.
+// (right-looking): for each column block s, the diagonal block is factored and inverted (the
+// diagonal block of X, row-major in the T workspace; k_fac_potrf for block 0, later blocks by the
+// workgroup of k_fac_syrk that completes them), k_fac_trsm forms the rows below it (times that
+// inverse) and row block s of X, k_fac_syrk updates the trailing lower blocks and accumulates the
+// X blocks below: X_IJ = -X_II W_IJ with W_IJ = sum_{K=J}^{I-1} L_IK X_KJ, the term K = s added at
+// step s (in X's place), so W_sJ is complete when step s begins.
 constexpr int kFb = 64;
+// operands of the MFMA products are staged in LDS kMk values of k at a time, rows kMpad doubles apart
+constexpr int kMk = 32;
+constexpr int kMpad = 66;
 
 // Diagonal block s, by four wavefronts (unrolled over the block's columns: a loop over them with
-// the same work per step measured almost twice as slow). Cholesky (unblocked, right-looking):
-// lane i of wave v holds row i of the block's columns [16 v, 16 v + 16) (a[j] = A(kb + i, kb +
-// 16 v + j), then L). The wave owning a panel of kPanel columns factors it in registers as an
-// unscaled LDL^T: at step k it takes the pivot d_k from lane k (v_readlane, two 32-bit halves per
-// double), forms 1/d_k (v_rcp_f64 and one cubic Newton step) and updates the panel's later columns,
-// a(i, c) -= a(i, k) a(c, k) / d_k (entries above the diagonal too, never used). The next pivot
-// waits only on 1/d_k; 1/sqrt(d_k) (v_rsq_f64 and two Newton steps) scales the panel's columns to
-// L afterwards. The panel goes to LDS, and after one barrier every wave applies the rank-kPanel
-// update to its columns past the panel.
+// the same work per step measured almost twice as slow). Cholesky (unblocked, right-looking) as an
+// unscaled LDL^T: lane i of wave v holds row i of the block's columns [16 v, 16 v + 16) (a[j] =
+// A(kb + i, kb + 16 v + j), then u). The wave owning a panel of kPanel columns factors it in
+// registers: at step k it takes the pivot d_k = u_k(k) from lane k (v_readlane, two 32-bit halves
+// per double), forms 1/d_k (v_rcp_f64 and one cubic Newton step) and updates the panel's later
+// columns, a(i, c) -= (u_k(i) / d_k) u_k(c) (entries above the diagonal too, never used). It puts
+// u_k and u_k / d_k in LDS; after one barrier the waves with columns past the panel apply the
+// rank-kPanel update, the next panel's wave first and in the same basic block as that panel's
+// factorisation, so the chain starts while the rest of its update is still being issued. At the
+// end every wave scales its columns, L(:, k) = u_k / sqrt(d_k) (v_rsq_f64 and two Newton steps).
 //
 // Inverse X = L^{-1} by 16 x 16 blocks: wave v inverts diagonal block v by columns (four lanes per
 // column, lane r of column i holding X(t, i) for t = r mod 4; X(k, i) = (delta_ki - sum_{t<k}
@@ -1588,59 +1593,62 @@ constexpr int kLdsPad = kFb + 1;
 constexpr int kIb = 16;              // blocks of the inverse (one per wave)
 static_assert(kFb == 4 * kIb && kFacBlock == 4 * 64, "four wavefronts, one 16-row block of the inverse each");
 
-// kPanel columns per barrier: the owning wave factors the panel k .. k + kPanel - 1 in its
-// registers, puts it in LDS, and after one barrier every wave applies the rank-kPanel update to
-// its columns past the panel.
+// kPanel columns per barrier. FULL: a block of kFb columns (no per-column bound checks, so a
+// panel's chain is one basic block); else b columns, the rest zero. Panel buffers: cu[parity][k][i]
+// = u_k(i), cv[parity][k][i] = u_k(i) / d_k.
 constexpr int kPanel = 8;
 static_assert(kPotrfCols % kPanel == 0, "a panel lies within one wave's columns");
-template <int K>
-__device__ __forceinline__ void chol_panel(double (&a)[kPotrfCols], double (*col)[kPanel][kFb], double &rdiag, int wv,
-                                           int lane, int b, double amax, bool &bad) {
+template <int K, bool FULL>
+__device__ __forceinline__ void factor_panel(double (&a)[kPotrfCols], double (*cu)[kPanel][kFb], double (*cv)[kPanel][kFb],
+                                             int lane, int b, double amax, bool &bad) {
+    constexpr int j0 = K % kPotrfCols, buf = (K / kPanel) & 1;
+#pragma unroll
+    for (int kk = 0; kk < kPanel; ++kk) {
+        double v = 0.0;  // columns past the block stay zero
+        if (FULL || K + kk < b) {
+            const double d = readlane_f64(a[j0 + kk], K + kk);
+            if (!(d > 1e-14 * amax)) bad = true;
+            // 1/d: v_rcp_f64 (relative error below 2^-22) and y (1 + e + e^2), e = 1 - d y
+            double y = __builtin_amdgcn_rcp(d);
+            const double e = __builtin_fma(-d, y, 1.0);
+            y = __builtin_fma(y, __builtin_fma(e, e, e), y);
+            v = a[j0 + kk] * y;
+#pragma unroll
+            for (int c = kk + 1; c < kPanel; ++c) a[j0 + c] -= v * readlane_f64(a[j0 + kk], K + c);
+        }
+        cu[buf][kk][lane] = a[j0 + kk];
+        cv[buf][kk][lane] = v;
+    }
+}
+
+// Panel K is in LDS once the barrier is passed: the update of the columns past it (from column jn of
+// the next panel's wave, all columns of the waves after it), then the next panel's factorisation.
+template <int K, bool FULL>
+__device__ __forceinline__ void chol_step(double (&a)[kPotrfCols], double (*cu)[kPanel][kFb], double (*cv)[kPanel][kFb],
+                                          int wv, int lane, int b, double amax, bool &bad) {
     if constexpr (K < kFb) {
-        if (K < b) {
-            constexpr int j0 = K % kPotrfCols, buf = (K / kPanel) & 1;
-            if (wv == K / kPotrfCols) {
-                double rs[kPanel];
-#pragma unroll
-                for (int kk = 0; kk < kPanel; ++kk) {
-                    rs[kk] = 0.0;  // columns past the block stay zero
-                    if (K + kk < b) {
-                        const double d = readlane_f64(a[j0 + kk], K + kk);
-                        if (!(d > 1e-14 * amax)) bad = true;
-                        // 1/d: v_rcp_f64 (relative error below 2^-22) and y (1 + e + e^2), e = 1 - d y
-                        double y = __builtin_amdgcn_rcp(d);
-                        const double e = __builtin_fma(-d, y, 1.0);
-                        y = __builtin_fma(y, __builtin_fma(e, e, e), y);
-#pragma unroll
-                        for (int c = kk + 1; c < kPanel; ++c)
-                            a[j0 + c] -= a[j0 + kk] * (readlane_f64(a[j0 + kk], K + c) * y);
-                        double rd = __builtin_amdgcn_rsq(d);
-                        rd = rd * (1.5 - 0.5 * d * rd * rd);
-                        rd = rd * (1.5 - 0.5 * d * rd * rd);
-                        rs[kk] = rd;
-                    }
-                }
-                // L(:, k) = a(:, k) / sqrt(d_k); on lane k, d_k / sqrt(d_k) = L(k, k)
-#pragma unroll
-                for (int kk = 0; kk < kPanel; ++kk) {
-                    if (lane == K + kk) rdiag = rs[kk];
-                    a[j0 + kk] *= rs[kk];
-                    col[buf][kk][lane] = a[j0 + kk];
-                }
-            }
+        if (FULL || K < b) {
+            constexpr int buf = (K / kPanel) & 1;
             __syncthreads();
-            double lik[kPanel];
+            constexpr int nxt = K + kPanel, nown = nxt / kPotrfCols, jn = nxt % kPotrfCols;
+            if constexpr (nxt < kFb) {
+                double lik[kPanel];
 #pragma unroll
-            for (int kk = 0; kk < kPanel; ++kk) lik[kk] = col[buf][kk][lane];
+                for (int kk = 0; kk < kPanel; ++kk) lik[kk] = cv[buf][kk][lane];
+                if (wv == nown) {
 #pragma unroll
-            for (int j = 0; j < kPotrfCols; ++j) {
-                const int c = kPotrfCols * wv + j;
-                if (c >= K + kPanel) {
+                    for (int j = jn; j < kPotrfCols; ++j)
 #pragma unroll
-                    for (int kk = 0; kk < kPanel; ++kk) a[j] -= lik[kk] * col[buf][kk][c];
+                        for (int kk = 0; kk < kPanel; ++kk) a[j] -= lik[kk] * cu[buf][kk][kPotrfCols * nown + j];
+                    if (FULL || nxt < b) factor_panel<nxt, FULL>(a, cu, cv, lane, b, amax, bad);
+                } else if (wv > nown) {
+#pragma unroll
+                    for (int j = 0; j < kPotrfCols; ++j)
+#pragma unroll
+                        for (int kk = 0; kk < kPanel; ++kk) a[j] -= lik[kk] * cu[buf][kk][kPotrfCols * wv + j];
                 }
             }
-            chol_panel<K + kPanel>(a, col, rdiag, wv, lane, b, amax, bad);
+            chol_step<nxt, FULL>(a, cu, cv, wv, lane, b, amax, bad);
         }
     }
 }
@@ -1700,16 +1708,24 @@ __device__ __forceinline__ void inv_blockcol(const double (*Lsh)[kLdsPad], const
     }
 }
 
-__global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int s) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
+// LDS of the pivot block, two arrays (separate arrays let the compiler keep the panel buffers' and
+// L's accesses apart): the panel buffers (later the inverse's diagonal blocks; sized to also hold
+// the staged A operand of k_fac_syrk), and L (also holds the staged B operand).
+constexpr int kPbPanels = 2 * 2 * kPanel * kFb;
+constexpr int kPbPan = kPbPanels > kMk * kMpad ? kPbPanels : kMk * kMpad;
+constexpr int kPbL = kFb * kLdsPad > kMk * kMpad ? kFb * kLdsPad : kMk * kMpad;
+static_assert(kFb / kIb * kIb * (kIb + 1) <= kPbPanels, "the diagonal blocks of the inverse fit the panel buffers");
+
+// Diagonal block s of front S by the whole workgroup: L into the panel, X = L^{-1} into T.
+__device__ __forceinline__ void pivot_block(const FacArgs &A, const FacDev &S, int s, double *pan, double *lsh) {
     const int w = S.w, m = S.m, kb = s * kFb;
     if (kb >= w) return;
     const int b = min(kFb, w - kb), wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     MD_G double *P = A.P + S.poff;
     MD_G double *T = A.T + S.toff;
-    __shared__ double col[2][kPanel][kFb];
-    __shared__ double Lsh[kFb][kLdsPad];
-    __shared__ double Xd[kFb / kIb][kIb][kIb + 1];
+    double (*cu)[kPanel][kFb] = reinterpret_cast<double (*)[kPanel][kFb]>(pan);
+    double (*cv)[kPanel][kFb] = cu + 2;
+    double (*Lsh)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(lsh);
     double a[kPotrfCols];
 #pragma unroll
     for (int j = 0; j < kPotrfCols; ++j) {
@@ -1719,10 +1735,30 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
     }
     const double amax = __longlong_as_double((long long)*A.amax);
     bool bad = false;
-    double rdiag = 0.0;
-    chol_panel<0>(a, col, rdiag, wv, lane, b, amax, bad);
+    if (b == kFb) {
+        if (wv == 0) factor_panel<0, true>(a, cu, cv, lane, b, amax, bad);
+        chol_step<0, true>(a, cu, cv, wv, lane, b, amax, bad);
+    } else {
+        if (wv == 0) factor_panel<0, false>(a, cu, cv, lane, b, amax, bad);
+        chol_step<0, false>(a, cu, cv, wv, lane, b, amax, bad);
+    }
     if (bad && threadIdx.x == 0) *A.flag = 1;
-    // L: lower part to the panel, all of the wave's columns to LDS (the diagonal as 1 / L(i, i))
+    // L(:, k) = u_k / sqrt(d_k), d_k = u_k(k); lower part to the panel, all of the wave's columns to
+    // LDS (the diagonal as 1 / L(i, i))
+    double rdiag = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPotrfCols; ++j) {
+        const int c = kPotrfCols * wv + j;
+        const double d = readlane_f64(a[j], c);
+        double rd = 0.0;
+        if (c < b) {
+            rd = __builtin_amdgcn_rsq(d);
+            rd = rd * (1.5 - 0.5 * d * rd * rd);
+            rd = rd * (1.5 - 0.5 * d * rd * rd);
+        }
+        if (lane == c) rdiag = rd;
+        a[j] *= rd;
+    }
 #pragma unroll
     for (int j = 0; j < kPotrfCols; ++j) {
         const int c = kPotrfCols * wv + j;
@@ -1730,6 +1766,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
         Lsh[lane][c] = c == lane ? rdiag : a[j];
     }
     __syncthreads();
+    double (*Xd)[kIb][kIb + 1] = reinterpret_cast<double (*)[kIb][kIb + 1]>(pan);
     {
         const int r = lane & 3, ii = lane >> 2;
         double x[kIb / 4] = {0.0, 0.0, 0.0, 0.0};
@@ -1744,6 +1781,12 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
     else inv_blockcol<3>(Lsh, Xd, lane, T, w, kb, b);
 }
 
+__global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fac_potrf(FacArgs A, int h0, int s) {
+    __shared__ double pan[kPbPanels];
+    __shared__ double lsh[kFb * kLdsPad];
+    pivot_block(A, A.sn[A.hlist[h0 + blockIdx.x]], s, pan, lsh);
+}
+
 // ---- FP64 dense products of the factorization on the matrix cores ----------------------
 //
 // v_mfma_f64_16x16x4f64 (one wavefront): D(16x16) = A(16x4) B(4x16) + C with A(i, k) from
@@ -1751,8 +1794,6 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_potrf(FacArgs A, int h0, int 
 // i / 4. A 64 x 64 output tile per workgroup of 256 threads: wave v computes rows
 // [32 (v / 2), +32) and columns [32 (v % 2), +32) as 2 x 2 such blocks, with the operands
 // staged in LDS kMk values of k at a time (As[k][i] = A(i, k), Bs[k][j] = B(k, j)).
-constexpr int kMk = 32;
-constexpr int kMpad = 66;  // row stride of the staged operands (doubles)
 
 struct MmaAcc {
     dv4 c[2][2];
@@ -2070,13 +2111,18 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_trsm(FacArgs A, int h0, int s
 // others W tiles). Lower trailing tile q = (I, J), s < J <= I: A(r, c) -= sum_{t<64} L(r, kb + t)
 // L(c, kb + t). W tile (I, J), I = s + 1 + (q - split) / (s + 1), J = (q - split) % (s + 1):
 // W_IJ += L_Is X_sJ (stored by the first contribution, J = s), for the X blocks of row block I.
-__global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s, int split, int per) {
+// Tile 0 = (s + 1, s + 1) completes diagonal block s + 1, and its workgroup then factors and
+// inverts that block (pivot_block): no other tile of the launch touches it or its block of T, so
+// the next step's chain starts while the other tiles still run.
+__global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fac_syrk(FacArgs A, int h0, int s, int split, int per) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
     MD_G double *P = A.P + S.poff;
     MD_G double *T = A.T + S.toff;
-    __shared__ double As[kMk][kMpad];
-    __shared__ double Bs[kMk][kMpad];
+    __shared__ double pan[kPbPan];
+    __shared__ double lsh[kPbL];
+    double (*As)[kMpad] = reinterpret_cast<double (*)[kMpad]>(pan);
+    double (*Bs)[kMpad] = reinterpret_cast<double (*)[kMpad]>(lsh);
     MmaAcc acc;
     mma_zero(acc);
     if (q < split) {
@@ -2114,6 +2160,10 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_syrk(FacArgs A, int h0, int s
                     const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
                     if (rr < w && cc < w && rr >= cc) P[(long long)cc * m + rr] -= acc.c[x][y][r];
                 }
+        if (q == 0) {
+            __syncthreads();  // the tile's entries, and the end of the staged operands' use
+            pivot_block(A, S, s + 1, pan, lsh);
+        }
         return;
     }
     const int qq = q - split, r0 = (s + 1 + qq / (s + 1)) * kFb, J = qq % (s + 1), c0 = J * kFb;
@@ -2302,10 +2352,11 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
                        int per_dinv, int per_g, int per_u) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm, asm_cols);
-    // per step b: the diagonal block, then its row blocks and X blocks (X_bJ, J < b), then the
-    // trailing tiles and the W tiles of the row blocks below (W_IJ, I > b, J <= b)
+    // per step b: the row blocks and X blocks (X_bJ, J < b) of diagonal block b, then the trailing
+    // tiles and the W tiles of the row blocks below (W_IJ, I > b, J <= b), with diagonal block b + 1
+    hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, 0);
     for (int b = 0; b < nb; ++b) {
-        hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, b);
+        // diagonal block b is factored: by k_fac_potrf (b = 0) or by tile 0 of step b - 1's k_fac_syrk
         const int below = nb - 1 - b;
         if (below + b > 0)
             hipLaunchKernelGGL(k_fac_trsm, dim3(count * (below + b)), dim3(kFacBlock), 0, s, A, h0, b, below, below + b);
