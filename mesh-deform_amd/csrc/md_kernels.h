@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 namespace md {
@@ -112,6 +113,7 @@ struct FacArgs {
     MD_G int *flag;                 // 1: a pivot block is not positive definite; 2: structure
     MD_G void *pc, *pr;             // the sweeps' panels (float or double), written by the D^{-1}
     int solve64;                    //   and G kernels in the solve precision (1: double)
+    MD_G double *Wk;                // partial products of the split-K launches (fac_split_k_doubles)
 };
 
 // A set of meshes processed by one launch. Block b belongs to mesh k where
@@ -155,16 +157,30 @@ void launch_pcg_step(hipStream_t s, LaunchSet L, int blocks, int step, bool fp64
 void launch_pcg_apply(hipStream_t s, LaunchSet L, int blocks, bool fp64);
 // Device numeric factorization: diagonal and its maximum, then per height (the `count`
 // supernodes from hlist[h0]) the front assembly and the pivot block's Cholesky and inverse (one
-// workgroup per supernode), D^{-1}, G and the update matrix (per_* workgroups per supernode;
-// per_g = 0 when no supernode of the height has off rows); D^{-1} and G also go to the
-// solve-precision panels.
+// workgroup per supernode), D^{-1}, G and the update matrix (one workgroup per output tile, or
+// several split over K); D^{-1} and G also go to the solve-precision panels.
 void launch_fac_diag(hipStream_t s, const FacArgs &A);
-// One height: assembly, blocked Cholesky and inverse of the pivot blocks (nb = column blocks
-// of 64 in the widest one), D^{-1}, G and the update matrices.
-// per_*: workgroups per supernode of the stages (the height's most): assembly, one per asm_cols
-// front columns; D^{-1}, G and U, one per 64 x 64 output tile (lower tiles for the symmetric ones).
+// A product stage of a height: workgroups per supernode (one per 64 x 64 output tile, lower tiles
+// for the symmetric ones; the height's most) and splits over K (1: none); a split stage needs
+// fac_split_k_doubles(count * per * ks) doubles at A.Wk.
+struct FacSplit {
+    int per, ks;
+};
+// One height: assembly (per_asm workgroups per supernode, one per asm_cols front columns), blocked
+// Cholesky and inverse of the pivot blocks (nb = column blocks of 64 in the widest one), D^{-1},
+// G and the update matrices (g.per = 0 when no supernode of the height has off rows).
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
-                       int per_dinv, int per_g, int per_u);
+                       const FacSplit &dinv, const FacSplit &g, const FacSplit &u);
+// Split of a product over K: `tiles` output tiles of 64 x 64 and `chunks` chunks of K (32 values).
+// Splits only launches of few tiles (tiles x 2 <= kSplitWorkgroups) with at least 8 chunks, into
+// shares of at least 4 chunks, at most kSplitWorkgroups workgroups in all.
+constexpr int kSplitWorkgroups = 1024;
+inline int fac_split_k(long long tiles, int chunks) {
+    if (tiles <= 0 || tiles * 2 > kSplitWorkgroups || chunks < 8) return 1;
+    const long long ks = std::min<long long>(chunks / 4, kSplitWorkgroups / tiles);
+    return ks > 1 ? (int)ks : 1;
+}
+constexpr long long fac_split_k_doubles(long long workgroups) { return workgroups * 16 * 256; }
 // A height whose fronts all have m <= mm <= kFacSmallFront and w <= mw: every step of
 // launch_fac_height for one front in one wavefront (front in mm^2 + mm mw doubles of LDS: under
 // 64 KiB), without the FP64 panel workspace.

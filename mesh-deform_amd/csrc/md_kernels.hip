@@ -1446,6 +1446,7 @@ __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
 // then the children's update matrices in child order. `per` workgroups per supernode, each
 // owning `cols` columns of the front (pivot columns: P and F; later ones: U), so no two write
 // the same entry and every entry receives its additions in that fixed order.
+constexpr int kAsmSlotLanes = 16;  // threads per pivot column for its original entries
 __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, int per, int cols) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
@@ -1453,30 +1454,35 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     if (t0 >= m) return;
     MD_G double *P = A.P + S.poff, *F = A.S + S.soff, *U = A.U + S.uoff;
     const MD_G int *rows = A.offrows + S.roff;
-    // the front's row list in LDS for the searches of the original entries (pivot columns)
+    // the front's row list in LDS for the searches of the original entries (pivot columns), and
+    // each child's row map in turn
     __shared__ int rsh[kAsmRows];
+    __shared__ int relsh[kAsmRows];
     const bool staged = t0 < w && no <= kAsmRows;
     if (staged)
         for (int e = tid; e < no; e += kFacBlock) rsh[e] = rows[e];
-    for (int t = t0; t < t1; ++t) {
-        if (t < w) {
-            for (int i = tid; i < m; i += kFacBlock) P[(long long)t * m + i] = 0.0;
-            for (int i = tid; i < no; i += kFacBlock) F[(long long)t * no + i] = 0.0;
-        } else {
-            for (int i = tid; i < no; i += kFacBlock) U[(long long)(t - w) * no + i] = 0.0;
-        }
+    // zero the owned columns: contiguous ranges of P and F (pivot columns) and of U
+    const int kend = min(t1, w);
+    if (t0 < w) {
+        for (long long e = (long long)t0 * m + tid; e < (long long)kend * m; e += kFacBlock) P[e] = 0.0;
+        for (long long e = (long long)t0 * no + tid; e < (long long)kend * no; e += kFacBlock) F[e] = 0.0;
     }
+    if (t1 > w)
+        for (long long e = (long long)(max(t0, w) - w) * no + tid; e < (long long)(t1 - w) * no; e += kFacBlock)
+            U[e] = 0.0;
     __syncthreads();
-    for (int k = t0 + tid; k < min(t1, w); k += kFacBlock) {
+    // original entries, kAsmSlotLanes threads per pivot column (one slot each); each entry of the
+    // front receives at most one, right after its zero, so it is stored
+    for (int k = t0 + tid / kAsmSlotLanes; k < kend; k += kFacBlock / kAsmSlotLanes) {
         const int j = S.c0 + k, sl = j / kSlice, l = j % kSlice;
-        P[(long long)k * m + k] = A.diag[j];
-        for (int q = A.slice_ptr[sl]; q < A.slice_ptr[sl + 1]; ++q) {
+        if (tid % kAsmSlotLanes == 0) P[(long long)k * m + k] = A.diag[j];
+        for (int q = A.slice_ptr[sl] + tid % kAsmSlotLanes; q < A.slice_ptr[sl + 1]; q += kAsmSlotLanes) {
             const long long g = (long long)q * kSlice + l;
             const int i = A.sell_col[g];
             if (i <= j || i >= A.nf) continue;
             const double v = -A.w64[g];
             if (i < S.c0 + w) {
-                P[(long long)k * m + (i - S.c0)] += v;
+                P[(long long)k * m + (i - S.c0)] = v;
             } else {
                 int lo = 0, hi = no;  // first off row >= i
                 while (lo < hi) {
@@ -1484,33 +1490,31 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
                     if ((staged ? rsh[mid] : rows[mid]) < i) lo = mid + 1;
                     else hi = mid;
                 }
-                if (lo < no && (staged ? rsh[lo] : rows[lo]) == i) F[(long long)k * no + lo] += v;
+                if (lo < no && (staged ? rsh[lo] : rows[lo]) == i) F[(long long)k * no + lo] = v;
                 else *A.flag = 2;
             }
         }
     }
     __syncthreads();
     // children's update matrices, child by child: the child's lower entries (a >= b) of its
-    // columns b landing in [t0, t1) (at most `cols`: rel is increasing), flattened over the
-    // workgroup, kAsmU entries per thread in flight (their loads, then the read-modify-writes:
-    // distinct entries of one child land on distinct front entries)
+    // columns b landing in [t0, t1) (rel is increasing, so they are the columns [blo, bend),
+    // counted over the workgroup), flattened over the workgroup, kAsmU entries per thread in
+    // flight (their loads, then the read-modify-writes: distinct entries of one child land on
+    // distinct front entries)
     for (int ci = S.cb; ci < S.ce; ++ci) {
         const FacDev C = A.sn[A.child[ci]];
         const int nc = C.noff;
-        const int *rel = A.rel + C.roff;
+        const MD_G int *relg = A.rel + C.roff;
+        const bool rstaged = nc <= kAsmRows;
+        int blo = 0, bend = 0;
+        for (int a0 = 0; a0 < nc; a0 += kFacBlock) {
+            const int a = a0 + tid;
+            const int r = a < nc ? relg[a] : m;
+            if (rstaged && a < nc) relsh[a] = r;
+            blo += __syncthreads_count(r < t0);
+            bend += __syncthreads_count(r < t1);
+        }
         const MD_G double *Uc = A.U + C.uoff;
-        int blo = 0, bhi = nc;
-        while (blo < bhi) {
-            const int mid = (blo + bhi) >> 1;
-            if (rel[mid] < t0) blo = mid + 1;
-            else bhi = mid;
-        }
-        int bend = blo, hi = nc;
-        while (bend < hi) {
-            const int mid = (bend + hi) >> 1;
-            if (rel[mid] < t1) bend = mid + 1;
-            else hi = mid;
-        }
         const int total = (bend - blo) * nc;
         for (int e0 = tid; e0 < total; e0 += kFacBlock * kAsmU) {
             double v[kAsmU];
@@ -1521,7 +1525,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
                 dst[u] = nullptr;
                 v[u] = 0.0;
                 if (e < total && a >= b) {
-                    const int ra = rel[a], rb = rel[b];
+                    const int ra = rstaged ? relsh[a] : relg[a], rb = rstaged ? relsh[b] : relg[b];
                     v[u] = Uc[(long long)b * nc + a];
                     if (ra >= m || rb > ra) *A.flag = 2;
                     else if (rb < w) dst[u] = ra < w ? P + (long long)rb * m + ra : F + (long long)rb * no + (ra - w);
@@ -1855,12 +1859,54 @@ __device__ __forceinline__ void lower_tile(int q, int &I, int &J) {
     J = q - i * (i + 1) / 2;
 }
 
+// Split-K of the long products (few tiles, long K: the top heights): each of `ks` workgroups of a
+// tile multiplies over its share of the K chunks and stores its accumulators (kAccVals doubles per
+// workgroup, coalesced; MODE 1), and a second launch (MODE 2) sums the ks partials in split order
+// (deterministic) and finishes the tile as the unsplit kernel (MODE 0) does. (A last-arriving
+// workgroup summing in the same launch measured several times slower: every workgroup's
+// device-scope fence writes back its L2.)
+constexpr int kAccVals = 16 * kFacBlock;
+static_assert(kAccVals == fac_split_k_doubles(1), "partials per workgroup as the host sizes them");
+__device__ __forceinline__ void acc_store(MD_G double *wp, const MmaAcc &acc) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) wp[((x * 2 + y) * 4 + r) * kFacBlock + threadIdx.x] = acc.c[x][y][r];
+}
+// Sums in split order; the loads of four splits are issued before their additions.
+__device__ __forceinline__ void acc_sum(const MD_G double *wp, int ks, MmaAcc &acc) {
+    for (int z0 = 0; z0 < ks; z0 += 4) {
+        double v[4][16];
+#pragma unroll
+        for (int dz = 0; dz < 4; ++dz)
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                v[dz][t] = z0 + dz < ks ? wp[(long long)(z0 + dz) * kAccVals + t * kFacBlock + threadIdx.x] : 0.0;
+#pragma unroll
+        for (int dz = 0; dz < 4; ++dz)
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc.c[x][y][r] += v[dz][(x * 2 + y) * 4 + r];
+    }
+}
+
 // D^{-1} = X^T X into the pivot block of the panel (symmetric): lower 64 x 64 tile (I, J)
 // of D^{-1}(r, c) = sum_{k >= r} X(k, r) X(k, c), X = L^{-1} row-major in T (zero above its
 // diagonal inside the diagonal blocks), `per` workgroups per supernode.
-__global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, m = S.m, nb = (w + 63) / 64, q = (int)(blockIdx.x % per);
+template <int MODE>
+__global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int per, int ks) {
+    const int tile = MODE == 1 ? (int)blockIdx.x / ks : (int)blockIdx.x, z = MODE == 1 ? (int)blockIdx.x % ks : 0;
+    MmaAcc acc;
+    mma_zero(acc);
+    // MODE 2: the partials' loads go out first (a tile with no work stored none and returns below)
+    if constexpr (MODE == 2) acc_sum(A.Wk + (long long)tile * ks * kAccVals, ks, acc);
+    const FacDev S = A.sn[A.hlist[h0 + tile / per]];
+    const int w = S.w, m = S.m, nb = (w + 63) / 64, q = tile % per;
     if (q >= nb * (nb + 1) / 2) return;
     int I, J;
     lower_tile(q, I, J);
@@ -1869,28 +1915,33 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int p
     const double *X = A.T + S.toff;
     __shared__ double As[kMk][kMpad];
     __shared__ double Bs[kMk][kMpad];
-    MmaAcc acc;
-    mma_zero(acc);
-    mma_pipeline(
-        (w - r0 + kMk - 1) / kMk,
-        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+    if constexpr (MODE != 2) {
+        const int nch = (w - r0 + kMk - 1) / kMk, cb = z * nch / ks, ce = (z + 1) * nch / ks;
+        mma_pipeline(
+            ce - cb,
+            [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = r0 + c * kMk + e / 64;
-                const bool kin = k < w;
-                ra[i] = kin && r0 + x < w ? X[(long long)k * w + r0 + x] : 0.0;
-                rb[i] = kin && c0 + x < w ? X[(long long)k * w + c0 + x] : 0.0;
-            }
-        },
-        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = r0 + (cb + c) * kMk + e / 64;
+                    const bool kin = k < w;
+                    ra[i] = kin && r0 + x < w ? X[(long long)k * w + r0 + x] : 0.0;
+                    rb[i] = kin && c0 + x < w ? X[(long long)k * w + c0 + x] : 0.0;
+                }
+            },
+            [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i;
-                As[e / 64][e % 64] = ra[i];
-                Bs[e / 64][e % 64] = rb[i];
-            }
-        },
-        As, Bs, acc);
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i;
+                    As[e / 64][e % 64] = ra[i];
+                    Bs[e / 64][e % 64] = rb[i];
+                }
+            },
+            As, Bs, acc);
+    }
+    if constexpr (MODE == 1) {
+        acc_store(A.Wk + ((long long)tile * ks + z) * kAccVals, acc);
+        return;
+    }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1917,39 +1968,50 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int p
 
 // G = F_os D^{-1} into panel rows [w, m): 64 x 64 tile (rows of G, columns), `per`
 // workgroups per supernode. A(i, k) = F_os(r0 + i, k) (column-major), B(k, j) = D^{-1}(k, q0 + j).
-__global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
+template <int MODE>
+__global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per, int ks) {
+    const int tile = MODE == 1 ? (int)blockIdx.x / ks : (int)blockIdx.x, z = MODE == 1 ? (int)blockIdx.x % ks : 0;
+    MmaAcc acc;
+    mma_zero(acc);
+    // MODE 2: the partials' loads go out first (a tile with no work stored none and returns below)
+    if constexpr (MODE == 2) acc_sum(A.Wk + (long long)tile * ks * kAccVals, ks, acc);
+    const FacDev S = A.sn[A.hlist[h0 + tile / per]];
     const int w = S.w, no = S.noff, m = S.m, nr = (no + 63) / 64, nq = (w + 63) / 64;
-    const int q = (int)(blockIdx.x % per);
+    const int q = tile % per;
     if (q >= nr * nq) return;
     const int r0 = 64 * (q % nr), q0 = 64 * (q / nr);
     double *P = A.P + S.poff;
     const double *F = A.S + S.soff;
     __shared__ double As[kMk][kMpad];
     __shared__ double Bs[kMk][kMpad];
-    MmaAcc acc;
-    mma_zero(acc);
-    mma_pipeline(
-        (w + kMk - 1) / kMk,
-        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+    if constexpr (MODE != 2) {
+        const int nch = (w + kMk - 1) / kMk, cb = z * nch / ks, ce = (z + 1) * nch / ks;
+        mma_pipeline(
+            ce - cb,
+            [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = c * kMk + e / 64;
-                ra[i] = k < w && r0 + x < no ? F[(long long)k * no + r0 + x] : 0.0;
-                // B(k, q0 + x): consecutive threads read consecutive k of one column
-                const int xb = e / kMk, kg = c * kMk + e % kMk;
-                rb[i] = kg < w && q0 + xb < w ? P[(long long)(q0 + xb) * m + kg] : 0.0;
-            }
-        },
-        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = (cb + c) * kMk + e / 64;
+                    ra[i] = k < w && r0 + x < no ? F[(long long)k * no + r0 + x] : 0.0;
+                    // B(k, q0 + x): consecutive threads read consecutive k of one column
+                    const int xb = e / kMk, kg = (cb + c) * kMk + e % kMk;
+                    rb[i] = kg < w && q0 + xb < w ? P[(long long)(q0 + xb) * m + kg] : 0.0;
+                }
+            },
+            [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i;
-                As[e / 64][e % 64] = ra[i];
-                Bs[e % kMk][e / kMk] = rb[i];
-            }
-        },
-        As, Bs, acc);
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i;
+                    As[e / 64][e % 64] = ra[i];
+                    Bs[e % kMk][e / kMk] = rb[i];
+                }
+            },
+            As, Bs, acc);
+    }
+    if constexpr (MODE == 1) {
+        acc_store(A.Wk + ((long long)tile * ks + z) * kAccVals, acc);
+        return;
+    }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1975,9 +2037,15 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per)
 
 // U -= G F_os^T (lower part): lower 64 x 64 tile (I, J) of U(a, b), a >= b, `per` workgroups
 // per supernode. A(i, k) = G(a0 + i, k) (panel rows [w, m)), B(k, j) = F_os(b0 + j, k).
-__global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per) {
-    const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
-    const int w = S.w, no = S.noff, m = S.m, nb = (no + 63) / 64, q = (int)(blockIdx.x % per);
+template <int MODE>
+__global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per, int ks) {
+    const int tile = MODE == 1 ? (int)blockIdx.x / ks : (int)blockIdx.x, z = MODE == 1 ? (int)blockIdx.x % ks : 0;
+    MmaAcc acc;
+    mma_zero(acc);
+    // MODE 2: the partials' loads go out first (a tile with no work stored none and returns below)
+    if constexpr (MODE == 2) acc_sum(A.Wk + (long long)tile * ks * kAccVals, ks, acc);
+    const FacDev S = A.sn[A.hlist[h0 + tile / per]];
+    const int w = S.w, no = S.noff, m = S.m, nb = (no + 63) / 64, q = tile % per;
     if (q >= nb * (nb + 1) / 2) return;
     int I, J;
     lower_tile(q, I, J);
@@ -1986,27 +2054,32 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per)
     double *U = A.U + S.uoff;
     __shared__ double As[kMk][kMpad];
     __shared__ double Bs[kMk][kMpad];
-    MmaAcc acc;
-    mma_zero(acc);
-    mma_pipeline(
-        (w + kMk - 1) / kMk,
-        [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
+    if constexpr (MODE != 2) {
+        const int nch = (w + kMk - 1) / kMk, cb = z * nch / ks, ce = (z + 1) * nch / ks;
+        mma_pipeline(
+            ce - cb,
+            [&](int c, double (&ra)[kPer], double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = c * kMk + e / 64;
-                ra[i] = k < w && a0 + x < no ? P[(long long)k * m + w + a0 + x] : 0.0;
-                rb[i] = k < w && b0 + x < no ? F[(long long)k * no + b0 + x] : 0.0;
-            }
-        },
-        [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i, x = e % 64, k = (cb + c) * kMk + e / 64;
+                    ra[i] = k < w && a0 + x < no ? P[(long long)k * m + w + a0 + x] : 0.0;
+                    rb[i] = k < w && b0 + x < no ? F[(long long)k * no + b0 + x] : 0.0;
+                }
+            },
+            [&](const double (&ra)[kPer], const double (&rb)[kPer]) {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + kFacBlock * i;
-                As[e / 64][e % 64] = ra[i];
-                Bs[e / 64][e % 64] = rb[i];
-            }
-        },
-        As, Bs, acc);
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + kFacBlock * i;
+                    As[e / 64][e % 64] = ra[i];
+                    Bs[e / 64][e % 64] = rb[i];
+                }
+            },
+            As, Bs, acc);
+    }
+    if constexpr (MODE == 1) {
+        acc_store(A.Wk + ((long long)tile * ks + z) * kAccVals, acc);
+        return;
+    }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -2349,12 +2422,12 @@ void launch_fac_small(hipStream_t s, const FacArgs &A, int h0, int count, int mm
                            mm, mw);
 }
 void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int nb, int asm_cols, int per_asm,
-                       int per_dinv, int per_g, int per_u) {
+                       const FacSplit &dinv, const FacSplit &g, const FacSplit &u) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fac_assemble, dim3(count * per_asm), dim3(kFacBlock), 0, s, A, h0, per_asm, asm_cols);
+    hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, 0);
     // per step b: the row blocks and X blocks (X_bJ, J < b) of diagonal block b, then the trailing
     // tiles and the W tiles of the row blocks below (W_IJ, I > b, J <= b), with diagonal block b + 1
-    hipLaunchKernelGGL(k_fac_potrf, dim3(count), dim3(kFacBlock), 0, s, A, h0, 0);
     for (int b = 0; b < nb; ++b) {
         // diagonal block b is factored: by k_fac_potrf (b = 0) or by tile 0 of step b - 1's k_fac_syrk
         const int below = nb - 1 - b;
@@ -2365,10 +2438,19 @@ void launch_fac_height(hipStream_t s, const FacArgs &A, int h0, int count, int n
             hipLaunchKernelGGL(k_fac_syrk, dim3(count * (tiles + wtiles)), dim3(kFacBlock), 0, s, A, h0, b, tiles,
                                tiles + wtiles);
     }
-    hipLaunchKernelGGL(k_fac_dinv, dim3(count * per_dinv), dim3(kFacBlock), 0, s, A, h0, per_dinv);
-    if (per_g > 0) {
-        hipLaunchKernelGGL(k_fac_g, dim3(count * per_g), dim3(kFacBlock), 0, s, A, h0, per_g);
-        hipLaunchKernelGGL(k_fac_u, dim3(count * per_u), dim3(kFacBlock), 0, s, A, h0, per_u);
+    // the products, split over K where ks > 1
+    auto product = [&](auto plain, auto part, auto fix, const FacSplit &p) {
+        if (p.ks > 1) {
+            hipLaunchKernelGGL(part, dim3(count * p.per * p.ks), dim3(kFacBlock), 0, s, A, h0, p.per, p.ks);
+            hipLaunchKernelGGL(fix, dim3(count * p.per), dim3(kFacBlock), 0, s, A, h0, p.per, p.ks);
+        } else {
+            hipLaunchKernelGGL(plain, dim3(count * p.per), dim3(kFacBlock), 0, s, A, h0, p.per, 1);
+        }
+    };
+    product(k_fac_dinv<0>, k_fac_dinv<1>, k_fac_dinv<2>, dinv);
+    if (g.per > 0) {
+        product(k_fac_g<0>, k_fac_g<1>, k_fac_g<2>, g);
+        product(k_fac_u<0>, k_fac_u<1>, k_fac_u<2>, u);
     }
 }
 

@@ -835,9 +835,42 @@ int factor_device(md_mesh *m) {
     const Factor &f = m->fac;
     const int ns = f.ns, nf = m->topo.nf;
     hipStream_t st = m->ctx->stream;
+    // per height: workgroups per supernode of the multi-workgroup stages, for the height's largest
+    // front (assembly one per asm_cols front columns: 64, halved down to 8 while the height has
+    // fewer than 2048 assembly workgroups, since the top fronts take the largest children; the
+    // products one per 64 x 64 output tile), and the products' splits over K
+    struct HeightLaunch {
+        int count, mw, mm, asm_cols;
+        FacSplit dinv, g, u;
+    };
+    const int nh = (int)f.hptr.size() - 1;
+    std::vector<HeightLaunch> hl(nh);
+    size_t nW = 0;
+    for (int h = 0; h < nh; ++h) {
+        HeightLaunch &L = hl[h];
+        L = HeightLaunch{f.hptr[h + 1] - f.hptr[h], 0, 1, 64, {1, 1}, {0, 1}, {0, 1}};
+        auto lower = [](int b) { return b * (b + 1) / 2; };
+        for (int i = f.hptr[h]; i < f.hptr[h + 1]; ++i) {
+            const int w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
+            const int bw = (w + 63) / 64, bo = (no + 63) / 64;
+            L.mw = std::max(L.mw, w);
+            L.mm = std::max(L.mm, w + no);
+            L.dinv.per = std::max(L.dinv.per, lower(bw));
+            L.g.per = std::max(L.g.per, bo * bw);
+            L.u.per = std::max(L.u.per, lower(bo));
+        }
+        if (L.mm <= kFacSmallFront) continue;  // small fronts only: one fused launch
+        while (L.asm_cols > 8 && (long long)L.count * ((L.mm + L.asm_cols - 1) / L.asm_cols) < 2048) L.asm_cols /= 2;
+        const int chunks = (L.mw + 31) / 32;
+        for (FacSplit *p : {&L.dinv, &L.g, &L.u}) {
+            if (p->per == 0) continue;
+            p->ks = fac_split_k((long long)L.count * p->per, chunks);
+            if (p->ks > 1) nW = std::max(nW, (size_t)fac_split_k_doubles((long long)L.count * p->per * p->ks));
+        }
+    }
     const size_t nP = (size_t)f.poff[ns], nS = (size_t)f.woff_s[ns], nT = (size_t)f.woff_t[ns],
                  nU = (size_t)f.woff_u[ns];
-    const size_t bytes = sizeof(double) * (nP + nS + nT + nU + (size_t)nf) + 2 * sizeof(unsigned long long);
+    const size_t bytes = sizeof(double) * (nP + nS + nT + nU + nW + (size_t)nf) + 2 * sizeof(unsigned long long);
     const bool keep = m->cfg.rest_mode == MD_REST_REFERENCE;  // refactors on every deform call
     if (m->fac_ws && m->fac_ws_bytes != bytes) m->release_fac_ws();
     if (!m->fac_ws) {
@@ -853,7 +886,7 @@ int factor_device(md_mesh *m) {
     A.sn = m->d_fac; A.hlist = m->d_hlist; A.offrows = m->d_offrows; A.rel = m->d_rel; A.child = m->dev.child;
     A.slice_ptr = m->dev.slice_ptr; A.sell_col = m->dev.sell_col; A.w64 = m->dev.w64; A.nf = nf;
     double *base = static_cast<double *>(ws);
-    A.P = base; A.S = A.P + nP; A.T = A.S + nS; A.U = A.T + nT; A.diag = A.U + nU;
+    A.P = base; A.S = A.P + nP; A.T = A.S + nS; A.U = A.T + nT; A.Wk = A.U + nU; A.diag = A.Wk + nW;
     A.amax = reinterpret_cast<unsigned long long *>(A.diag + nf);
     A.flag = reinterpret_cast<int *>(A.amax + 1);
     A.pc = const_cast<void *>(m->dev.pc);
@@ -862,32 +895,14 @@ int factor_device(md_mesh *m) {
     hipError_t e = hipMemsetAsync(A.amax, 0, 2 * sizeof(unsigned long long), st);
     if (e == hipSuccess) {
         launch_fac_diag(st, A);
-        const int nh = (int)f.hptr.size() - 1;
         for (int h = 0; h < nh; ++h) {
-            // workgroups per supernode of the multi-workgroup stages, for the height's largest
-            // front: assembly one per asm_cols front columns (64, halved down to 8 while the
-            // height has fewer than 2048 assembly workgroups: the top fronts take the largest
-            // children), the products one per 64 x 64 output tile
-            int per_dinv = 1, per_g = 0, per_u = 0, mw = 0, mm = 1;
-            const int count = f.hptr[h + 1] - f.hptr[h];
-            auto lower = [](int b) { return b * (b + 1) / 2; };
-            for (int i = f.hptr[h]; i < f.hptr[h + 1]; ++i) {
-                const int w = f.w[f.hlist[i]], no = f.noff[f.hlist[i]];
-                const int bw = (w + 63) / 64, bo = (no + 63) / 64;
-                mw = std::max(mw, w);
-                mm = std::max(mm, w + no);
-                per_dinv = std::max(per_dinv, lower(bw));
-                per_g = std::max(per_g, bo * bw);
-                per_u = std::max(per_u, lower(bo));
-            }
-            if (mm <= kFacSmallFront) {  // small fronts only: one fused launch
-                launch_fac_small(st, A, f.hptr[h], count, mm, mw);
+            const HeightLaunch &L = hl[h];
+            if (L.mm <= kFacSmallFront) {  // small fronts only: one fused launch
+                launch_fac_small(st, A, f.hptr[h], L.count, L.mm, L.mw);
                 continue;
             }
-            int asm_cols = 64;
-            while (asm_cols > 8 && (long long)count * ((mm + asm_cols - 1) / asm_cols) < 2048) asm_cols /= 2;
-            launch_fac_height(st, A, f.hptr[h], count, (mw + 63) / 64, asm_cols, (mm + asm_cols - 1) / asm_cols,
-                              per_dinv, per_g, per_u);
+            launch_fac_height(st, A, f.hptr[h], L.count, (L.mw + 63) / 64, L.asm_cols,
+                              (L.mm + L.asm_cols - 1) / L.asm_cols, L.dinv, L.g, L.u);
         }
         e = hipGetLastError();
     }
