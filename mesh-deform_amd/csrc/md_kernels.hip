@@ -1447,6 +1447,7 @@ __global__ __launch_bounds__(kBlock) void k_fac_diag(FacArgs A) {
 // owning `cols` columns of the front (pivot columns: P and F; later ones: U), so no two write
 // the same entry and every entry receives its additions in that fixed order.
 constexpr int kAsmSlotLanes = 16;  // threads per pivot column for its original entries
+constexpr int kAsmChildren = 64;   // children whose records are staged in LDS at a time
 __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, int per, int cols) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
@@ -1458,6 +1459,21 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     // each child's row map in turn
     __shared__ int rsh[kAsmRows];
     __shared__ int relsh[kAsmRows];
+    // the children's (rows, row map, update matrix) records, kAsmChildren at a time (the first
+    // batch loaded up front, beside the zeroing)
+    __shared__ int chn[kAsmChildren], chr[kAsmChildren];
+    __shared__ long long chu[kAsmChildren];
+    const int nch = S.ce - S.cb;
+    auto stage_children = [&](int c0) {
+        if (tid < min(nch - c0, kAsmChildren)) {
+            const FacDev C = A.sn[A.child[S.cb + c0 + tid]];
+            chn[tid] = C.noff;
+            chr[tid] = C.roff;
+            chu[tid] = C.uoff;
+        }
+    };
+    static_assert(kAsmChildren <= kFacBlock, "one child record per thread");
+    stage_children(0);
     const bool staged = t0 < w && no <= kAsmRows;
     if (staged)
         for (int e = tid; e < no; e += kFacBlock) rsh[e] = rows[e];
@@ -1501,10 +1517,14 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
     // counted over the workgroup), flattened over the workgroup, kAsmU entries per thread in
     // flight (their loads, then the read-modify-writes: distinct entries of one child land on
     // distinct front entries)
-    for (int ci = S.cb; ci < S.ce; ++ci) {
-        const FacDev C = A.sn[A.child[ci]];
-        const int nc = C.noff;
-        const MD_G int *relg = A.rel + C.roff;
+    for (int c = 0; c < nch; ++c) {
+        if (c > 0 && c % kAsmChildren == 0) {  // the previous child's loop ended with a barrier
+            stage_children(c);
+            __syncthreads();
+        }
+        const int nc = chn[c % kAsmChildren], roff = chr[c % kAsmChildren];
+        const long long uoff = chu[c % kAsmChildren];
+        const MD_G int *relg = A.rel + roff;
         const bool rstaged = nc <= kAsmRows;
         int blo = 0, bend = 0;
         for (int a0 = 0; a0 < nc; a0 += kFacBlock) {
@@ -1514,7 +1534,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_assemble(FacArgs A, int h0, i
             blo += __syncthreads_count(r < t0);
             bend += __syncthreads_count(r < t1);
         }
-        const MD_G double *Uc = A.U + C.uoff;
+        const MD_G double *Uc = A.U + uoff;
         const int total = (bend - blo) * nc;
         for (int e0 = tid; e0 < total; e0 += kFacBlock * kAsmU) {
             double v[kAsmU];
