@@ -203,13 +203,18 @@ def test_deform_sweep_layouts_match_golden(ctx, golden_dir, name, budget):
         assert err < tol
 
 
-@pytest.mark.parametrize("which", ["sphere", "ico", "bar"])
+@pytest.mark.parametrize("which", ["sphere", "ico", "bar", "bar16k", "bar67k", "ico164k"])
 def test_device_factorization_matches_host(ctx, which):
     """The device numeric factorization (FP64, multifrontal on the GPU) and the host one give
-    the same iterates: FP64 positions after 3 iterations agree to 1e-11 relative, and both
-    meet the oracle at 1e-9."""
+    the same iterates: FP64 positions after 3 iterations agree to 1e-11 relative, and both meet
+    the oracle at 1e-9. Every case has pivot blocks past the first
+    (factored inside the trailing update's launch); the three larger ones also split the top
+    heights' products over K (2 splits on the 16.8k bar; 2 and 4 on the 67k bar; 2 to 6 on the
+    164k icosphere)."""
     pr = {"sphere": lambda: meshgen.sphere_problem(), "ico": lambda: meshgen.noisy_icosphere_problem(5),
-          "bar": lambda: meshgen.bar_problem(120, 12, 12)}[which]()
+          "bar": lambda: meshgen.bar_problem(120, 12, 12), "bar16k": lambda: meshgen.bar_problem(200, 20, 20),
+          "bar67k": lambda: meshgen.bar_problem(400, 40, 40),
+          "ico164k": lambda: meshgen.noisy_icosphere_problem(7)}[which]()
     say("case: device factorization", pr.name, "n", pr.n)
     idx, tgt = pr.constraints()
     out = []
