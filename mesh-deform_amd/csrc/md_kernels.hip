@@ -1740,7 +1740,9 @@ constexpr int kPbPan = kPbPanels > kMk * kMpad ? kPbPanels : kMk * kMpad;
 constexpr int kPbL = kFb * kLdsPad > kMk * kMpad ? kFb * kLdsPad : kMk * kMpad;
 static_assert(kFb / kIb * kIb * (kIb + 1) <= kPbPanels, "the diagonal blocks of the inverse fit the panel buffers");
 
-// Diagonal block s of front S by the whole workgroup: L into the panel, X = L^{-1} into T.
+// Diagonal block s of front S by the whole workgroup: L into the panel, X = L^{-1} into T. FROM_LDS:
+// the block's lower entries are in lsh (row-major, rows kLdsPad apart), not yet in the panel.
+template <bool FROM_LDS>
 __device__ __forceinline__ void pivot_block(const FacArgs &A, const FacDev &S, int s, double *pan, double *lsh) {
     const int w = S.w, m = S.m, kb = s * kFb;
     if (kb >= w) return;
@@ -1755,8 +1757,10 @@ __device__ __forceinline__ void pivot_block(const FacArgs &A, const FacDev &S, i
     for (int j = 0; j < kPotrfCols; ++j) {
         // column-major panel: A(kb + lane, kb + c) at (kb + c) m + kb + lane (lower part only)
         const int c = kPotrfCols * wv + j;
-        a[j] = (lane < b && c < b && c <= lane) ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
+        if constexpr (FROM_LDS) a[j] = (lane < b && c < b && c <= lane) ? lsh[lane * kLdsPad + c] : 0.0;
+        else a[j] = (lane < b && c < b && c <= lane) ? P[(long long)(kb + c) * m + kb + lane] : 0.0;
     }
+    if constexpr (FROM_LDS) __syncthreads();  // every entry read before lsh takes L
     const double amax = __longlong_as_double((long long)*A.amax);
     bool bad = false;
     if (b == kFb) {
@@ -1808,7 +1812,7 @@ __device__ __forceinline__ void pivot_block(const FacArgs &A, const FacDev &S, i
 __global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fac_potrf(FacArgs A, int h0, int s) {
     __shared__ double pan[kPbPanels];
     __shared__ double lsh[kFb * kLdsPad];
-    pivot_block(A, A.sn[A.hlist[h0 + blockIdx.x]], s, pan, lsh);
+    pivot_block<false>(A, A.sn[A.hlist[h0 + blockIdx.x]], s, pan, lsh);
 }
 
 // ---- FP64 dense products of the factorization on the matrix cores ----------------------
@@ -2244,6 +2248,30 @@ __global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(3))) 
                 }
             },
             As, Bs, acc);
+        if (q == 0) {
+            // tile (s + 1, s + 1): its entries go to LDS for the pivot block (which writes L over
+            // their place in the panel), after every wave's last read of the staged operands
+            double v[2][2][4];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
+                        v[x][y][r] = rr < w && cc < w && rr >= cc ? P[(long long)cc * m + rr] - acc.c[x][y][r] : 0.0;
+                    }
+            __syncthreads();
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) lsh[mma_row(x, r) * kLdsPad + mma_col(y)] = v[x][y][r];
+            __syncthreads();
+            pivot_block<true>(A, S, s + 1, pan, lsh);
+            return;
+        }
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -2253,10 +2281,6 @@ __global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(3))) 
                     const int rr = r0 + mma_row(x, r), cc = c0 + mma_col(y);
                     if (rr < w && cc < w && rr >= cc) P[(long long)cc * m + rr] -= acc.c[x][y][r];
                 }
-        if (q == 0) {
-            __syncthreads();  // the tile's entries, and the end of the staged operands' use
-            pivot_block(A, S, s + 1, pan, lsh);
-        }
         return;
     }
     const int qq = q - split, r0 = (s + 1 + qq / (s + 1)) * kFb, J = qq % (s + 1), c0 = J * kFb;
