@@ -937,7 +937,9 @@ bool numeric_device_emulation(const std::vector<int>& ptr, const std::vector<int
                 }
             }
             // Cholesky factor and its inverse X of the pivot block, blocked by 64 columns
-            // (right-looking); the device kernels follow the same operation order.
+            // (right-looking), in the device's workspaces; the device orders the work within a
+            // front differently (an unscaled LDL^T chain per block, the X blocks inside the
+            // right-looking loop, products split over K), so the two agree to rounding.
             constexpr int kFb = 64;
             const int nb = (w + kFb - 1) / kFb;
             for (int s = 0; s < nb; ++s) {
