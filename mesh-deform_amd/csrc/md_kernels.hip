@@ -2327,6 +2327,7 @@ __global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(3))) 
 // solve precision. The FP64 panel workspace is not used.
 constexpr int kSmallFront = 64;
 constexpr int kSmallBlock = 64;
+constexpr int kSmallSlotLanes = 4;  // lanes per pivot column for its original entries
 __global__ __launch_bounds__(kSmallBlock) void k_fac_small(FacArgs A, int h0, int mm, int mw) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x]];
     const int tid = threadIdx.x, w = S.w, no = S.noff, m = S.m;
@@ -2341,11 +2342,12 @@ __global__ __launch_bounds__(kSmallBlock) void k_fac_small(FacArgs A, int h0, in
     for (int e = tid; e < no; e += kSmallBlock) rows[e] = A.offrows[S.roff + e];
     for (int e = tid; e < m * mm; e += kSmallBlock) Fs[e] = 0.0;
     __syncthreads();
-    // original entries, column by column (one lane per pivot column)
-    for (int k = tid; k < w; k += kSmallBlock) {
+    // original entries, kSmallSlotLanes lanes per pivot column (one slot each; every front entry
+    // receives at most one)
+    for (int k = tid / kSmallSlotLanes; k < w; k += kSmallBlock / kSmallSlotLanes) {
         const int j = S.c0 + k, sl = j / kSlice, l = j % kSlice;
-        Fs[k * mm + k] = A.diag[j];
-        for (int q = A.slice_ptr[sl]; q < A.slice_ptr[sl + 1]; ++q) {
+        if (tid % kSmallSlotLanes == 0) Fs[k * mm + k] = A.diag[j];
+        for (int q = A.slice_ptr[sl] + tid % kSmallSlotLanes; q < A.slice_ptr[sl + 1]; q += kSmallSlotLanes) {
             const long long g = (long long)q * kSlice + l;
             const int i = A.sell_col[g];
             if (i <= j || i >= A.nf) continue;
@@ -2381,14 +2383,21 @@ __global__ __launch_bounds__(kSmallBlock) void k_fac_small(FacArgs A, int h0, in
         }
         __syncthreads();
     }
-    // Cholesky of the pivot block
+    // Cholesky of the pivot block (1 / sqrt(pivot): v_rsq_f64 and two Newton steps, kept for X)
     const double amax = __longlong_as_double((long long)*A.amax);
+    __shared__ double rdg[kSmallFront];
     for (int k = 0; k < w; ++k) {
         const double piv = Fs[k * mm + k];
         if (tid == 0 && !(piv > 1e-14 * amax)) *A.flag = 1;
-        const double d = sqrt(piv), rd = 1.0 / d;
+        double rd = __builtin_amdgcn_rsq(piv);
+        rd = rd * (1.5 - 0.5 * piv * rd * rd);
+        rd = rd * (1.5 - 0.5 * piv * rd * rd);
+        const double d = piv * rd;
         __syncthreads();  // every lane has read the pivot before it changes
-        if (tid == 0) Fs[k * mm + k] = d;
+        if (tid == 0) {
+            Fs[k * mm + k] = d;
+            rdg[k] = rd;
+        }
         for (int i = k + 1 + tid; i < w; i += kSmallBlock) Fs[i * mm + k] *= rd;
         __syncthreads();
         const int t = w - k - 1;  // trailing lower triangle, t x t
@@ -2404,7 +2413,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_fac_small(FacArgs A, int h0, in
         for (int r = i; r < w; ++r) {
             double acc = r == i ? 1.0 : 0.0;
             for (int t = i; t < r; ++t) acc -= Fs[r * mm + t] * Xs[t * mw + i];
-            Xs[r * mw + i] = acc / Fs[r * mm + r];
+            Xs[r * mw + i] = acc * rdg[r];
         }
     }
     __syncthreads();
