@@ -1923,7 +1923,7 @@ __device__ __forceinline__ void acc_sum(const MD_G double *wp, int ks, MmaAcc &a
 // of D^{-1}(r, c) = sum_{k >= r} X(k, r) X(k, c), X = L^{-1} row-major in T (zero above its
 // diagonal inside the diagonal blocks), `per` workgroups per supernode.
 template <int MODE>
-__global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int per, int ks) {
+__global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 2 : 4))) void k_fac_dinv(FacArgs A, int h0, int per, int ks) {
     const int tile = MODE == 1 ? (int)blockIdx.x / ks : (int)blockIdx.x, z = MODE == 1 ? (int)blockIdx.x % ks : 0;
     MmaAcc acc;
     mma_zero(acc);
@@ -1993,7 +1993,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_dinv(FacArgs A, int h0, int p
 // G = F_os D^{-1} into panel rows [w, m): 64 x 64 tile (rows of G, columns), `per`
 // workgroups per supernode. A(i, k) = F_os(r0 + i, k) (column-major), B(k, j) = D^{-1}(k, q0 + j).
 template <int MODE>
-__global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per, int ks) {
+__global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 2 : 4))) void k_fac_g(FacArgs A, int h0, int per, int ks) {
     const int tile = MODE == 1 ? (int)blockIdx.x / ks : (int)blockIdx.x, z = MODE == 1 ? (int)blockIdx.x % ks : 0;
     MmaAcc acc;
     mma_zero(acc);
@@ -2062,7 +2062,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_g(FacArgs A, int h0, int per,
 // U -= G F_os^T (lower part): lower 64 x 64 tile (I, J) of U(a, b), a >= b, `per` workgroups
 // per supernode. A(i, k) = G(a0 + i, k) (panel rows [w, m)), B(k, j) = F_os(b0 + j, k).
 template <int MODE>
-__global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per, int ks) {
+__global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 2 : 4))) void k_fac_u(FacArgs A, int h0, int per, int ks) {
     const int tile = MODE == 1 ? (int)blockIdx.x / ks : (int)blockIdx.x, z = MODE == 1 ? (int)blockIdx.x % ks : 0;
     MmaAcc acc;
     mma_zero(acc);
@@ -2122,7 +2122,7 @@ __global__ __launch_bounds__(kFacBlock) void k_fac_u(FacArgs A, int h0, int per,
 // = X(kb + j, kb + t) (row-major X). X block (s, J), J = q - split < s: X_sJ = -X_ss W_sJ, where
 // W_sJ = sum_{K=J}^{s-1} L_sK X_KJ, accumulated in X's place by the second launches of the
 // earlier steps. Each tile is read whole before it is written.
-__global__ __launch_bounds__(kFacBlock) void k_fac_trsm(FacArgs A, int h0, int s, int split, int per) {
+__global__ __launch_bounds__(kFacBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fac_trsm(FacArgs A, int h0, int s, int split, int per) {
     const FacDev S = A.sn[A.hlist[h0 + blockIdx.x / per]];
     const int w = S.w, m = S.m, kb = s * kFb, q = (int)(blockIdx.x % per);
     MD_G double *P = A.P + S.poff;
