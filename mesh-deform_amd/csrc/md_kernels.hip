@@ -1732,9 +1732,8 @@ __device__ __forceinline__ void inv_blockcol(const double (*Lsh)[kLdsPad], const
     }
 }
 
-// LDS of the pivot block, two arrays (separate arrays let the compiler keep the panel buffers' and
-// L's accesses apart): the panel buffers (later the inverse's diagonal blocks; sized to also hold
-// the staged A operand of k_fac_syrk), and L (also holds the staged B operand).
+// LDS of the pivot block, two arrays: the panel buffers (later the inverse's diagonal blocks; sized
+// to also hold the staged A operand of k_fac_syrk), and L (also holds the staged B operand).
 constexpr int kPbPanels = 2 * 2 * kPanel * kFb;
 constexpr int kPbPan = kPbPanels > kMk * kMpad ? kPbPanels : kMk * kMpad;
 constexpr int kPbL = kFb * kLdsPad > kMk * kMpad ? kFb * kLdsPad : kMk * kMpad;
