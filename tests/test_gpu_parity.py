@@ -230,6 +230,22 @@ def test_device_factorization_matches_host(ctx, which):
     assert meshgen.relative_error(out[0], a.mesh) < POS_TOL_FP64
 
 
+def test_device_factorization_bitwise_repeatable(ctx):
+    """Two fresh meshes with the same frame give bitwise-identical positions after 3 iterations:
+    the device factorization has no floating-point atomics, and the partial products of the top
+    heights, split over K (2 and 4 ways on this 67k bar), are summed in split order."""
+    pr = meshgen.bar_problem(400, 40, 40)
+    say("case: repeatable device factorization", pr.name, "n", pr.n)
+    idx, tgt = pr.constraints()
+    outs = []
+    for _ in range(2):
+        m, ok = run_gpu(ctx, pr.xyz, pr.tri, idx, tgt, 3)
+        assert ok
+        outs.append(np.array(m.mesh, copy=True))
+        m.close()
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 def test_energy_trace_monotone(ctx):
     say("case: energy trace")
     pr = meshgen.sphere_problem()
