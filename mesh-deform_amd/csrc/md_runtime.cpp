@@ -712,6 +712,37 @@ hipError_t copy_on_stream(md_mesh *m, void *dst, const void *src, size_t bytes, 
 // Uploads the caller's positions (original order, 3 per vertex) and spreads them to internal
 // order on the device: into p (rest, may be null) and pp (current positions with the targets
 // of the constrained vertices pasted, may be null). Stream-ordered, no synchronisation.
+// Pinned staging of the packed positions on their way back: downloads of at least kPinMinBytes,
+// from a mesh's second one on, in at most kPinChunks chunks of at least kPinChunkMin bytes, each
+// chunk copied to the caller's buffer as soon as its DMA has arrived. For 3.9 MB (the 164k
+// icosphere) a pageable copy back measured 231 us, a pinned DMA 77 us and the host copy out of
+// the pinned buffer 131 us; the chunked form took a frame's download from 0.24 to 0.20 ms. (The
+// same staging toward the device made the upload slower: 0.32 against 0.21 ms.)
+constexpr size_t kPinMinBytes = size_t(2) << 20, kPinChunkMin = size_t(1) << 20;
+static bool pin_ready(md_mesh *m, size_t bytes, int calls) {
+    if (bytes < kPinMinBytes || calls < 2) return false;
+    if (m->xyz_pin_bytes >= bytes) return true;
+    m->release_pin();
+    void *h = nullptr;
+    if (hipHostMalloc(&h, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;  // the direct copy remains
+    }
+    m->xyz_pin = static_cast<double *>(h);
+    m->xyz_pin_bytes = bytes;
+    for (auto &ev : m->pin_ev)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            m->release_pin();
+            return false;
+        }
+    return true;
+}
+static size_t pin_chunk(size_t bytes) {
+    const size_t c = std::max(kPinChunkMin, (bytes + md_mesh::kPinChunks - 1) / md_mesh::kPinChunks);
+    return (c + 7) & ~size_t(7);
+}
+
 hipError_t upload_positions(md_mesh *m, const double *xyz, d4 *p, d4 *pp) {
     hipStream_t st = m->ctx->stream;
     const int n = m->n, nf = m->topo.nf;
@@ -736,8 +767,26 @@ hipError_t download_positions(md_mesh *m, double *xyz) {
     if (m->n == 0) return hipSuccess;
     launch_pack_xyz(st, m->dev.pp, m->d_orig, m->n, m->d_xyz);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(xyz, m->d_xyz, sizeof(double) * 3 * (size_t)m->n, hipMemcpyDeviceToHost, st);
+    const size_t bytes = sizeof(double) * 3 * (size_t)m->n;
+    if (e == hipSuccess && pin_ready(m, bytes, ++m->downloads)) {
+        // every chunk's DMA queued (each followed by its event), then each chunk copied out as
+        // soon as it has arrived
+        const size_t ch = pin_chunk(bytes);
+        char *pin = reinterpret_cast<char *>(m->xyz_pin), *out = reinterpret_cast<char *>(xyz);
+        const char *src = reinterpret_cast<const char *>(m->d_xyz);
+        int c = 0;
+        for (size_t off = 0; off < bytes && e == hipSuccess; off += ch, ++c) {
+            e = hipMemcpyAsync(pin + off, src + off, std::min(ch, bytes - off), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipEventRecord(m->pin_ev[c], st);
+        }
+        c = 0;
+        for (size_t off = 0; off < bytes && e == hipSuccess; off += ch, ++c) {
+            e = hipEventSynchronize(m->pin_ev[c]);
+            if (e == hipSuccess) std::memcpy(out + off, pin + off, std::min(ch, bytes - off));
+        }
+        return e;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(xyz, m->d_xyz, bytes, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e;
 }

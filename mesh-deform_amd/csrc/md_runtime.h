@@ -204,6 +204,27 @@ struct md_mesh {
         fac_ws = nullptr;
         fac_ws_bytes = 0;
     }
+    // pinned staging of the packed positions for meshes that download them on every call (from
+    // the second download on): chunked copies whose DMA overlaps the host copy to the caller's
+    // buffer; pin_ev[c] marks the arrival of chunk c
+    static constexpr int kPinChunks = 8;
+    double *xyz_pin = nullptr;
+    size_t xyz_pin_bytes = 0;
+    hipEvent_t pin_ev[kPinChunks] = {};
+    int downloads = 0;
+    void release_pin() {
+        if (xyz_pin) hipHostFree(xyz_pin);
+        xyz_pin = nullptr;
+        xyz_pin_bytes = 0;
+        for (auto &ev : pin_ev)
+            if (ev) {
+                hipEventDestroy(ev);
+                ev = nullptr;
+            }
+    }
     bool fp64() const { return cfg.precision == MD_PREC_FP64; }
-    ~md_mesh() { release_fac_ws(); }
+    ~md_mesh() {
+        release_fac_ws();
+        release_pin();
+    }
 };

@@ -348,6 +348,44 @@ def test_reference_mode_keeps_factor_workspace(ctx, golden_dir):
         assert err < POS_TOL_FP64, (frame, err)
 
 
+def test_large_mesh_staged_position_copies_match_oracle(ctx):
+    """The 164k icosphere (3.9 MB of positions): from a mesh's second download on, its positions
+    come back through the pinned, chunked staging. Reference rest mode over three frames, then
+    fixed rest mode with the constrained set changed (the rest pose and the current positions
+    uploaded in one preparation): every frame matches the oracle."""
+    pr = meshgen.noisy_icosphere_problem(7)
+    idx, tgt = pr.constraints()
+    say("case: staged position downloads, reference frames", pr.name, "n", pr.n)
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(rest_mode=mdarap.REST_REFERENCE))
+    a = oracle.Arap(pr.xyz, pr.tri)
+    for frame in range(3):
+        t = tgt + 0.01 * frame
+        m.set_constraints(idx, t)
+        a.set_constraints(idx, t)
+        assert m.deform(2)
+        assert a.deform(2)
+        err = meshgen.relative_error(m.mesh, a.mesh)
+        say("  frame", frame, "rel err", err)
+        assert err < POS_TOL_FP64, (frame, err)
+    m.close()
+    say("case: staged position downloads, fixed rest with a new constrained set")
+    m = mdarap.Mesh(ctx, pr.xyz, pr.tri, mdarap.config(rest_mode=mdarap.REST_FIXED))
+    a = oracle.Arap(pr.xyz, pr.tri)
+    extra = int(np.setdiff1d(np.arange(pr.n), idx)[0])
+    for obj in (m, a):
+        obj.set_constraints(idx, tgt)
+    assert m.deform(2)
+    assert a.deform(2, rest_mode=1)
+    for obj in (m, a):
+        obj.set_constraint(extra, pr.xyz[extra] + [0.0, 0.0, 0.05])
+    assert m.deform(2)
+    assert a.deform(2, rest_mode=1)
+    err = meshgen.relative_error(m.mesh, a.mesh)
+    say("  rel err", err)
+    assert err < POS_TOL_FP64
+    m.close()
+
+
 def test_fixed_rest_new_constraint_set_matches_oracle(ctx, golden_dir):
     """Fixed rest mode with the constrained set changed between frames: the rest pose and its
     weights stay those of the first call while the current mesh is the initial guess (the
